@@ -1,0 +1,194 @@
+/*
+ * qs.h — C ABI of libqs, the MI355X engine for the second-quantization hot path:
+ * operator x wavefunction on Slater-determinant bitstrings (H*psi), Lanczos
+ * tridiagonalization, block-Lanczos, lowest eigenstates and the
+ * continued-fraction Green's function.
+ *
+ * Conventions shared by every entry point
+ *  - Determinants of NF <= 128 fermion modes are packed into NW = 1 (NF <= 64)
+ *    or NW = 2 (NF <= 128) uint64 words, word 0 most significant. Mode i sits at
+ *    bit (64*NW - 1 - i) of the 64*NW-bit integer, so numeric order equals the
+ *    memcmp order of the reference's byte layout (mode i = bit 128>>(i&7) of
+ *    byte i>>3, Core/OperatorWaveFunction.cpp:10-12).
+ *  - A ladder term of length l is l uint16 codes o[0..l-1]; bit 15 set = create,
+ *    low 15 bits = mode. o[0] is applied first, i.e. it is the rightmost written
+ *    factor (Core/StructDef.cpp:46-47, Core/OperatorWaveFunction.cpp:98-116).
+ *  - Vectors are complex FP64, interleaved (re, im), one entry per basis
+ *    determinant in the basis' order, resident in device memory (HBM).
+ *  - Every function returns a qs_status; on failure qs_last_error() holds a
+ *    message. No function calls exit().
+ *  - One qs_ctx per host thread. A ctx owns one HIP stream; all device work of
+ *    a call is ordered on it, and calls return after that work has completed
+ *    unless stated otherwise.
+ */
+#ifndef QS_H
+#define QS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  QS_OK = 0,
+  QS_ENOMEM_DEVICE = 1, /* hipMalloc failed (cf. reference status 1 = allocation) */
+  QS_ENOMEM_HOST = 2,
+  QS_EINVAL = 3,
+  QS_EHIP = 4,
+  QS_ENOTCONVERGED = 5,
+  QS_EUNSUPPORTED = 6
+} qs_status;
+
+typedef struct qs_ctx qs_ctx;     /* device, stream, scratch, last error */
+typedef struct qs_basis qs_basis; /* sorted determinant set on the device (+ rank tables) */
+typedef struct qs_op qs_op;       /* compiled ladder-term table on the device */
+typedef struct qs_hop qs_hop;     /* an operator bound to (row basis, column basis) */
+
+/* ---- context ------------------------------------------------------------ */
+qs_status qs_ctx_create(int device, qs_ctx **ctx);
+qs_status qs_ctx_destroy(qs_ctx *ctx);
+const char *qs_last_error(const qs_ctx *ctx);
+/* Device memory helpers so C callers need no HIP headers. */
+qs_status qs_dev_alloc(qs_ctx *ctx, size_t bytes, void **ptr);
+qs_status qs_dev_free(qs_ctx *ctx, void *ptr);
+qs_status qs_memcpy_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes);
+qs_status qs_memcpy_d2h(qs_ctx *ctx, void *dst, const void *src, size_t bytes);
+qs_status qs_sync(qs_ctx *ctx);
+/* The ctx's HIP stream (a hipStream_t), for callers that order their own work
+ * or record timing events on it. */
+void *qs_ctx_stream(qs_ctx *ctx);
+/* async != 0: entry points whose results stay on the device (qs_hop_apply,
+ * qs_hop_apply_multi, qs_apply) return after enqueueing their kernels instead
+ * of waiting for them; the caller orders later work on qs_ctx_stream() or
+ * calls qs_sync(). Default 0. */
+qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
+
+/* ---- basis (restricted-occupation determinant sets) --------------------- */
+/* Enumerate every determinant of nf modes whose occupation inside each mask r
+ * lies in [lo[r], hi[r]] (the test of QDetNotRestricted,
+ * Core/OperatorWaveFunction.cpp:2260-2286, restriction tables as parsed by
+ * luaL_setarray_DeterminatRestrictionType, TableToArray.cpp:3005-3068).
+ * masks: nres rows of ceil(nf/8) bytes in the reference byte layout.
+ * The result is sorted ascending and carries exact rank tables, so lookups are
+ * O(ceil(nf/8)) table steps. Replaces the reference's hash-set basis
+ * (Core/Hash.cpp, Core/ComplexWaveFunctionArithmetic.cpp:64-353). */
+qs_status qs_basis_enumerate(qs_ctx *ctx, int nf, int nres, const uint8_t *masks,
+                             const int *lo, const int *hi, qs_basis **out);
+/* Count only (no device allocation of the determinant list). */
+qs_status qs_basis_count(qs_ctx *ctx, int nf, int nres, const uint8_t *masks,
+                         const int *lo, const int *hi, int64_t *n);
+/* Build a basis from an explicit list of n determinants (host, NW words each,
+ * any order, duplicates removed). Lookups use binary search. */
+qs_status qs_basis_from_words(qs_ctx *ctx, int nf, const uint64_t *words, int64_t n,
+                              qs_basis **out);
+/* Closure of a determinant set under an operator: the smallest set containing
+ * `seed` and every target t|d> != 0 of op's terms, optionally filtered by
+ * restrictions (nres may be 0), iterated until nothing new appears or the set
+ * exceeds max_dim (QS_EINVAL then). This is the reachable space the reference
+ * grows implicitly by repeated H*psi (Core/BlockLanczos.cpp:5849-5858,
+ * LuaInterface/ssp.cpp:2265-2316). */
+qs_status qs_basis_closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,
+                           const uint8_t *masks, const int *lo, const int *hi,
+                           int64_t max_dim, qs_basis **out);
+int64_t qs_basis_size(const qs_basis *b);
+int qs_basis_nf(const qs_basis *b);
+int qs_basis_words(const qs_basis *b); /* NW: 1 or 2 */
+/* Copy the sorted determinant words to host (n*NW uint64). */
+qs_status qs_basis_export(qs_ctx *ctx, const qs_basis *b, uint64_t *words);
+/* Device pointer to the sorted determinant words (n*NW uint64). */
+const uint64_t *qs_basis_device_words(const qs_basis *b);
+/* Index of each of n host determinants in b (-1 when absent). */
+qs_status qs_basis_index(qs_ctx *ctx, const qs_basis *b, const uint64_t *words, int64_t n,
+                         int64_t *idx);
+qs_status qs_basis_destroy(qs_basis *b);
+
+/* ---- operators ---------------------------------------------------------- */
+/* nterms ladder strings: len[t] codes each, concatenated in `codes`;
+ * values: 2*nterms doubles (re, im). Terms that vanish identically (e.g. c+_i c+_i)
+ * are dropped. Fermions only (the reference rejects NB>0 wavefunctions,
+ * LuaInterface/ssp.cpp:950-953). Replaces OperatorType / OperatorFixedLengthType
+ * (Core/StructDef.cpp:223-249) as input to OperatorPsi (Core/OperatorWaveFunction.cpp:5598). */
+qs_status qs_op_create(qs_ctx *ctx, int nf, int nterms, const int *len, const uint16_t *codes,
+                       const double *values, qs_op **out);
+int qs_op_nterms(const qs_op *op);
+qs_status qs_op_destroy(qs_op *op);
+
+/* ---- operator on a basis: apply ---------------------------------------- */
+enum { QS_HOP_MATFREE = 0, QS_HOP_ASSEMBLED = 1, QS_HOP_AUTO = 2 };
+/* Bind op to rows = `rows` basis and columns = `cols` basis. ASSEMBLED builds
+ * the sliced connection stream (column index + term id per nonzero) once and
+ * reuses it on every apply; MATFREE recomputes targets on the fly; AUTO
+ * assembles when the stream fits in `mem_budget_bytes` (0 = 50% of free HBM). */
+qs_status qs_hop_create(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols,
+                        int mode, size_t mem_budget_bytes, qs_hop **out);
+/* Number of (row, term) pairs with nonzero phase whose source lies in cols,
+ * and the stored off-diagonal stream length (diagonal terms are pre-summed). */
+qs_status qs_hop_stats(const qs_hop *h, int64_t *nnz_eff, int64_t *nnz_stored,
+                       int64_t *stream_bytes, int *assembled);
+/* y = O x (pull form: y_i = sum_t V_t <d_i| t |d_j> x_j). x over cols, y over
+ * rows, device pointers. Equals OperatorPsiRestricted (Core/OperatorWaveFunction.cpp:2288)
+ * with the row basis as restriction set, and OperatorPsiConserveBasis (:245) when
+ * rows == cols. */
+qs_status qs_hop_apply(qs_ctx *ctx, const qs_hop *h, const double *x, double *y);
+/* Same for k vectors stored back to back (vector v at x + 2*v*ncols). */
+qs_status qs_hop_apply_multi(qs_ctx *ctx, const qs_hop *h, int k, const double *x, double *y);
+qs_status qs_hop_destroy(qs_hop *h);
+
+/* One-shot apply through a temporary matrix-free binding. */
+qs_status qs_apply(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols,
+                   const double *x, double *y);
+/* <x|O|x> over one basis (PsiOperatorPsi, Core/OperatorWaveFunction.cpp:5680). */
+qs_status qs_expect(qs_ctx *ctx, const qs_op *op, const qs_basis *b, const double *x,
+                    double *re, double *im);
+
+/* ---- vector helpers (device pointers, n complex entries) ---------------- */
+/* <a|b> with a conjugated (ComplexWaveFunctionDotProductSameBasis,
+ * Core/ComplexWaveFunctionArithmetic.cpp:961). Fixed-order, deterministic. */
+qs_status qs_dot(qs_ctx *ctx, int64_t n, const double *a, const double *b, double *re,
+                 double *im);
+qs_status qs_norm(qs_ctx *ctx, int64_t n, const double *a, double *nrm);
+
+/* ---- Lanczos tridiagonalization (spectra) ------------------------------- */
+/* Scalar Lanczos on h (rows == cols) from v0 (device, not modified):
+ * a[0]=0, b[0]=|v0|, a[i]=Re<Hv_{i-1}|v_{i-1}>, b[i]=|r_i|, no
+ * reorthogonalization — the recurrence of LanczosTriDiagonalizeCC
+ * (Core/Lanczos.cpp:2169-2264) on a fixed basis, epsilon = 0.
+ * a_out: nsteps+1 doubles, b_out: nsteps doubles (host). */
+qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, int nsteps, double *a_out,
+                     double *b_out);
+/* Block Lanczos for k start vectors V0 (k vectors back to back, device):
+ * block recurrence of LanczosBlockTriDiagonalizeCC (Core/BlockLanczos.cpp:2131-2371):
+ * Loewdin-orthonormalised start block, B0 = <v|V0>, per step A_k = V_k^+ H V_k,
+ * local re-orthogonalization against the two previous blocks, singular values
+ * below sv_cut dropped. A_out: (nsteps+1)*k*k complex, B_out: nsteps*k*k complex
+ * (row-major blocks, host); *kept receives the block size of each step
+ * (nsteps+1 ints, may be NULL). */
+qs_status qs_block_lanczos(qs_ctx *ctx, const qs_hop *h, int k, const double *V0, int nsteps,
+                           double sv_cut, double *A_out, double *B_out, int *kept);
+
+/* ---- lowest eigenstates (Eigensystem) ----------------------------------- */
+/* nstates lowest eigenpairs of Hermitian h (rows == cols) by thick-restarted
+ * block Lanczos with full re-orthogonalization; block size = max(nstates, 1).
+ * Converged when every wanted state has |H y - E y|^2 <= tol * max(1, sum E^2)/nstates
+ * (the variance test of BlockGroundState, Core/BlockLanczos.cpp:5869-5883).
+ * start: nstates device vectors or NULL (seeded pseudo-random start).
+ * E_out: nstates doubles (host, ascending); vecs: nstates device vectors (out). */
+qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int max_krylov,
+                           int max_restarts, uint64_t seed, const double *start, double *E_out,
+                           double *vecs, double *variance_out, int *iterations_out);
+
+/* ---- continued-fraction Green's function -------------------------------- */
+/* G(w_i) = a0 + b0^2/(z - a1 - b1^2/(z - a2 - ...)), z = w_i + E0 + i Gamma/2,
+ * w_i = emin + i*(emax-emin)/ne, i = 0..ne (ne+1 points) — TriDiagonalMatrixToSpectrum
+ * (Core/Spectral.cpp:1433-1480). nspec spectra, a: nspec*(n+1), b: nspec*n,
+ * E0: nspec (host arrays); re/im: nspec*(ne+1) (host). Evaluated on the GPU. */
+qs_status qs_cf_spectrum(qs_ctx *ctx, int nspec, int n, const double *a, const double *b,
+                         const double *E0, double emin, double emax, int ne, double gamma,
+                         double *re, double *im);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QS_H */

@@ -1,0 +1,656 @@
+// Operator compilation and H*psi on a fixed basis, in pull form:
+//   y_i = sum_t V_t <d_i| t |d_j> x_j,   d_j = t^+ d_i
+// computed row by row without atomics. Two execution paths share one hit
+// enumerator: the matrix-free kernel recomputes every (row, term) hit on each
+// call; the assembled path stores each row's off-diagonal hits once as a
+// sliced-ELL stream (64 rows per slice, column-major inside the slice) of
+// column index + signed term id, pre-sums the diagonal terms per row, and then
+// applies the operator as a streaming gather.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "qs_internal.h"
+#include "qs_kernels.h"
+
+// ---------------------------------------------------------------------------
+// term compilation (host)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct ModeBits {
+  int nw;
+  void set(uint64_t *m, int i) const { m[i >> 6] |= 1ULL << (63 - (i & 63)); }
+  // modes with index < i
+  void low(uint64_t *m, int i) const {
+    for (int w = 0; w < QS_MAX_WORDS; ++w) m[w] = 0;
+    int wi = i >> 6, b = i & 63;
+    for (int w = 0; w < wi; ++w) m[w] = ~0ULL;
+    if (b > 0) m[wi] = ~0ULL << (64 - b);
+  }
+};
+
+int parity_and(const uint64_t *a, const uint64_t *b, int nw) {
+  int p = 0;
+  for (int w = 0; w < nw; ++w) p ^= __builtin_popcountll(a[w] & b[w]) & 1;
+  return p;
+}
+
+// Follows the action of OperatorLader (Core/OperatorWaveFunction.cpp:98-116):
+// o[0] is applied first; each c+/c of mode q contributes (-1)^(occupied modes
+// below q) in the intermediate determinant (OperatorCr/OperatorAn, :7-96).
+// Returns false when the string vanishes on every determinant.
+bool compile_term(int nf, int len, const uint16_t *o, double re, double im, TermHost &t,
+                  std::string &err) {
+  const int nw = nf <= 64 ? 1 : 2;
+  ModeBits mb{nw};
+  std::map<int, std::pair<int, int>> st;  // mode -> (initial occupancy, current occupancy)
+  uint64_t G[QS_MAX_WORDS] = {0, 0};
+  t = TermHost();
+  int c = 0;
+  for (int k = 0; k < len; ++k) {
+    const bool cr = (o[k] & 0x8000u) != 0;
+    const int q = o[k] & 0x7FFF;
+    if (q >= nf) {
+      err = "ladder operator index beyond the number of fermion modes (bosons unsupported)";
+      return false;
+    }
+    auto it = st.find(q);
+    if (it == st.end()) {
+      st[q] = {cr ? 0 : 1, cr ? 0 : 1};
+      it = st.find(q);
+    }
+    if (it->second.second != (cr ? 0 : 1)) return false;  // Pauli: always zero
+    uint64_t L[QS_MAX_WORDS];
+    mb.low(L, q);
+    for (int w = 0; w < nw; ++w) t.P[w] ^= L[w];
+    c ^= parity_and(G, L, nw);
+    it->second.second ^= 1;
+    uint64_t bit[QS_MAX_WORDS] = {0, 0};
+    mb.set(bit, q);
+    for (int w = 0; w < nw; ++w) G[w] ^= bit[w];
+    t.nchange += cr ? 1 : -1;
+  }
+  for (auto &kv : st) {
+    mb.set(t.C, kv.first);
+    if (kv.second.second) mb.set(t.V, kv.first);
+  }
+  for (int w = 0; w < nw; ++w) t.F[w] = G[w];
+  c ^= parity_and(t.F, t.P, nw);
+  t.sgn = (uint32_t)c;
+  t.re = re;
+  t.im = im;
+  return true;
+}
+
+}  // namespace
+
+extern "C" qs_status qs_op_create(qs_ctx *ctx, int nf, int nterms, const int *len,
+                                  const uint16_t *codes, const double *values, qs_op **out) {
+  if (!ctx || !out || nf <= 0 || nf > 128 || nterms < 0) return QS_EINVAL;
+  *out = nullptr;
+  const int nw = nf <= 64 ? 1 : 2;
+  // compile, then merge terms with identical masks (sign folded into the value)
+  std::map<std::tuple<uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t>,
+           size_t>
+      seen;
+  std::vector<TermHost> terms;
+  size_t off = 0;
+  for (int t = 0; t < nterms; ++t) {
+    if (len[t] < 0) return qs_fail(ctx, QS_EINVAL, "term %d has negative length", t);
+    TermHost th;
+    std::string err;
+    bool ok = compile_term(nf, len[t], codes + off, values[2 * t], values[2 * t + 1], th, err);
+    off += (size_t)len[t];
+    if (!err.empty()) return qs_fail(ctx, QS_EINVAL, "term %d: %s", t, err.c_str());
+    if (!ok) continue;
+    if (th.sgn) {
+      th.re = -th.re;
+      th.im = -th.im;
+      th.sgn = 0;
+    }
+    auto key = std::make_tuple(th.C[0], th.C[1], th.V[0], th.V[1], th.F[0], th.F[1], th.P[0], th.P[1]);
+    auto it = seen.find(key);
+    if (it == seen.end()) {
+      seen[key] = terms.size();
+      terms.push_back(th);
+    } else {
+      terms[it->second].re += th.re;
+      terms[it->second].im += th.im;
+    }
+  }
+  std::vector<TermHost> kept;
+  for (auto &t : terms)
+    if (t.re != 0.0 || t.im != 0.0) kept.push_back(t);
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  qs_op *op = new qs_op();
+  op->nf = nf;
+  op->nw = nw;
+  op->nterms = (int)kept.size();
+  op->h_terms = kept;
+  const size_t tsz = nw == 1 ? sizeof(TermD<1>) : sizeof(TermD<2>);
+  std::vector<char> buf(std::max<size_t>(1, kept.size()) * tsz, 0);
+  std::vector<double> coef(std::max<size_t>(1, kept.size()) * 4, 0.0);
+  for (size_t i = 0; i < kept.size(); ++i) {
+    const TermHost &h = kept[i];
+    const bool diag = (h.F[0] | h.F[1]) == 0;
+    if (nw == 1) {
+      TermD<1> d;
+      d.C[0] = h.C[0]; d.V[0] = h.V[0]; d.F[0] = h.F[0]; d.P[0] = h.P[0];
+      d.re = h.re; d.im = h.im; d.sgn = h.sgn; d.diag = diag;
+      memcpy(buf.data() + i * tsz, &d, sizeof d);
+    } else {
+      TermD<2> d;
+      for (int w = 0; w < 2; ++w) { d.C[w] = h.C[w]; d.V[w] = h.V[w]; d.F[w] = h.F[w]; d.P[w] = h.P[w]; }
+      d.re = h.re; d.im = h.im; d.sgn = h.sgn; d.diag = diag;
+      memcpy(buf.data() + i * tsz, &d, sizeof d);
+    }
+    coef[4 * i + 0] = h.re;
+    coef[4 * i + 1] = h.im;
+    coef[4 * i + 2] = -h.re;
+    coef[4 * i + 3] = -h.im;
+  }
+  qs_status st = qs_malloc(ctx, &op->d_terms, buf.size());
+  if (st == QS_OK) st = qs_malloc(ctx, (void **)&op->d_coef, coef.size() * 8);
+  if (st != QS_OK) {
+    qs_op_destroy(op);
+    return st;
+  }
+  hipError_t e = hipMemcpyAsync(op->d_terms, buf.data(), buf.size(), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(op->d_coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    qs_op_destroy(op);
+    return qs_hip_fail(ctx, e, "qs_op_create", __FILE__, __LINE__);
+  }
+  *out = op;
+  return QS_OK;
+}
+
+extern "C" int qs_op_nterms(const qs_op *op) { return op ? op->nterms : -1; }
+
+extern "C" qs_status qs_op_destroy(qs_op *op) {
+  if (!op) return QS_OK;
+  if (op->d_terms) (void)hipFree(op->d_terms);
+  if (op->d_coef) (void)hipFree(op->d_coef);
+  delete op;
+  return QS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hit enumeration (device)
+// ---------------------------------------------------------------------------
+#define QS_TILE 256   // terms staged in LDS per pass
+#define QS_BLOCK 256  // threads per block; 4 wavefronts = 4 slices of 64 rows
+
+struct Lookup {
+  const uint2 *tbl;      // rank table of the column basis, or null
+  uint32_t smax;
+  int nch;
+  const uint64_t *words; // sorted column determinants (binary search when tbl is null)
+  int64_t n;
+};
+
+template <int NW>
+__device__ __forceinline__ int64_t col_lookup(const uint64_t *s, const Lookup &L) {
+  if (L.tbl) {
+    uint32_t st = 0, r = 0;
+    for (int c = 0; c < L.nch; ++c) {
+      uint32_t v = (uint32_t)(s[c >> 3] >> (56 - 8 * (c & 7))) & 0xFFu;
+      uint2 e = L.tbl[((size_t)c * L.smax + st) * 256 + v];
+      if (e.x == QS_DEAD) return -1;
+      r += e.y;
+      st = e.x;
+    }
+    return r;
+  }
+  int64_t lo = 0, hi = L.n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    const uint64_t *m = L.words + mid * NW;
+    bool lt = NW == 1 ? (m[0] < s[0]) : (m[0] < s[0] || (m[0] == s[0] && m[NW - 1] < s[NW - 1]));
+    if (lt) lo = mid + 1; else hi = mid;
+  }
+  if (lo < L.n) {
+    bool eq = true;
+    for (int w = 0; w < NW; ++w) eq = eq && (L.words[lo * NW + w] == s[w]);
+    if (eq) return lo;
+  }
+  return -1;
+}
+
+// Stage terms [base, base+cnt) into LDS.
+template <int NW>
+__device__ __forceinline__ void stage_terms(TermD<NW> *sh, const TermD<NW> *__restrict__ g, int base,
+                                            int cnt) {
+  const int nwords = cnt * (int)(sizeof(TermD<NW>) / 8);
+  const uint64_t *src = reinterpret_cast<const uint64_t *>(g + base);
+  uint64_t *dst = reinterpret_cast<uint64_t *>(sh);
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) dst[i] = src[i];
+}
+
+// Visit every hit of row `row` (determinant d) for the staged terms.
+// f(term_index, column, parity, is_diagonal)
+template <int NW, class F>
+__device__ __forceinline__ void visit_tile(const uint64_t *d, int64_t row, const TermD<NW> *sh, int cnt,
+                                           int tbase, bool same, const Lookup &L, F &&f) {
+  for (int j = 0; j < cnt; ++j) {
+    const TermD<NW> &t = sh[j];
+    bool hit = true;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) hit = hit && ((d[w] & t.C[w]) == t.V[w]);
+    if (!hit) continue;
+    const bool dg = t.diag && same;
+    int64_t col;
+    if (dg) {
+      col = row;
+    } else {
+      uint64_t s[NW];
+#pragma unroll
+      for (int w = 0; w < NW; ++w) s[w] = d[w] ^ t.F[w];
+      col = col_lookup<NW>(s, L);
+    }
+    if (col < 0) continue;
+    uint32_t par = t.sgn;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) par ^= (uint32_t)__popcll(d[w] & t.P[w]) & 1u;
+    f(tbase + j, col, par, dg);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// matrix-free apply: one lane per row
+// ---------------------------------------------------------------------------
+template <int NW>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_apply_mf(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms,
+           int nterms, Lookup L, int same, const double2 *__restrict__ x, double2 *__restrict__ y) {
+  __shared__ TermD<NW> sh[QS_TILE];
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = row < nrows;
+  uint64_t d[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) d[w] = active ? rows[row * NW + w] : 0;
+  double ar = 0.0, ai = 0.0;
+  for (int base = 0; base < nterms; base += QS_TILE) {
+    const int cnt = min(QS_TILE, nterms - base);
+    __syncthreads();
+    stage_terms<NW>(sh, terms, base, cnt);
+    __syncthreads();
+    if (active) {
+      visit_tile<NW>(d, row, sh, cnt, base, same != 0, L, [&](int tj, int64_t col, uint32_t par, bool) {
+        const TermD<NW> &t = sh[tj - base];
+        const double2 xv = x[col];
+        double cr = par ? -t.re : t.re, ci = par ? -t.im : t.im;
+        ar = fma(cr, xv.x, fma(-ci, xv.y, ar));
+        ai = fma(cr, xv.y, fma(ci, xv.x, ai));
+      });
+    }
+  }
+  if (active) y[row] = make_double2(ar, ai);
+}
+
+// ---------------------------------------------------------------------------
+// assembly: count pass and fill pass
+// ---------------------------------------------------------------------------
+template <int NW>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_count(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms, int nterms,
+        Lookup L, int same, uint16_t *__restrict__ rowcnt, uint32_t *__restrict__ slice_len,
+        double2 *__restrict__ diag, unsigned long long *__restrict__ nnz_eff) {
+  __shared__ TermD<NW> sh[QS_TILE];
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = row < nrows;
+  uint64_t d[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) d[w] = active ? rows[row * NW + w] : 0;
+  uint32_t off = 0, eff = 0;
+  double dr = 0.0, di = 0.0;
+  for (int base = 0; base < nterms; base += QS_TILE) {
+    const int cnt = min(QS_TILE, nterms - base);
+    __syncthreads();
+    stage_terms<NW>(sh, terms, base, cnt);
+    __syncthreads();
+    if (active) {
+      visit_tile<NW>(d, row, sh, cnt, base, same != 0, L, [&](int tj, int64_t, uint32_t par, bool dg) {
+        ++eff;
+        if (dg) {
+          const TermD<NW> &t = sh[tj - base];
+          dr += par ? -t.re : t.re;
+          di += par ? -t.im : t.im;
+        } else {
+          ++off;
+        }
+      });
+    }
+  }
+  if (active) {
+    rowcnt[row] = (uint16_t)off;
+    if (diag) diag[row] = make_double2(dr, di);
+  }
+  // slice = wavefront: max row length and the wave's nnz_eff
+  uint32_t m = off;
+  unsigned long long e = eff;
+  for (int o = 32; o > 0; o >>= 1) {
+    m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    e += (unsigned long long)__shfl_xor((long long)e, o);
+  }
+  const int64_t slice = row >> 6;
+  if ((threadIdx.x & 63) == 0 && slice * 64 < nrows) {
+    slice_len[slice] = m;
+    atomicAdd(nnz_eff, e);
+  }
+}
+
+template <int NW>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_fill(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms, int nterms,
+       Lookup L, int same, const uint64_t *__restrict__ slice_ptr, uint32_t *__restrict__ colv,
+       uint16_t *__restrict__ tidv) {
+  __shared__ TermD<NW> sh[QS_TILE];
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = row < nrows;
+  uint64_t d[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) d[w] = active ? rows[row * NW + w] : 0;
+  const int lane = threadIdx.x & 63;
+  const uint64_t base_e = active ? slice_ptr[row >> 6] : 0;
+  uint32_t j = 0;
+  for (int base = 0; base < nterms; base += QS_TILE) {
+    const int cnt = min(QS_TILE, nterms - base);
+    __syncthreads();
+    stage_terms<NW>(sh, terms, base, cnt);
+    __syncthreads();
+    if (active) {
+      visit_tile<NW>(d, row, sh, cnt, base, same != 0, L, [&](int tj, int64_t col, uint32_t par, bool dg) {
+        if (dg) return;
+        const uint64_t e = base_e + (uint64_t)j * 64 + lane;
+        colv[e] = (uint32_t)col;
+        tidv[e] = (uint16_t)(2 * tj + par);
+        ++j;
+      });
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// assembled apply: one wavefront per slice, lane per row
+// ---------------------------------------------------------------------------
+template <int KC>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_spmv(int64_t nrows, int64_t nslices, const uint64_t *__restrict__ slice_ptr,
+       const uint32_t *__restrict__ slice_len, const uint16_t *__restrict__ rowcnt,
+       const uint32_t *__restrict__ colv, const uint16_t *__restrict__ tidv,
+       const double2 *__restrict__ coef, const double2 *__restrict__ diag,
+       const double2 *__restrict__ x, int64_t ldx, double2 *__restrict__ y, int64_t ldy) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t s = wave; s < nslices; s += nwaves) {
+    const int64_t r = s * 64 + lane;
+    const bool active = r < nrows;
+    double ar[KC], ai[KC];
+    spmv_row<KC>(r, active, slice_ptr[s], slice_len[s], rowcnt, colv, tidv, coef, diag, x, ldx, lane,
+                 ar, ai);
+    if (active) {
+#pragma unroll
+      for (int v = 0; v < KC; ++v) y[r * ldy + v] = make_double2(ar[v], ai[v]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static Lookup make_lookup(const qs_basis *cols) {
+  Lookup L;
+  L.tbl = cols->has_rank ? cols->d_tbl : nullptr;
+  L.smax = cols->smax;
+  L.nch = cols->nchunks;
+  L.words = cols->d_words;
+  L.n = cols->n;
+  return L;
+}
+
+static qs_status check_pair(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols) {
+  if (!op || !rows || !cols) return qs_fail(ctx, QS_EINVAL, "null operator or basis");
+  if (op->nf != rows->nf || op->nf != cols->nf)
+    return qs_fail(ctx, QS_EINVAL, "operator has %d modes, bases have %d and %d", op->nf, rows->nf,
+                   cols->nf);
+  return QS_OK;
+}
+
+static qs_status launch_mf(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols,
+                           bool same, const double *x, double *y) {
+  if (rows->n == 0) return QS_OK;
+  Lookup L = make_lookup(cols);
+  const unsigned blocks = (unsigned)((rows->n + QS_BLOCK - 1) / QS_BLOCK);
+  if (op->nw == 1)
+    hipLaunchKernelGGL(k_apply_mf<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, rows->n,
+                       (const TermD<1> *)op->d_terms, op->nterms, L, (int)same, (const double2 *)x,
+                       (double2 *)y);
+  else
+    hipLaunchKernelGGL(k_apply_mf<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, rows->n,
+                       (const TermD<2> *)op->d_terms, op->nterms, L, (int)same, (const double2 *)x,
+                       (double2 *)y);
+  QS_HIP(ctx, hipGetLastError());
+  return QS_OK;
+}
+
+qs_status qs_launch_mf_ext(qs_ctx *ctx, const qs_hop *h, const double *x, double *y) {
+  return launch_mf(ctx, h->op, h->rows, h->cols, h->same, x, y);
+}
+
+static void free_stream(qs_hop *h) {
+  if (h->d_slice_ptr) (void)hipFree(h->d_slice_ptr);
+  if (h->d_slice_len) (void)hipFree(h->d_slice_len);
+  if (h->d_rowcnt) (void)hipFree(h->d_rowcnt);
+  if (h->d_col) (void)hipFree(h->d_col);
+  if (h->d_tid) (void)hipFree(h->d_tid);
+  if (h->d_diag) (void)hipFree(h->d_diag);
+  h->d_slice_ptr = nullptr;
+  h->d_slice_len = nullptr;
+  h->d_rowcnt = nullptr;
+  h->d_col = nullptr;
+  h->d_tid = nullptr;
+  h->d_diag = nullptr;
+}
+
+static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
+  const qs_op *op = h->op;
+  const qs_basis *rows = h->rows;
+  const int64_t n = rows->n;
+  const int64_t nslices = (n + 63) / 64;
+  h->nslices = nslices;
+  if (op->nterms > 32767) return qs_fail(ctx, QS_EUNSUPPORTED, "more than 32767 terms: not assembled");
+  QS_TRY(qs_malloc(ctx, (void **)&h->d_slice_ptr, (size_t)(nslices + 1) * 8));
+  QS_TRY(qs_malloc(ctx, (void **)&h->d_slice_len, (size_t)std::max<int64_t>(nslices, 1) * 4));
+  QS_TRY(qs_malloc(ctx, (void **)&h->d_rowcnt, (size_t)std::max<int64_t>(n, 1) * 2));
+  if (h->same) QS_TRY(qs_malloc(ctx, (void **)&h->d_diag, (size_t)std::max<int64_t>(n, 1) * 16));
+  unsigned long long *d_eff;
+  void *scr;
+  QS_TRY(qs_scratch(ctx, 256, &scr));
+  d_eff = (unsigned long long *)scr;
+  QS_HIP(ctx, hipMemsetAsync(d_eff, 0, 8, ctx->stream));
+  Lookup L = make_lookup(h->cols);
+  const unsigned blocks = (unsigned)((n + QS_BLOCK - 1) / QS_BLOCK);
+  if (n > 0) {
+    if (op->nw == 1)
+      hipLaunchKernelGGL(k_count<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, n,
+                         (const TermD<1> *)op->d_terms, op->nterms, L, (int)h->same, h->d_rowcnt,
+                         h->d_slice_len, (double2 *)h->d_diag, d_eff);
+    else
+      hipLaunchKernelGGL(k_count<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, n,
+                         (const TermD<2> *)op->d_terms, op->nterms, L, (int)h->same, h->d_rowcnt,
+                         h->d_slice_len, (double2 *)h->d_diag, d_eff);
+    QS_HIP(ctx, hipGetLastError());
+  }
+  std::vector<uint32_t> slen((size_t)nslices);
+  unsigned long long eff = 0;
+  if (nslices)
+    QS_HIP(ctx, hipMemcpyAsync(slen.data(), h->d_slice_len, (size_t)nslices * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  QS_HIP(ctx, hipMemcpyAsync(&eff, d_eff, 8, hipMemcpyDeviceToHost, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<uint64_t> sptr((size_t)nslices + 1, 0);
+  for (int64_t s = 0; s < nslices; ++s) sptr[s + 1] = sptr[s] + (uint64_t)slen[s] * 64;
+  const uint64_t ent = sptr[nslices];
+  h->nnz_eff = (int64_t)eff;
+  h->nnz_stored = (int64_t)ent;
+  h->stream_bytes = (int64_t)(ent * 6 + (uint64_t)n * 2 + (uint64_t)(nslices + 1) * 12 + (h->same ? n * 16 : 0));
+  if (budget && (size_t)h->stream_bytes > budget)
+    return qs_fail(ctx, QS_ENOMEM_DEVICE, "assembled stream needs %lld bytes, budget %zu",
+                   (long long)h->stream_bytes, budget);
+  QS_TRY(qs_malloc(ctx, (void **)&h->d_col, (size_t)std::max<uint64_t>(ent, 1) * 4));
+  QS_TRY(qs_malloc(ctx, (void **)&h->d_tid, (size_t)std::max<uint64_t>(ent, 1) * 2));
+  QS_HIP(ctx, hipMemcpyAsync(h->d_slice_ptr, sptr.data(), sptr.size() * 8, hipMemcpyHostToDevice,
+                             ctx->stream));
+  if (n > 0) {
+    if (op->nw == 1)
+      hipLaunchKernelGGL(k_fill<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, n,
+                         (const TermD<1> *)op->d_terms, op->nterms, L, (int)h->same, h->d_slice_ptr,
+                         h->d_col, h->d_tid);
+    else
+      hipLaunchKernelGGL(k_fill<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, n,
+                         (const TermD<2> *)op->d_terms, op->nterms, L, (int)h->same, h->d_slice_ptr,
+                         h->d_col, h->d_tid);
+    QS_HIP(ctx, hipGetLastError());
+  }
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  h->assembled = 1;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_hop_create(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols,
+                                   int mode, size_t budget, qs_hop **out) {
+  if (!ctx || !out) return QS_EINVAL;
+  *out = nullptr;
+  QS_TRY(check_pair(ctx, op, rows, cols));
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  qs_hop *h = new qs_hop();
+  h->op = op;
+  h->rows = rows;
+  h->cols = cols;
+  h->same = rows == cols;
+  if (mode == QS_HOP_ASSEMBLED || mode == QS_HOP_AUTO) {
+    size_t bud = budget;
+    if (mode == QS_HOP_AUTO && bud == 0) {
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess) bud = fr / 2;
+    }
+    qs_status st = assemble(ctx, h, mode == QS_HOP_ASSEMBLED ? budget : bud);
+    if (st != QS_OK) {
+      free_stream(h);
+      h->assembled = 0;
+      if (mode == QS_HOP_ASSEMBLED) {
+        delete h;
+        return st;
+      }
+      (void)hipGetLastError();
+      ctx->err.clear();
+    }
+  }
+  if (!h->assembled) {
+    // nnz_eff for the matrix-free binding is computed lazily by qs_hop_stats
+    h->nslices = (rows->n + 63) / 64;
+  }
+  *out = h;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_hop_stats(const qs_hop *h, int64_t *nnz_eff, int64_t *nnz_stored,
+                                  int64_t *stream_bytes, int *assembled) {
+  if (!h) return QS_EINVAL;
+  if (nnz_eff) *nnz_eff = h->nnz_eff;
+  if (nnz_stored) *nnz_stored = h->nnz_stored;
+  if (stream_bytes) *stream_bytes = h->stream_bytes;
+  if (assembled) *assembled = h->assembled;
+  return QS_OK;
+}
+
+qs_status qs_hop_apply_strided(qs_ctx *ctx, const qs_hop *h, int k, const double *x, int64_t ldx,
+                               double *y, int64_t ldy) {
+  // vectors interleaved: entry j of vector v at (j*ld + v)
+  const int64_t n = h->rows->n;
+  if (n == 0 || k == 0) return QS_OK;
+  const int64_t nslices = h->nslices;
+  int blocks = (int)std::min<int64_t>((nslices + 3) / 4, (int64_t)ctx->num_cu * 8);
+  blocks = std::max(blocks, 1);
+  for (int v0 = 0; v0 < k;) {
+    int kc = k - v0 >= 4 ? 4 : (k - v0 >= 2 ? 2 : 1);
+    if (k - v0 == 3) kc = 3;
+    const double2 *xs = (const double2 *)x + v0;
+    double2 *ys = (double2 *)y + v0;
+#define QS_LAUNCH_SPMV(KC)                                                                     \
+  hipLaunchKernelGGL(k_spmv<KC>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, n, nslices,      \
+                     h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,           \
+                     (const double2 *)h->op->d_coef, (const double2 *)h->d_diag, xs, ldx, ys, ldy)
+    if (kc == 4) QS_LAUNCH_SPMV(4);
+    else if (kc == 3) QS_LAUNCH_SPMV(3);
+    else if (kc == 2) QS_LAUNCH_SPMV(2);
+    else QS_LAUNCH_SPMV(1);
+#undef QS_LAUNCH_SPMV
+    QS_HIP(ctx, hipGetLastError());
+    v0 += kc;
+  }
+  return QS_OK;
+}
+
+extern "C" qs_status qs_hop_apply(qs_ctx *ctx, const qs_hop *h, const double *x, double *y) {
+  if (!ctx || !h || (h->rows->n && (!x || !y))) return QS_EINVAL;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  if (h->assembled) {
+    QS_TRY(qs_hop_apply_strided(ctx, h, 1, x, 1, y, 1));
+  } else {
+    QS_TRY(launch_mf(ctx, h->op, h->rows, h->cols, h->same, x, y));
+  }
+  if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+extern "C" qs_status qs_hop_apply_multi(qs_ctx *ctx, const qs_hop *h, int k, const double *x, double *y) {
+  if (!ctx || !h || k < 0) return QS_EINVAL;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t nc = h->cols->n, nr = h->rows->n;
+  for (int v = 0; v < k; ++v) {
+    if (h->assembled)
+      QS_TRY(qs_hop_apply_strided(ctx, h, 1, x + 2 * v * nc, 1, y + 2 * v * nr, 1));
+    else
+      QS_TRY(launch_mf(ctx, h->op, h->rows, h->cols, h->same, x + 2 * v * nc, y + 2 * v * nr));
+  }
+  if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+extern "C" qs_status qs_hop_destroy(qs_hop *h) {
+  if (!h) return QS_OK;
+  free_stream(h);
+  delete h;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_apply(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols,
+                              const double *x, double *y) {
+  if (!ctx) return QS_EINVAL;
+  QS_TRY(check_pair(ctx, op, rows, cols));
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  QS_TRY(launch_mf(ctx, op, rows, cols, rows == cols, x, y));
+  if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+extern "C" qs_status qs_expect(qs_ctx *ctx, const qs_op *op, const qs_basis *b, const double *x, double *re,
+                               double *im) {
+  if (!ctx || !re || !im) return QS_EINVAL;
+  QS_TRY(check_pair(ctx, op, b, b));
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  double *y = nullptr;
+  QS_TRY(qs_malloc(ctx, (void **)&y, (size_t)std::max<int64_t>(b->n, 1) * 16));
+  qs_status st = launch_mf(ctx, op, b, b, true, x, y);
+  // <x|O x>
+  if (st == QS_OK) st = qs_dot_internal(ctx, b->n, x, y, re, im);
+  (void)hipFree(y);
+  return st;
+}

@@ -1,0 +1,491 @@
+// Context, device memory, and restricted-occupation bases (enumeration by
+// exact ranking, lookups by rank table or binary search).
+#include <algorithm>
+#include <cstdarg>
+#include <cstring>
+#include <map>
+
+#include "qs_internal.h"
+
+// ---------------------------------------------------------------------------
+// errors and memory
+// ---------------------------------------------------------------------------
+qs_status qs_fail(qs_ctx *ctx, qs_status st, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return st;
+}
+
+qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *file, int line) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return qs_fail(ctx, QS_ENOMEM_DEVICE, "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+  return qs_fail(ctx, QS_EHIP, "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+}
+
+qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return qs_fail(ctx, QS_ENOMEM_DEVICE, "hipMalloc of %zu bytes failed: %s", bytes,
+                   hipGetErrorString(e));
+  }
+  return QS_OK;
+}
+
+qs_status qs_scratch(qs_ctx *ctx, size_t bytes, void **p) {
+  if (bytes > ctx->scratch_bytes) {
+    if (ctx->scratch) {
+      QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      QS_HIP(ctx, hipFree(ctx->scratch));
+      ctx->scratch = nullptr;
+      ctx->scratch_bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    QS_TRY(qs_malloc(ctx, &ctx->scratch, want));
+    ctx->scratch_bytes = want;
+  }
+  *p = ctx->scratch;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_ctx_create(int device, qs_ctx **out) {
+  if (!out) return QS_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return QS_EHIP;
+  if (device < 0 || device >= ndev) return QS_EINVAL;
+  qs_ctx *ctx = new qs_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete ctx;
+    return QS_EHIP;
+  }
+  // default-priority stream (no priority other than the default is used)
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return QS_EHIP;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+  *out = ctx;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_ctx_destroy(qs_ctx *ctx) {
+  if (!ctx) return QS_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return QS_OK;
+}
+
+extern "C" const char *qs_last_error(const qs_ctx *ctx) { return ctx ? ctx->err.c_str() : "null ctx"; }
+
+extern "C" qs_status qs_dev_alloc(qs_ctx *ctx, size_t bytes, void **ptr) {
+  if (!ctx || !ptr) return QS_EINVAL;
+  return qs_malloc(ctx, ptr, bytes);
+}
+extern "C" qs_status qs_dev_free(qs_ctx *ctx, void *ptr) {
+  if (!ctx) return QS_EINVAL;
+  if (ptr) QS_HIP(ctx, hipFree(ptr));
+  return QS_OK;
+}
+extern "C" qs_status qs_memcpy_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return QS_EINVAL;
+  QS_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+extern "C" qs_status qs_memcpy_d2h(qs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return QS_EINVAL;
+  QS_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+extern "C" void *qs_ctx_stream(qs_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+extern "C" qs_status qs_ctx_set_async(qs_ctx *ctx, int async) {
+  if (!ctx) return QS_EINVAL;
+  ctx->async = async != 0;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_sync(qs_ctx *ctx) {
+  if (!ctx) return QS_EINVAL;
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// rank tables (host)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct StateKey {
+  std::vector<int16_t> c;
+  bool operator<(const StateKey &o) const { return c < o.c; }
+};
+
+}  // namespace
+
+qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int *lo,
+                              const int *hi, RankTable &rt, std::string &err) {
+  if (nf <= 0 || nf > 128) {
+    err = "nf must be in 1..128";
+    return QS_EINVAL;
+  }
+  if (nres < 0 || nres > 64) {
+    err = "at most 64 restrictions";
+    return QS_EINVAL;
+  }
+  const int nbytes = (nf + 7) / 8;
+  const int nch = nbytes;
+  // per chunk: list of valid byte values and their per-restriction counts
+  std::vector<std::vector<int>> vals(nch);
+  for (int c = 0; c < nch; ++c) {
+    int rem = nf - 8 * c;  // modes present in this chunk
+    uint32_t forbid = rem >= 8 ? 0u : ((1u << (8 - rem)) - 1u);
+    for (int v = 0; v < 256; ++v)
+      if ((v & forbid) == 0) vals[c].push_back(v);
+  }
+  auto count = [&](int r, int c, int v) {
+    return __builtin_popcount((unsigned)(masks[r * nbytes + c] & v));
+  };
+  // forward reachable states per boundary (counts clamped to hi+1 = overflow => dropped)
+  std::vector<std::map<StateKey, int64_t>> bnd(nch + 1);
+  StateKey s0;
+  s0.c.assign(nres, 0);
+  bnd[0][s0] = 0;
+  for (int c = 0; c < nch; ++c) {
+    for (auto &kv : bnd[c]) {
+      for (int v : vals[c]) {
+        StateKey t = kv.first;
+        bool ok = true;
+        for (int r = 0; r < nres && ok; ++r) {
+          t.c[r] = (int16_t)(t.c[r] + count(r, c, v));
+          if (t.c[r] > hi[r]) ok = false;
+        }
+        if (ok) bnd[c + 1].emplace(t, 0);
+      }
+    }
+    if (bnd[c + 1].size() > 2000000) {
+      err = "restriction state space too large";
+      return QS_EINVAL;
+    }
+  }
+  // completions, backward
+  for (auto &kv : bnd[nch]) {
+    bool acc = true;
+    for (int r = 0; r < nres; ++r)
+      if (kv.first.c[r] < lo[r] || kv.first.c[r] > hi[r]) acc = false;
+    kv.second = acc ? 1 : 0;
+  }
+  for (int c = nch - 1; c >= 0; --c) {
+    for (auto &kv : bnd[c]) {
+      int64_t tot = 0;
+      for (int v : vals[c]) {
+        StateKey t = kv.first;
+        bool ok = true;
+        for (int r = 0; r < nres && ok; ++r) {
+          t.c[r] = (int16_t)(t.c[r] + count(r, c, v));
+          if (t.c[r] > hi[r]) ok = false;
+        }
+        if (!ok) continue;
+        auto it = bnd[c + 1].find(t);
+        if (it != bnd[c + 1].end()) tot += it->second;
+      }
+      kv.second = tot;
+    }
+  }
+  rt.total = bnd[0].begin()->second;
+  if (rt.total >= ((int64_t)1 << 32) - 1) {
+    err = "basis larger than 2^32-2 determinants";
+    return QS_EINVAL;
+  }
+  // ids of viable states per boundary
+  std::vector<std::map<StateKey, uint32_t>> ids(nch + 1);
+  uint32_t smax = 1;
+  for (int c = 0; c < nch; ++c) {
+    uint32_t k = 0;
+    for (auto &kv : bnd[c])
+      if (kv.second > 0) ids[c][kv.first] = k++;
+    smax = std::max(smax, k);
+  }
+  rt.nchunks = nch;
+  rt.smax = smax;
+  rt.next.assign((size_t)nch * smax * 256, QS_DEAD);
+  rt.less.assign((size_t)nch * smax * 256, 0);
+  for (int c = 0; c < nch; ++c) {
+    for (auto &kv : ids[c]) {
+      const uint32_t sid = kv.second;
+      size_t base = ((size_t)c * smax + sid) * 256;
+      int64_t run = 0;
+      for (int v = 0; v < 256; ++v) {
+        rt.less[base + v] = (uint32_t)run;
+        if ((std::find(vals[c].begin(), vals[c].end(), v) == vals[c].end())) continue;
+        StateKey t = kv.first;
+        bool ok = true;
+        for (int r = 0; r < nres && ok; ++r) {
+          t.c[r] = (int16_t)(t.c[r] + count(r, c, v));
+          if (t.c[r] > hi[r]) ok = false;
+        }
+        if (!ok) continue;
+        auto it = bnd[c + 1].find(t);
+        if (it == bnd[c + 1].end() || it->second == 0) continue;
+        if (c + 1 == nch) {
+          rt.next[base + v] = 0;  // accepting
+        } else {
+          rt.next[base + v] = ids[c + 1][t];
+        }
+        run += it->second;
+      }
+    }
+  }
+  return QS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device kernels: unrank (enumeration) and lookup
+// ---------------------------------------------------------------------------
+template <int NW>
+__global__ void k_unrank(const uint2 *__restrict__ tbl, uint32_t smax, int nch, int64_t n,
+                         uint64_t *__restrict__ out) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint32_t rem = (uint32_t)k;
+  uint32_t s = 0;
+  uint64_t w[NW];
+#pragma unroll
+  for (int i = 0; i < NW; ++i) w[i] = 0;
+  for (int c = 0; c < nch; ++c) {
+    const uint2 *row = tbl + ((size_t)c * smax + s) * 256;
+    int lo = 0, hi = 256;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) >> 1;
+      if (row[mid].y <= rem) lo = mid; else hi = mid;
+    }
+    uint2 e = row[lo];
+    rem -= e.y;
+    s = e.x;
+    w[c >> 3] |= (uint64_t)lo << (56 - 8 * (c & 7));
+  }
+#pragma unroll
+  for (int i = 0; i < NW; ++i) out[k * NW + i] = w[i];
+}
+
+template <int NW>
+__device__ __forceinline__ int64_t rank_lookup(const uint64_t *w, const uint2 *__restrict__ tbl,
+                                               uint32_t smax, int nch) {
+  uint32_t s = 0, r = 0;
+  for (int c = 0; c < nch; ++c) {
+    uint32_t v = (uint32_t)(w[c >> 3] >> (56 - 8 * (c & 7))) & 0xFFu;
+    uint2 e = tbl[((size_t)c * smax + s) * 256 + v];
+    if (e.x == QS_DEAD) return -1;
+    r += e.y;
+    s = e.x;
+  }
+  return r;
+}
+
+template <int NW>
+__device__ __forceinline__ bool det_less(const uint64_t *a, const uint64_t *b) {
+  if (NW == 1) return a[0] < b[0];
+  return a[0] < b[0] || (a[0] == b[0] && a[1] < b[1]);
+}
+
+template <int NW>
+__device__ __forceinline__ int64_t bsearch_lookup(const uint64_t *w, const uint64_t *__restrict__ words,
+                                                  int64_t n) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (det_less<NW>(words + mid * NW, w)) lo = mid + 1; else hi = mid;
+  }
+  if (lo < n) {
+    bool eq = true;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) eq = eq && (words[lo * NW + i] == w[i]);
+    if (eq) return lo;
+  }
+  return -1;
+}
+
+template <int NW>
+__global__ void k_index(const uint64_t *__restrict__ q, int64_t nq, const uint2 *__restrict__ tbl,
+                        uint32_t smax, int nch, const uint64_t *__restrict__ words, int64_t n,
+                        int64_t *__restrict__ idx) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nq) return;
+  uint64_t w[NW];
+#pragma unroll
+  for (int i = 0; i < NW; ++i) w[i] = q[k * NW + i];
+  idx[k] = tbl ? rank_lookup<NW>(w, tbl, smax, nch) : bsearch_lookup<NW>(w, words, n);
+}
+
+// ---------------------------------------------------------------------------
+// basis API
+// ---------------------------------------------------------------------------
+static qs_status basis_alloc_words(qs_ctx *ctx, qs_basis *b) {
+  return qs_malloc(ctx, (void **)&b->d_words, (size_t)std::max<int64_t>(b->n, 1) * b->nw * 8);
+}
+
+extern "C" qs_status qs_basis_count(qs_ctx *ctx, int nf, int nres, const uint8_t *masks,
+                                    const int *lo, const int *hi, int64_t *n) {
+  RankTable rt;
+  std::string err;
+  qs_status st = qs_build_rank_table(nf, nres, masks, lo, hi, rt, err);
+  if (st != QS_OK) return qs_fail(ctx, st, "%s", err.c_str());
+  *n = rt.total;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_basis_enumerate(qs_ctx *ctx, int nf, int nres, const uint8_t *masks,
+                                        const int *lo, const int *hi, qs_basis **out) {
+  if (!ctx || !out) return QS_EINVAL;
+  *out = nullptr;
+  RankTable rt;
+  std::string err;
+  qs_status st = qs_build_rank_table(nf, nres, masks, lo, hi, rt, err);
+  if (st != QS_OK) return qs_fail(ctx, st, "qs_basis_enumerate: %s", err.c_str());
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  qs_basis *b = new qs_basis();
+  b->nf = nf;
+  b->nw = nf <= 64 ? 1 : 2;
+  b->n = rt.total;
+  b->has_rank = true;
+  b->nchunks = rt.nchunks;
+  b->smax = rt.smax;
+  b->nres = nres;
+  const int nbytes = (nf + 7) / 8;
+  b->masks.assign(masks, masks + (size_t)nres * nbytes);
+  b->lo.assign(lo, lo + nres);
+  b->hi.assign(hi, hi + nres);
+  std::vector<uint2> tbl(rt.next.size());
+  for (size_t i = 0; i < tbl.size(); ++i) tbl[i] = make_uint2(rt.next[i], rt.less[i]);
+  st = qs_malloc(ctx, (void **)&b->d_tbl, tbl.size() * sizeof(uint2));
+  if (st == QS_OK) st = basis_alloc_words(ctx, b);
+  if (st != QS_OK) {
+    qs_basis_destroy(b);
+    return st;
+  }
+  hipError_t e = hipMemcpyAsync(b->d_tbl, tbl.data(), tbl.size() * sizeof(uint2),
+                                hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && b->n > 0) {
+    int64_t blocks = (b->n + 255) / 256;
+    if (b->nw == 1)
+      hipLaunchKernelGGL(k_unrank<1>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, b->d_tbl,
+                         b->smax, b->nchunks, b->n, b->d_words);
+    else
+      hipLaunchKernelGGL(k_unrank<2>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, b->d_tbl,
+                         b->smax, b->nchunks, b->n, b->d_words);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    qs_basis_destroy(b);
+    return qs_hip_fail(ctx, e, "qs_basis_enumerate", __FILE__, __LINE__);
+  }
+  *out = b;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_basis_from_words(qs_ctx *ctx, int nf, const uint64_t *words, int64_t n,
+                                         qs_basis **out) {
+  if (!ctx || !out || n < 0 || nf <= 0 || nf > 128) return QS_EINVAL;
+  *out = nullptr;
+  const int nw = nf <= 64 ? 1 : 2;
+  std::vector<std::pair<uint64_t, uint64_t>> v((size_t)n);
+  const int shift_unused = 64 * nw - nf;  // low bits that must be zero
+  for (int64_t i = 0; i < n; ++i) {
+    uint64_t hi = words[i * nw];
+    uint64_t lo = nw == 2 ? words[i * nw + 1] : 0;
+    uint64_t tail = nw == 1 ? hi : lo;
+    int unused = nw == 1 ? shift_unused : shift_unused;  // bits of the last word above nf
+    if (unused > 0 && unused < 64 && (tail & ((1ULL << unused) - 1)) != 0)
+      return qs_fail(ctx, QS_EINVAL, "determinant %lld has bits set beyond mode %d", (long long)i,
+                     nf - 1);
+    v[(size_t)i] = {hi, lo};
+  }
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  qs_basis *b = new qs_basis();
+  b->nf = nf;
+  b->nw = nw;
+  b->n = (int64_t)v.size();
+  std::vector<uint64_t> flat((size_t)b->n * nw);
+  for (size_t i = 0; i < v.size(); ++i) {
+    flat[i * nw] = v[i].first;
+    if (nw == 2) flat[i * nw + 1] = v[i].second;
+  }
+  qs_status st = basis_alloc_words(ctx, b);
+  if (st != QS_OK) {
+    delete b;
+    return st;
+  }
+  hipError_t e = hipMemcpyAsync(b->d_words, flat.data(), flat.size() * 8, hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    qs_basis_destroy(b);
+    return qs_hip_fail(ctx, e, "qs_basis_from_words", __FILE__, __LINE__);
+  }
+  *out = b;
+  return QS_OK;
+}
+
+extern "C" int64_t qs_basis_size(const qs_basis *b) { return b ? b->n : -1; }
+extern "C" int qs_basis_nf(const qs_basis *b) { return b ? b->nf : -1; }
+extern "C" int qs_basis_words(const qs_basis *b) { return b ? b->nw : -1; }
+extern "C" const uint64_t *qs_basis_device_words(const qs_basis *b) { return b ? b->d_words : nullptr; }
+
+extern "C" qs_status qs_basis_export(qs_ctx *ctx, const qs_basis *b, uint64_t *words) {
+  if (!ctx || !b || !words) return QS_EINVAL;
+  if (b->n == 0) return QS_OK;
+  QS_HIP(ctx, hipMemcpyAsync(words, b->d_words, (size_t)b->n * b->nw * 8, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+extern "C" qs_status qs_basis_index(qs_ctx *ctx, const qs_basis *b, const uint64_t *words, int64_t n,
+                                    int64_t *idx) {
+  if (!ctx || !b || (n > 0 && (!words || !idx))) return QS_EINVAL;
+  if (n == 0) return QS_OK;
+  size_t qbytes = (size_t)n * b->nw * 8;
+  void *scr;
+  QS_TRY(qs_scratch(ctx, qbytes + (size_t)n * 8, &scr));
+  uint64_t *dq = (uint64_t *)scr;
+  int64_t *di = (int64_t *)((char *)scr + qbytes);
+  QS_HIP(ctx, hipMemcpyAsync(dq, words, qbytes, hipMemcpyHostToDevice, ctx->stream));
+  int64_t blocks = (n + 255) / 256;
+  const uint2 *tbl = b->has_rank ? b->d_tbl : nullptr;
+  if (b->nw == 1)
+    hipLaunchKernelGGL(k_index<1>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, dq, n, tbl,
+                       b->smax, b->nchunks, b->d_words, b->n, di);
+  else
+    hipLaunchKernelGGL(k_index<2>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, dq, n, tbl,
+                       b->smax, b->nchunks, b->d_words, b->n, di);
+  QS_HIP(ctx, hipGetLastError());
+  QS_HIP(ctx, hipMemcpyAsync(idx, di, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+extern "C" qs_status qs_basis_destroy(qs_basis *b) {
+  if (!b) return QS_OK;
+  if (b->d_words) (void)hipFree(b->d_words);
+  if (b->d_tbl) (void)hipFree(b->d_tbl);
+  delete b;
+  return QS_OK;
+}

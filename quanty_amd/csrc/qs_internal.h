@@ -1,0 +1,141 @@
+// Internal declarations shared by the libqs translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/qs.h"
+
+#define QS_MAX_WORDS 2
+#define QS_MAX_CHUNKS 16          // 128 modes / 8 modes per chunk
+#define QS_DEAD 0xFFFFFFFFu        // rank-table successor of an invalid prefix
+
+struct qs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // reusable device scratch (grown on demand, freed with the ctx)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  int num_cu = 256;
+  int async = 0;                   // skip the end-of-call stream sync for device-only results
+};
+
+// Exact rank of a determinant among all determinants satisfying the
+// restrictions, in ascending numeric order. The modes are cut into chunks of 8
+// (one reference byte each). After chunks 0..c-1 the "state" is the vector of
+// occupation counts inside every restriction mask; only states that can still
+// be completed into a valid determinant get an id. For chunk c, state s and
+// byte value v, entry (c, s, v) holds the id of the successor state (or
+// QS_DEAD) and the number of valid determinants that share the prefix and have
+// a smaller byte at chunk c. rank = sum of the entries along the path.
+struct RankTable {
+  int nchunks = 0;
+  uint32_t smax = 0;               // state stride (max viable states at any boundary)
+  std::vector<uint32_t> next;      // nchunks * smax * 256
+  std::vector<uint32_t> less;      // nchunks * smax * 256
+  int64_t total = 0;               // number of valid determinants
+};
+
+struct qs_basis {
+  int nf = 0;
+  int nw = 1;
+  int64_t n = 0;
+  uint64_t *d_words = nullptr;     // n * nw, ascending
+  bool has_rank = false;
+  int nchunks = 0;
+  uint32_t smax = 0;
+  uint2 *d_tbl = nullptr;          // {next, less} per (chunk, state, byte)
+  // restriction description (kept for filtering)
+  int nres = 0;
+  std::vector<uint8_t> masks;
+  std::vector<int> lo, hi;
+};
+
+// A ladder string compiled for the pull form. For a row determinant d the
+// term contributes iff (d & C) == V; its source determinant is s = d ^ F and
+// the fermion sign is (-1)^(popcount(d & P) + sgn).
+struct TermHost {
+  uint64_t C[QS_MAX_WORDS] = {0, 0};
+  uint64_t V[QS_MAX_WORDS] = {0, 0};
+  uint64_t F[QS_MAX_WORDS] = {0, 0};
+  uint64_t P[QS_MAX_WORDS] = {0, 0};
+  uint32_t sgn = 0;
+  double re = 0, im = 0;
+  int nchange = 0;                 // signed change of particle number
+};
+
+template <int NW>
+struct TermD {
+  uint64_t C[NW], V[NW], F[NW], P[NW];
+  double re, im;
+  uint32_t sgn;
+  uint32_t diag;                   // F == 0
+};
+
+struct qs_op {
+  int nf = 0;
+  int nw = 1;
+  int nterms = 0;                  // compiled (non-vanishing) terms
+  std::vector<TermHost> h_terms;
+  void *d_terms = nullptr;         // TermD<nw>[nterms]
+  double *d_coef = nullptr;        // 2*nterms complex: +V_t, -V_t
+};
+
+struct qs_hop {
+  const qs_op *op = nullptr;
+  const qs_basis *rows = nullptr;
+  const qs_basis *cols = nullptr;
+  bool same = false;
+  int assembled = 0;
+  int64_t nnz_eff = 0;             // (row, term) pairs with a source in cols
+  int64_t nnz_stored = 0;          // stored off-diagonal entries (incl. slice padding)
+  int64_t stream_bytes = 0;
+  // sliced-ELL (64 rows per slice, column-major inside a slice)
+  int64_t nslices = 0;
+  uint64_t *d_slice_ptr = nullptr; // nslices + 1 entry offsets
+  uint32_t *d_slice_len = nullptr; // max row length per slice
+  uint16_t *d_rowcnt = nullptr;    // off-diagonal entries per row
+  uint32_t *d_col = nullptr;
+  uint16_t *d_tid = nullptr;       // 2*term + sign
+  double *d_diag = nullptr;        // complex pre-summed diagonal per row (rows == cols)
+};
+
+// ---- error helpers --------------------------------------------------------
+qs_status qs_fail(qs_ctx *ctx, qs_status st, const char *fmt, ...);
+qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *file, int line);
+
+#define QS_HIP(ctx, call)                                                       \
+  do {                                                                          \
+    hipError_t _e = (call);                                                     \
+    if (_e != hipSuccess) return qs_hip_fail((ctx), _e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define QS_TRY(call)                                                            \
+  do {                                                                          \
+    qs_status _s = (call);                                                      \
+    if (_s != QS_OK) return _s;                                                 \
+  } while (0)
+
+qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes);
+qs_status qs_scratch(qs_ctx *ctx, size_t bytes, void **p);
+
+// ---- internal entry points across translation units ----------------------
+qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int *lo,
+                              const int *hi, RankTable &rt, std::string &err);
+qs_status qs_dot_internal(qs_ctx *ctx, int64_t n, const double *a, const double *b,
+                          double *re, double *im);
+
+// Host-side dense Hermitian eigen-solver (column-major, complex interleaved).
+// A: n*n complex (overwritten), w: n eigenvalues ascending, Z: n*n eigenvectors
+// (column j = eigenvector j).
+void qs_heev(int n, const double *A, double *w, double *Z);
+
+// ---- device helpers -------------------------------------------------------
+__device__ __forceinline__ uint32_t qs_chunk_byte1(uint64_t w, int c) {
+  return (uint32_t)(w >> (56 - 8 * c)) & 0xFFu;
+}

@@ -1,0 +1,213 @@
+"""Thin object layer over libqs handles (bases, operators, bound operators).
+
+Device vectors are torch complex128 tensors on the GPU (PyTorch is used only
+for device memory); every computation goes through libqs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import det as D
+from ._lib import QS_HOP_ASSEMBLED, QS_HOP_AUTO, QS_HOP_MATFREE, Context, QSError, default_context, load, ptr
+
+
+def _device(ctx: Context):
+    return torch.device("cuda", ctx.device)
+
+
+def restriction_arrays(nf: int, restrictions):
+    """[(mask_string, lo, hi), ...] -> (masks uint8 (nres, nbytes), lo int32, hi int32)."""
+    nres = len(restrictions)
+    nb = D.nbytes(nf)
+    masks = np.zeros((max(nres, 1), nb), dtype=np.uint8)
+    lo = np.zeros(max(nres, 1), dtype=np.int32)
+    hi = np.zeros(max(nres, 1), dtype=np.int32)
+    for r, (m, a, b) in enumerate(restrictions):
+        masks[r] = D.mask_bytes(m, nf) if isinstance(m, str) else np.asarray(m, dtype=np.uint8)
+        lo[r], hi[r] = int(a), int(b)
+    return nres, masks, lo, hi
+
+
+class Basis:
+    """Sorted determinant set on the device (qs_basis)."""
+
+    def __init__(self, handle, ctx: Context, restrictions=None):
+        self.h = handle
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.n = int(self.lib.qs_basis_size(handle))
+        self.nf = int(self.lib.qs_basis_nf(handle))
+        self.nw = int(self.lib.qs_basis_words(handle))
+        self.restrictions = restrictions
+
+    @classmethod
+    def enumerate(cls, nf: int, restrictions, ctx: Context | None = None) -> "Basis":
+        ctx = ctx or default_context()
+        nres, masks, lo, hi = restriction_arrays(nf, restrictions)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.qs_basis_enumerate(ctx.h, nf, nres, ptr(masks), ptr(lo), ptr(hi), C.byref(h)),
+                  "qs_basis_enumerate")
+        return cls(h, ctx, list(restrictions))
+
+    @staticmethod
+    def count(nf: int, restrictions) -> int:
+        """Sector size from the rank tables (host only; needs no GPU)."""
+        lib = load()
+        nres, masks, lo, hi = restriction_arrays(nf, restrictions)
+        n = C.c_int64()
+        st = lib.qs_basis_count(None, nf, nres, ptr(masks), ptr(lo), ptr(hi), C.byref(n))
+        if st != 0:
+            raise QSError(f"qs_basis_count failed with status {st}")
+        return int(n.value)
+
+    @classmethod
+    def from_words(cls, nf: int, words: np.ndarray, ctx: Context | None = None) -> "Basis":
+        ctx = ctx or default_context()
+        w = np.ascontiguousarray(np.asarray(words, dtype=np.uint64).reshape(-1, D.nwords(nf)))
+        h = C.c_void_p()
+        ctx.check(ctx.lib.qs_basis_from_words(ctx.h, nf, ptr(w), w.shape[0], C.byref(h)), "qs_basis_from_words")
+        return cls(h, ctx)
+
+    def words(self) -> np.ndarray:
+        out = np.zeros((self.n, self.nw), dtype=np.uint64)
+        if self.n:
+            self.ctx.check(self.lib.qs_basis_export(self.ctx.h, self.h, ptr(out)), "qs_basis_export")
+        return out
+
+    def index(self, words: np.ndarray) -> np.ndarray:
+        w = np.ascontiguousarray(np.asarray(words, dtype=np.uint64).reshape(-1, self.nw))
+        out = np.zeros(w.shape[0], dtype=np.int64)
+        if w.shape[0]:
+            self.ctx.check(self.lib.qs_basis_index(self.ctx.h, self.h, ptr(w), w.shape[0], ptr(out)),
+                           "qs_basis_index")
+        return out
+
+    def zeros(self, k: int = 1) -> torch.Tensor:
+        shape = (self.n,) if k == 1 else (k, self.n)
+        return torch.zeros(shape, dtype=torch.complex128, device=_device(self.ctx))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.qs_basis_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def pack_terms(terms):
+    """[(codes (o[0] first), complex value), ...] -> (len int32, codes uint16, values float64)."""
+    lens = np.array([len(c) for c, _ in terms], dtype=np.int32)
+    codes = np.array([x for c, _ in terms for x in c] or [0], dtype=np.uint16)
+    vals = np.zeros(2 * max(len(terms), 1), dtype=np.float64)
+    for i, (_, v) in enumerate(terms):
+        vals[2 * i] = complex(v).real
+        vals[2 * i + 1] = complex(v).imag
+    return lens, codes, vals
+
+
+class Op:
+    """Compiled ladder-term table on the device (qs_op)."""
+
+    def __init__(self, nf: int, terms, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.nf = nf
+        lens, codes, vals = pack_terms(terms)
+        h = C.c_void_p()
+        self.ctx.check(self.lib.qs_op_create(self.ctx.h, nf, len(terms), ptr(lens), ptr(codes), ptr(vals),
+                                             C.byref(h)), "qs_op_create")
+        self.h = h
+        self.nterms = int(self.lib.qs_op_nterms(h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.qs_op_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Hop:
+    """An operator bound to (row basis, column basis) (qs_hop)."""
+
+    MODES = {"matfree": QS_HOP_MATFREE, "assembled": QS_HOP_ASSEMBLED, "auto": QS_HOP_AUTO}
+
+    def __init__(self, op: Op, rows: Basis, cols: Basis | None = None, mode: str = "auto", budget: int = 0):
+        cols = rows if cols is None else cols
+        self.op, self.rows, self.cols = op, rows, cols
+        self.ctx = op.ctx
+        self.lib = self.ctx.lib
+        h = C.c_void_p()
+        self.ctx.check(self.lib.qs_hop_create(self.ctx.h, op.h, rows.h, cols.h, self.MODES[mode], budget,
+                                              C.byref(h)), "qs_hop_create")
+        self.h = h
+
+    def stats(self) -> dict:
+        a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
+        self.ctx.check(self.lib.qs_hop_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)), "qs_hop_stats")
+        return {"nnz_eff": a.value, "nnz_stored": b.value, "stream_bytes": c.value, "assembled": bool(d.value)}
+
+    def apply(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+        assert x.dtype == torch.complex128 and x.is_contiguous() and x.numel() == self.cols.n
+        if y is None:
+            y = self.rows.zeros()
+        torch.cuda.synchronize(x.device)
+        self.ctx.check(self.lib.qs_hop_apply(self.ctx.h, self.h, ptr(x), ptr(y)), "qs_hop_apply")
+        return y
+
+    def lanczos(self, v0: torch.Tensor, nsteps: int):
+        a = np.zeros(nsteps + 1)
+        b = np.zeros(nsteps)
+        torch.cuda.synchronize(v0.device)
+        self.ctx.check(self.lib.qs_lanczos(self.ctx.h, self.h, ptr(v0), nsteps, ptr(a), ptr(b)), "qs_lanczos")
+        return a, b
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.qs_hop_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def apply(op: Op, rows: Basis, cols: Basis, x: torch.Tensor) -> torch.Tensor:
+    y = rows.zeros()
+    torch.cuda.synchronize(x.device)
+    op.ctx.check(op.lib.qs_apply(op.ctx.h, op.h, rows.h, cols.h, ptr(x), ptr(y)), "qs_apply")
+    return y
+
+
+def expect(op: Op, basis: Basis, x: torch.Tensor) -> complex:
+    re, im = C.c_double(), C.c_double()
+    torch.cuda.synchronize(x.device)
+    op.ctx.check(op.lib.qs_expect(op.ctx.h, op.h, basis.h, ptr(x), C.byref(re), C.byref(im)), "qs_expect")
+    return complex(re.value, im.value)
+
+
+def dot(a: torch.Tensor, b: torch.Tensor, ctx: Context | None = None) -> complex:
+    ctx = ctx or default_context()
+    re, im = C.c_double(), C.c_double()
+    torch.cuda.synchronize(a.device)
+    ctx.check(ctx.lib.qs_dot(ctx.h, a.numel(), ptr(a), ptr(b), C.byref(re), C.byref(im)), "qs_dot")
+    return complex(re.value, im.value)
+
+
+def cf_spectrum(a: np.ndarray, b: np.ndarray, E0, emin: float, emax: float, ne: int, gamma: float,
+                ctx: Context | None = None) -> np.ndarray:
+    """Continued fractions on the GPU. a: (nspec, n+1), b: (nspec, n). Returns complex (nspec, ne+1)."""
+    ctx = ctx or default_context()
+    a = np.ascontiguousarray(np.atleast_2d(a), dtype=np.float64)
+    b = np.ascontiguousarray(np.atleast_2d(b), dtype=np.float64)
+    nspec, n = b.shape
+    e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(E0, dtype=np.float64), (nspec,)))
+    re = np.zeros((nspec, ne + 1))
+    im = np.zeros((nspec, ne + 1))
+    ctx.check(ctx.lib.qs_cf_spectrum(ctx.h, nspec, n, ptr(a), ptr(b), ptr(e0), emin, emax, ne, gamma,
+                                     ptr(re), ptr(im)), "qs_cf_spectrum")
+    return re + 1j * im

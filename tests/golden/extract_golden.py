@@ -1,0 +1,52 @@
+"""Lift the golden data held by the reference's own test scripts into JSON
+fixtures (run in a container where the reference tree is mounted; the JSON
+files are committed, so tests never read the reference).
+
+  createspectra_tri.json  <- TestScripts/CreateSpectra.Quanty:188-260, 664-736
+  eigensystem_f6.json     <- TestScripts/Eigensystem.Quanty:571, 669
+"""
+import json
+import os
+import re
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def main():
+    src = open(os.path.join(REF, "TestScripts/CreateSpectra.Quanty")).read()
+    blk = src[src.index("G0Dn1 = {"):src.index("-- some reference frequencies")]
+    tabs = {}
+    for m in re.finditer(r"(G\d(?:Dn|Up)\d) = \{ \{([^}]*)\} ,\s*\{([^}]*)\}", blk):
+        tabs[m.group(1)] = {"a": [float(x) for x in m.group(2).split(",")],
+                            "b": [float(x) for x in m.group(3).split(",")]}
+    out = {
+        "source": "TestScripts/CreateSpectra.Quanty:188-260 (NBath=5, unrestricted) and :664-736 "
+                  "(NBath=6, last site restricted empty)",
+        "omega": [-1.1, -0.5463, 0.0, 4.75, 100.1],
+        "gamma": 0.1,
+        "threshold_block1": 1e-12,
+        "threshold_block2": 1e-13,
+        "model": "impurity (modes 0,1) + chain; H0 = U n0 n1 + t hopping, H1 = U n0 n1 + i t hopping, "
+                 "H2 = i t hopping; U=3, t=1; TDn = c+_0, TUp = c+_1; psi1 = (|10..>+|01..>)/sqrt2, "
+                 "psi2 = (|10..> - 2i|01..>)/sqrt5",
+        "tables": tabs,
+    }
+    json.dump(out, open(os.path.join(HERE, "createspectra_tri.json"), "w"), indent=1)
+    src = open(os.path.join(REF, "TestScripts/Eigensystem.Quanty")).read()
+
+    def grab(key):
+        i = src.index(key)
+        return [float(x) for x in src[i:src.index("}", i)].split("{")[1].split(",")]
+
+    json.dump({"source": "TestScripts/Eigensystem.Quanty:571 (real) and :669 (complex); energies from Mathematica",
+               "model": "f shell, 14 modes (dn = even, up = odd), 6 electrons; H = U(F0=0,F2=10,F4=6.25,F6=4) "
+                        "+ 0.1 ldots; complex case adds 1e-6 Ly + 2e-6 Sy",
+               "real": grab("expectedEn = {-12.11458547046453"),
+               "complex": grab("expectedEn = {-12.11458548596329")},
+              open(os.path.join(HERE, "eigensystem_f6.json"), "w"), indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,213 @@
+"""HIP path (libqs) against the oracle: bases bit-exact, H*psi to 1e-13,
+Lanczos coefficients to 1e-10, continued fractions to 1e-13, golden tables.
+All calls go through the C ABI."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from models import chain, chain_states, golden, str_to_bytes
+from quanty_amd import det as D
+from quanty_amd import operators as O
+from quanty_amd import problems as P
+from quanty_amd.engine import Basis, Hop, Op, apply, cf_spectrum, dot, expect
+
+pytestmark = pytest.mark.gpu
+
+
+def cvec(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=n) + 1j * rng.normal(size=n)
+
+
+def oracle_apply_dense(nf, terms, basis_words, x, restrictions, rows: Basis):
+    """Oracle push-form O*x (restricted) scattered onto the row basis."""
+    psi = oracle.Vec.from_arrays(nf, D.words_to_bytes(basis_words, nf), x)
+    d, v = oracle.Op(nf, terms).apply(psi, restrictions=restrictions).arrays()
+    out = np.zeros(rows.n, dtype=complex)
+    if len(v):
+        idx = rows.index(D.bytes_to_words(d, nf))
+        assert np.all(idx >= 0)
+        out[idx] = v
+    return out
+
+
+SMALL = [
+    dict(seed=3, nbath=1, nelec=12, full_baths=1, max_holes=3, max_elec=0, d_range=(0, 10)),
+    dict(seed=5, nbath=2, nelec=18, full_baths=1, max_holes=2, max_elec=2),
+    dict(seed=9, nbath=3, nelec=22, full_baths=1, max_holes=2, max_elec=3),
+]
+
+
+@pytest.mark.parametrize("kw", SMALL)
+def test_enumeration_bit_exact(kw):
+    pr = P.anderson(**kw)
+    nf = pr["NF"]
+    b = Basis.enumerate(nf, pr["restrictions"])
+    ref = oracle.enumerate_sector(nf, pr["restrictions"])
+    assert b.n == len(ref)
+    assert np.array_equal(b.words(), D.bytes_to_words(ref, nf))
+    # rank lookup: every determinant maps to its position, foreign ones to -1
+    assert np.array_equal(b.index(b.words()), np.arange(b.n))
+    foreign = b.words().copy()
+    foreign[:, -1] ^= np.uint64(1) << np.uint64(64 * b.nw - nf)  # flip the last mode
+    idx = b.index(foreign)
+    ok = np.array([all(lo <= bin(D.words_to_int(w) >> (64 * b.nw - nf) & int(m.replace(" ", ""), 2)).count("1") <= hi
+                       for m, lo, hi in pr["restrictions"]) for w in foreign[:500]])
+    assert np.array_equal(idx[:500] >= 0, ok)
+
+
+def test_enumeration_uint128_and_overlapping_masks():
+    nf = 76
+    restr = [("1" * nf, 4, 4), ("1" * 20 + "0" * 56, 1, 3), ("0" * 10 + "1" * 60 + "0" * 6, 1, 3),
+             ("0" * 70 + "1" * 6, 0, 1)]
+    b = Basis.enumerate(nf, restr)
+    assert b.nw == 2
+    assert b.n == Basis.count(nf, restr) == 634745
+    w = b.words()
+    assert np.array_equal(w, D.bytes_to_words(oracle.enumerate_sector(nf, restr), nf))
+    ints = [D.words_to_int(x) for x in w[:: max(1, b.n // 2000)]]
+    assert all(x < y for x, y in zip(ints, ints[1:]))
+    for x in ints:
+        s = format(x >> (128 - nf), f"0{nf}b")
+        for m, lo, hi in restr:
+            assert lo <= sum(1 for p, q in zip(s, m) if p == q == "1") <= hi
+    assert np.array_equal(b.index(w[:4096]), np.arange(min(4096, b.n)))
+
+
+@pytest.mark.parametrize("kw", SMALL)
+@pytest.mark.parametrize("mode", ["matfree", "assembled"])
+def test_hpsi_matches_oracle(kw, mode):
+    pr = P.anderson(**kw)
+    nf = pr["NF"]
+    b = Basis.enumerate(nf, pr["restrictions"])
+    op = Op(nf, pr["H"])
+    hop = Hop(op, b, mode=mode)
+    assert hop.stats()["assembled"] == (mode == "assembled")
+    x = cvec(b.n, 11)
+    y = hop.apply(torch.from_numpy(x).cuda()).cpu().numpy()
+    want = oracle_apply_dense(nf, pr["H"], b.words(), x, pr["restrictions"], b)
+    assert np.max(np.abs(y - want)) <= 1e-13 * np.max(np.abs(want))
+
+
+def test_optimespsi_known_answer_gpu():
+    terms = [O.written([O.cr(0), O.an(0)], 1j), O.written([O.cr(0), O.cr(1), O.an(0), O.an(1)], 1.0)]
+    b = Basis.enumerate(2, [("11", 2, 2)])
+    y = apply(Op(2, terms), b, b, torch.tensor([1.0 + 0j], dtype=torch.complex128).cuda()).cpu().numpy()
+    assert y[0] == complex(-1, 1)
+
+
+def test_transition_between_sectors():
+    """T maps the 2p6 ground sector to the 2p5 final sector (rows != cols)."""
+    ni = P.ni_xas()
+    gs = Basis.enumerate(26, ni["gs_restrictions"])
+    fs = Basis.enumerate(26, ni["fs_restrictions"])
+    x = cvec(gs.n, 2)
+    for T in ni["T"]:
+        for mode in ["matfree", "assembled"]:
+            hop = Hop(Op(26, T), fs, gs, mode=mode)
+            y = hop.apply(torch.from_numpy(x).cuda()).cpu().numpy()
+            want = oracle_apply_dense(26, T, gs.words(), x, ni["fs_restrictions"], fs)
+            assert np.max(np.abs(y - want)) <= 1e-13 * np.max(np.abs(want))
+
+
+def test_hermiticity_and_expectation():
+    pr = P.anderson(**SMALL[2])
+    nf = pr["NF"]
+    b = Basis.enumerate(nf, pr["restrictions"])
+    op = Op(nf, pr["H"])
+    hop = Hop(op, b, mode="assembled")
+    x = torch.from_numpy(cvec(b.n, 1)).cuda()
+    z = torch.from_numpy(cvec(b.n, 2)).cuda()
+    lhs = dot(z, hop.apply(x))
+    rhs = dot(hop.apply(z), x)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    e = expect(op, b, x)
+    assert abs(e - dot(x, hop.apply(x))) < 1e-12 * abs(e)
+    assert abs(e.imag) < 1e-10 * abs(e)
+
+
+@pytest.mark.parametrize("kw", SMALL[1:])
+@pytest.mark.parametrize("mode", ["matfree", "assembled"])
+def test_lanczos_matches_oracle(kw, mode):
+    pr = P.anderson(**kw)
+    nf = pr["NF"]
+    b = Basis.enumerate(nf, pr["restrictions"])
+    hop = Hop(Op(nf, pr["H"]), b, mode=mode)
+    x = cvec(b.n, 4)
+    a, bb = hop.lanczos(torch.from_numpy(x).cuda(), 40)
+    psi = oracle.Vec.from_arrays(nf, D.words_to_bytes(b.words(), nf), x)
+    ra, rb = oracle.Op(nf, pr["H"]).lanczos(psi, 40, restrictions=pr["restrictions"])
+    assert np.allclose(a, ra, rtol=1e-10, atol=1e-12)
+    assert np.allclose(bb, rb, rtol=1e-10, atol=0)
+
+
+@pytest.mark.parametrize("name", ["G0Dn1", "G0Dn2", "G0Up1", "G0Up2", "G1Dn1", "G1Dn2", "G1Up1", "G1Up2",
+                                  "G2Dn1", "G2Dn2", "G2Up1", "G2Up2"])
+def test_lanczos_golden_tri_tables_gpu(name):
+    """GPU Lanczos on the restricted 6-site chain (TestScripts/CreateSpectra.Quanty:596-760):
+    golden a/b and the G(w, 0.1) checks at 1e-13."""
+    g = golden("createspectra_tri.json")
+    ref = g["tables"][name]
+    nf, terms = chain(6, int(name[1]))
+    restr = [("00" + "0" * 10 + "11", 0, 0)]
+    strings, coefs = chain_states(nf, int(name[-1]))
+    gs = Basis.enumerate(nf, restr + [("1" * nf, 1, 1)])
+    fs = Basis.enumerate(nf, restr + [("1" * nf, 2, 2)])
+    x = np.zeros(gs.n, dtype=complex)
+    idx = gs.index(np.array([D.string_to_words(s, nf) for s in strings], dtype=np.uint64))
+    x[idx] = coefs
+    T = [((O.cr(0 if name[2:4] == "Dn" else 1),), 1.0)]
+    tx = Hop(Op(nf, T), fs, gs, mode="matfree").apply(torch.from_numpy(x).cuda())
+    hop = Hop(Op(nf, terms), fs, mode="assembled")
+    a, b = hop.lanczos(tx, 200)
+    na, nb = len(ref["a"]), len(ref["b"])
+    assert np.allclose(a[:na], ref["a"], atol=5e-13, rtol=1e-12)
+    assert np.allclose(b[:nb], ref["b"], atol=5e-13, rtol=1e-12)
+    for w in g["omega"]:
+        got = cf_spectrum(a, b, 0.0, w, w + 1.0, 1, g["gamma"])[0, 0]
+        want = oracle.cf(np.array(ref["a"]), np.array(ref["b"]), 0.0, w, w + 1.0, 1, g["gamma"])[0]
+        assert abs(got - want) < g["threshold_block2"]
+
+
+def test_cf_matches_oracle():
+    rng = np.random.default_rng(5)
+    nspec, n, ne = 3, 150, 2000
+    a = rng.normal(size=(nspec, n + 1))
+    b = np.abs(rng.normal(size=(nspec, n))) + 0.05
+    E0 = rng.normal(size=nspec)
+    g = cf_spectrum(a, b, E0, -10.0, 20.0, ne, 0.4)
+    for s in range(nspec):
+        r = oracle.cf(a[s], b[s], E0[s], -10.0, 20.0, ne, 0.4)
+        assert np.max(np.abs(g[s] - r)) <= 1e-13 * np.max(np.abs(r))
+
+
+def test_anderson_full_size_properties():
+    """The 50-mode, 2,179,235-determinant workload: assembled == matrix-free,
+    Hermiticity, and columns H e_j equal to the oracle's push-form columns."""
+    pr = P.anderson(seed=1)
+    nf = pr["NF"]
+    b = Basis.enumerate(nf, pr["restrictions"])
+    assert b.n == 2179235
+    op = Op(nf, pr["H"])
+    asm = Hop(op, b, mode="assembled")
+    mf = Hop(op, b, mode="matfree")
+    x = torch.from_numpy(cvec(b.n, 8)).cuda()
+    y1 = asm.apply(x)
+    y2 = mf.apply(x)
+    assert (torch.max(torch.abs(y1 - y2)) / torch.max(torch.abs(y2))).item() < 1e-13
+    z = torch.from_numpy(cvec(b.n, 9)).cuda()
+    lhs, rhs = dot(z, y1), dot(asm.apply(z), x)
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+    words = b.words()
+    rng = np.random.default_rng(0)
+    for j in rng.integers(0, b.n, size=6):
+        e = torch.zeros(b.n, dtype=torch.complex128, device="cuda")
+        e[int(j)] = 1.0
+        col = asm.apply(e).cpu().numpy()
+        want = oracle_apply_dense(nf, pr["H"], words[j:j + 1], np.array([1.0 + 0j]), pr["restrictions"], b)
+        assert np.max(np.abs(col - want)) < 1e-13 * np.max(np.abs(want))
+    st = asm.stats()
+    assert st["nnz_eff"] > 0 and st["nnz_stored"] >= 0
