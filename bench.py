@@ -102,6 +102,22 @@ def main():
     ctx.lib.qs_ctx_set_async(ctx.h, 0)
     hpsi_ms = e0.elapsed_time(e1) / args.hpsi_reps
     n = basis.n
+
+    # plain device copy of a known size: the practical HBM ceiling, and the
+    # calibration of rocprof's FETCH_SIZE/WRITE_SIZE counters
+    src = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)   # 1 GiB
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    c1.record()
+    c1.synchronize()
+    copy_ms = c0.elapsed_time(c1) / 5
+    copy_gbps = 2 * src.numel() * 8 / (copy_ms * 1e-3) / 1e9
+    del src, dst
     bytes_model = n * (8 + 16 + 16) + 16 * st["nnz_eff"]          # determinant + x_i + y_i + gathered x_j
     bytes_stream = st["stream_bytes"] + n * 16 * 3 + 16 * st["nnz_stored"]
     achieved = bytes_model / (hpsi_ms * 1e-3) / 1e9
@@ -130,6 +146,7 @@ def main():
                  "stream_bytes": st["stream_bytes"], "bytes_model": bytes_model, "bytes_stream_model": bytes_stream,
                  "GBps_model": achieved, "GBps_stream": bytes_stream / (hpsi_ms * 1e-3) / 1e9},
         "setup_s": {"enumerate": t_enum, "assemble": t_asm},
+        "copy_1GiB": {"ms": copy_ms, "GBps": copy_gbps},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_spmv<1> (assembled H*psi)"},

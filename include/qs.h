@@ -179,6 +179,22 @@ qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates, double tol
                            int max_restarts, uint64_t seed, const double *start, double *E_out,
                            double *vecs, double *variance_out, int *iterations_out);
 
+/* Tensor Green's function from qs_block_lanczos output, evaluated on the GPU:
+ * G(w) = B0^* G_1 B0^T, G_j = (z - A_j - B_j G_{j+1} B_j^+)^-1 bottom-up,
+ * z = w + E0 + i Gamma/2, w on the same ne+1 point grid as qs_cf_spectrum —
+ * BlockTriDiagonalMatrixToSpectraWithBroadening (BasicMath_BlockTypeConversion.cpp:1636-1726).
+ * A: (n+1)*k*k, B: n*k*k complex, kept: n+1 ints (as written by
+ * qs_block_lanczos; k <= 16); out: (ne+1)*k*k complex, out[i][r][c] =
+ * <X_r|(z-H)^-1|X_c> for the start vectors X. Host arrays. */
+qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const double *A, const double *B,
+                               const int *kept, double E0, double emin, double emax, int ne,
+                               double gamma, double *out);
+
+/* Dense Hermitian eigen-decomposition on the host (Householder + implicit QL),
+ * used for the small projected problems. A: n*n complex column-major;
+ * w: n eigenvalues ascending; Z: n*n complex column-major eigenvectors. */
+qs_status qs_heev_host(int n, const double *A, double *w, double *Z);
+
 /* ---- continued-fraction Green's function -------------------------------- */
 /* G(w_i) = a0 + b0^2/(z - a1 - b1^2/(z - a2 - ...)), z = w_i + E0 + i Gamma/2,
  * w_i = emin + i*(emax-emin)/ne, i = 0..ne (ne+1 points) — TriDiagonalMatrixToSpectrum

@@ -167,6 +167,33 @@ class Hop:
         self.ctx.check(self.lib.qs_lanczos(self.ctx.h, self.h, ptr(v0), nsteps, ptr(a), ptr(b)), "qs_lanczos")
         return a, b
 
+    def block_lanczos(self, V0: torch.Tensor, nsteps: int, sv_cut: float = 1.4901161193847656e-08):
+        """V0: (k, n) complex128 start vectors. Returns A (nsteps+1, k, k), B (nsteps, k, k), kept (nsteps+1)."""
+        V0 = V0.contiguous()
+        k = V0.shape[0]
+        A = np.zeros((nsteps + 1, k, k), dtype=np.complex128)
+        B = np.zeros((nsteps, k, k), dtype=np.complex128)
+        kept = np.zeros(nsteps + 1, dtype=np.int32)
+        torch.cuda.synchronize(V0.device)
+        self.ctx.check(self.lib.qs_block_lanczos(self.ctx.h, self.h, k, ptr(V0), nsteps, sv_cut, ptr(A), ptr(B),
+                                                 ptr(kept)), "qs_block_lanczos")
+        return A, B, kept
+
+    def ground_states(self, nstates: int, tol: float = 1e-12, max_krylov: int = 500, max_restarts: int = 100,
+                      seed: int = 1, start: torch.Tensor | None = None):
+        """Lowest eigenpairs. Returns (E (nstates,), vecs (nstates, n) tensor, variances, iterations)."""
+        E = np.zeros(nstates)
+        var = np.zeros(nstates)
+        it = C.c_int()
+        vecs = torch.zeros((nstates, self.rows.n), dtype=torch.complex128, device=_device(self.ctx))
+        if start is not None:
+            start = start.contiguous()
+        torch.cuda.synchronize()
+        self.ctx.check(self.lib.qs_ground_states(self.ctx.h, self.h, nstates, tol, max_krylov, max_restarts, seed,
+                                                 ptr(start), ptr(E), ptr(vecs), ptr(var), C.byref(it)),
+                       "qs_ground_states")
+        return E, vecs, var, it.value
+
     def __del__(self):
         try:
             if self.h:
@@ -211,3 +238,30 @@ def cf_spectrum(a: np.ndarray, b: np.ndarray, E0, emin: float, emax: float, ne: 
     ctx.check(ctx.lib.qs_cf_spectrum(ctx.h, nspec, n, ptr(a), ptr(b), ptr(e0), emin, emax, ne, gamma,
                                      ptr(re), ptr(im)), "qs_cf_spectrum")
     return re + 1j * im
+
+
+def block_cf_spectrum(A: np.ndarray, B: np.ndarray, kept: np.ndarray, E0: float, emin: float, emax: float, ne: int,
+                      gamma: float, ctx: Context | None = None) -> np.ndarray:
+    """Tensor continued fraction on the GPU: returns (ne+1, k, k) complex, G[i][r][c] = <X_r|(z-H)^-1|X_c>."""
+    ctx = ctx or default_context()
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    B = np.ascontiguousarray(B, dtype=np.complex128)
+    kept = np.ascontiguousarray(kept, dtype=np.int32)
+    n, k = B.shape[0], B.shape[1]
+    out = np.zeros((ne + 1, k, k), dtype=np.complex128)
+    ctx.check(ctx.lib.qs_block_cf_spectrum(ctx.h, k, n, ptr(A), ptr(B), ptr(kept), E0, emin, emax, ne, gamma,
+                                           ptr(out)), "qs_block_cf_spectrum")
+    return out
+
+
+def heev_host(A: np.ndarray):
+    """Dense Hermitian eigen-decomposition by libqs' host solver (no GPU needed)."""
+    lib = load()
+    A = np.asfortranarray(np.asarray(A, dtype=np.complex128))
+    n = A.shape[0]
+    w = np.zeros(n)
+    Z = np.zeros((n, n), dtype=np.complex128, order="F")
+    st = lib.qs_heev_host(n, ptr(A), ptr(w), ptr(Z))
+    if st != 0:
+        raise QSError(f"qs_heev_host failed with status {st}")
+    return w, Z

@@ -1,0 +1,57 @@
+"""Host-side pieces of libqs and the C ABI surface (no GPU needed)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from quanty_amd import _lib
+from quanty_amd.engine import Basis, heev_host
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    hdr = open(os.path.join(ROOT, "include", "qs.h")).read()
+    declared = set(re.findall(r"\b(qs_[a-z0-9_]+)\s*\(", hdr))
+    assert declared, "no declarations found"
+    for name in declared:
+        assert hasattr(lib, name), f"libqs does not export {name}"
+    assert declared == set(_lib.EXPORTED), sorted(declared ^ set(_lib.EXPORTED))
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 17, 64, 150])
+def test_heev_matches_numpy(n):
+    rng = np.random.default_rng(n)
+    M = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    A = M + M.conj().T
+    w, Z = heev_host(A)
+    assert np.allclose(w, np.linalg.eigvalsh(A), atol=1e-11 * max(1, np.abs(w).max()))
+    assert np.allclose(Z.conj().T @ Z, np.eye(n), atol=1e-11)
+    assert np.allclose(A @ Z, Z * w, atol=1e-10 * max(1, np.abs(w).max()))
+
+
+def test_heev_degenerate_and_tridiagonal():
+    n = 40
+    d = np.repeat(np.arange(10.0), 4)
+    Q, _ = np.linalg.qr(np.random.default_rng(1).normal(size=(n, n)) + 1j * np.random.default_rng(2).normal(size=(n, n)))
+    A = (Q * d) @ Q.conj().T
+    w, Z = heev_host(A)
+    assert np.allclose(w, d, atol=1e-11)
+    assert np.allclose(A @ Z, Z * w, atol=1e-10)
+    T = np.diag(np.linspace(-1, 1, 30)) + np.diag(np.full(29, 0.3), 1) + np.diag(np.full(29, 0.3), -1)
+    w, Z = heev_host(T)
+    assert np.allclose(w, np.linalg.eigvalsh(T), atol=1e-13)
+
+
+def test_rank_table_counts_match_oracle():
+    import oracle
+    cases = [
+        (20, [("1" * 20, 8, 8)]),
+        (24, [("1" * 24, 10, 10), ("111111000000111111000000", 2, 5), ("000111111111111000000000", 4, 9)]),
+        (30, [("1" * 30, 12, 14), ("101010101010101010101010101010", 5, 9)]),
+        (17, []),
+    ]
+    for nf, restr in cases:
+        assert Basis.count(nf, restr) == len(oracle.enumerate_sector(nf, restr))
