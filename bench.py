@@ -23,6 +23,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -34,7 +38,40 @@ def parse():
     p.add_argument("--hpsi-reps", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--stream", type=int, default=128, help="requests in the served/plain stream sample")
+    p.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate of the stream (req/s)")
+    p.add_argument("--slots", type=int, default=64)
+    p.add_argument("--no-serve", action="store_true")
     return p.parse_args()
+
+
+def bench_serving(args, ctx):
+    """The same Poisson stream of impurity-spectrum requests served with
+    continuous batching and with one request at a time."""
+    from quanty_amd import serve
+
+    reqs = serve.request_stream(args.stream, args.rate, seed=args.seed + 100)
+    warm = serve.request_stream(4, 0.0, seed=args.seed + 200, dim_lo=1e3, dim_hi=1e4)
+    serve.SpectrumServer(ctx, max_slots=4).serve(warm, realtime=False)
+    t0 = time.perf_counter()
+    served = serve.SpectrumServer(ctx, max_slots=args.slots).serve(reqs, realtime=True)
+    t_served = time.perf_counter() - t0
+    log(f"served stream done in {t_served:.1f} s")
+    reqs2 = serve.request_stream(args.stream, args.rate, seed=args.seed + 100)
+    t0 = time.perf_counter()
+    plain = serve.SpectrumServer(ctx, max_slots=1).serve(reqs2, realtime=True, plain=True)
+    t_plain = time.perf_counter() - t0
+    log(f"plain stream done in {t_plain:.1f} s")
+    same = all(np.array_equal(a.spectrum, b.spectrum) for a, b in
+               zip(sorted(served, key=lambda r: r.rid), sorted(plain, key=lambda r: r.rid)))
+    s1, s2 = serve.latency_stats(served), serve.latency_stats(plain)
+    dims = np.array([r.dim for r in reqs])
+    return {"served": s1, "plain": s2, "speedup": s1["spectra_per_s"] / s2["spectra_per_s"],
+            "wall_s": {"served": t_served, "plain": t_plain}, "identical_spectra": bool(same),
+            "stream": {"requests": args.stream, "rate_per_s": args.rate, "dim_min": int(dims.min()),
+                       "dim_max": int(dims.max()), "dim_median": float(np.median(dims)), "depth": "U{100..400}",
+                       "gamma": "U(0.1,0.5)", "ne_points": 2001, "gs": "two-pass Lanczos, 200 steps",
+                       "slots": args.slots}}
 
 
 def main():
@@ -84,6 +121,7 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     assert np.all(np.isfinite(g_last))
+    log(f"plain spectra on the 2.18M basis: {ms_per_step:.1f} ms each")
 
     # H*psi kernel time with HIP events on the library's stream
     stream = torch.cuda.ExternalStream(ctx.lib.qs_ctx_stream(ctx.h), device=dev)
@@ -125,6 +163,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pr, basis, args)
+        log("cpu baseline done")
+    srv = None if args.no_serve else bench_serving(args, ctx)
 
     out = {
         "metric": "spectra/s (plain one-at-a-time GPU Lanczos, 50-mode Anderson impurity, 2,179,235 dets)",
@@ -151,6 +191,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_spmv<1> (assembled H*psi)"},
         "cpu_baseline": cpu,
+        "serving": srv,
     }
     print(json.dumps(out))
 

@@ -168,6 +168,36 @@ qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, int nsteps,
 qs_status qs_block_lanczos(qs_ctx *ctx, const qs_hop *h, int k, const double *V0, int nsteps,
                            double sv_cut, double *A_out, double *B_out, int *kept);
 
+/* ---- batched Lanczos (many problems in lockstep) ------------------------ */
+/* No single counterpart in the reference, which runs one (T, psi) Lanczos at a
+ * time (LuaInterface/ssp.cpp:2598-2639 loops over LanczosTriDiagonalize). A
+ * batch holds up to max_slots independent recurrences; every qs_batch_run step
+ * advances all of them with one set of launches. A slot's coefficients are
+ * bit-identical whether it runs alone or with others. */
+typedef struct qs_batch qs_batch;
+qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out);
+/* Start a recurrence of `depth` steps on an assembled h (rows == cols) from
+ * device vector v0 (copied); max_depth (>= depth) sizes the coefficient arrays
+ * so qs_batch_set_depth can extend the run. ycoef (host, depth doubles, or
+ * NULL): also build y = sum_i ycoef[i] v_i, the Ritz vector of a previous pass
+ * over the same start. */
+qs_status qs_batch_add(qs_batch *b, const qs_hop *h, const double *v0, int depth, int max_depth,
+                       const double *ycoef, int *slot);
+/* Change the depth of a coefficient-only slot (<= its max_depth); a finished
+ * slot resumes from its last step when the depth grows. */
+qs_status qs_batch_set_depth(qs_batch *b, int slot, int depth);
+/* Advance every active slot by `steps` Lanczos steps (slots stop at their depth). */
+qs_status qs_batch_run(qs_batch *b, int steps);
+/* Completed steps of a slot; a/b (host, depth+1 doubles each, may be NULL)
+ * receive a[0..depth] and b[0..depth] (b[depth] is the norm after the last
+ * step); y (device, n complex, may be NULL) receives the accumulated vector. */
+qs_status qs_batch_poll(qs_batch *b, int slot, int *step, double *a, double *bb, double *y);
+qs_status qs_batch_remove(qs_batch *b, int slot);
+int qs_batch_active(const qs_batch *b);
+/* Completed steps of every slot in one transfer: steps[max_slots], -1 for free slots. */
+qs_status qs_batch_steps(qs_batch *b, int *steps);
+qs_status qs_batch_destroy(qs_batch *b);
+
 /* ---- lowest eigenstates (Eigensystem) ----------------------------------- */
 /* nstates lowest eigenpairs of Hermitian h (rows == cols) by thick-restarted
  * block Lanczos with full re-orthogonalization; block size = max(nstates, 1).
@@ -194,6 +224,13 @@ qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const double *A, const
  * used for the small projected problems. A: n*n complex column-major;
  * w: n eigenvalues ascending; Z: n*n complex column-major eigenvectors. */
 qs_status qs_heev_host(int n, const double *A, double *w, double *Z);
+/* Real symmetric tridiagonal T (diagonal d[0..m-1], off-diagonal e[0..m-2]) on
+ * the host: the lowest eigenpair (Sturm bisection + inverse iteration; z has
+ * unit norm, largest component positive), and the smallest/largest
+ * eigenvalues — the pole range of a Lanczos table
+ * (TriDiagonalMatrixToEigenvalues, BasicMath_MatrixTypeConversion.cpp:1538). */
+qs_status qs_tridiag_lowest_host(int m, const double *d, const double *e, double *E, double *z);
+qs_status qs_tridiag_extremes_host(int m, const double *d, const double *e, double *lo, double *hi);
 
 /* ---- continued-fraction Green's function -------------------------------- */
 /* G(w_i) = a0 + b0^2/(z - a1 - b1^2/(z - a2 - ...)), z = w_i + E0 + i Gamma/2,

@@ -63,6 +63,17 @@ _SIGS = [
     ("qs_cf_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _D, _D, C.c_int, _D, _P, _P]),
     ("qs_block_cf_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _D, _D, _D, C.c_int, _D, _P]),
     ("qs_heev_host", C.c_int, [C.c_int, _P, _P, _P]),
+    ("qs_tridiag_lowest_host", C.c_int, [C.c_int, _P, _P, C.POINTER(_D), _P]),
+    ("qs_tridiag_extremes_host", C.c_int, [C.c_int, _P, _P, C.POINTER(_D), C.POINTER(_D)]),
+    ("qs_batch_create", C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    ("qs_batch_add", C.c_int, [_P, _P, _P, C.c_int, C.c_int, _P, C.POINTER(C.c_int)]),
+    ("qs_batch_set_depth", C.c_int, [_P, C.c_int, C.c_int]),
+    ("qs_batch_run", C.c_int, [_P, C.c_int]),
+    ("qs_batch_poll", C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _P, _P, _P]),
+    ("qs_batch_remove", C.c_int, [_P, C.c_int]),
+    ("qs_batch_active", C.c_int, [_P]),
+    ("qs_batch_steps", C.c_int, [_P, _P]),
+    ("qs_batch_destroy", C.c_int, [_P]),
 ]
 EXPORTED = [s[0] for s in _SIGS]
 

@@ -1,0 +1,610 @@
+"""The Lua-facing API of the spectra path, mirrored in Python with the same
+names, argument meaning and error behaviour (LuaInterface/ssp.cpp:572-700
+NewOperator, :927 NewWavefunction, :1906-3089 CreateSpectra, :4039-4272
+Eigensystem; LuaInterface/Operator.cpp:4775 and Wavefunction.cpp:200-398 for
+`H*psi`, `psi*H*psi`). Every numerical step runs in libqs on the GPU; this
+module only decides bases and marshals data.
+
+Conventions kept from the reference:
+* raw operator terms are written products, positive = create, negative (or -0.0)
+  = annihilate; `{i, -j, v}` is v c+_i c_j;
+* Restrictions are `[NF, NB, ["mask", min, max], ...]`;
+* options are lists of pairs `[["Emin", -10], ["NE", 2000], ...]` (or a dict);
+  an unknown option name is an error, as LuaCheckOptions makes it;
+* CreateSpectra returns (Spectra, [Tri tables]) where each table holds
+  a, b (shifted by -E0 on a[1..N]), mu = 0 and type = "Tri".
+"""
+from __future__ import annotations
+
+import math
+from typing import Iterable
+
+import numpy as np
+import torch
+
+from . import det as D
+from . import operators as O
+from ._lib import QSError, default_context
+from .engine import Basis, Hop, Op, block_cf_spectrum, cf_spectrum, tridiag_extremes
+
+SQRT_EPS = math.sqrt(np.finfo(float).eps)
+
+
+# ---------------------------------------------------------------------------
+# options
+# ---------------------------------------------------------------------------
+def _options(opts, allowed: dict) -> dict:
+    """Parse a list of {name, value} pairs against `allowed` (name -> default)."""
+    out = dict(allowed)
+    if opts is None:
+        return out
+    items = opts.items() if isinstance(opts, dict) else opts
+    for item in items:
+        name, value = item[0], item[1]
+        if name not in allowed:
+            raise QSError(f"unknown option {name!r}; possible: {', '.join(allowed)}")
+        out[name] = value
+    return out
+
+
+def _restrictions(R, nf):
+    """[NF, NB, [mask, min, max], ...] -> [(mask, min, max), ...]."""
+    if R is None:
+        return []
+    if R[0] != nf:
+        raise QSError(f"restrictions are for {R[0]} fermions, the operator has {nf}")
+    if R[1] != 0:
+        raise QSError("bosons are not supported")
+    return [(str(m), int(a), int(b)) for m, a, b in R[2:]]
+
+
+# ---------------------------------------------------------------------------
+# operators
+# ---------------------------------------------------------------------------
+class Operator:
+    """A many-body operator: ladder terms (codes, o[0] applied first) -> value."""
+
+    def __init__(self, nf: int, terms: dict | Iterable = (), restrictions=None, name: str = ""):
+        self.NF = nf
+        self.NB = 0
+        self.terms: dict = {}
+        for c, v in (terms.items() if isinstance(terms, dict) else terms):
+            c = tuple(int(x) for x in c)
+            self.terms[c] = self.terms.get(c, 0) + complex(v)
+        self.Restrictions = restrictions
+        self.Name = name
+        self._op = None
+
+    # -- algebra ------------------------------------------------------------
+    def _compatible(self, other):
+        if other.NF != self.NF:
+            raise QSError(f"operators on {self.NF} and {other.NF} fermions cannot be combined")
+
+    def __add__(self, other):
+        if isinstance(other, (int, float, complex)):
+            other = Operator(self.NF, [((), other)])
+        self._compatible(other)
+        out = Operator(self.NF, self.terms)
+        for c, v in other.terms.items():
+            out.terms[c] = out.terms.get(c, 0) + v
+        return out
+
+    __radd__ = __add__
+
+    def __neg__(self):
+        return Operator(self.NF, {c: -v for c, v in self.terms.items()})
+
+    def __sub__(self, other):
+        return self + (-other if isinstance(other, Operator) else -complex(other))
+
+    def __rsub__(self, other):
+        return (-self) + other
+
+    def __mul__(self, other):
+        if isinstance(other, (int, float, complex)):
+            return Operator(self.NF, {c: v * other for c, v in self.terms.items()})
+        if isinstance(other, Operator):
+            self._compatible(other)
+            out = {}
+            for ca, va in self.terms.items():
+                for cb, vb in other.terms.items():
+                    c = cb + ca  # other acts first
+                    out[c] = out.get(c, 0) + va * vb
+            return Operator(self.NF, out)
+        if isinstance(other, Wavefunction):
+            return operator_psi(self, other)
+        return NotImplemented
+
+    def __rmul__(self, other):
+        if isinstance(other, (int, float, complex)):
+            return self * other
+        return NotImplemented
+
+    def __truediv__(self, other):
+        return self * (1.0 / other)
+
+    def ConjugateTranspose(self):
+        return Operator(self.NF, dagger_terms(self.terms))
+
+    def Chop(self, eps=1e-14):
+        return Operator(self.NF, {c: v for c, v in self.terms.items() if abs(v) > eps})
+
+    @property
+    def NTerms(self):
+        return len(self.terms)
+
+    def term_list(self):
+        return list(self.terms.items())
+
+    def device(self, ctx=None):
+        if self._op is None:
+            self._op = Op(self.NF, self.term_list(), ctx or default_context())
+        return self._op
+
+    def particle_change(self):
+        """Set of particle-number changes of the terms."""
+        return {sum(1 if x & O.CR else -1 for x in c) for c in self.terms}
+
+    def __repr__(self):
+        return f"Operator(NF={self.NF}, terms={len(self.terms)})"
+
+
+def dagger_terms(terms: dict) -> dict:
+    return {tuple(x ^ O.CR for x in reversed(c)): complex(v).conjugate() for c, v in terms.items()}
+
+
+def _shell_args(args):
+    return list(args[0]), list(args[1])
+
+
+def NewOperator(*args, **kw):
+    """NewOperator(NF, NB, {{i, -j, ..., v}, ...}) or a named operator:
+    "Cr"/"An" (NF, i), "Number" (NF, idxA, idxB, values), "U" (NF, up, dn, F) or
+    (NF, upA, dnA, upB, dnB, F, G), "ldots"/"Lx"/"Ly"/"Lz"/"Sx"/"Sy"/"Sz" (NF, up, dn),
+    "CF" (NF, up, dn, Akm) or (NF, upA, dnA, upB, dnB, Akm)."""
+    if not args:
+        raise QSError("NewOperator called with 0 arguments, expect at least one")
+    if isinstance(args[0], (int, np.integer)):
+        if len(args) not in (3, 4):
+            raise QSError(f"NewOperator called with {len(args)} arguments, expected 3 or 4")
+        nf, nb, table = int(args[0]), int(args[1]), args[2]
+        if nb != 0:
+            raise QSError("bosons are not supported")
+        opts = _options(args[3] if len(args) == 4 else None, {"Restrictions": None, "Name": ""})
+        out = {}
+        for pos, t in enumerate(table, 1):
+            if len(t) == 0:
+                raise QSError(f"NewOperator called an empty table as operator at position {pos}")
+            *ops, v = t
+            codes = []
+            for j, o in enumerate(ops, 1):
+                if abs(o) > nf - 1 or abs(abs(o) - round(abs(o))) > 10 * np.finfo(float).eps:
+                    raise QSError(f"Operator {pos} at position {j} has invalid index {o}")
+                idx = int(round(abs(o)))
+                codes.append(idx if math.copysign(1.0, o) < 0 else (O.CR | idx))
+            c = tuple(reversed(codes))  # the rightmost written factor acts first
+            out[c] = out.get(c, 0) + complex(v)
+        return Operator(nf, out, restrictions=opts["Restrictions"], name=opts["Name"])
+    name, rest = args[0], args[1:]
+    nf = int(rest[0])
+    if name in ("Cr", "An"):
+        i = int(rest[1])
+        return Operator(nf, [((O.cr(i) if name == "Cr" else O.an(i),), 1.0)])
+    if name == "Number":
+        a, b = list(rest[1]), list(rest[2])
+        vals = list(rest[3]) if len(rest) > 3 else [1.0] * len(a)
+        return Operator(nf, [O.written([O.cr(i), O.an(j)], v) for i, j, v in zip(a, b, vals)])
+    if name in ("U", "Coulomb"):
+        if len(rest) == 4:
+            up, dn = _shell_args(rest[1:3])
+            return Operator(nf, O.coulomb(up, dn, list(rest[3])))
+        upa, dna, upb, dnb = (list(x) for x in rest[1:5])
+        return Operator(nf, O.coulomb_two_shells(upa, dna, upb, dnb, list(rest[5]), list(rest[6])))
+    if name in ("ldots", "Lx", "Ly", "Lz", "Sx", "Sy", "Sz"):
+        up, dn = _shell_args(rest[1:3])
+        return Operator(nf, O.shell_operator(up, dn, name))
+    if name == "CF":
+        if len(rest) == 4:
+            up, dn = _shell_args(rest[1:3])
+            return Operator(nf, O.dipole_p_to_d(up, dn, up, dn, [tuple(x) for x in rest[3]]))
+        upa, dna, upb, dnb = (list(x) for x in rest[1:5])
+        return Operator(nf, O.dipole_p_to_d(upa, dna, upb, dnb, [tuple(x) for x in rest[5]]))
+    raise QSError(f"unknown operator name {name!r}")
+
+
+# ---------------------------------------------------------------------------
+# wavefunctions
+# ---------------------------------------------------------------------------
+class Wavefunction:
+    """A wavefunction: sorted determinants and complex coefficients. It may be
+    host-resident (words, coef) or device-resident (Basis, torch vector)."""
+
+    def __init__(self, nf, words=None, coef=None, basis: Basis | None = None, vec: torch.Tensor | None = None):
+        self.NF = nf
+        self.NB = 0
+        self._words = words
+        self._coef = coef
+        self._basis = basis
+        self._vec = vec
+
+    # -- storage ------------------------------------------------------------
+    def host(self):
+        if self._words is None:
+            w = self._basis.words()
+            c = self._vec.detach().cpu().numpy()
+            keep = c != 0
+            self._words, self._coef = w[keep], c[keep]
+        return self._words, self._coef
+
+    def device(self, ctx=None):
+        if self._basis is None:
+            ctx = ctx or default_context()
+            w, c = self.host()
+            self._basis = Basis.from_words(self.NF, w, ctx)
+            # from_words sorts: place the coefficients by lookup
+            idx = self._basis.index(w)
+            v = np.zeros(self._basis.n, dtype=np.complex128)
+            v[idx] = c
+            self._vec = torch.from_numpy(v).to(torch.device("cuda", ctx.device))
+        return self._basis, self._vec
+
+    @property
+    def N(self):
+        if self._words is not None:
+            return len(self._words)
+        return int(torch.count_nonzero(self._vec).item())
+
+    def particle_numbers(self):
+        w, _ = self.host()
+        return {sum(bin(int(x)).count("1") for x in row) for row in w}
+
+    # -- algebra ------------------------------------------------------------
+    def _combine(self, other, fa, fb):
+        wa, ca = self.host()
+        wb, cb = other.host()
+        keys = {}
+        for w, c in ((wa, ca * fa), (wb, cb * fb)):
+            for row, v in zip(map(tuple, w.tolist()), c):
+                keys[row] = keys.get(row, 0) + v
+        rows = sorted(keys)
+        words = np.array(rows, dtype=np.uint64).reshape(-1, D.nwords(self.NF))
+        return Wavefunction(self.NF, words, np.array([keys[r] for r in rows], dtype=np.complex128))
+
+    def __add__(self, other):
+        return self._combine(other, 1.0, 1.0)
+
+    def __sub__(self, other):
+        return self._combine(other, 1.0, -1.0)
+
+    def __mul__(self, other):
+        if isinstance(other, (int, float, complex)):
+            if self._words is not None:
+                return Wavefunction(self.NF, self._words, self._coef * other)
+            return Wavefunction(self.NF, basis=self._basis, vec=self._vec * other)
+        if isinstance(other, Operator):
+            # psi * H is the bra <psi|H, represented by H^+ |psi> (LuaWavefunctionMul)
+            return operator_psi(Operator(other.NF, dagger_terms(other.terms), other.Restrictions), self)
+        if isinstance(other, Wavefunction):
+            return dot(self, other)
+        return NotImplemented
+
+    __rmul__ = lambda self, other: self * other if isinstance(other, (int, float, complex)) else NotImplemented
+
+    def __truediv__(self, other):
+        return self * (1.0 / other)
+
+    def Normalize(self):
+        n = math.sqrt(abs(dot(self, self)))
+        if self._words is not None:
+            self._coef = self._coef / n
+        else:
+            self._vec = self._vec / n
+        return self
+
+    def __eq__(self, other):
+        d = self - other
+        _, c = d.host()
+        return bool(np.all(np.abs(c) <= 100 * np.finfo(float).eps * max(1.0, float(np.abs(self.host()[1]).max(initial=0)))))
+
+    def __repr__(self):
+        return f"Wavefunction(NF={self.NF}, N={self.N})"
+
+
+def NewWavefunction(nf, nb, table, options=None):
+    """NewWavefunction(NF, NB, {{"0101", coef}, ...}) (LuaInterface/ssp.cpp:927)."""
+    if nb != 0:
+        raise QSError("NewWavefunction with NB > 0 is not supported (the reference rejects it too)")
+    keys = {}
+    for s, c in table:
+        w = D.string_to_words(str(s), nf)
+        keys[w] = keys.get(w, 0) + complex(c)
+    rows = sorted(keys)
+    words = np.array(rows, dtype=np.uint64).reshape(-1, D.nwords(nf))
+    return Wavefunction(nf, words, np.array([keys[r] for r in rows], dtype=np.complex128))
+
+
+def dot(a: Wavefunction, b: Wavefunction):
+    """<a|b>, a conjugated; real when |Im| < 10(eps|Re| + DBL_MIN) (LuaWavefunctionMul)."""
+    if a._basis is not None and a._basis is b._basis:
+        va, vb = a._vec, b._vec
+        r = complex(torch.vdot(va, vb).item())
+    else:
+        wa, ca = a.host()
+        wb, cb = b.host()
+        ia = {tuple(x): i for i, x in enumerate(wa.tolist())}
+        r = 0j
+        for j, x in enumerate(wb.tolist()):
+            i = ia.get(tuple(x))
+            if i is not None:
+                r += np.conj(ca[i]) * cb[j]
+    if abs(r.imag) < 10 * (np.finfo(float).eps * abs(r.real) + np.finfo(float).tiny):
+        return r.real
+    return r
+
+
+# ---------------------------------------------------------------------------
+# bases
+# ---------------------------------------------------------------------------
+def _sector(nf, restrictions, npart, ctx, max_dim=50_000_000) -> Basis:
+    """Sector of `npart` particles inside the restrictions."""
+    restr = list(restrictions) + [("1" * nf, npart, npart)]
+    n = Basis.count(nf, restr)
+    if n > max_dim:
+        raise QSError(f"the {npart}-particle space has {n} determinants; give Restrictions to reduce it")
+    return Basis.enumerate(nf, restr, ctx)
+
+
+def conserved_groups(H: Operator):
+    """Finest partition of the modes found by merging, term by term, the groups
+    whose occupation a term changes; the occupation of every group is then
+    conserved by H. The reference grows the space of Eigensystem and
+    CreateSpectra as the closure of H*psi (Core/BlockLanczos.cpp:5849-5858,
+    LuaInterface/ssp.cpp:2265-2316); fixing these occupations reproduces that
+    closure's sector without iterating."""
+    nf = H.NF
+    parent = list(range(nf))
+
+    def find(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    changed = True
+    while changed:
+        changed = False
+        for c in H.terms:
+            net = {}
+            for x in c:
+                g = find(x & 0x7FFF)
+                net[g] = net.get(g, 0) + (1 if x & O.CR else -1)
+            moving = [g for g, d in net.items() if d != 0]
+            for g in moving[1:]:
+                a, b = find(moving[0]), find(g)
+                if a != b:
+                    parent[b] = a
+                    changed = True
+    groups = {}
+    for m in range(nf):
+        groups.setdefault(find(m), []).append(m)
+    return sorted(groups.values())
+
+
+def _group_mask(nf, modes):
+    s = ["0"] * nf
+    for m in modes:
+        s[m] = "1"
+    return "".join(s)
+
+
+def _sector_of_counts(nf, groups, counts, restrictions, ctx, max_dim=50_000_000) -> Basis:
+    restr = list(restrictions) + [(_group_mask(nf, g), c, c) for g, c in zip(groups, counts)]
+    n = Basis.count(nf, restr)
+    if n > max_dim:
+        raise QSError(f"the reachable space has {n} determinants; give Restrictions to reduce it")
+    return Basis.enumerate(nf, restr, ctx)
+
+
+def _counts_of_words(nf, groups, words):
+    """Occupation of every group for each determinant (host words)."""
+    ints = [D.words_to_int(w) >> (64 * D.nwords(nf) - nf) for w in words]
+    masks = [int(_group_mask(nf, g), 2) for g in groups]
+    return {tuple(bin(x & m).count("1") for m in masks) for x in ints}
+
+
+def operator_psi(op: Operator, psi: Wavefunction, restrictions=None) -> Wavefunction:
+    """O|psi> (OperatorPsi / OperatorPsiRestricted): output over the sector the
+    operator maps psi to, filtered by the operator's or the given restrictions."""
+    ctx = default_context()
+    if op.NF != psi.NF:
+        raise QSError("operator and wavefunction have different numbers of fermions")
+    nf = op.NF
+    R = _restrictions(restrictions if restrictions is not None else op.Restrictions, nf)
+    cols, x = psi.device(ctx)
+    npsi = psi.particle_numbers()
+    dn = op.particle_change()
+    if len(npsi) != 1 or len(dn) != 1:
+        raise QSError("H*psi needs a wavefunction and an operator with one fixed particle number change")
+    rows = _sector(nf, R, npsi.pop() + dn.pop(), ctx)
+    hop = Hop(op.device(ctx), rows, cols, mode="matfree")
+    y = hop.apply(x)
+    return Wavefunction(nf, basis=rows, vec=y)
+
+
+def _expect(op: Operator, psi: Wavefunction) -> complex:
+    """<psi|O|psi> over psi's own support (PsiHermitianOperatorPsi)."""
+    ctx = default_context()
+    b, x = psi.device(ctx)
+    hop = Hop(op.device(ctx), b, b, mode="matfree")
+    y = hop.apply(x)
+    return complex(torch.vdot(x, y).item())
+
+
+# ---------------------------------------------------------------------------
+# Eigensystem
+# ---------------------------------------------------------------------------
+def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
+    """Lowest Npsi eigenstates (LuaEigensystem, LuaInterface/ssp.cpp:4039).
+    The basis is the particle sector of the start determinants, inside the
+    Restrictions when given (the closure of H from StartRestrictions stays in it)."""
+    o = _options(options, {"Restrictions": None, "Zero": 100 * SQRT_EPS, "Epsilon": SQRT_EPS / 100,
+                           "NKrylovMax": 500, "NKrylovStart": 100, "NKrylovStep": 10, "NKrylovMin": 50,
+                           "DenseBorder": 1513, "NBitsKey": 16, "ShiftScheme": "Variance", "Seed": 1,
+                           "MaxRestarts": 200})
+    ctx = default_context()
+    nf = H.NF
+    if H.particle_change() - {0}:
+        raise QSError("Eigensystem needs a particle-number conserving Hamiltonian")
+    start = _restrictions(StartRestrictions, nf)
+    groups = conserved_groups(H)
+    counts = []
+    for g in groups:
+        m = _group_mask(nf, g)
+        cs = [c for c in range(len(g) + 1) if Basis.count(nf, start + [(m, c, c)]) > 0]
+        if len(cs) != 1:
+            raise QSError(f"StartRestrictions allow occupations {cs} of the conserved mode group {g}; "
+                          "one start sector is supported")
+        counts.append(cs[0])
+    basis = _sector_of_counts(nf, groups, counts, _restrictions(o["Restrictions"], nf), ctx)
+    if basis.n < Npsi:
+        raise QSError(f"the space has {basis.n} determinants, fewer than Npsi = {Npsi}")
+    hop = Hop(H.device(ctx), basis, mode="auto")
+    E, vecs, var, it = hop.ground_states(Npsi, tol=float(o["Zero"]), max_krylov=int(o["NKrylovMax"]),
+                                         max_restarts=int(o["MaxRestarts"]), seed=int(o["Seed"]))
+    return [Wavefunction(nf, basis=basis, vec=vecs[i].clone()) for i in range(Npsi)]
+
+
+# ---------------------------------------------------------------------------
+# CreateSpectra
+# ---------------------------------------------------------------------------
+class TriResponse(dict):
+    """A response function in tridiagonal form: {1: a, 2: b, mu, type="Tri", name}
+    callable as G(omega, gamma) (EvaluateTriDiagonalMatrix, BasicMath_GreenFunctionArithmetic.cpp:4621)."""
+
+    def __init__(self, a, b, mu=0.0, name=""):
+        super().__init__(a=np.asarray(a, dtype=float), b=np.asarray(b, dtype=float), mu=mu, type="Tri", name=name)
+
+    def __call__(self, omega, gamma):
+        g = cf_spectrum(self["a"], self["b"], 0.0, float(omega), float(omega) + 1.0, 1, float(gamma))
+        return complex(g[0, 0])
+
+
+class Spectra:
+    """Spectra on the grid w_i = Emin + i (Emax-Emin)/NE, i = 0..NE (SpectraTypeArray)."""
+
+    def __init__(self, emin, emax, ne, gamma, G, eminpole=None, emaxpole=None):
+        self.Emin, self.Emax, self.NE, self.Gamma = emin, emax, ne, gamma
+        self.G = np.asarray(G)  # (nspec, NE+1) complex
+        self.EminPole, self.EmaxPole = eminpole, emaxpole
+
+    @property
+    def energies(self):
+        return self.Emin + np.arange(self.NE + 1) * (self.Emax - self.Emin) / self.NE
+
+    def Sum(self, weights):
+        return Spectra(self.Emin, self.Emax, self.NE, self.Gamma,
+                       [np.tensordot(np.asarray(weights), self.G.reshape(len(weights), -1), 1)])
+
+    def Print(self, file=None):
+        """Energy column then Re/Im of every spectrum, 17 significant digits."""
+        lines = []
+        for i, w in enumerate(self.energies):
+            row = [w]
+            for g in self.G:
+                row += [g[i].real, g[i].imag]
+            lines.append(" ".join(f"{x:23.16E}" for x in row))
+        text = "\n".join(lines) + "\n"
+        if file is None:
+            return text
+        with open(file, "w") as f:
+            f.write(text)
+
+
+def _pole_range(a, b, E0):
+    n = len(b)
+    lo, hi = tridiag_extremes(a[1:n + 1], b[1:n])
+    return lo - E0, hi - E0
+
+
+def CreateSpectra(H: Operator, T, psi, options=None):
+    """Spectra of transition operators T on wavefunctions psi (LuaCreateSpectra,
+    LuaInterface/ssp.cpp:1906): E0 = <psi|H|psi> unless given, T psi (restricted
+    when Restrictions are given), NTri-step Lanczos per (T, psi) — or one block
+    Lanczos over all of them with Tensor = true — and the continued fraction on
+    NE+1 points with Lorentzian FWHM Gamma. Spectra of the dense method are
+    computed by Lanczos here (results agree; the Krylov space is exhausted)."""
+    o = _options(options, {"Tensor": False, "KrylovBasis": False, "NTri": 200, "epsilon": SQRT_EPS / 100,
+                           "Epsilon": SQRT_EPS / 100, "SingularValue": SQRT_EPS, "restrictions": None,
+                           "Restrictions": None, "Emin": None, "Emax": None, "NE": 1000, "Gamma": 0.0,
+                           "E0": None, "DenseBorder": 1513, "ForceDenseMethod": False, "Method": None})
+    ctx = default_context()
+    Ts = list(T) if isinstance(T, (list, tuple)) else [T]
+    psis = list(psi) if isinstance(psi, (list, tuple)) else [psi]
+    if not Ts or not psis:
+        raise QSError("CreateSpectra called with an empty table of operators or wavefunctions")
+    nf = H.NF
+    R = _restrictions(o["Restrictions"] if o["Restrictions"] is not None else o["restrictions"], nf)
+    ntri = int(o["NTri"])
+    E0s = []
+    for p in psis:
+        E0s.append(float(o["E0"]) if o["E0"] is not None else _expect(H, p).real)
+    # T psi for every (T, psi), T outer, psi inner
+    tpsi = [operator_psi(t, p, restrictions=[nf, 0, *R] if R else None) for t in Ts for p in psis]
+    # the space H explores from T psi: fix the occupation of every group of modes H conserves
+    groups = conserved_groups(H)
+    cv = set()
+    for tp in tpsi:
+        w, _ = tp.host()
+        cv |= _counts_of_words(nf, groups, w)
+    if len(cv) != 1:
+        raise QSError(f"T psi spans {len(cv)} sectors of the conserved mode groups; one is supported")
+    rows = _sector_of_counts(nf, groups, cv.pop(), R, ctx)
+    for i, tp in enumerate(tpsi):
+        w, c = tp.host()
+        v = np.zeros(rows.n, dtype=np.complex128)
+        if len(w):
+            idx = rows.index(w)
+            if np.any(idx < 0):
+                raise QSError("T psi has determinants outside its sector")
+            v[idx] = c
+        tpsi[i] = Wavefunction(nf, basis=rows, vec=torch.from_numpy(v).to(torch.device("cuda", ctx.device)))
+    hop = Hop(H.device(ctx), rows, mode="auto")
+    nsp = len(tpsi)
+    psiEn = [E0s[i % len(psis)] for i in range(nsp)]
+    if not o["Tensor"]:
+        A = np.zeros((nsp, ntri + 1))
+        Bv = np.zeros((nsp, ntri))
+        for i, tp in enumerate(tpsi):
+            A[i], Bv[i] = hop.lanczos(tp._vec.contiguous(), ntri)
+        emin, emax, gamma, ne = o["Emin"], o["Emax"], float(o["Gamma"]), int(o["NE"])
+        lo, hi = zip(*[_pole_range(A[i], Bv[i], psiEn[i]) for i in range(nsp)])
+        eminpole, emaxpole = min(lo), max(hi)
+        if emin is None or emax is None:
+            span = emaxpole - eminpole
+            if span == 0:
+                dmin, dmax = (-1.0, 1.0) if eminpole == 0 else ((2.125 * eminpole, -0.125 * eminpole) if eminpole < 0
+                                                                 else (-0.125 * eminpole, 2.125 * eminpole))
+            else:
+                dmin, dmax = eminpole - 0.125 * span, emaxpole + 0.125 * span
+            emin = dmin if emin is None else emin
+            emax = dmax if emax is None else emax
+        if gamma == 0:
+            gamma = 10 * (emax - emin) / ne
+        G = cf_spectrum(A, Bv, np.array(psiEn), float(emin), float(emax), ne, gamma, ctx)
+        tables = []
+        for i in range(nsp):
+            a = A[i].copy()
+            a[1:] -= psiEn[i]  # ShiftTriDiagonalMatrix shifts a[1..N] (BasicMath_TriDiagonalMatrixVector.cpp:133)
+            tables.append(TriResponse(a, Bv[i], 0.0, f"Spectrum {i}"))
+        return Spectra(float(emin), float(emax), ne, gamma, G, eminpole, emaxpole), tables
+    # tensor: one block Lanczos over all T psi, (nsp x nsp) spectra
+    if len(set(psiEn)) != 1:
+        raise QSError("Tensor spectra need one E0 for all wavefunctions (pass E0)")
+    V0 = torch.stack([tp._vec for tp in tpsi]).contiguous()
+    A, B, kept = hop.block_lanczos(V0, ntri, float(o["SingularValue"]))
+    emin = -10.0 if o["Emin"] is None else float(o["Emin"])
+    emax = 10.0 if o["Emax"] is None else float(o["Emax"])
+    ne = int(o["NE"])
+    gamma = float(o["Gamma"]) or 10 * (emax - emin) / ne
+    G = block_cf_spectrum(A, B, kept, psiEn[0], emin, emax, ne, gamma, ctx)
+    return Spectra(emin, emax, ne, gamma, G.reshape(ne + 1, -1).T), {"A": A, "B": B, "kept": kept}

@@ -1,0 +1,461 @@
+// Batched Lanczos: many independent recurrences (different operators, bases,
+// start vectors and depths) advanced in lockstep by one set of launches per
+// step. Each slot holds an assembled operator, its two Lanczos vectors and
+// its coefficient arrays; a device worklist of (slot, slice) and (slot, row
+// chunk) entries drives the kernels, and per-slot partial sums are reduced in a
+// fixed order, so a slot's coefficients are bit-identical whether it runs
+// alone or among others. Step counters live on the device; a slot stops when
+// it reaches its depth.
+//
+// Per step (recurrence of LanczosTriDiagonalizeCC, Core/Lanczos.cpp:2169-2264):
+//   K1: h = H u_i, per-entry partial of Re<h|u_i>, w = s_i h - b_i s_{i-1} u_{i-1}
+//   R1: a_{i+1} = s_i^2 * sum of the slot's partials
+//   K2: u_{i+1} = w - a_{i+1} s_i u_i, partial |u_{i+1}|^2 (and y += z_i s_i u_i
+//       when the slot rebuilds a Ritz vector)
+//   R2: b_{i+1} = sqrt(sum), s_{i+1} = 1/b_{i+1}, step += 1, swap u_i/u_{i+1}
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "qs_internal.h"
+#include "qs_kernels.h"
+
+#define QS_K2_CHUNK 4096  // rows per K2 work item
+
+struct SlotDev {
+  int64_t n;
+  int64_t nslices;
+  const uint64_t *slice_ptr;
+  const uint32_t *slice_len;
+  const uint16_t *rowcnt;
+  const uint32_t *col;
+  const uint16_t *tid;
+  const double2 *coef;
+  const double2 *diag;
+  double2 *uc;
+  double2 *up;
+  double2 *y;            // Ritz-vector accumulator or null
+  const double *ycoef;   // z_i, length depth
+  double *a;             // depth + 1
+  double *b;             // depth + 1
+  double *s;             // depth + 1
+  int step;              // completed steps
+  int depth;
+  int active;
+  int pad;               // capacity (maximum depth) of a, b, s, ycoef
+};
+
+struct WorkItem {
+  int slot;
+  int idx;  // slice (K1) or row chunk (K2)
+};
+
+struct qs_batch {
+  qs_ctx *ctx = nullptr;
+  int cap = 0;
+  SlotDev *d_slots = nullptr;
+  std::vector<SlotDev> h_slots;
+  std::vector<int> used;
+  std::vector<char *> mem;     // per-slot allocation (vectors + scalars)
+  std::vector<const qs_hop *> hops;
+  // worklists (rebuilt when the set of slots changes)
+  bool dirty = true;
+  WorkItem *d_w1 = nullptr, *d_w2 = nullptr;
+  int64_t n1 = 0, n2 = 0, cap1 = 0, cap2 = 0;
+  int *d_r1 = nullptr, *d_r2 = nullptr;  // per slot: [begin, end) of its entries, 2 ints each
+  double *d_p1 = nullptr, *d_p2 = nullptr;  // partials per work item
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b_init(SlotDev *slots, int slot, const double2 *__restrict__ v0, double *__restrict__ part) {
+  __shared__ double red[QS_BLOCK];
+  SlotDev &S = slots[slot];
+  double sum = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 x = v0[i];
+    S.uc[i] = x;
+    S.up[i] = make_double2(0.0, 0.0);
+    if (S.y) S.y[i] = make_double2(0.0, 0.0);
+    sum += x.x * x.x + x.y * x.y;
+  }
+  const double t = block_sum_values(sum, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b_init_finish(SlotDev *slots, int slot, const double *__restrict__ part, int np) {
+  __shared__ double red[QS_BLOCK];
+  const double nrm2 = block_sum_fixed(part, np, red);
+  if (threadIdx.x == 0) {
+    SlotDev &S = slots[slot];
+    const double b0 = sqrt(nrm2);
+    S.a[0] = 0.0;
+    S.b[0] = b0;
+    S.s[0] = b0 > 0.0 ? 1.0 / b0 : 0.0;
+    S.step = 0;
+  }
+}
+
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64_t nwork, double *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t w = wave; w < nwork; w += nwaves) {
+    const WorkItem it = work[w];
+    const SlotDev &S = slots[it.slot];
+    if (!S.active || S.step >= S.depth) continue;  // uniform across the wave
+    const int i = S.step;
+    const double sc = S.s[i];
+    const double cp = i > 0 ? S.b[i] * S.s[i - 1] : 0.0;
+    const int64_t sl = it.idx;
+    const int64_t r = sl * 64 + lane;
+    const bool act = r < S.n;
+    double hr[1], hi[1];
+    spmv_row<1>(r, act, S.slice_ptr[sl], S.slice_len[sl], S.rowcnt, S.col, S.tid, S.coef, S.diag, S.uc, 1, lane, hr,
+                hi);
+    double dot = 0.0;
+    if (act) {
+      const double2 u = S.uc[r];
+      dot = hr[0] * u.x + hi[0] * u.y;
+      const double2 p = S.up[r];
+      S.up[r] = make_double2(sc * hr[0] - cp * p.x, sc * hi[0] - cp * p.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);  // fixed butterfly: same order every call
+    if (lane == 0) part[w] = dot;
+  }
+}
+
+// one wavefront per slot: fixed-order sum of the slot's work-item partials
+__global__ void k_b_r1(SlotDev *slots, int nslots, const int *__restrict__ range, const double *__restrict__ part) {
+  const int slot = blockIdx.x;
+  if (slot >= nslots) return;
+  SlotDev &S = slots[slot];
+  if (!S.active || S.step >= S.depth) return;
+  const int lane = threadIdx.x;
+  const int b = range[2 * slot], e = range[2 * slot + 1];
+  double sum = 0.0;
+  for (int k = b + lane; k < e; k += 64) sum += part[k];
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+  if (lane == 0) {
+    const double sc = S.s[S.step];
+    S.a[S.step + 1] = sc * sc * sum;
+  }
+}
+
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b2(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64_t nwork, double *__restrict__ part) {
+  __shared__ double red[QS_BLOCK];
+  for (int64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const WorkItem it = work[w];
+    const SlotDev &S = slots[it.slot];
+    if (!S.active || S.step >= S.depth) continue;  // uniform across the block
+    const int i = S.step;
+    const double sc = S.s[i];
+    const double f = S.a[i + 1] * sc;
+    const double zc = S.y ? S.ycoef[i] * sc : 0.0;
+    const int64_t r0 = (int64_t)it.idx * QS_K2_CHUNK;
+    const int64_t r1 = min(S.n, r0 + QS_K2_CHUNK);
+    double nrm = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+      const double2 wv = S.up[r];
+      const double2 u = S.uc[r];
+      if (S.y) {
+        const double2 yv = S.y[r];
+        S.y[r] = make_double2(yv.x + zc * u.x, yv.y + zc * u.y);
+      }
+      const double2 un = make_double2(wv.x - f * u.x, wv.y - f * u.y);
+      S.up[r] = un;
+      nrm += un.x * un.x + un.y * un.y;
+    }
+    const double t = block_sum_values(nrm, red);
+    if (threadIdx.x == 0) part[w] = t;
+  }
+}
+
+__global__ void k_b_r2(SlotDev *slots, int nslots, const int *__restrict__ range, const double *__restrict__ part) {
+  const int slot = blockIdx.x;
+  if (slot >= nslots) return;
+  SlotDev &S = slots[slot];
+  if (!S.active || S.step >= S.depth) return;
+  const int lane = threadIdx.x;
+  const int b = range[2 * slot], e = range[2 * slot + 1];
+  double sum = 0.0;
+  for (int k = b + lane; k < e; k += 64) sum += part[k];
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+  if (lane == 0) {
+    const int i = S.step + 1;
+    const double bn = sqrt(sum);
+    S.b[i] = bn;
+    S.s[i] = bn > 0.0 ? 1.0 / bn : 0.0;
+    double2 *t = S.uc;
+    S.uc = S.up;
+    S.up = t;
+    S.step = i;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+static qs_status rebuild(qs_batch *B) {
+  qs_ctx *ctx = B->ctx;
+  std::vector<WorkItem> w1, w2;
+  std::vector<int> r1(2 * B->cap, 0), r2(2 * B->cap, 0);
+  for (int s = 0; s < B->cap; ++s) {
+    r1[2 * s] = (int)w1.size();
+    r2[2 * s] = (int)w2.size();
+    if (B->used[s]) {
+      const SlotDev &S = B->h_slots[s];
+      for (int64_t k = 0; k < S.nslices; ++k) w1.push_back({s, (int)k});
+      for (int64_t k = 0; k * QS_K2_CHUNK < S.n; ++k) w2.push_back({s, (int)k});
+    }
+    r1[2 * s + 1] = (int)w1.size();
+    r2[2 * s + 1] = (int)w2.size();
+  }
+  auto grow = [&](WorkItem **p, int64_t *cap, int64_t need) -> qs_status {
+    if (need > *cap) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+      int64_t c = std::max<int64_t>(need, 2 * *cap);
+      QS_TRY(qs_malloc(ctx, (void **)p, (size_t)c * sizeof(WorkItem)));
+      *cap = c;
+    }
+    return QS_OK;
+  };
+  const int64_t c1 = B->cap1, c2 = B->cap2;
+  QS_TRY(grow(&B->d_w1, &B->cap1, std::max<size_t>(1, w1.size())));
+  QS_TRY(grow(&B->d_w2, &B->cap2, std::max<size_t>(1, w2.size())));
+  if (B->cap1 != c1 || !B->d_p1) {
+    if (B->d_p1) (void)hipFree(B->d_p1);
+    QS_TRY(qs_malloc(ctx, (void **)&B->d_p1, (size_t)B->cap1 * 8));
+  }
+  if (B->cap2 != c2 || !B->d_p2) {
+    if (B->d_p2) (void)hipFree(B->d_p2);
+    QS_TRY(qs_malloc(ctx, (void **)&B->d_p2, (size_t)B->cap2 * 8));
+  }
+  if (!w1.empty())
+    QS_HIP(ctx, hipMemcpyAsync(B->d_w1, w1.data(), w1.size() * sizeof(WorkItem), hipMemcpyHostToDevice, ctx->stream));
+  if (!w2.empty())
+    QS_HIP(ctx, hipMemcpyAsync(B->d_w2, w2.data(), w2.size() * sizeof(WorkItem), hipMemcpyHostToDevice, ctx->stream));
+  QS_HIP(ctx, hipMemcpyAsync(B->d_r1, r1.data(), r1.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  QS_HIP(ctx, hipMemcpyAsync(B->d_r2, r2.data(), r2.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  B->n1 = (int64_t)w1.size();
+  B->n2 = (int64_t)w2.size();
+  B->dirty = false;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out) {
+  if (!ctx || !out || max_slots < 1) return QS_EINVAL;
+  *out = nullptr;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  qs_batch *B = new qs_batch();
+  B->ctx = ctx;
+  B->cap = max_slots;
+  B->h_slots.assign(max_slots, SlotDev{});
+  B->used.assign(max_slots, 0);
+  B->mem.assign(max_slots, nullptr);
+  B->hops.assign(max_slots, nullptr);
+  qs_status st = qs_malloc(ctx, (void **)&B->d_slots, (size_t)max_slots * sizeof(SlotDev));
+  if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r1, (size_t)max_slots * 8);
+  if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r2, (size_t)max_slots * 8);
+  if (st == QS_OK) {
+    hipError_t e = hipMemsetAsync(B->d_slots, 0, (size_t)max_slots * sizeof(SlotDev), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) st = qs_hip_fail(ctx, e, "qs_batch_create", __FILE__, __LINE__);
+  }
+  if (st != QS_OK) {
+    qs_batch_destroy(B);
+    return st;
+  }
+  *out = B;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_batch_add(qs_batch *B, const qs_hop *h, const double *v0, int depth, int max_depth,
+                                  const double *ycoef, int *slot_out) {
+  if (!B || !h || !v0 || depth < 1 || !slot_out) return QS_EINVAL;
+  if (max_depth < depth) max_depth = depth;
+  qs_ctx *ctx = B->ctx;
+  if (!h->assembled || h->rows != h->cols)
+    return qs_fail(ctx, QS_EINVAL, "qs_batch_add needs an assembled operator with rows == cols");
+  int slot = -1;
+  for (int s = 0; s < B->cap; ++s)
+    if (!B->used[s]) {
+      slot = s;
+      break;
+    }
+  if (slot < 0) return qs_fail(ctx, QS_EINVAL, "batch is full (%d slots)", B->cap);
+  const int64_t n = h->rows->n;
+  const size_t vb = (size_t)std::max<int64_t>(n, 1) * 16;
+  const bool want_y = ycoef != nullptr;
+  const size_t sc = (size_t)(max_depth + 1) * 8;
+  char *m = nullptr;
+  QS_TRY(qs_malloc(ctx, (void **)&m, (want_y ? 3 : 2) * vb + 4 * sc + 256));
+  SlotDev S{};
+  S.n = n;
+  S.nslices = h->nslices;
+  S.slice_ptr = h->d_slice_ptr;
+  S.slice_len = h->d_slice_len;
+  S.rowcnt = h->d_rowcnt;
+  S.col = h->d_col;
+  S.tid = h->d_tid;
+  S.coef = (const double2 *)h->op->d_coef;
+  S.diag = (const double2 *)h->d_diag;
+  S.uc = (double2 *)m;
+  S.up = (double2 *)(m + vb);
+  S.y = want_y ? (double2 *)(m + 2 * vb) : nullptr;
+  double *scal = (double *)(m + (want_y ? 3 : 2) * vb);
+  S.a = scal;
+  S.b = scal + (max_depth + 1);
+  S.s = scal + 2 * (max_depth + 1);
+  double *yc = scal + 3 * (max_depth + 1);
+  S.ycoef = want_y ? yc : nullptr;
+  S.step = 0;
+  S.depth = depth;
+  S.active = 1;
+  S.pad = max_depth;  // capacity of the coefficient arrays
+  hipError_t e = hipSuccess;
+  if (want_y) e = hipMemcpyAsync(yc, ycoef, (size_t)depth * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(scal, 0, 3 * sc, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(B->d_slots + slot, &S, sizeof(SlotDev), hipMemcpyHostToDevice, ctx->stream);
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + QS_BLOCK - 1) / QS_BLOCK));
+  double *part = nullptr;
+  void *scr;
+  qs_status st = qs_scratch(ctx, (size_t)nb * 8, &scr);
+  part = (double *)scr;
+  if (st == QS_OK && e == hipSuccess) {
+    hipLaunchKernelGGL(k_b_init, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, (const double2 *)v0, part);
+    hipLaunchKernelGGL(k_b_init_finish, dim3(1), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, part, nb);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (st != QS_OK || e != hipSuccess) {
+    (void)hipFree(m);
+    return st != QS_OK ? st : qs_hip_fail(ctx, e, "qs_batch_add", __FILE__, __LINE__);
+  }
+  B->h_slots[slot] = S;
+  B->used[slot] = 1;
+  B->mem[slot] = m;
+  B->hops[slot] = h;
+  B->dirty = true;
+  *slot_out = slot;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_batch_run(qs_batch *B, int steps) {
+  if (!B || steps < 0) return QS_EINVAL;
+  qs_ctx *ctx = B->ctx;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  if (B->dirty) QS_TRY(rebuild(B));
+  if (B->n1 == 0) return QS_OK;
+  const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>((B->n1 + 3) / 4, (int64_t)ctx->num_cu * 8));
+  const int g2 = (int)std::max<int64_t>(1, std::min<int64_t>(B->n2, (int64_t)ctx->num_cu * 4));
+  for (int k = 0; k < steps; ++k) {
+    hipLaunchKernelGGL(k_b1, dim3(g1), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_w1, B->n1, B->d_p1);
+    hipLaunchKernelGGL(k_b_r1, dim3(B->cap), dim3(64), 0, ctx->stream, B->d_slots, B->cap, B->d_r1, B->d_p1);
+    hipLaunchKernelGGL(k_b2, dim3(g2), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_w2, B->n2, B->d_p2);
+    hipLaunchKernelGGL(k_b_r2, dim3(B->cap), dim3(64), 0, ctx->stream, B->d_slots, B->cap, B->d_r2, B->d_p2);
+  }
+  QS_HIP(ctx, hipGetLastError());
+  if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+__global__ void k_b_set_depth(SlotDev *slots, int slot, int depth) { slots[slot].depth = depth; }
+
+extern "C" qs_status qs_batch_set_depth(qs_batch *B, int slot, int depth) {
+  if (!B || slot < 0 || slot >= B->cap || !B->used[slot]) return QS_EINVAL;
+  qs_ctx *ctx = B->ctx;
+  SlotDev &H = B->h_slots[slot];
+  if (depth < 1 || depth > H.pad) return qs_fail(ctx, QS_EINVAL, "depth %d outside 1..%d", depth, H.pad);
+  if (H.y) return qs_fail(ctx, QS_EINVAL, "the depth of a vector-building slot is fixed");
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(k_b_set_depth, dim3(1), dim3(1), 0, ctx->stream, B->d_slots, slot, depth);
+  QS_HIP(ctx, hipGetLastError());
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  H.depth = depth;
+  return QS_OK;
+}
+
+extern "C" qs_status qs_batch_poll(qs_batch *B, int slot, int *step, double *a, double *b, double *y) {
+  if (!B || slot < 0 || slot >= B->cap || !B->used[slot]) return QS_EINVAL;
+  qs_ctx *ctx = B->ctx;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  SlotDev S;
+  QS_HIP(ctx, hipMemcpyAsync(&S, B->d_slots + slot, sizeof(SlotDev), hipMemcpyDeviceToHost, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (step) *step = S.step;
+  const SlotDev &H = B->h_slots[slot];
+  if (a) QS_HIP(ctx, hipMemcpyAsync(a, H.a, (size_t)(H.depth + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (b) QS_HIP(ctx, hipMemcpyAsync(b, H.b, (size_t)(H.depth + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (y) {
+    if (!H.y) return qs_fail(ctx, QS_EINVAL, "slot %d accumulates no vector", slot);
+    QS_HIP(ctx, hipMemcpyAsync(y, H.y, (size_t)H.n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+extern "C" qs_status qs_batch_remove(qs_batch *B, int slot) {
+  if (!B || slot < 0 || slot >= B->cap || !B->used[slot]) return QS_EINVAL;
+  qs_ctx *ctx = B->ctx;
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SlotDev z{};
+  QS_HIP(ctx, hipMemcpyAsync(B->d_slots + slot, &z, sizeof(SlotDev), hipMemcpyHostToDevice, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  (void)hipFree(B->mem[slot]);
+  B->mem[slot] = nullptr;
+  B->used[slot] = 0;
+  B->hops[slot] = nullptr;
+  B->h_slots[slot] = SlotDev{};
+  B->dirty = true;
+  return QS_OK;
+}
+
+__global__ void k_b_steps(const SlotDev *__restrict__ slots, int n, int *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = slots[i].active ? slots[i].step : -1;
+}
+
+extern "C" qs_status qs_batch_steps(qs_batch *B, int *steps) {
+  if (!B || !steps) return QS_EINVAL;
+  qs_ctx *ctx = B->ctx;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  void *scr;
+  QS_TRY(qs_scratch(ctx, (size_t)B->cap * 4, &scr));
+  hipLaunchKernelGGL(k_b_steps, dim3((B->cap + 255) / 256), dim3(256), 0, ctx->stream, B->d_slots, B->cap, (int *)scr);
+  QS_HIP(ctx, hipGetLastError());
+  QS_HIP(ctx, hipMemcpyAsync(steps, scr, (size_t)B->cap * 4, hipMemcpyDeviceToHost, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+extern "C" int qs_batch_active(const qs_batch *B) {
+  if (!B) return -1;
+  int k = 0;
+  for (int u : B->used) k += u;
+  return k;
+}
+
+extern "C" qs_status qs_batch_destroy(qs_batch *B) {
+  if (!B) return QS_OK;
+  if (B->ctx) (void)hipStreamSynchronize(B->ctx->stream);
+  for (char *m : B->mem)
+    if (m) (void)hipFree(m);
+  if (B->d_slots) (void)hipFree(B->d_slots);
+  if (B->d_w1) (void)hipFree(B->d_w1);
+  if (B->d_w2) (void)hipFree(B->d_w2);
+  if (B->d_r1) (void)hipFree(B->d_r1);
+  if (B->d_r2) (void)hipFree(B->d_r2);
+  if (B->d_p1) (void)hipFree(B->d_p1);
+  if (B->d_p2) (void)hipFree(B->d_p2);
+  delete B;
+  return QS_OK;
+}

@@ -152,6 +152,127 @@ void qs_heev(int n, const double *A, double *w, double *Z) {
   heev_impl(n, reinterpret_cast<const cd *>(A), w, reinterpret_cast<cd *>(Z));
 }
 
+// ---------------------------------------------------------------------------
+// real symmetric tridiagonal T (diagonal d[0..m-1], off-diagonal e[0..m-2])
+// ---------------------------------------------------------------------------
+namespace {
+
+// number of eigenvalues of T below x (Sturm sequence of the LDL^T pivots)
+int sturm_count(int m, const double *d, const double *e, double x) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (q < 0) ++cnt;
+  for (int i = 1; i < m; ++i) {
+    const double qq = q != 0.0 ? q : std::numeric_limits<double>::epsilon() * (std::fabs(e[i - 1]) + 1e-300);
+    q = d[i] - x - e[i - 1] * e[i - 1] / qq;
+    if (q < 0) ++cnt;
+  }
+  return cnt;
+}
+
+// k-th smallest eigenvalue (k = 0..m-1) by bisection on the Gershgorin interval
+double kth_eigenvalue(int m, const double *d, const double *e, int k) {
+  double lo = d[0], hi = d[0];
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+  }
+  const double span = std::max(hi - lo, 1e-300);
+  lo -= 1e-12 * span;
+  hi += 1e-12 * span;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_count(m, d, e, mid) > k) hi = mid; else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+}  // namespace
+
+// Lowest eigenpair of T: eigenvalue by Sturm bisection, eigenvector by inverse
+// iteration with the tridiagonal LU (Thomas algorithm, partial pivoting).
+extern "C" qs_status qs_tridiag_lowest_host(int m, const double *d, const double *e, double *E, double *z) {
+  if (m < 1 || !d || !E || !z || (m > 1 && !e)) return QS_EINVAL;
+  const double lam = kth_eigenvalue(m, d, e, 0);
+  *E = lam;
+  if (m == 1) {
+    z[0] = 1.0;
+    return QS_OK;
+  }
+  double scale = 0.0;
+  for (int i = 0; i < m; ++i) scale = std::max(scale, std::fabs(d[i]) + (i + 1 < m ? std::fabs(e[i]) : 0.0));
+  const double shift = lam - 1e-14 * std::max(scale, 1e-300);
+  // LU of (T - shift) with partial pivoting: rows i, i+1 may swap; U has 3 diagonals
+  std::vector<double> u0(m), u1(m, 0.0), u2(m, 0.0), l(m, 0.0);
+  std::vector<int> swp(m, 0);
+  double a = d[0] - shift, bnext = e[0];  // current row: diagonal a, super-diagonal bnext
+  double c = 0.0;                            // second super-diagonal carried into the current row
+  for (int i = 0; i < m - 1; ++i) {
+    const double sub = e[i];                  // T[i+1][i]
+    const double dn = d[i + 1] - shift;       // T[i+1][i+1]
+    const double en = i + 2 < m ? e[i + 1] : 0.0;
+    if (std::fabs(sub) > std::fabs(a)) {
+      // swap rows i and i+1
+      swp[i] = 1;
+      u0[i] = sub;
+      u1[i] = dn;
+      u2[i] = en;
+      const double f = a / sub;
+      l[i] = f;
+      a = bnext - f * dn;
+      bnext = c - f * en;
+      c = 0.0;
+    } else {
+      u0[i] = a != 0.0 ? a : 1e-300;
+      u1[i] = bnext;
+      u2[i] = c;
+      const double f = sub / u0[i];
+      l[i] = f;
+      a = dn - f * bnext;
+      bnext = en - f * c;
+      c = 0.0;
+    }
+  }
+  u0[m - 1] = a != 0.0 ? a : 1e-300;
+  std::vector<double> x(m, 1.0), y(m);
+  for (int iter = 0; iter < 4; ++iter) {
+    // forward: apply the row swaps and L
+    for (int i = 0; i < m; ++i) y[i] = x[i];
+    for (int i = 0; i < m - 1; ++i) {
+      if (swp[i]) std::swap(y[i], y[i + 1]);
+      y[i + 1] -= l[i] * y[i];
+    }
+    // back substitution with U
+    for (int i = m - 1; i >= 0; --i) {
+      double s = y[i];
+      if (i + 1 < m) s -= u1[i] * x[i + 1];
+      if (i + 2 < m) s -= u2[i] * x[i + 2];
+      x[i] = s / u0[i];
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < m; ++i) nrm += x[i] * x[i];
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < m; ++i) x[i] /= nrm;
+  }
+  // fix the sign: largest component positive
+  int im = 0;
+  for (int i = 1; i < m; ++i)
+    if (std::fabs(x[i]) > std::fabs(x[im])) im = i;
+  const double sg = x[im] < 0 ? -1.0 : 1.0;
+  for (int i = 0; i < m; ++i) z[i] = sg * x[i];
+  return QS_OK;
+}
+
+// Smallest and largest eigenvalue of T (pole range of a Lanczos table).
+extern "C" qs_status qs_tridiag_extremes_host(int m, const double *d, const double *e, double *lo, double *hi) {
+  if (m < 1 || !d || !lo || !hi || (m > 1 && !e)) return QS_EINVAL;
+  *lo = kth_eigenvalue(m, d, e, 0);
+  *hi = kth_eigenvalue(m, d, e, m - 1);
+  return QS_OK;
+}
+
 extern "C" qs_status qs_heev_host(int n, const double *A, double *w, double *Z) {
   if (n < 0 || (n > 0 && (!A || !w || !Z))) return QS_EINVAL;
   if (!heev_impl(n, reinterpret_cast<const cd *>(A), w, reinterpret_cast<cd *>(Z))) return QS_ENOTCONVERGED;

@@ -265,3 +265,92 @@ def heev_host(A: np.ndarray):
     if st != 0:
         raise QSError(f"qs_heev_host failed with status {st}")
     return w, Z
+
+
+class Batch:
+    """Batched Lanczos recurrences advanced in lockstep (qs_batch)."""
+
+    def __init__(self, max_slots: int, ctx: Context | None = None):
+        self.cap = max_slots
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        h = C.c_void_p()
+        self.ctx.check(self.lib.qs_batch_create(self.ctx.h, max_slots, C.byref(h)), "qs_batch_create")
+        self.h = h
+        self.depth = {}
+        self.n = {}
+
+    def add(self, hop: Hop, v0: torch.Tensor, depth: int, ycoef: np.ndarray | None = None,
+            max_depth: int | None = None) -> int:
+        slot = C.c_int()
+        yc = None if ycoef is None else np.ascontiguousarray(ycoef, dtype=np.float64)
+        md = depth if max_depth is None else max(depth, max_depth)
+        torch.cuda.synchronize(v0.device)
+        self.ctx.check(self.lib.qs_batch_add(self.h, hop.h, ptr(v0), depth, md, ptr(yc), C.byref(slot)),
+                       "qs_batch_add")
+        self.depth[slot.value] = depth
+        self.maxdepth = getattr(self, "maxdepth", {})
+        self.maxdepth[slot.value] = md
+        self.n[slot.value] = hop.rows.n
+        return slot.value
+
+    def set_depth(self, slot: int, depth: int):
+        self.ctx.check(self.lib.qs_batch_set_depth(self.h, slot, depth), "qs_batch_set_depth")
+        self.depth[slot] = depth
+
+    def run(self, steps: int):
+        self.ctx.check(self.lib.qs_batch_run(self.h, steps), "qs_batch_run")
+
+    def poll(self, slot: int, want_coef=True, y: torch.Tensor | None = None):
+        st = C.c_int()
+        d = self.maxdepth[slot]
+        a = np.zeros(d + 1) if want_coef else None
+        b = np.zeros(d + 1) if want_coef else None
+        self.ctx.check(self.lib.qs_batch_poll(self.h, slot, C.byref(st), ptr(a), ptr(b), ptr(y)), "qs_batch_poll")
+        return st.value, a, b
+
+    def remove(self, slot: int):
+        self.ctx.check(self.lib.qs_batch_remove(self.h, slot), "qs_batch_remove")
+        self.depth.pop(slot, None)
+        self.n.pop(slot, None)
+
+    def active(self) -> int:
+        return int(self.lib.qs_batch_active(self.h))
+
+    def steps(self, cap: int) -> np.ndarray:
+        out = np.zeros(cap, dtype=np.int32)
+        self.ctx.check(self.lib.qs_batch_steps(self.h, ptr(out)), "qs_batch_steps")
+        return out
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.qs_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def tridiag_lowest(d: np.ndarray, e: np.ndarray):
+    """Lowest eigenpair of the real symmetric tridiagonal matrix (d, e) (host, libqs)."""
+    lib = load()
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    m = len(d)
+    E = C.c_double()
+    z = np.zeros(m)
+    st = lib.qs_tridiag_lowest_host(m, ptr(d), ptr(e) if m > 1 else None, C.byref(E), ptr(z))
+    if st != 0:
+        raise QSError(f"qs_tridiag_lowest_host failed with status {st}")
+    return E.value, z
+
+
+def tridiag_extremes(d: np.ndarray, e: np.ndarray):
+    lib = load()
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    lo, hi = C.c_double(), C.c_double()
+    st = lib.qs_tridiag_extremes_host(len(d), ptr(d), ptr(e) if len(d) > 1 else None, C.byref(lo), C.byref(hi))
+    if st != 0:
+        raise QSError(f"qs_tridiag_extremes_host failed with status {st}")
+    return lo.value, hi.value

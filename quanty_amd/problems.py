@@ -14,6 +14,7 @@ All randomness comes from SplitMix64 streams so a seed fixes the problem.
 from __future__ import annotations
 
 import math
+from functools import lru_cache
 
 import numpy as np
 
@@ -116,6 +117,34 @@ def ni_xas(params=None):
 # ---------------------------------------------------------------------------
 # Anderson impurity: d shell + bath shells
 # ---------------------------------------------------------------------------
+def anderson_restrictions(nbath: int = 4, nelec: int = 28, full_baths: int = 2, d_range=(6, 10),
+                          max_holes: int = 2, max_elec: int = 2):
+    """Restrictions of the Anderson sector: total N, d occupation window, at most
+    `max_holes` holes in the nominally full baths, at most `max_elec` electrons in
+    the nominally empty ones."""
+    NF = 10 * (1 + nbath)
+    dmodes = list(range(10))
+    full = list(range(10, 10 + 10 * full_baths))
+    empty = list(range(10 + 10 * full_baths, NF))
+    restr = [(mask(NF, range(NF)), nelec, nelec), (mask(NF, dmodes), d_range[0], d_range[1])]
+    if full:
+        restr.append((mask(NF, full), len(full) - max_holes, len(full)))
+    if empty:
+        restr.append((mask(NF, empty), 0, max_elec))
+    return restr
+
+
+@lru_cache(maxsize=8)
+def _impurity_d_terms(coulomb, zeta, tenDq):
+    up_d, dn_d = shell(0, 2)
+    dmodes = O.shell_modes(up_d, dn_d)
+    Udd, F2, F4 = coulomb
+    F0 = Udd + (F2 + F4) * 2 / 63
+    eg, t2g = O.oh_d_projectors()
+    return tuple(O.coulomb(up_d, dn_d, [F0, F2, F4]) + O.shell_operator(up_d, dn_d, "ldots", zeta)
+                 + O.one_body(dmodes, dmodes, tenDq * O.d_shell_matrix_to_spin(0.6 * eg - 0.4 * t2g)))
+
+
 def anderson(seed: int = 1, nbath: int = 4, nelec: int = 28, full_baths: int = 2, d_range=(6, 10),
              max_holes: int = 2, max_elec: int = 2, coulomb=(7.3, 9.79, 6.08), zeta=0.083, tenDq=1.1):
     """Anderson impurity with `nbath` bath shells; the first `full_baths` are
@@ -125,12 +154,7 @@ def anderson(seed: int = 1, nbath: int = 4, nelec: int = 28, full_baths: int = 2
     NF = 10 * (1 + nbath)
     up_d, dn_d = shell(0, 2)
     dmodes = O.shell_modes(up_d, dn_d)
-    Udd, F2, F4 = coulomb
-    F0 = Udd + (F2 + F4) * 2 / 63
-    eg, t2g = O.oh_d_projectors()
-    terms = [O.coulomb(up_d, dn_d, [F0, F2, F4]),
-             O.shell_operator(up_d, dn_d, "ldots", zeta),
-             O.one_body(dmodes, dmodes, tenDq * O.d_shell_matrix_to_spin(0.6 * eg - 0.4 * t2g))]
+    terms = [list(_impurity_d_terms(tuple(coulomb), zeta, tenDq))]
     baths = []
     for b in range(nbath):
         up_b, dn_b = shell(10 + 10 * b, 2)
@@ -155,13 +179,7 @@ def anderson(seed: int = 1, nbath: int = 4, nelec: int = 28, full_baths: int = 2
                 T[j, i] = z.conjugate()
         terms.append(O.one_body(bm, bm, T))
     H = O.add(*terms)
-    full = [m for b in baths[:full_baths] for m in b]
-    empty = [m for b in baths[full_baths:] for m in b]
-    restr = [(mask(NF, range(NF)), nelec, nelec), (mask(NF, dmodes), d_range[0], d_range[1])]
-    if full:
-        restr.append((mask(NF, full), len(full) - max_holes, len(full)))
-    if empty:
-        restr.append((mask(NF, empty), 0, max_elec))
+    restr = anderson_restrictions(nbath, nelec, full_baths, d_range, max_holes, max_elec)
     return dict(NF=NF, H=H, restrictions=restr, dmodes=dmodes, baths=baths, seed=seed)
 
 

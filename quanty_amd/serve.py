@@ -1,0 +1,248 @@
+"""Continuous-batching spectrum service.
+
+A request is one impurity spectrum: a seeded Anderson impurity (d shell plus
+bath shells, sizes from 10^3 to a few 10^6 determinants), its ground state
+(two-pass Lanczos: coefficients extended in steps until the lowest Ritz pair
+has |H psi - E psi|^2 <= tol max(1, E^2), then the Ritz vector rebuilt from the
+same start), T|psi0> for a random one-body d<->bath transition T, a Lanczos
+tridiagonalization of the requested depth, and the continued fraction on the
+grid. Requests are admitted as slots and HBM budget free up; every admitted
+request is one slot of a `Batch`, so all of them advance together, one set of
+launches per Lanczos step. Bases, operators and vectors stay on the device
+from admission to completion. The plain baseline runs the same requests with
+one slot, one at a time.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import problems as P
+from .engine import Basis, Batch, Hop, Op, cf_spectrum, tridiag_lowest
+
+# (nbath, full_baths, nelec, d_range, max_holes, max_elec) -> sector size, cached
+_FAMILY = None
+
+
+_FAMILY_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "impurity_family.json")
+
+
+def impurity_family():
+    """Anderson-impurity shapes with their sector sizes, from about 10^2 to 10^8
+    (the table is cached in impurity_family.json next to this module)."""
+    global _FAMILY
+    if _FAMILY is not None:
+        return _FAMILY
+    if os.path.exists(_FAMILY_FILE):
+        data = json.load(open(_FAMILY_FILE))
+        _FAMILY = [(int(n), dict(kw, d_range=tuple(kw["d_range"]))) for n, kw in data]
+        return _FAMILY
+    out = []
+    for nbath in (1, 2, 3, 4):
+        for full in range(0, nbath + 1):
+            for nd in (6, 7, 8, 9):
+                nelec = 10 * full + nd
+                for mh in range(0, 4):
+                    for me in range(0, 4):
+                        if full == 0 and mh:
+                            continue
+                        if full == nbath and me:
+                            continue
+                        for dw in (1, 2):
+                            kw = dict(nbath=nbath, full_baths=full, nelec=nelec, d_range=(max(0, nd - dw), min(10, nd + dw)),
+                                      max_holes=mh, max_elec=me)
+                            n = Basis.count(10 * (1 + nbath), P.anderson_restrictions(**kw))
+                            if n >= 50:
+                                out.append((n, kw))
+    out.sort(key=lambda t: t[0])
+    _FAMILY = out
+    with open(_FAMILY_FILE, "w") as f:
+        json.dump(out, f)
+    return out
+
+
+def pick_shape(target_dim: float):
+    fam = impurity_family()
+    best = min(fam, key=lambda t: abs(math.log(t[0]) - math.log(target_dim)))
+    return best
+
+
+@dataclass(order=True)
+class Request:
+    arrival: float
+    rid: int = field(compare=False)
+    seed: int = field(compare=False)
+    shape: dict = field(compare=False, default_factory=dict)
+    dim: int = field(compare=False, default=0)
+    depth: int = field(compare=False, default=200)
+    gamma: float = field(compare=False, default=0.3)
+    # results
+    t_admit: float = field(compare=False, default=-1.0)
+    t_done: float = field(compare=False, default=-1.0)
+    E0: float = field(compare=False, default=0.0)
+    gs_residual: float = field(compare=False, default=0.0)
+    gs_steps: int = field(compare=False, default=0)
+    spectrum: np.ndarray | None = field(compare=False, default=None)
+    a: np.ndarray | None = field(compare=False, default=None)
+    b: np.ndarray | None = field(compare=False, default=None)
+
+
+def request_stream(n: int, rate: float, seed: int = 7, dim_lo=1e3, dim_hi=3e6, depth=(100, 400),
+                   gamma=(0.1, 0.5)) -> list[Request]:
+    """Poisson arrivals at `rate` req/s; dim log-uniform; depth U{lo..hi}; Gamma U(lo, hi)."""
+    rng = P.SplitMix64(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += -math.log(max(rng.uniform(), 1e-300)) / rate if rate > 0 else 0.0
+        target = math.exp(rng.uniform(math.log(dim_lo), math.log(dim_hi)))
+        dim, shape = pick_shape(target)
+        d = depth[0] + int(rng.uniform() * (depth[1] - depth[0] + 1))
+        g = rng.uniform(*gamma)
+        out.append(Request(arrival=t, rid=i, seed=int(rng.next() & 0x7FFFFFFF), shape=shape, dim=dim, depth=d, gamma=g))
+    return out
+
+
+class _Job:
+    """Device-resident state of an admitted request."""
+
+    def __init__(self, req: Request, ctx, gs_depth: int):
+        self.req = req
+        pr = P.anderson(seed=req.seed, **req.shape)
+        self.nf = pr["NF"]
+        self.basis = Basis.enumerate(self.nf, pr["restrictions"], ctx)
+        self.H = Op(self.nf, pr["H"], ctx)
+        self.hop = Hop(self.H, self.basis, mode="assembled")
+        rng = P.SplitMix64(req.seed ^ 0x5DEECE66D)
+        bath = [m for b in pr["baths"] for m in b]
+        self.T = Op(self.nf, P.random_one_body_transition(rng, pr["dmodes"], bath), ctx)
+        self.thop = Hop(self.T, self.basis, mode="matfree")
+        g = torch.Generator(device="cuda").manual_seed(req.seed)
+        dev = torch.device("cuda", ctx.device)
+        v = torch.randn(self.basis.n, dtype=torch.float64, device=dev, generator=g) + \
+            1j * torch.randn(self.basis.n, dtype=torch.float64, device=dev, generator=g)
+        self.v0 = v.to(torch.complex128).contiguous()
+        self.gs_depth = min(gs_depth, self.basis.n)
+        self.phase = "gs1"
+        self.slot = -1
+        self.bytes = self.hop.stats()["stream_bytes"] + 6 * 16 * self.basis.n
+
+    def release(self):
+        for k in ("hop", "thop", "H", "T", "basis", "v0"):
+            setattr(self, k, None)
+
+
+def estimate_bytes(dim: int) -> int:
+    return dim * (6 * 16 + 8 * 110)
+
+
+class SpectrumServer:
+    """Admits requests into a `Batch` by slot and HBM budget and advances all
+    admitted requests together."""
+
+    def __init__(self, ctx, max_slots: int = 64, hbm_budget: float = 200e9, gs_depth: int = 60,
+                 gs_max: int = 500, gs_step: int = 40, gs_tol: float = 1e-12, chunk: int = 8, ne: int = 2000,
+                 emin: float = -10.0, emax: float = 20.0):
+        self.ctx = ctx
+        self.max_slots = max_slots
+        self.budget = hbm_budget
+        self.gs_depth = gs_depth
+        self.gs_max = gs_max
+        self.gs_step = gs_step
+        self.gs_tol = gs_tol
+        self.chunk = chunk
+        self.ne, self.emin, self.emax = ne, emin, emax
+        self.batch = Batch(max_slots, ctx)
+        self.jobs: dict[int, _Job] = {}
+        self.used = 0
+
+    def _advance(self, job: _Job):
+        """Move a job whose slot finished to its next phase."""
+        r = job.req
+        if job.phase == "gs1":
+            st, a, b = self.batch.poll(job.slot)
+            m = st
+            E, z = tridiag_lowest(a[1:m + 1], b[1:m])
+            w = [E]
+            res = float(abs(b[m] * z[m - 1]))
+            if res ** 2 > self.gs_tol * max(1.0, E ** 2) and m < job.gs_max:
+                self.batch.set_depth(job.slot, min(job.gs_max, m + self.gs_step))
+                return False
+            self.batch.remove(job.slot)
+            r.E0 = float(w[0])
+            r.gs_residual = res
+            r.gs_steps = m
+            job.slot = self.batch.add(job.hop, job.v0, m, ycoef=z)
+            job.gs_depth = m
+            job.phase = "gs2"
+            return False
+        if job.phase == "gs2":
+            y = job.basis.zeros()
+            self.batch.poll(job.slot, want_coef=False, y=y)
+            self.batch.remove(job.slot)
+            y /= torch.linalg.vector_norm(y)
+            tpsi = job.thop.apply(y.contiguous())
+            job.slot = self.batch.add(job.hop, tpsi, r.depth)
+            job.phase = "spec"
+            return False
+        st, a, b = self.batch.poll(job.slot)
+        self.batch.remove(job.slot)
+        r.a, r.b = a, b[:r.depth]
+        r.spectrum = cf_spectrum(a, b[:r.depth], r.E0, self.emin, self.emax, self.ne, r.gamma, self.ctx)[0]
+        return True
+
+    def serve(self, requests: list[Request], realtime: bool = True, plain: bool = False) -> list[Request]:
+        """Run the stream. realtime: arrivals are honoured on the wall clock;
+        plain: one request at a time (the baseline)."""
+        pending = sorted(requests)
+        waiting: list[Request] = []
+        done: list[Request] = []
+        t0 = time.perf_counter()
+        i = 0
+        while i < len(pending) or waiting or self.jobs:
+            now = time.perf_counter() - t0
+            while i < len(pending) and (not realtime or pending[i].arrival <= now):
+                waiting.append(pending[i])
+                i += 1
+            # admission: FIFO, by free slots and HBM budget
+            while waiting and len(self.jobs) < (1 if plain else self.max_slots):
+                r = waiting[0]
+                if self.jobs and self.used + estimate_bytes(r.dim) > self.budget:
+                    break
+                waiting.pop(0)
+                r.t_admit = time.perf_counter() - t0
+                job = _Job(r, self.ctx, self.gs_depth)
+                job.gs_max = min(self.gs_max, job.basis.n)
+                job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max)
+                self.jobs[r.rid] = job
+                self.used += job.bytes
+            if not self.jobs:
+                if i < len(pending) and realtime:
+                    time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
+                continue
+            self.batch.run(self.chunk)
+            steps = self.batch.steps(self.max_slots)
+            for rid in list(self.jobs):
+                job = self.jobs[rid]
+                depth = job.gs_depth if job.phase in ("gs1", "gs2") else job.req.depth
+                if steps[job.slot] >= depth:
+                    if self._advance(job):
+                        job.req.t_done = time.perf_counter() - t0
+                        done.append(job.req)
+                        self.used -= job.bytes
+                        job.release()
+                        del self.jobs[rid]
+        return done
+
+
+def latency_stats(done: list[Request]) -> dict:
+    lat = np.array([r.t_done - r.arrival for r in done])
+    span = max(r.t_done for r in done) - min(r.arrival for r in done)
+    return {"n": len(done), "spectra_per_s": len(done) / span, "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()), "span_s": span}

@@ -1,0 +1,132 @@
+"""Batched Lanczos and the continuous-batching spectrum server on the GPU."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from quanty_amd import det as D
+from quanty_amd import problems as P
+from quanty_amd import serve
+from quanty_amd.engine import Basis, Batch, Hop, Op, heev_host
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [
+    dict(seed=3, nbath=1, nelec=12, full_baths=1, max_holes=3, max_elec=0, d_range=(0, 10)),
+    dict(seed=5, nbath=2, nelec=18, full_baths=1, max_holes=2, max_elec=2),
+    dict(seed=9, nbath=3, nelec=22, full_baths=1, max_holes=2, max_elec=3),
+    dict(seed=11, nbath=2, nelec=17, full_baths=1, max_holes=1, max_elec=1, d_range=(5, 9)),
+]
+
+
+def _setup(kw):
+    pr = P.anderson(**kw)
+    b = Basis.enumerate(pr["NF"], pr["restrictions"])
+    hop = Hop(Op(pr["NF"], pr["H"]), b, mode="assembled")
+    g = np.random.default_rng(kw["seed"])
+    v0 = torch.from_numpy(g.normal(size=b.n) + 1j * g.normal(size=b.n)).cuda()
+    return pr, b, hop, v0
+
+
+def test_batch_matches_single_and_is_mix_invariant():
+    probs = [_setup(kw) for kw in SHAPES]
+    depths = [40, 55, 30, 47]
+    mixed = Batch(8)
+    slots = [mixed.add(hop, v0, d) for (_, _, hop, v0), d in zip(probs, depths)]
+    for _ in range(8):
+        mixed.run(7)
+    for (pr, b, hop, v0), d, s in zip(probs, depths, slots):
+        st, a, bb = mixed.poll(s)
+        assert st == d
+        # alone: bit-identical coefficients
+        alone = Batch(1)
+        s1 = alone.add(hop, v0, d)
+        alone.run(d + 5)
+        st1, a1, b1 = alone.poll(s1)
+        assert np.array_equal(a, a1) and np.array_equal(bb, b1)
+        # against the single-problem Lanczos and the oracle (epsilon = 0)
+        ra, rb = hop.lanczos(v0, d)
+        assert np.allclose(a, ra, rtol=1e-10, atol=1e-12)
+        assert np.allclose(bb[:d], rb, rtol=1e-10, atol=0)
+        if b.n > 20000:
+            continue  # the oracle comparison is kept to the smaller problems
+        x = v0.cpu().numpy()
+        psi = oracle.Vec.from_arrays(pr["NF"], D.words_to_bytes(b.words(), pr["NF"]), x)
+        oa, ob = oracle.Op(pr["NF"], pr["H"]).lanczos(psi, d, restrictions=pr["restrictions"])
+        assert np.allclose(a, oa, rtol=1e-10, atol=1e-12) and np.allclose(bb[:d], ob, rtol=1e-10)
+
+
+def test_two_pass_ritz_vector():
+    """Coefficients, then the Ritz vector rebuilt from the same start: the
+    lowest eigenpair of the tridiagonal matrix becomes an eigenvector of H."""
+    pr, b, hop, v0 = _setup(SHAPES[0])
+    m = 300
+    bt = Batch(2)
+    s = bt.add(hop, v0, m)
+    bt.run(m)
+    _, a, bb = bt.poll(s)
+    T = np.diag(a[1:m + 1]).astype(complex)
+    for i in range(1, m):
+        T[i, i - 1] = T[i - 1, i] = bb[i]
+    w, Z = heev_host(T)
+    z = (Z[:, 0] * np.exp(-1j * np.angle(Z[np.argmax(np.abs(Z[:, 0])), 0]))).real
+    s2 = bt.add(hop, v0, m, ycoef=z)
+    bt.run(m)
+    y = b.zeros()
+    bt.poll(s2, want_coef=False, y=y)
+    y /= torch.linalg.vector_norm(y)
+    hy = hop.apply(y.contiguous())
+    res = torch.linalg.vector_norm(hy - w[0] * y).item()
+    assert res < 1e-6
+    E, _, _, _ = hop.ground_states(1, tol=1e-14)
+    assert abs(E[0] - w[0]) < 1e-9 * abs(E[0])
+
+
+def test_server_results_same_alone_and_in_the_mix():
+    """A stream of requests of different sizes: every spectrum produced in the
+    continuous batch equals the one produced by the plain one-at-a-time run, bit
+    for bit."""
+    from quanty_amd import default_context
+    reqs = serve.request_stream(10, rate=0.0, seed=3, dim_lo=1e2, dim_hi=4e4)
+    srv = serve.SpectrumServer(default_context(), max_slots=16, gs_depth=40, chunk=5)
+    mixed = {r.rid: r.spectrum for r in srv.serve(reqs, realtime=False)}
+    reqs2 = serve.request_stream(10, rate=0.0, seed=3, dim_lo=1e2, dim_hi=4e4)
+    plain = {r.rid: r.spectrum for r in serve.SpectrumServer(default_context(), max_slots=1, gs_depth=40,
+                                                             chunk=5).serve(reqs2, realtime=False, plain=True)}
+    assert len(mixed) == len(plain) == 10
+    for rid in mixed:
+        assert np.array_equal(mixed[rid], plain[rid]), rid
+        assert np.all(np.isfinite(mixed[rid]))
+
+
+def test_server_spectrum_matches_oracle():
+    """One served request re-done by the oracle from the served ground state."""
+    from quanty_amd import default_context
+    reqs = serve.request_stream(1, rate=0.0, seed=11, dim_lo=800, dim_hi=1500)
+    srv = serve.SpectrumServer(default_context(), max_slots=4, chunk=8)
+    keep = {}
+    orig = srv._advance
+
+    def spy(job):
+        if job.phase == "gs2":
+            keep["basis"] = job.basis.words()
+            y = job.basis.zeros()
+            srv.batch.poll(job.slot, want_coef=False, y=y)
+            keep["psi"] = (y / torch.linalg.vector_norm(y)).cpu().numpy()
+            keep["nf"] = job.nf
+            keep["req"] = job.req
+        return orig(job)
+
+    srv._advance = spy
+    (r,) = srv.serve(reqs, realtime=False)
+    pr = P.anderson(seed=r.seed, **r.shape)
+    nf = keep["nf"]
+    psi = oracle.Vec.from_arrays(nf, D.words_to_bytes(keep["basis"], nf), keep["psi"])
+    rng = P.SplitMix64(r.seed ^ 0x5DEECE66D)
+    bath = [m for b in pr["baths"] for m in b]
+    T = P.random_one_body_transition(rng, pr["dmodes"], bath)
+    tpsi = oracle.Op(nf, T).apply(psi, restrictions=pr["restrictions"])
+    a, b = oracle.Op(nf, pr["H"]).lanczos(tpsi, r.depth, restrictions=pr["restrictions"])
+    want = oracle.cf(a, b, r.E0, srv.emin, srv.emax, srv.ne, r.gamma)
+    assert np.max(np.abs(r.spectrum - want)) <= 1e-8 * np.max(np.abs(want))
+    assert r.gs_residual ** 2 <= 1e-12 * max(1.0, r.E0 ** 2)

@@ -55,3 +55,21 @@ def test_rank_table_counts_match_oracle():
     ]
     for nf, restr in cases:
         assert Basis.count(nf, restr) == len(oracle.enumerate_sector(nf, restr))
+
+
+@pytest.mark.parametrize("m", [1, 2, 7, 60, 400])
+def test_tridiagonal_lowest_and_extremes(m):
+    from quanty_amd.engine import tridiag_extremes, tridiag_lowest
+    rng = np.random.default_rng(m)
+    d = rng.normal(size=m) * 3
+    e = np.abs(rng.normal(size=m - 1)) + (1e-3 if m > 2 else 0.0)
+    if m > 5:
+        e[m // 2] = 0.0  # a split block
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    w, V = np.linalg.eigh(T)
+    E, z = tridiag_lowest(d, e)
+    assert abs(E - w[0]) < 1e-12 * max(1, abs(w[0]))
+    assert np.linalg.norm(T @ z - E * z) < 1e-9 * max(1, np.abs(w).max())
+    assert abs(np.linalg.norm(z) - 1) < 1e-12
+    lo, hi = tridiag_extremes(d, e)
+    assert abs(lo - w[0]) < 1e-12 * max(1, abs(w[0])) and abs(hi - w[-1]) < 1e-12 * max(1, abs(w[-1]))
