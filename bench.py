@@ -183,6 +183,7 @@ def main():
                    "gamma": 0.4, "psi0": "random normalized vector", "transition": "random one-body d<->bath",
                    "parallelism": "single GPU"},
         "hpsi": {"ms": hpsi_ms, "nnz_eff": st["nnz_eff"], "nnz_stored": st["nnz_stored"],
+                 "nnz_offdiag": st["nnz_offdiag"],
                  "stream_bytes": st["stream_bytes"], "bytes_model": bytes_model, "bytes_stream_model": bytes_stream,
                  "GBps_model": achieved, "GBps_stream": bytes_stream / (hpsi_ms * 1e-3) / 1e9},
         "setup_s": {"enumerate": t_enum, "assemble": t_asm},

@@ -124,9 +124,11 @@ enum { QS_HOP_MATFREE = 0, QS_HOP_ASSEMBLED = 1, QS_HOP_AUTO = 2 };
 qs_status qs_hop_create(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols,
                         int mode, size_t mem_budget_bytes, qs_hop **out);
 /* Number of (row, term) pairs with nonzero phase whose source lies in cols,
- * and the stored off-diagonal stream length (diagonal terms are pre-summed). */
+ * the stored off-diagonal stream length (incl. slice padding; diagonal terms
+ * are pre-summed), its bytes, and the off-diagonal entries without padding.
+ * Any pointer may be NULL. */
 qs_status qs_hop_stats(const qs_hop *h, int64_t *nnz_eff, int64_t *nnz_stored,
-                       int64_t *stream_bytes, int *assembled);
+                       int64_t *stream_bytes, int *assembled, int64_t *nnz_offdiag);
 /* y = O x (pull form: y_i = sum_t V_t <d_i| t |d_j> x_j). x over cols, y over
  * rows, device pointers. Equals OperatorPsiRestricted (Core/OperatorWaveFunction.cpp:2288)
  * with the row basis as restriction set, and OperatorPsiConserveBasis (:245) when

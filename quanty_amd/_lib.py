@@ -49,7 +49,8 @@ _SIGS = [
     ("qs_op_nterms", C.c_int, [_P]),
     ("qs_op_destroy", C.c_int, [_P]),
     ("qs_hop_create", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_size_t, C.POINTER(_P)]),
-    ("qs_hop_stats", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int)]),
+    ("qs_hop_stats", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
+                               C.POINTER(_I64)]),
     ("qs_hop_apply", C.c_int, [_P, _P, _P, _P]),
     ("qs_hop_apply_multi", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("qs_hop_destroy", C.c_int, [_P]),

@@ -332,15 +332,17 @@ k_count(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__res
   }
   // slice = wavefront: max row length and the wave's nnz_eff
   uint32_t m = off;
-  unsigned long long e = eff;
+  unsigned long long e = eff, od = off;
   for (int o = 32; o > 0; o >>= 1) {
     m = max(m, (uint32_t)__shfl_xor((int)m, o));
     e += (unsigned long long)__shfl_xor((long long)e, o);
+    od += (unsigned long long)__shfl_xor((long long)od, o);
   }
   const int64_t slice = row >> 6;
   if ((threadIdx.x & 63) == 0 && slice * 64 < nrows) {
     slice_len[slice] = m;
-    atomicAdd(nnz_eff, e);
+    atomicAdd(nnz_eff, e);      // counters: [0] (row, term) hits, [1] off-diagonal entries
+    atomicAdd(nnz_eff + 1, od);
   }
 }
 
@@ -378,22 +380,28 @@ k_fill(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__rest
 // ---------------------------------------------------------------------------
 // assembled apply: one wavefront per slice, lane per row
 // ---------------------------------------------------------------------------
-template <int KC>
+template <int KC, bool LDSC>
 __global__ void __launch_bounds__(QS_BLOCK)
 k_spmv(int64_t nrows, int64_t nslices, const uint64_t *__restrict__ slice_ptr,
        const uint32_t *__restrict__ slice_len, const uint16_t *__restrict__ rowcnt,
        const uint32_t *__restrict__ colv, const uint16_t *__restrict__ tidv,
-       const double2 *__restrict__ coef, const double2 *__restrict__ diag,
+       const double2 *__restrict__ coef, int nterms, const double2 *__restrict__ diag,
        const double2 *__restrict__ x, int64_t ldx, double2 *__restrict__ y, int64_t ldy) {
+  // the term values (+V of each term) are staged in LDS when they fit
+  extern __shared__ double2 s_coef[];
+  if (LDSC) {
+    for (int i = threadIdx.x; i < nterms; i += blockDim.x) s_coef[i] = coef[2 * i];
+    __syncthreads();
+  }
+  const double2 *cf = LDSC ? s_coef : coef;
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t s = wave; s < nslices; s += nwaves) {
+  int64_t s0, s1, ds;
+  xcd_items(nslices, s0, s1, ds);
+  for (int64_t s = s0; s < s1; s += ds) {
     const int64_t r = s * 64 + lane;
     const bool active = r < nrows;
     double ar[KC], ai[KC];
-    spmv_row<KC>(r, active, slice_ptr[s], slice_len[s], rowcnt, colv, tidv, coef, diag, x, ldx, lane,
-                 ar, ai);
+    spmv_row<KC, LDSC>(r, active, slice_ptr[s], slice_len[s], rowcnt, colv, tidv, cf, diag, x, ldx, lane, ar, ai);
     if (active) {
 #pragma unroll
       for (int v = 0; v < KC; ++v) y[r * ldy + v] = make_double2(ar[v], ai[v]);
@@ -473,7 +481,7 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   void *scr;
   QS_TRY(qs_scratch(ctx, 256, &scr));
   d_eff = (unsigned long long *)scr;
-  QS_HIP(ctx, hipMemsetAsync(d_eff, 0, 8, ctx->stream));
+  QS_HIP(ctx, hipMemsetAsync(d_eff, 0, 16, ctx->stream));
   Lookup L = make_lookup(h->cols);
   const unsigned blocks = (unsigned)((n + QS_BLOCK - 1) / QS_BLOCK);
   if (n > 0) {
@@ -488,16 +496,17 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
     QS_HIP(ctx, hipGetLastError());
   }
   std::vector<uint32_t> slen((size_t)nslices);
-  unsigned long long eff = 0;
+  unsigned long long eff[2] = {0, 0};
   if (nslices)
     QS_HIP(ctx, hipMemcpyAsync(slen.data(), h->d_slice_len, (size_t)nslices * 4, hipMemcpyDeviceToHost,
                                ctx->stream));
-  QS_HIP(ctx, hipMemcpyAsync(&eff, d_eff, 8, hipMemcpyDeviceToHost, ctx->stream));
+  QS_HIP(ctx, hipMemcpyAsync(eff, d_eff, 16, hipMemcpyDeviceToHost, ctx->stream));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<uint64_t> sptr((size_t)nslices + 1, 0);
   for (int64_t s = 0; s < nslices; ++s) sptr[s + 1] = sptr[s] + (uint64_t)slen[s] * 64;
   const uint64_t ent = sptr[nslices];
-  h->nnz_eff = (int64_t)eff;
+  h->nnz_eff = (int64_t)eff[0];
+  h->nnz_offdiag = (int64_t)eff[1];
   h->nnz_stored = (int64_t)ent;
   h->stream_bytes = (int64_t)(ent * 6 + (uint64_t)n * 2 + (uint64_t)(nslices + 1) * 12 + (h->same ? n * 16 : 0));
   if (budget && (size_t)h->stream_bytes > budget)
@@ -561,8 +570,9 @@ extern "C" qs_status qs_hop_create(qs_ctx *ctx, const qs_op *op, const qs_basis 
 }
 
 extern "C" qs_status qs_hop_stats(const qs_hop *h, int64_t *nnz_eff, int64_t *nnz_stored,
-                                  int64_t *stream_bytes, int *assembled) {
+                                  int64_t *stream_bytes, int *assembled, int64_t *nnz_offdiag) {
   if (!h) return QS_EINVAL;
+  if (nnz_offdiag) *nnz_offdiag = h->nnz_offdiag;
   if (nnz_eff) *nnz_eff = h->nnz_eff;
   if (nnz_stored) *nnz_stored = h->nnz_stored;
   if (stream_bytes) *stream_bytes = h->stream_bytes;
@@ -576,21 +586,35 @@ qs_status qs_hop_apply_strided(qs_ctx *ctx, const qs_hop *h, int k, const double
   const int64_t n = h->rows->n;
   if (n == 0 || k == 0) return QS_OK;
   const int64_t nslices = h->nslices;
-  int blocks = (int)std::min<int64_t>((nslices + 3) / 4, (int64_t)ctx->num_cu * 8);
-  blocks = std::max(blocks, 1);
+  const int nterms = h->op->nterms;
+  const bool lds = (size_t)nterms * 16 <= QS_COEF_LDS_BYTES;
+  const size_t shm = lds ? (size_t)nterms * 16 : 0;
   for (int v0 = 0; v0 < k;) {
     int kc = k - v0 >= 4 ? 4 : (k - v0 >= 2 ? 2 : 1);
     if (k - v0 == 3) kc = 3;
     const double2 *xs = (const double2 *)x + v0;
     double2 *ys = (double2 *)y + v0;
-#define QS_LAUNCH_SPMV(KC)                                                                     \
-  hipLaunchKernelGGL(k_spmv<KC>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, n, nslices,      \
-                     h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,           \
-                     (const double2 *)h->op->d_coef, (const double2 *)h->d_diag, xs, ldx, ys, ldy)
-    if (kc == 4) QS_LAUNCH_SPMV(4);
-    else if (kc == 3) QS_LAUNCH_SPMV(3);
-    else if (kc == 2) QS_LAUNCH_SPMV(2);
-    else QS_LAUNCH_SPMV(1);
+    // grid-stride over slices with no more workgroups than stay resident
+#define QS_LAUNCH_SPMV(KC)                                                                                 \
+  if (lds) {                                                                                                \
+    const int blocks = (int)std::max<int64_t>(                                                              \
+        1, std::min<int64_t>((nslices + 3) / 4,                                                             \
+                             qs_resident_blocks(ctx, (const void *)k_spmv<KC, true>, shm)));               \
+    hipLaunchKernelGGL((k_spmv<KC, true>), dim3(blocks), dim3(QS_BLOCK), shm, ctx->stream, n, nslices,     \
+                       h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,                    \
+                       (const double2 *)h->op->d_coef, nterms, (const double2 *)h->d_diag, xs, ldx, ys, ldy); \
+  } else {                                                                                                  \
+    const int blocks = (int)std::max<int64_t>(                                                              \
+        1, std::min<int64_t>((nslices + 3) / 4,                                                             \
+                             qs_resident_blocks(ctx, (const void *)k_spmv<KC, false>, 0)));                \
+    hipLaunchKernelGGL((k_spmv<KC, false>), dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, n, nslices,      \
+                       h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,                    \
+                       (const double2 *)h->op->d_coef, nterms, (const double2 *)h->d_diag, xs, ldx, ys, ldy); \
+  }
+    if (kc == 4) { QS_LAUNCH_SPMV(4); }
+    else if (kc == 3) { QS_LAUNCH_SPMV(3); }
+    else if (kc == 2) { QS_LAUNCH_SPMV(2); }
+    else { QS_LAUNCH_SPMV(1); }
 #undef QS_LAUNCH_SPMV
     QS_HIP(ctx, hipGetLastError());
     v0 += kc;

@@ -103,9 +103,9 @@ k_b_init_finish(SlotDev *slots, int slot, const double *__restrict__ part, int n
 __global__ void __launch_bounds__(QS_BLOCK)
 k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64_t nwork, double *__restrict__ part) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t w = wave; w < nwork; w += nwaves) {
+  int64_t w0, w1, dw;
+  xcd_items(nwork, w0, w1, dw);
+  for (int64_t w = w0; w < w1; w += dw) {
     const WorkItem it = work[w];
     const SlotDev &S = slots[it.slot];
     if (!S.active || S.step >= S.depth) continue;  // uniform across the wave

@@ -55,6 +55,23 @@ qs_status qs_scratch(qs_ctx *ctx, size_t bytes, void **p) {
   return QS_OK;
 }
 
+int64_t qs_resident_blocks(qs_ctx *ctx, const void *kernel, size_t shm) {
+  const auto key = std::make_pair(kernel, shm);
+  auto it = ctx->resident.find(key);
+  int per_cu;
+  if (it != ctx->resident.end()) {
+    per_cu = it->second;
+  } else {
+    per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, shm) != hipSuccess || per_cu < 1) {
+      (void)hipGetLastError();
+      per_cu = 1;
+    }
+    ctx->resident[key] = per_cu;
+  }
+  return (int64_t)per_cu * ctx->num_cu;
+}
+
 extern "C" qs_status qs_ctx_create(int device, qs_ctx **out) {
   if (!out) return QS_EINVAL;
   *out = nullptr;

@@ -5,7 +5,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/qs.h"
@@ -23,6 +25,8 @@ struct qs_ctx {
   size_t scratch_bytes = 0;
   int num_cu = 256;
   int async = 0;                   // skip the end-of-call stream sync for device-only results
+  // resident workgroups per CU, by (kernel, dynamic LDS bytes)
+  std::map<std::pair<const void *, size_t>, int> resident;
 };
 
 // Exact rank of a determinant among all determinants satisfying the
@@ -94,6 +98,7 @@ struct qs_hop {
   int assembled = 0;
   int64_t nnz_eff = 0;             // (row, term) pairs with a source in cols
   int64_t nnz_stored = 0;          // stored off-diagonal entries (incl. slice padding)
+  int64_t nnz_offdiag = 0;         // off-diagonal entries without padding
   int64_t stream_bytes = 0;
   // sliced-ELL (64 rows per slice, column-major inside a slice)
   int64_t nslices = 0;
@@ -123,6 +128,12 @@ qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *f
 
 qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes);
 qs_status qs_scratch(qs_ctx *ctx, size_t bytes, void **p);
+
+// Workgroups of QS_BLOCK (256) threads of `kernel` with `shm` bytes of dynamic
+// LDS that the whole device keeps resident at once. Grid-stride kernels are
+// launched with at most this many, so no workgroup waits for a slot while the
+// others already drain (which would leave its share of the work as a tail).
+int64_t qs_resident_blocks(qs_ctx *ctx, const void *kernel, size_t shm);
 
 // ---- internal entry points across translation units ----------------------
 qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int *lo,

@@ -122,24 +122,30 @@ __device__ __forceinline__ void lz_k1_prologue(int step, int nb2, LzState S, dou
 }
 
 // K1, assembled operator (fused SpMV + epilogue)
+template <bool LDSC>
 __global__ void __launch_bounds__(QS_BLOCK)
 k_lz1_asm(int step, int nb2, LzState S, int64_t nrows, int64_t nslices, const uint64_t *__restrict__ slice_ptr,
           const uint32_t *__restrict__ slice_len, const uint16_t *__restrict__ rowcnt,
           const uint32_t *__restrict__ colv, const uint16_t *__restrict__ tidv,
-          const double2 *__restrict__ coef, const double2 *__restrict__ diag, const double2 *__restrict__ uc,
-          double2 *__restrict__ up) {
+          const double2 *__restrict__ coef, int nterms, const double2 *__restrict__ diag,
+          const double2 *__restrict__ uc, double2 *__restrict__ up) {
   __shared__ double red[QS_BLOCK];
+  extern __shared__ double2 s_coef[];
+  if (LDSC) {
+    for (int i = threadIdx.x; i < nterms; i += blockDim.x) s_coef[i] = coef[2 * i];
+  }
   double sc, cp;
-  lz_k1_prologue(step, nb2, S, red, sc, cp);
+  lz_k1_prologue(step, nb2, S, red, sc, cp);  // ends with a barrier
+  const double2 *cf = LDSC ? s_coef : coef;
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  int64_t s0, s1, ds;
+  xcd_items(nslices, s0, s1, ds);
   double dot = 0.0;
-  for (int64_t s = wave; s < nslices; s += nwaves) {
+  for (int64_t s = s0; s < s1; s += ds) {
     const int64_t r = s * 64 + lane;
     const bool active = r < nrows;
     double hr[1], hi[1];
-    spmv_row<1>(r, active, slice_ptr[s], slice_len[s], rowcnt, colv, tidv, coef, diag, uc, 1, lane, hr, hi);
+    spmv_row<1, LDSC>(r, active, slice_ptr[s], slice_len[s], rowcnt, colv, tidv, cf, diag, uc, 1, lane, hr, hi);
     if (active) {
       const double2 u = uc[r];
       dot += hr[0] * u.x + hi[0] * u.y;
@@ -199,8 +205,14 @@ extern "C" qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, 
   if (n == 0) return qs_fail(ctx, QS_EINVAL, "empty basis");
   const int nb2 = red_blocks(ctx, n);
   int nb1 = nb2;
-  if (h->assembled)
-    nb1 = (int)std::max<int64_t>(1, std::min<int64_t>((h->nslices + 3) / 4, (int64_t)ctx->num_cu * 8));
+  const int nterms = h->op->nterms;
+  const bool lds = (size_t)nterms * 16 <= QS_COEF_LDS_BYTES;
+  const size_t shm = lds ? (size_t)nterms * 16 : 0;
+  if (h->assembled) {
+    // grid-stride over slices with no more workgroups than stay resident
+    const void *kern = lds ? (const void *)k_lz1_asm<true> : (const void *)k_lz1_asm<false>;
+    nb1 = (int)std::max<int64_t>(1, std::min<int64_t>((h->nslices + 3) / 4, qs_resident_blocks(ctx, kern, shm)));
+  }
   const size_t vbytes = (size_t)n * 16;
   const size_t sbytes = ((size_t)(nsteps + 1) + 2 * (size_t)nsteps + nb1 + nb2 + 64) * 8;
   char *mem = nullptr;
@@ -226,9 +238,16 @@ extern "C" qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, 
   double2 *uc = u0, *up = u1;
   for (int i = 0; i < nsteps && e == hipSuccess && st == QS_OK; ++i) {
     if (h->assembled) {
-      hipLaunchKernelGGL(k_lz1_asm, dim3(nb1), dim3(QS_BLOCK), 0, ctx->stream, i, nb2, S, n, h->nslices,
-                         h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,
-                         (const double2 *)h->op->d_coef, (const double2 *)h->d_diag, (const double2 *)uc, up);
+      if (lds)
+        hipLaunchKernelGGL(k_lz1_asm<true>, dim3(nb1), dim3(QS_BLOCK), shm, ctx->stream, i, nb2, S, n, h->nslices,
+                           h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,
+                           (const double2 *)h->op->d_coef, nterms, (const double2 *)h->d_diag, (const double2 *)uc,
+                           up);
+      else
+        hipLaunchKernelGGL(k_lz1_asm<false>, dim3(nb1), dim3(QS_BLOCK), 0, ctx->stream, i, nb2, S, n, h->nslices,
+                           h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,
+                           (const double2 *)h->op->d_coef, nterms, (const double2 *)h->d_diag, (const double2 *)uc,
+                           up);
     } else {
       // matrix-free: h = H u_c into hb, then the epilogue
       st = qs_launch_mf_ext(ctx, h, (const double *)uc, (double *)hb);

@@ -148,9 +148,11 @@ class Hop:
         self.h = h
 
     def stats(self) -> dict:
-        a, b, c, d = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
-        self.ctx.check(self.lib.qs_hop_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)), "qs_hop_stats")
-        return {"nnz_eff": a.value, "nnz_stored": b.value, "stream_bytes": c.value, "assembled": bool(d.value)}
+        a, b, c, d, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int(), C.c_int64()
+        self.ctx.check(self.lib.qs_hop_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d), C.byref(e)),
+                       "qs_hop_stats")
+        return {"nnz_eff": a.value, "nnz_stored": b.value, "stream_bytes": c.value, "assembled": bool(d.value),
+                "nnz_offdiag": e.value}
 
     def apply(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
         assert x.dtype == torch.complex128 and x.is_contiguous() and x.numel() == self.cols.n

@@ -144,6 +144,51 @@ def test_lanczos_matches_oracle(kw, mode):
     assert np.allclose(bb, rb, rtol=1e-10, atol=0)
 
 
+def _many_term_operator(nf, nterms, seed):
+    """Hermitian two-body operator with more distinct terms than the LDS
+    staging of the term values takes, so the assembled kernels read them from
+    global memory."""
+    rng = np.random.default_rng(seed)
+    terms = []
+    seen = set()
+    while len(terms) < nterms:
+        i, j, k, l = (int(q) for q in rng.integers(0, nf, 4))
+        if i == j or k == l or (i, j, k, l) in seen:
+            continue
+        seen.add((i, j, k, l))
+        terms.append(O.written([O.cr(i), O.cr(j), O.an(k), O.an(l)], complex(*rng.normal(size=2))))
+    return O.add(terms, O.dagger(terms))
+
+
+def test_many_terms_global_coef_path():
+    """H*psi, Lanczos and the interleaved block product when the term table is
+    too large for LDS (2000+ terms on a 16-mode, 8-electron basis)."""
+    nf = 16
+    terms = _many_term_operator(nf, 1100, 21)
+    restr = [("1" * nf, 8, 8)]
+    b = Basis.enumerate(nf, restr)
+    op = Op(nf, terms)
+    assert op.nterms > 1024
+    hop = Hop(op, b, mode="assembled")
+    x = cvec(b.n, 3)
+    y = hop.apply(torch.from_numpy(x).cuda()).cpu().numpy()
+    want = oracle_apply_dense(nf, terms, b.words(), x, restr, b)
+    assert np.max(np.abs(y - want)) <= 1e-13 * np.max(np.abs(want))
+    a, bb = hop.lanczos(torch.from_numpy(x).cuda(), 30)
+    psi = oracle.Vec.from_arrays(nf, D.words_to_bytes(b.words(), nf), x)
+    ra, rb = oracle.Op(nf, terms).lanczos(psi, 30, restrictions=restr)
+    assert np.allclose(a, ra, rtol=1e-10, atol=1e-12)
+    assert np.allclose(bb, rb, rtol=1e-10, atol=0)
+    # 3 orthonormal start vectors (the first Krylov block is then the start
+    # block itself): A_1 = V^+ H V through the 3-vector kernel
+    q, _ = np.linalg.qr(np.stack([cvec(b.n, 30 + v) for v in range(3)], axis=1))
+    A, _, kept = hop.block_lanczos(torch.from_numpy(np.ascontiguousarray(q.T)).cuda(), 1)
+    assert kept[1] == 3
+    hq = np.stack([oracle_apply_dense(nf, terms, b.words(), q[:, v], restr, b) for v in range(3)], axis=1)
+    A1 = q.conj().T @ hq
+    assert np.max(np.abs(A[1] - A1)) <= 1e-11 * np.max(np.abs(A1))
+
+
 @pytest.mark.parametrize("name", ["G0Dn1", "G0Dn2", "G0Up1", "G0Up2", "G1Dn1", "G1Dn2", "G1Up1", "G1Up2",
                                   "G2Dn1", "G2Dn2", "G2Up1", "G2Up2"])
 def test_lanczos_golden_tri_tables_gpu(name):
