@@ -64,6 +64,17 @@ void *qs_ctx_stream(qs_ctx *ctx);
  * of waiting for them; the caller orders later work on qs_ctx_stream() or
  * calls qs_sync(). Default 0. */
 qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
+/* Tuning parameters of a ctx (no counterpart in the reference, whose
+ * operator application has no assembled form).
+ * QS_PARAM_SHIFT_MIN (default 56, range 1..65): in operators assembled after
+ * the call, a term's hits inside a 64-row slice whose columns are a common
+ * shift of the row positions are stored as one block header when at least
+ * this many rows share the shift; 65 stores every entry individually.
+ * QS_PARAM_ELL_PACK (default 1): 1 stores an individual entry as one uint32
+ * (column and term id) when both fit, 0 always as uint32 column + uint16 term. */
+enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2 };
+qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value);
+qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
 
 /* ---- basis (restricted-occupation determinant sets) --------------------- */
 /* Enumerate every determinant of nf modes whose occupation inside each mask r
@@ -118,9 +129,11 @@ qs_status qs_op_destroy(qs_op *op);
 /* ---- operator on a basis: apply ---------------------------------------- */
 enum { QS_HOP_MATFREE = 0, QS_HOP_ASSEMBLED = 1, QS_HOP_AUTO = 2 };
 /* Bind op to rows = `rows` basis and columns = `cols` basis. ASSEMBLED builds
- * the sliced connection stream (column index + term id per nonzero) once and
- * reuses it on every apply; MATFREE recomputes targets on the fly; AUTO
- * assembles when the stream fits in `mem_budget_bytes` (0 = 50% of free HBM). */
+ * the connection stream once and reuses it on every apply: per 64-row slice,
+ * shift blocks (a term whose rows read consecutive columns, one header each)
+ * and the remaining entries (column index + term id each); MATFREE recomputes
+ * targets on the fly; AUTO assembles when the stream fits in
+ * `mem_budget_bytes` (0 = 50% of free HBM). */
 qs_status qs_hop_create(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols,
                         int mode, size_t mem_budget_bytes, qs_hop **out);
 /* Number of (row, term) pairs with nonzero phase whose source lies in cols,
@@ -129,6 +142,11 @@ qs_status qs_hop_create(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, cons
  * Any pointer may be NULL. */
 qs_status qs_hop_stats(const qs_hop *h, int64_t *nnz_eff, int64_t *nnz_stored,
                        int64_t *stream_bytes, int *assembled, int64_t *nnz_offdiag);
+/* Layout of an assembled binding: shift blocks, the entries they hold, the
+ * remaining per-entry slots (incl. padding), whether an entry is one packed
+ * uint32, and the shift threshold used. Any pointer may be NULL. */
+qs_status qs_hop_layout(const qs_hop *h, int64_t *nblocks, int64_t *block_hits, int64_t *ell_slots,
+                        int *packed, int *shift_min);
 /* y = O x (pull form: y_i = sum_t V_t <d_i| t |d_j> x_j). x over cols, y over
  * rows, device pointers. Equals OperatorPsiRestricted (Core/OperatorWaveFunction.cpp:2288)
  * with the row basis as restriction set, and OperatorPsiConserveBasis (:245) when

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libqs.so")
 
 QS_OK = 0
 QS_HOP_MATFREE, QS_HOP_ASSEMBLED, QS_HOP_AUTO = 0, 1, 2
+QS_PARAM_SHIFT_MIN, QS_PARAM_ELL_PACK = 1, 2
 STATUS = {0: "QS_OK", 1: "QS_ENOMEM_DEVICE", 2: "QS_ENOMEM_HOST", 3: "QS_EINVAL",
           4: "QS_EHIP", 5: "QS_ENOTCONVERGED", 6: "QS_EUNSUPPORTED"}
 
@@ -34,6 +35,8 @@ _SIGS = [
     ("qs_sync", C.c_int, [_P]),
     ("qs_ctx_stream", _P, [_P]),
     ("qs_ctx_set_async", C.c_int, [_P, C.c_int]),
+    ("qs_ctx_set_param", C.c_int, [_P, C.c_int, _I64]),
+    ("qs_ctx_get_param", C.c_int, [_P, C.c_int, C.POINTER(_I64)]),
     ("qs_basis_enumerate", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, C.POINTER(_P)]),
     ("qs_basis_count", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, C.POINTER(_I64)]),
     ("qs_basis_from_words", C.c_int, [_P, C.c_int, _P, _I64, C.POINTER(_P)]),
@@ -51,6 +54,8 @@ _SIGS = [
     ("qs_hop_create", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_size_t, C.POINTER(_P)]),
     ("qs_hop_stats", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
                                C.POINTER(_I64)]),
+    ("qs_hop_layout", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
+                                C.POINTER(C.c_int)]),
     ("qs_hop_apply", C.c_int, [_P, _P, _P, _P]),
     ("qs_hop_apply_multi", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("qs_hop_destroy", C.c_int, [_P]),
@@ -126,6 +131,14 @@ class Context:
         if st != QS_OK:
             msg = self.lib.qs_last_error(self.h)
             raise QSError(f"{what}: {STATUS.get(st, st)}: {msg.decode() if msg else ''}")
+
+    def set_param(self, param: int, value: int):
+        self.check(self.lib.qs_ctx_set_param(self.h, param, int(value)), "qs_ctx_set_param")
+
+    def get_param(self, param: int) -> int:
+        v = C.c_int64()
+        self.check(self.lib.qs_ctx_get_param(self.h, param, C.byref(v)), "qs_ctx_get_param")
+        return v.value
 
     def close(self):
         if getattr(self, "h", None):

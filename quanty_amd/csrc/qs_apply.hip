@@ -131,7 +131,8 @@ extern "C" qs_status qs_op_create(qs_ctx *ctx, int nf, int nterms, const int *le
   op->h_terms = kept;
   const size_t tsz = nw == 1 ? sizeof(TermD<1>) : sizeof(TermD<2>);
   std::vector<char> buf(std::max<size_t>(1, kept.size()) * tsz, 0);
-  std::vector<double> coef(std::max<size_t>(1, kept.size()) * 4, 0.0);
+  std::vector<double> val(std::max<size_t>(1, kept.size()) * 2, 0.0);
+  std::vector<uint64_t> par(std::max<size_t>(1, kept.size()) * nw, 0);
   for (size_t i = 0; i < kept.size(); ++i) {
     const TermHost &h = kept[i];
     const bool diag = (h.F[0] | h.F[1]) == 0;
@@ -146,20 +147,22 @@ extern "C" qs_status qs_op_create(qs_ctx *ctx, int nf, int nterms, const int *le
       d.re = h.re; d.im = h.im; d.sgn = h.sgn; d.diag = diag;
       memcpy(buf.data() + i * tsz, &d, sizeof d);
     }
-    coef[4 * i + 0] = h.re;
-    coef[4 * i + 1] = h.im;
-    coef[4 * i + 2] = -h.re;
-    coef[4 * i + 3] = -h.im;
+    val[2 * i + 0] = h.re;  // sign constant already folded in (sgn == 0)
+    val[2 * i + 1] = h.im;
+    for (int w = 0; w < nw; ++w) par[(size_t)i * nw + w] = h.P[w];
   }
   qs_status st = qs_malloc(ctx, &op->d_terms, buf.size());
-  if (st == QS_OK) st = qs_malloc(ctx, (void **)&op->d_coef, coef.size() * 8);
+  if (st == QS_OK) st = qs_malloc(ctx, (void **)&op->d_val, val.size() * 8);
+  if (st == QS_OK) st = qs_malloc(ctx, (void **)&op->d_par, par.size() * 8);
   if (st != QS_OK) {
     qs_op_destroy(op);
     return st;
   }
   hipError_t e = hipMemcpyAsync(op->d_terms, buf.data(), buf.size(), hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(op->d_coef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, ctx->stream);
+    e = hipMemcpyAsync(op->d_val, val.data(), val.size() * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(op->d_par, par.data(), par.size() * 8, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) {
     qs_op_destroy(op);
@@ -174,7 +177,8 @@ extern "C" int qs_op_nterms(const qs_op *op) { return op ? op->nterms : -1; }
 extern "C" qs_status qs_op_destroy(qs_op *op) {
   if (!op) return QS_OK;
   if (op->d_terms) (void)hipFree(op->d_terms);
-  if (op->d_coef) (void)hipFree(op->d_coef);
+  if (op->d_val) (void)hipFree(op->d_val);
+  if (op->d_par) (void)hipFree(op->d_par);
   delete op;
   return QS_OK;
 }
@@ -293,118 +297,159 @@ k_apply_mf(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__
 }
 
 // ---------------------------------------------------------------------------
-// assembly: count pass and fill pass
+// assembly: one wavefront per 64-row slice, count pass then fill pass
 // ---------------------------------------------------------------------------
-template <int NW>
+// For every term the wave finds the hit lanes and their columns, then groups
+// the hit lanes by shift (column - lane): a group of at least `hmin` lanes
+// becomes one shift block, the lanes of smaller groups get ELL entries. Both
+// passes visit terms and groups in the same order, so the fill pass writes
+// exactly the layout the count pass sized. Count outputs: ELL entries per row,
+// ELL positions per slice, blocks per slice, the pre-summed diagonal and the
+// counters [hits, off-diagonal hits, block hits].
+template <int NW, bool FILL>
 __global__ void __launch_bounds__(QS_BLOCK)
-k_count(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms, int nterms,
-        Lookup L, int same, uint16_t *__restrict__ rowcnt, uint32_t *__restrict__ slice_len,
-        double2 *__restrict__ diag, unsigned long long *__restrict__ nnz_eff) {
+k_sell_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms, int nterms,
+             Lookup L, int same, int hmin, uint16_t *__restrict__ rowcnt, uint32_t *__restrict__ slice_len,
+             uint32_t *__restrict__ blk_cnt, double2 *__restrict__ diag, unsigned long long *__restrict__ ctr,
+             const uint64_t *__restrict__ blk_ptr, uint4 *__restrict__ blk, const uint64_t *__restrict__ slice_ptr,
+             uint32_t *__restrict__ colv, uint16_t *__restrict__ tidv, int packed, uint32_t colbits) {
   __shared__ TermD<NW> sh[QS_TILE];
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = row < nrows;
+  const int lane = threadIdx.x & 63;
+  const int64_t slice = row >> 6;
+  const bool live = slice * 64 < nrows;  // wave-uniform
   uint64_t d[NW];
 #pragma unroll
   for (int w = 0; w < NW; ++w) d[w] = active ? rows[row * NW + w] : 0;
-  uint32_t off = 0, eff = 0;
+  uint64_t bbase = 0, ebase = 0;
+  if (FILL && live) {
+    bbase = blk_ptr[slice];
+    ebase = slice_ptr[slice];
+  }
+  uint32_t nell = 0, nblk = 0;
+  unsigned long long eff = 0, bhits = 0;
   double dr = 0.0, di = 0.0;
   for (int base = 0; base < nterms; base += QS_TILE) {
     const int cnt = min(QS_TILE, nterms - base);
     __syncthreads();
     stage_terms<NW>(sh, terms, base, cnt);
     __syncthreads();
-    if (active) {
-      visit_tile<NW>(d, row, sh, cnt, base, same != 0, L, [&](int tj, int64_t, uint32_t par, bool dg) {
-        ++eff;
-        if (dg) {
-          const TermD<NW> &t = sh[tj - base];
-          dr += par ? -t.re : t.re;
-          di += par ? -t.im : t.im;
-        } else {
-          ++off;
+    if (!live) continue;
+    for (int j = 0; j < cnt; ++j) {
+      const TermD<NW> &t = sh[j];
+      bool hit = active;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) hit = hit && ((d[w] & t.C[w]) == t.V[w]);
+      if (__ballot(hit) == 0) continue;
+      if (t.diag && same) {
+        if (hit) {
+          uint32_t p = 0;
+#pragma unroll
+          for (int w = 0; w < NW; ++w) p += __popcll(d[w] & t.P[w]);
+          ++eff;
+          dr += (p & 1u) ? -t.re : t.re;
+          di += (p & 1u) ? -t.im : t.im;
         }
-      });
+        continue;
+      }
+      int64_t col = -1;
+      if (hit) {
+        uint64_t sdet[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sdet[w] = d[w] ^ t.F[w];
+        col = col_lookup<NW>(sdet, L);
+      }
+      hit = col >= 0;
+      eff += hit ? 1 : 0;
+      const int off = hit ? (int)(col - lane) : 0;
+      uint64_t un = __ballot(hit);
+      const uint32_t tj = (uint32_t)(base + j);
+      while (un) {
+        const int leader = __ffsll((unsigned long long)un) - 1;
+        const int cand = __shfl(off, leader);
+        const uint64_t grp = __ballot(hit && off == cand);
+        const int ng = __popcll(grp);
+        if (ng >= hmin) {
+          if (FILL && lane == 0)
+            blk[bbase + nblk] = make_uint4((uint32_t)cand, tj, (uint32_t)grp, (uint32_t)(grp >> 32));
+          ++nblk;
+          bhits += (unsigned long long)ng;
+        } else if ((grp >> lane) & 1u) {
+          if (FILL) {
+            const uint64_t e = ebase + (uint64_t)nell * 64 + lane;
+            if (packed) {
+              colv[e] = (uint32_t)col | (tj << colbits);
+            } else {
+              colv[e] = (uint32_t)col;
+              tidv[e] = (uint16_t)tj;
+            }
+          }
+          ++nell;
+        }
+        un &= ~grp;
+      }
     }
   }
+  if (!live) return;
+  if (FILL) {
+    // padding slots of the lane's ELL column: column 0, term 0
+    const uint32_t slen = slice_len[slice];
+    for (uint32_t j = nell; j < slen; ++j) {
+      const uint64_t e = ebase + (uint64_t)j * 64 + lane;
+      colv[e] = 0;
+      if (!packed) tidv[e] = 0;
+    }
+    return;
+  }
   if (active) {
-    rowcnt[row] = (uint16_t)off;
+    rowcnt[row] = (uint16_t)nell;
     if (diag) diag[row] = make_double2(dr, di);
   }
-  // slice = wavefront: max row length and the wave's nnz_eff
-  uint32_t m = off;
-  unsigned long long e = eff, od = off;
+  uint32_t m = nell;
+  unsigned long long e = eff;
   for (int o = 32; o > 0; o >>= 1) {
     m = max(m, (uint32_t)__shfl_xor((int)m, o));
     e += (unsigned long long)__shfl_xor((long long)e, o);
-    od += (unsigned long long)__shfl_xor((long long)od, o);
   }
-  const int64_t slice = row >> 6;
-  if ((threadIdx.x & 63) == 0 && slice * 64 < nrows) {
+  if (lane == 0) {
     slice_len[slice] = m;
-    atomicAdd(nnz_eff, e);      // counters: [0] (row, term) hits, [1] off-diagonal entries
-    atomicAdd(nnz_eff + 1, od);
+    blk_cnt[slice] = nblk;
+    atomicAdd(ctr, e);
+    atomicAdd(ctr + 2, bhits);
   }
-}
-
-template <int NW>
-__global__ void __launch_bounds__(QS_BLOCK)
-k_fill(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms, int nterms,
-       Lookup L, int same, const uint64_t *__restrict__ slice_ptr, uint32_t *__restrict__ colv,
-       uint16_t *__restrict__ tidv) {
-  __shared__ TermD<NW> sh[QS_TILE];
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = row < nrows;
-  uint64_t d[NW];
-#pragma unroll
-  for (int w = 0; w < NW; ++w) d[w] = active ? rows[row * NW + w] : 0;
-  const int lane = threadIdx.x & 63;
-  const uint64_t base_e = active ? slice_ptr[row >> 6] : 0;
-  uint32_t j = 0;
-  for (int base = 0; base < nterms; base += QS_TILE) {
-    const int cnt = min(QS_TILE, nterms - base);
-    __syncthreads();
-    stage_terms<NW>(sh, terms, base, cnt);
-    __syncthreads();
-    if (active) {
-      visit_tile<NW>(d, row, sh, cnt, base, same != 0, L, [&](int tj, int64_t col, uint32_t par, bool dg) {
-        if (dg) return;
-        const uint64_t e = base_e + (uint64_t)j * 64 + lane;
-        colv[e] = (uint32_t)col;
-        tidv[e] = (uint16_t)(2 * tj + par);
-        ++j;
-      });
-    }
-  }
+  // off-diagonal hits: ELL entries plus block hits
+  unsigned long long ne = nell;
+  for (int o = 32; o > 0; o >>= 1) ne += (unsigned long long)__shfl_xor((long long)ne, o);
+  if (lane == 0) atomicAdd(ctr + 1, ne + bhits);
 }
 
 // ---------------------------------------------------------------------------
 // assembled apply: one wavefront per slice, lane per row
 // ---------------------------------------------------------------------------
-template <int KC, bool LDSC>
+template <int KC, int NW, bool PACKED, bool LDS>
 __global__ void __launch_bounds__(QS_BLOCK)
-k_spmv(int64_t nrows, int64_t nslices, const uint64_t *__restrict__ slice_ptr,
-       const uint32_t *__restrict__ slice_len, const uint16_t *__restrict__ rowcnt,
-       const uint32_t *__restrict__ colv, const uint16_t *__restrict__ tidv,
-       const double2 *__restrict__ coef, int nterms, const double2 *__restrict__ diag,
-       const double2 *__restrict__ x, int64_t ldx, double2 *__restrict__ y, int64_t ldy) {
-  // the term values (+V of each term) are staged in LDS when they fit
-  extern __shared__ double2 s_coef[];
-  if (LDSC) {
-    for (int i = threadIdx.x; i < nterms; i += blockDim.x) s_coef[i] = coef[2 * i];
+k_spmv(SellD A, const double2 *__restrict__ x, int64_t ldx, double2 *__restrict__ y, int64_t ldy) {
+  extern __shared__ char smem[];
+  const double2 *tv = A.val;
+  const uint64_t *tp = A.par;
+  if (LDS) {
+    stage_term_tables<NW>(A, smem);
     __syncthreads();
+    tv = reinterpret_cast<const double2 *>(smem);
+    tp = reinterpret_cast<const uint64_t *>(smem + (size_t)A.nterms * 16);
   }
-  const double2 *cf = LDSC ? s_coef : coef;
   const int lane = threadIdx.x & 63;
   int64_t s0, s1, ds;
-  xcd_items(nslices, s0, s1, ds);
+  xcd_items(A.nslices, s0, s1, ds);
+  s0 = wave_uniform(s0);
   for (int64_t s = s0; s < s1; s += ds) {
-    const int64_t r = s * 64 + lane;
-    const bool active = r < nrows;
     double ar[KC], ai[KC];
-    spmv_row<KC, LDSC>(r, active, slice_ptr[s], slice_len[s], rowcnt, colv, tidv, cf, diag, x, ldx, lane, ar, ai);
-    if (active) {
+    sell_row<KC, NW, PACKED>(A, s, lane, tv, tp, x, ldx, ar, ai);
+    const int64_t r = s * 64 + lane;
+    if (r < A.nrows) {
 #pragma unroll
-      for (int v = 0; v < KC; ++v) y[r * ldy + v] = make_double2(ar[v], ai[v]);
+      for (int k = 0; k < KC; ++k) y[r * ldy + k] = make_double2(ar[k], ai[k]);
     }
   }
 }
@@ -452,18 +497,34 @@ qs_status qs_launch_mf_ext(qs_ctx *ctx, const qs_hop *h, const double *x, double
 }
 
 static void free_stream(qs_hop *h) {
-  if (h->d_slice_ptr) (void)hipFree(h->d_slice_ptr);
-  if (h->d_slice_len) (void)hipFree(h->d_slice_len);
-  if (h->d_rowcnt) (void)hipFree(h->d_rowcnt);
-  if (h->d_col) (void)hipFree(h->d_col);
-  if (h->d_tid) (void)hipFree(h->d_tid);
-  if (h->d_diag) (void)hipFree(h->d_diag);
+  void *bufs[] = {h->d_blk_ptr, h->d_blk, h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid, h->d_diag};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  h->d_blk_ptr = nullptr;
+  h->d_blk = nullptr;
   h->d_slice_ptr = nullptr;
   h->d_slice_len = nullptr;
   h->d_rowcnt = nullptr;
   h->d_col = nullptr;
   h->d_tid = nullptr;
   h->d_diag = nullptr;
+  h->view = SellD();
+}
+
+static uint32_t bits_for(uint64_t maxval) {
+  uint32_t b = 1;
+  while (b < 64 && (maxval >> b) != 0) ++b;
+  return b;
+}
+
+template <int NW, bool FILL>
+static void launch_build(qs_ctx *ctx, qs_hop *h, unsigned blocks, const Lookup &L, unsigned long long *ctr,
+                         uint32_t *blk_cnt) {
+  const qs_op *op = h->op;
+  hipLaunchKernelGGL((k_sell_build<NW, FILL>), dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, h->rows->d_words,
+                     h->rows->n, (const TermD<NW> *)op->d_terms, op->nterms, L, (int)h->same, h->shift_min,
+                     h->d_rowcnt, h->d_slice_len, blk_cnt, (double2 *)h->d_diag, ctr, h->d_blk_ptr, h->d_blk,
+                     h->d_slice_ptr, h->d_col, h->d_tid, h->view.packed, h->view.colbits);
 }
 
 static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
@@ -472,62 +533,86 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   const int64_t n = rows->n;
   const int64_t nslices = (n + 63) / 64;
   h->nslices = nslices;
-  if (op->nterms > 32767) return qs_fail(ctx, QS_EUNSUPPORTED, "more than 32767 terms: not assembled");
+  h->shift_min = ctx->shift_min;
+  if (op->nterms > 65535) return qs_fail(ctx, QS_EUNSUPPORTED, "more than 65535 terms: not assembled");
+  if (h->cols->n >= ((int64_t)1 << 31)) return qs_fail(ctx, QS_EUNSUPPORTED, "2^31 or more columns: not assembled");
   QS_TRY(qs_malloc(ctx, (void **)&h->d_slice_ptr, (size_t)(nslices + 1) * 8));
+  QS_TRY(qs_malloc(ctx, (void **)&h->d_blk_ptr, (size_t)(nslices + 1) * 8));
   QS_TRY(qs_malloc(ctx, (void **)&h->d_slice_len, (size_t)std::max<int64_t>(nslices, 1) * 4));
   QS_TRY(qs_malloc(ctx, (void **)&h->d_rowcnt, (size_t)std::max<int64_t>(n, 1) * 2));
   if (h->same) QS_TRY(qs_malloc(ctx, (void **)&h->d_diag, (size_t)std::max<int64_t>(n, 1) * 16));
-  unsigned long long *d_eff;
+  // scratch: 3 counters, then the per-slice block counts
   void *scr;
-  QS_TRY(qs_scratch(ctx, 256, &scr));
-  d_eff = (unsigned long long *)scr;
-  QS_HIP(ctx, hipMemsetAsync(d_eff, 0, 16, ctx->stream));
+  QS_TRY(qs_scratch(ctx, 64 + (size_t)std::max<int64_t>(nslices, 1) * 4, &scr));
+  unsigned long long *ctr = (unsigned long long *)scr;
+  uint32_t *blk_cnt = (uint32_t *)((char *)scr + 64);
+  QS_HIP(ctx, hipMemsetAsync(ctr, 0, 64, ctx->stream));
   Lookup L = make_lookup(h->cols);
   const unsigned blocks = (unsigned)((n + QS_BLOCK - 1) / QS_BLOCK);
+  // the ELL entry format is fixed before the count pass (it does not change the counts)
+  const uint32_t colbits = bits_for((uint64_t)std::max<int64_t>(h->cols->n - 1, 1));
+  const uint32_t termbits = bits_for((uint64_t)std::max(op->nterms - 1, 1));
+  h->view.packed = ctx->ell_pack && colbits + termbits <= 32;
+  h->view.colbits = colbits;
   if (n > 0) {
-    if (op->nw == 1)
-      hipLaunchKernelGGL(k_count<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, n,
-                         (const TermD<1> *)op->d_terms, op->nterms, L, (int)h->same, h->d_rowcnt,
-                         h->d_slice_len, (double2 *)h->d_diag, d_eff);
-    else
-      hipLaunchKernelGGL(k_count<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, n,
-                         (const TermD<2> *)op->d_terms, op->nterms, L, (int)h->same, h->d_rowcnt,
-                         h->d_slice_len, (double2 *)h->d_diag, d_eff);
+    if (op->nw == 1) launch_build<1, false>(ctx, h, blocks, L, ctr, blk_cnt);
+    else launch_build<2, false>(ctx, h, blocks, L, ctr, blk_cnt);
     QS_HIP(ctx, hipGetLastError());
   }
-  std::vector<uint32_t> slen((size_t)nslices);
-  unsigned long long eff[2] = {0, 0};
-  if (nslices)
+  std::vector<uint32_t> slen((size_t)nslices), bcnt((size_t)nslices);
+  unsigned long long c[3] = {0, 0, 0};
+  if (nslices) {
     QS_HIP(ctx, hipMemcpyAsync(slen.data(), h->d_slice_len, (size_t)nslices * 4, hipMemcpyDeviceToHost,
                                ctx->stream));
-  QS_HIP(ctx, hipMemcpyAsync(eff, d_eff, 16, hipMemcpyDeviceToHost, ctx->stream));
+    QS_HIP(ctx, hipMemcpyAsync(bcnt.data(), blk_cnt, (size_t)nslices * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  QS_HIP(ctx, hipMemcpyAsync(c, ctr, 24, hipMemcpyDeviceToHost, ctx->stream));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  std::vector<uint64_t> sptr((size_t)nslices + 1, 0);
-  for (int64_t s = 0; s < nslices; ++s) sptr[s + 1] = sptr[s] + (uint64_t)slen[s] * 64;
-  const uint64_t ent = sptr[nslices];
-  h->nnz_eff = (int64_t)eff[0];
-  h->nnz_offdiag = (int64_t)eff[1];
-  h->nnz_stored = (int64_t)ent;
-  h->stream_bytes = (int64_t)(ent * 6 + (uint64_t)n * 2 + (uint64_t)(nslices + 1) * 12 + (h->same ? n * 16 : 0));
+  std::vector<uint64_t> sptr((size_t)nslices + 1, 0), bptr((size_t)nslices + 1, 0);
+  for (int64_t s = 0; s < nslices; ++s) {
+    sptr[s + 1] = sptr[s] + (uint64_t)slen[s] * 64;
+    bptr[s + 1] = bptr[s] + bcnt[s];
+  }
+  const uint64_t ent = sptr[nslices], nblk = bptr[nslices];
+  h->nnz_eff = (int64_t)c[0];
+  h->nnz_offdiag = (int64_t)c[1];
+  h->block_hits = (int64_t)c[2];
+  h->nblocks = (int64_t)nblk;
+  h->ell_slots = (int64_t)ent;
+  h->nnz_stored = (int64_t)(ent + nblk);
+  const uint64_t per_entry = h->view.packed ? 4 : 6;
+  h->stream_bytes = (int64_t)(ent * per_entry + nblk * 16 + (uint64_t)n * (2 + 8 * op->nw) +
+                              (uint64_t)(nslices + 1) * 20 + (h->same ? n * 16 : 0));
   if (budget && (size_t)h->stream_bytes > budget)
     return qs_fail(ctx, QS_ENOMEM_DEVICE, "assembled stream needs %lld bytes, budget %zu",
                    (long long)h->stream_bytes, budget);
   QS_TRY(qs_malloc(ctx, (void **)&h->d_col, (size_t)std::max<uint64_t>(ent, 1) * 4));
-  QS_TRY(qs_malloc(ctx, (void **)&h->d_tid, (size_t)std::max<uint64_t>(ent, 1) * 2));
-  QS_HIP(ctx, hipMemcpyAsync(h->d_slice_ptr, sptr.data(), sptr.size() * 8, hipMemcpyHostToDevice,
-                             ctx->stream));
+  if (!h->view.packed) QS_TRY(qs_malloc(ctx, (void **)&h->d_tid, (size_t)std::max<uint64_t>(ent, 1) * 2));
+  QS_TRY(qs_malloc(ctx, (void **)&h->d_blk, (size_t)std::max<uint64_t>(nblk, 1) * 16));
+  QS_HIP(ctx, hipMemcpyAsync(h->d_slice_ptr, sptr.data(), sptr.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  QS_HIP(ctx, hipMemcpyAsync(h->d_blk_ptr, bptr.data(), bptr.size() * 8, hipMemcpyHostToDevice, ctx->stream));
   if (n > 0) {
-    if (op->nw == 1)
-      hipLaunchKernelGGL(k_fill<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, n,
-                         (const TermD<1> *)op->d_terms, op->nterms, L, (int)h->same, h->d_slice_ptr,
-                         h->d_col, h->d_tid);
-    else
-      hipLaunchKernelGGL(k_fill<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, n,
-                         (const TermD<2> *)op->d_terms, op->nterms, L, (int)h->same, h->d_slice_ptr,
-                         h->d_col, h->d_tid);
+    if (op->nw == 1) launch_build<1, true>(ctx, h, blocks, L, ctr, blk_cnt);
+    else launch_build<2, true>(ctx, h, blocks, L, ctr, blk_cnt);
     QS_HIP(ctx, hipGetLastError());
   }
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SellD &A = h->view;
+  A.nrows = n;
+  A.nslices = nslices;
+  A.nw = op->nw;
+  A.nterms = op->nterms;
+  A.rows = rows->d_words;
+  A.blk_ptr = h->d_blk_ptr;
+  A.blk = h->d_blk;
+  A.slice_ptr = h->d_slice_ptr;
+  A.slice_len = h->d_slice_len;
+  A.rowcnt = h->d_rowcnt;
+  A.col = h->d_col;
+  A.tid = h->d_tid;
+  A.val = (const double2 *)op->d_val;
+  A.par = op->d_par;
+  A.diag = (const double2 *)h->d_diag;
   h->assembled = 1;
   return QS_OK;
 }
@@ -580,42 +665,61 @@ extern "C" qs_status qs_hop_stats(const qs_hop *h, int64_t *nnz_eff, int64_t *nn
   return QS_OK;
 }
 
+extern "C" qs_status qs_hop_layout(const qs_hop *h, int64_t *nblocks, int64_t *block_hits, int64_t *ell_slots,
+                                   int *packed, int *shift_min) {
+  if (!h) return QS_EINVAL;
+  if (nblocks) *nblocks = h->nblocks;
+  if (block_hits) *block_hits = h->block_hits;
+  if (ell_slots) *ell_slots = h->ell_slots;
+  if (packed) *packed = h->view.packed;
+  if (shift_min) *shift_min = h->shift_min;
+  return QS_OK;
+}
+
+// Launch k_spmv for KC interleaved vectors with the term tables in LDS when
+// they fit, on no more workgroups than stay resident (grid stride over slices).
+template <int KC, int NW, bool PACKED>
+static void launch_spmv(qs_ctx *ctx, const qs_hop *h, const double2 *x, int64_t ldx, double2 *y, int64_t ldy) {
+  const SellD &A = h->view;
+  const size_t shm = term_table_bytes(A.nterms, NW);
+  const int64_t want = (A.nslices + 3) / 4;
+  if (shm <= QS_TERM_LDS_BYTES) {
+    const void *kern = (const void *)k_spmv<KC, NW, PACKED, true>;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, qs_resident_blocks(ctx, kern, shm)));
+    hipLaunchKernelGGL((k_spmv<KC, NW, PACKED, true>), dim3(blocks), dim3(QS_BLOCK), shm, ctx->stream, A, x, ldx,
+                       y, ldy);
+  } else {
+    const void *kern = (const void *)k_spmv<KC, NW, PACKED, false>;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, qs_resident_blocks(ctx, kern, 0)));
+    hipLaunchKernelGGL((k_spmv<KC, NW, PACKED, false>), dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, A, x, ldx,
+                       y, ldy);
+  }
+}
+
+template <int KC>
+static void launch_spmv_kc(qs_ctx *ctx, const qs_hop *h, const double2 *x, int64_t ldx, double2 *y, int64_t ldy) {
+  const SellD &A = h->view;
+  if (A.nw == 1) {
+    if (A.packed) launch_spmv<KC, 1, true>(ctx, h, x, ldx, y, ldy);
+    else launch_spmv<KC, 1, false>(ctx, h, x, ldx, y, ldy);
+  } else {
+    if (A.packed) launch_spmv<KC, 2, true>(ctx, h, x, ldx, y, ldy);
+    else launch_spmv<KC, 2, false>(ctx, h, x, ldx, y, ldy);
+  }
+}
+
 qs_status qs_hop_apply_strided(qs_ctx *ctx, const qs_hop *h, int k, const double *x, int64_t ldx,
                                double *y, int64_t ldy) {
   // vectors interleaved: entry j of vector v at (j*ld + v)
   const int64_t n = h->rows->n;
   if (n == 0 || k == 0) return QS_OK;
-  const int64_t nslices = h->nslices;
-  const int nterms = h->op->nterms;
-  const bool lds = (size_t)nterms * 16 <= QS_COEF_LDS_BYTES;
-  const size_t shm = lds ? (size_t)nterms * 16 : 0;
   for (int v0 = 0; v0 < k;) {
-    int kc = k - v0 >= 4 ? 4 : (k - v0 >= 2 ? 2 : 1);
-    if (k - v0 == 3) kc = 3;
+    const int kc = k - v0 >= 4 ? 4 : (k - v0 >= 2 ? 2 : 1);
     const double2 *xs = (const double2 *)x + v0;
     double2 *ys = (double2 *)y + v0;
-    // grid-stride over slices with no more workgroups than stay resident
-#define QS_LAUNCH_SPMV(KC)                                                                                 \
-  if (lds) {                                                                                                \
-    const int blocks = (int)std::max<int64_t>(                                                              \
-        1, std::min<int64_t>((nslices + 3) / 4,                                                             \
-                             qs_resident_blocks(ctx, (const void *)k_spmv<KC, true>, shm)));               \
-    hipLaunchKernelGGL((k_spmv<KC, true>), dim3(blocks), dim3(QS_BLOCK), shm, ctx->stream, n, nslices,     \
-                       h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,                    \
-                       (const double2 *)h->op->d_coef, nterms, (const double2 *)h->d_diag, xs, ldx, ys, ldy); \
-  } else {                                                                                                  \
-    const int blocks = (int)std::max<int64_t>(                                                              \
-        1, std::min<int64_t>((nslices + 3) / 4,                                                             \
-                             qs_resident_blocks(ctx, (const void *)k_spmv<KC, false>, 0)));                \
-    hipLaunchKernelGGL((k_spmv<KC, false>), dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, n, nslices,      \
-                       h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,                    \
-                       (const double2 *)h->op->d_coef, nterms, (const double2 *)h->d_diag, xs, ldx, ys, ldy); \
-  }
-    if (kc == 4) { QS_LAUNCH_SPMV(4); }
-    else if (kc == 3) { QS_LAUNCH_SPMV(3); }
-    else if (kc == 2) { QS_LAUNCH_SPMV(2); }
-    else { QS_LAUNCH_SPMV(1); }
-#undef QS_LAUNCH_SPMV
+    if (kc == 4) launch_spmv_kc<4>(ctx, h, xs, ldx, ys, ldy);
+    else if (kc == 2) launch_spmv_kc<2>(ctx, h, xs, ldx, ys, ldy);
+    else launch_spmv_kc<1>(ctx, h, xs, ldx, ys, ldy);
     QS_HIP(ctx, hipGetLastError());
     v0 += kc;
   }

@@ -26,13 +26,7 @@
 struct SlotDev {
   int64_t n;
   int64_t nslices;
-  const uint64_t *slice_ptr;
-  const uint32_t *slice_len;
-  const uint16_t *rowcnt;
-  const uint32_t *col;
-  const uint16_t *tid;
-  const double2 *coef;
-  const double2 *diag;
+  SellD A;               // the slot's assembled operator
   double2 *uc;
   double2 *up;
   double2 *y;            // Ritz-vector accumulator or null
@@ -105,6 +99,7 @@ k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64
   const int lane = threadIdx.x & 63;
   int64_t w0, w1, dw;
   xcd_items(nwork, w0, w1, dw);
+  w0 = wave_uniform(w0);
   for (int64_t w = w0; w < w1; w += dw) {
     const WorkItem it = work[w];
     const SlotDev &S = slots[it.slot];
@@ -116,8 +111,15 @@ k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64
     const int64_t r = sl * 64 + lane;
     const bool act = r < S.n;
     double hr[1], hi[1];
-    spmv_row<1>(r, act, S.slice_ptr[sl], S.slice_len[sl], S.rowcnt, S.col, S.tid, S.coef, S.diag, S.uc, 1, lane, hr,
-                hi);
+    // the slot's layout picks the row kernel (uniform across the wave)
+    const SellD &A = S.A;
+    if (A.nw == 1) {
+      if (A.packed) sell_row<1, 1, true>(A, sl, lane, A.val, A.par, S.uc, 1, hr, hi);
+      else sell_row<1, 1, false>(A, sl, lane, A.val, A.par, S.uc, 1, hr, hi);
+    } else {
+      if (A.packed) sell_row<1, 2, true>(A, sl, lane, A.val, A.par, S.uc, 1, hr, hi);
+      else sell_row<1, 2, false>(A, sl, lane, A.val, A.par, S.uc, 1, hr, hi);
+    }
     double dot = 0.0;
     if (act) {
       const double2 u = S.uc[r];
@@ -301,13 +303,7 @@ extern "C" qs_status qs_batch_add(qs_batch *B, const qs_hop *h, const double *v0
   SlotDev S{};
   S.n = n;
   S.nslices = h->nslices;
-  S.slice_ptr = h->d_slice_ptr;
-  S.slice_len = h->d_slice_len;
-  S.rowcnt = h->d_rowcnt;
-  S.col = h->d_col;
-  S.tid = h->d_tid;
-  S.coef = (const double2 *)h->op->d_coef;
-  S.diag = (const double2 *)h->d_diag;
+  S.A = h->view;
   S.uc = (double2 *)m;
   S.up = (double2 *)(m + vb);
   S.y = want_y ? (double2 *)(m + 2 * vb) : nullptr;

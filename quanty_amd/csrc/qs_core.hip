@@ -137,6 +137,36 @@ extern "C" qs_status qs_ctx_set_async(qs_ctx *ctx, int async) {
   return QS_OK;
 }
 
+extern "C" qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value) {
+  if (!ctx) return QS_EINVAL;
+  switch (param) {
+    case QS_PARAM_SHIFT_MIN:
+      if (value < 1 || value > 65) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_SHIFT_MIN must be in 1..65");
+      ctx->shift_min = (int)value;
+      return QS_OK;
+    case QS_PARAM_ELL_PACK:
+      if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_ELL_PACK must be 0 or 1");
+      ctx->ell_pack = (int)value;
+      return QS_OK;
+    default:
+      return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
+  }
+}
+
+extern "C" qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value) {
+  if (!ctx || !value) return QS_EINVAL;
+  switch (param) {
+    case QS_PARAM_SHIFT_MIN:
+      *value = ctx->shift_min;
+      return QS_OK;
+    case QS_PARAM_ELL_PACK:
+      *value = ctx->ell_pack;
+      return QS_OK;
+    default:
+      return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
+  }
+}
+
 extern "C" qs_status qs_sync(qs_ctx *ctx) {
   if (!ctx) return QS_EINVAL;
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));

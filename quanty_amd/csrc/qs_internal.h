@@ -27,6 +27,8 @@ struct qs_ctx {
   int async = 0;                   // skip the end-of-call stream sync for device-only results
   // resident workgroups per CU, by (kernel, dynamic LDS bytes)
   std::map<std::pair<const void *, size_t>, int> resident;
+  int shift_min = 56;              // QS_PARAM_SHIFT_MIN for operators assembled from now on
+  int ell_pack = 1;                // QS_PARAM_ELL_PACK
 };
 
 // Exact rank of a determinant among all determinants satisfying the
@@ -87,7 +89,37 @@ struct qs_op {
   int nterms = 0;                  // compiled (non-vanishing) terms
   std::vector<TermHost> h_terms;
   void *d_terms = nullptr;         // TermD<nw>[nterms]
-  double *d_coef = nullptr;        // 2*nterms complex: +V_t, -V_t
+  double *d_val = nullptr;         // V_t per term (complex, fermion-sign constant folded in)
+  uint64_t *d_par = nullptr;       // P_t per term (nw words): entry sign = (-1)^popcount(d & P_t)
+};
+
+// Device view of an assembled operator. Rows are cut into slices of 64 (one
+// wavefront, one lane per row). A slice's off-diagonal entries are split into
+//  - shift blocks: one term whose hit lanes l all read column base + l; a
+//    16-byte header {base (int32), term, lane mask lo, lane mask hi};
+//  - an ELL remainder, column-major inside the slice (entry e of lane l at
+//    slice_ptr + 64 e + l): col | term << colbits in one uint32 when both fit,
+//    else the column in col[] and the term in tid[].
+// The entry sign is (-1)^popcount(d_row & P_term); the diagonal terms are
+// pre-summed per row when rows == cols.
+struct SellD {
+  int64_t nrows = 0;
+  int64_t nslices = 0;
+  int nw = 1;
+  int packed = 0;
+  uint32_t colbits = 0;
+  int nterms = 0;
+  const uint64_t *rows = nullptr;      // row determinants (nw words each)
+  const uint64_t *blk_ptr = nullptr;   // nslices + 1 block offsets
+  const uint4 *blk = nullptr;          // shift-block headers
+  const uint64_t *slice_ptr = nullptr; // nslices + 1 ELL entry offsets
+  const uint32_t *slice_len = nullptr; // ELL positions per slice
+  const uint16_t *rowcnt = nullptr;    // ELL entries per row
+  const uint32_t *col = nullptr;       // ELL column (or packed column and term)
+  const uint16_t *tid = nullptr;       // ELL term when not packed
+  const double2 *val = nullptr;        // V_t
+  const uint64_t *par = nullptr;       // P_t
+  const double2 *diag = nullptr;       // pre-summed diagonal (rows == cols) or null
 };
 
 struct qs_hop {
@@ -97,17 +129,23 @@ struct qs_hop {
   bool same = false;
   int assembled = 0;
   int64_t nnz_eff = 0;             // (row, term) pairs with a source in cols
-  int64_t nnz_stored = 0;          // stored off-diagonal entries (incl. slice padding)
-  int64_t nnz_offdiag = 0;         // off-diagonal entries without padding
-  int64_t stream_bytes = 0;
-  // sliced-ELL (64 rows per slice, column-major inside a slice)
+  int64_t nnz_stored = 0;          // stored index entries: ELL slots (incl. padding) + block headers
+  int64_t nnz_offdiag = 0;         // off-diagonal entries
+  int64_t stream_bytes = 0;        // bytes of the assembled index stream and per-row data
+  int64_t nblocks = 0;             // shift blocks
+  int64_t block_hits = 0;          // off-diagonal entries held by shift blocks
+  int64_t ell_slots = 0;           // ELL slots incl. padding
+  int shift_min = 0;               // lanes a group needs to become a shift block
   int64_t nslices = 0;
-  uint64_t *d_slice_ptr = nullptr; // nslices + 1 entry offsets
-  uint32_t *d_slice_len = nullptr; // max row length per slice
-  uint16_t *d_rowcnt = nullptr;    // off-diagonal entries per row
+  SellD view;                      // device view (pointers below)
+  uint64_t *d_blk_ptr = nullptr;
+  uint4 *d_blk = nullptr;
+  uint64_t *d_slice_ptr = nullptr;
+  uint32_t *d_slice_len = nullptr;
+  uint16_t *d_rowcnt = nullptr;
   uint32_t *d_col = nullptr;
-  uint16_t *d_tid = nullptr;       // 2*term + sign
-  double *d_diag = nullptr;        // complex pre-summed diagonal per row (rows == cols)
+  uint16_t *d_tid = nullptr;
+  double *d_diag = nullptr;
 };
 
 // ---- error helpers --------------------------------------------------------

@@ -1,76 +1,152 @@
-// Device building blocks shared by the apply and Lanczos translation units.
+// Device building blocks shared by the apply, Lanczos and batch translation units.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+
+#include "qs_internal.h"
 
 #ifndef QS_BLOCK
 #define QS_BLOCK 256
 #endif
 
-// Term-value tables up to this size are staged in LDS by the assembled kernels
-// (one complex value per term; the entry's sign bit picks +V or -V).
-#define QS_COEF_LDS_BYTES (16 * 1024)
+// Term tables (V_t and P_t) up to this size are staged in LDS by the
+// assembled kernels.
+#define QS_TERM_LDS_BYTES (24 * 1024)
 
-// One row of the assembled operator for KC interleaved vectors
-// (entry j of vector v at x[j*ldx + v]). Lane `lane` of the slice's wavefront
-// owns row r; the slice stores its entries column-major (entry e of the lane's
-// row at sbase + 64*e + lane), so every load of the index stream is coalesced.
-// Entry id t = 2*term + sign: with HALF the table holds V_term only and the
-// sign is applied here, otherwise coef[t] = (-1)^sign V_term.
-template <int KC, bool HALF = false>
-__device__ __forceinline__ void spmv_row(int64_t r, bool active, uint64_t sbase, uint32_t slen,
-                                         const uint16_t *__restrict__ rowcnt,
-                                         const uint32_t *__restrict__ colv,
-                                         const uint16_t *__restrict__ tidv,
-                                         const double2 *__restrict__ coef,
-                                         const double2 *__restrict__ diag,
-                                         const double2 *__restrict__ x, int64_t ldx, int lane,
+// Wave-uniform copy of a value the compiler cannot prove uniform.
+__device__ __forceinline__ int64_t wave_uniform(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(uint64_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Term t applied to the row's determinant d: accumulate
+// (-1)^popcount(d & P_t) V_t x[c] for KC interleaved vectors when `hit`.
+template <int KC, int NW>
+__device__ __forceinline__ void sell_acc(bool hit, int64_t c, double2 v, const uint64_t *P, const uint64_t *d,
+                                         const double2 *__restrict__ x, int64_t ldx, double *ar, double *ai) {
+  uint32_t p = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) p += __popcll(d[w] & P[w]);
+  const uint64_t sb = (uint64_t)(p & 1u) << 63;
+  v.x = __longlong_as_double(__double_as_longlong(v.x) ^ sb);
+  v.y = __longlong_as_double(__double_as_longlong(v.y) ^ sb);
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    const double2 xv = x[c * ldx + k];
+    const double tr = fma(v.x, xv.x, fma(-v.y, xv.y, ar[k]));
+    const double ti = fma(v.x, xv.y, fma(v.y, xv.x, ai[k]));
+    ar[k] = hit ? tr : ar[k];
+    ai[k] = hit ? ti : ai[k];
+  }
+}
+
+// Row s*64 + lane of the assembled operator A applied to KC interleaved
+// vectors (entry j of vector k at x[j*ldx + k]). `s` must be wave-uniform.
+// tv / tp: the term values and parity masks (LDS copies or A.val / A.par).
+// Both loops are branch-free so that many gathers are in flight per wave:
+// lanes without an entry read x[0] and keep their sums.
+template <int KC, int NW, bool PACKED>
+__device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, const double2 *tv,
+                                         const uint64_t *tp, const double2 *__restrict__ x, int64_t ldx,
                                          double *ar, double *ai) {
+  constexpr int U = KC == 1 ? 8 : 4;  // ELL positions per batch
 #pragma unroll
-  for (int v = 0; v < KC; ++v) {
-    ar[v] = 0.0;
-    ai[v] = 0.0;
+  for (int k = 0; k < KC; ++k) {
+    ar[k] = 0.0;
+    ai[k] = 0.0;
   }
+  const int64_t r = s * 64 + lane;
+  const bool active = r < A.nrows;
+  uint64_t d[NW];
   uint32_t cnt = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) d[w] = 0;
   if (active) {
-    cnt = rowcnt[r];
-    if (diag) {
-      const double2 dg = diag[r];
 #pragma unroll
-      for (int v = 0; v < KC; ++v) {
-        const double2 xv = x[r * ldx + v];
-        ar[v] = dg.x * xv.x - dg.y * xv.y;
-        ai[v] = dg.x * xv.y + dg.y * xv.x;
+    for (int w = 0; w < NW; ++w) d[w] = A.rows[r * NW + w];
+    cnt = A.rowcnt[r];
+    if (A.diag) {
+      const double2 dg = A.diag[r];
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        const double2 xv = x[r * ldx + k];
+        ar[k] = dg.x * xv.x - dg.y * xv.y;
+        ai[k] = dg.x * xv.y + dg.y * xv.x;
       }
     }
   }
-  const uint32_t *cp = colv + sbase + lane;
-  const uint16_t *tp = tidv + sbase + lane;
-  // the index stream is read once per apply: non-temporal loads keep it from
-  // evicting the gathered vector from L2 / the Infinity Cache
+  // shift blocks: up to 64 headers are loaded at once (lane i holds header i)
+  // and broadcast one by one; the hit lanes of a block read x[base + lane]
+  const uint64_t b0 = A.blk_ptr[s], b1 = A.blk_ptr[s + 1];
+  for (uint64_t c0 = b0; c0 < b1; c0 += 64) {
+    const int nb = (int)((b1 - c0) < 64 ? (b1 - c0) : 64);
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 hv = {0u, 0u, 0u, 0u};
+    if (lane < nb) hv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(A.blk + c0 + lane));
 #pragma unroll 4
-  for (uint32_t j = 0; j < slen; ++j) {
-    if (j < cnt) {
-      const uint32_t c = __builtin_nontemporal_load(cp + (size_t)j * 64);
-      const uint16_t t = __builtin_nontemporal_load(tp + (size_t)j * 64);
-      double2 cv;
-      if (HALF) {
-        cv = coef[t >> 1];
-        if (t & 1) {
-          cv.x = -cv.x;
-          cv.y = -cv.y;
-        }
-      } else {
-        cv = coef[t];
-      }
+    for (int k = 0; k < nb; ++k) {
+      const int32_t base = (int32_t)__builtin_amdgcn_readlane(hv.x, k);
+      const uint32_t t = __builtin_amdgcn_readlane(hv.y, k);
+      const uint64_t mask = ((uint64_t)__builtin_amdgcn_readlane(hv.w, k) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane(hv.z, k);
+      const bool hit = (mask >> lane) & 1u;
+      const int64_t c = hit ? (int64_t)base + lane : 0;
+      sell_acc<KC, NW>(hit, c, tv[t], tp + (size_t)t * NW, d, x, ldx, ar, ai);
+    }
+  }
+  // ELL remainder, U positions per batch, software-pipelined: the stream
+  // loads of batch b+1 are issued before the gathers of batch b, so the two
+  // latencies overlap. Padding slots hold column 0 / term 0 and are masked.
+  // The stream is read once per apply, so non-temporal loads keep it from
+  // evicting the gathered vector from L2 / the Infinity Cache.
+  const uint64_t sb = A.slice_ptr[s];
+  const uint32_t slen = A.slice_len[s];
+  const uint32_t *cp = A.col + sb + lane;
+  const uint16_t *tq = PACKED ? nullptr : A.tid + sb + lane;
+  const uint32_t cbits = A.colbits;
+  const uint32_t cmask = PACKED ? ((1u << cbits) - 1u) : 0xFFFFFFFFu;
+  uint32_t e[U], tt[U];
+  auto load_batch = [&](uint32_t j0, uint32_t *eo, uint32_t *to) {
 #pragma unroll
-      for (int v = 0; v < KC; ++v) {
-        const double2 xv = x[(int64_t)c * ldx + v];
-        ar[v] = fma(cv.x, xv.x, fma(-cv.y, xv.y, ar[v]));
-        ai[v] = fma(cv.x, xv.y, fma(cv.y, xv.x, ai[v]));
+    for (int u = 0; u < U; ++u) {
+      const uint32_t j = j0 + u < slen ? j0 + u : slen - 1;
+      eo[u] = __builtin_nontemporal_load(cp + (size_t)j * 64);
+      to[u] = PACKED ? 0u : (uint32_t)__builtin_nontemporal_load(tq + (size_t)j * 64);
+    }
+  };
+  if (slen > 0) load_batch(0, e, tt);
+  for (uint32_t j0 = 0; j0 < slen; j0 += U) {
+    uint32_t en[U], tn[U];
+    const bool more = j0 + U < slen;
+    if (more) load_batch(j0 + U, en, tn);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = j0 + u < cnt;
+      const uint32_t c = ok ? (e[u] & cmask) : 0u;
+      const uint32_t t = ok ? (PACKED ? e[u] >> cbits : tt[u]) : 0u;
+      sell_acc<KC, NW>(ok, (int64_t)c, tv[t], tp + (size_t)t * NW, d, x, ldx, ar, ai);
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        e[u] = en[u];
+        tt[u] = tn[u];
       }
     }
   }
+}
+
+// Bytes of LDS that stage_term_tables uses.
+__host__ __device__ inline size_t term_table_bytes(int nterms, int nw) { return (size_t)nterms * (16 + 8 * nw); }
+
+// Stage the term tables of A into LDS at smem (no barrier).
+template <int NW>
+__device__ __forceinline__ void stage_term_tables(const SellD &A, char *smem) {
+  double2 *sv = reinterpret_cast<double2 *>(smem);
+  uint64_t *sp = reinterpret_cast<uint64_t *>(smem + (size_t)A.nterms * 16);
+  for (int i = threadIdx.x; i < A.nterms; i += blockDim.x) sv[i] = A.val[i];
+  for (int i = threadIdx.x; i < A.nterms * NW; i += blockDim.x) sp[i] = A.par[i];
 }
 
 // XCD-aware split of n wave-sized work items. Workgroups are dispatched
@@ -97,30 +173,29 @@ __device__ __forceinline__ void xcd_items(int64_t n, int64_t &first, int64_t &en
   stride = in_xcd * wpb;
 }
 
-// Sum of `np` partials in a fixed order, identical in every block.
-__device__ __forceinline__ double block_sum_fixed(const double *__restrict__ part, int np, double *red) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
-  red[threadIdx.x] = s;
+// Sum over a wavefront by an xor butterfly: every lane ends with the same
+// value, in the same order on every call.
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Block-wide sum of one value per thread in a fixed order; the result is valid
+// in every thread. red: at least blockDim.x / 64 doubles of LDS.
+__device__ __forceinline__ double block_sum_values(double v, double *red) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  const double r = red[0];
+  double r = 0.0;
+  const int nwv = blockDim.x >> 6;
+  for (int i = 0; i < nwv; ++i) r += red[i];
   __syncthreads();
   return r;
 }
 
-// Block-wide sum of one value per thread, fixed order; result valid in thread 0.
-__device__ __forceinline__ double block_sum_values(double v, double *red) {
-  red[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = blockDim.x >> 1; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  const double r = red[0];
-  __syncthreads();
-  return r;
+// Sum of `np` partials in a fixed order, identical in every block.
+__device__ __forceinline__ double block_sum_fixed(const double *__restrict__ part, int np, double *red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+  return block_sum_values(s, red);
 }

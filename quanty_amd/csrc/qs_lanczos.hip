@@ -122,31 +122,31 @@ __device__ __forceinline__ void lz_k1_prologue(int step, int nb2, LzState S, dou
 }
 
 // K1, assembled operator (fused SpMV + epilogue)
-template <bool LDSC>
-__global__ void __launch_bounds__(QS_BLOCK)
-k_lz1_asm(int step, int nb2, LzState S, int64_t nrows, int64_t nslices, const uint64_t *__restrict__ slice_ptr,
-          const uint32_t *__restrict__ slice_len, const uint16_t *__restrict__ rowcnt,
-          const uint32_t *__restrict__ colv, const uint16_t *__restrict__ tidv,
-          const double2 *__restrict__ coef, int nterms, const double2 *__restrict__ diag,
-          const double2 *__restrict__ uc, double2 *__restrict__ up) {
-  __shared__ double red[QS_BLOCK];
-  extern __shared__ double2 s_coef[];
-  if (LDSC) {
-    for (int i = threadIdx.x; i < nterms; i += blockDim.x) s_coef[i] = coef[2 * i];
+template <int NW, bool PACKED, bool LDS>
+// the packed one-word variant fits 64 VGPRs (8 waves per SIMD) without spills
+__global__ void __launch_bounds__(QS_BLOCK, (NW == 1 && PACKED) ? 8 : 1)
+k_lz1_asm(int step, int nb2, LzState S, SellD A, const double2 *__restrict__ uc, double2 *__restrict__ up) {
+  __shared__ double red[QS_BLOCK / 64];
+  extern __shared__ char smem[];
+  const double2 *tv = A.val;
+  const uint64_t *tp = A.par;
+  if (LDS) {
+    stage_term_tables<NW>(A, smem);
+    tv = reinterpret_cast<const double2 *>(smem);
+    tp = reinterpret_cast<const uint64_t *>(smem + (size_t)A.nterms * 16);
   }
   double sc, cp;
   lz_k1_prologue(step, nb2, S, red, sc, cp);  // ends with a barrier
-  const double2 *cf = LDSC ? s_coef : coef;
   const int lane = threadIdx.x & 63;
   int64_t s0, s1, ds;
-  xcd_items(nslices, s0, s1, ds);
+  xcd_items(A.nslices, s0, s1, ds);
+  s0 = wave_uniform(s0);
   double dot = 0.0;
   for (int64_t s = s0; s < s1; s += ds) {
-    const int64_t r = s * 64 + lane;
-    const bool active = r < nrows;
     double hr[1], hi[1];
-    spmv_row<1, LDSC>(r, active, slice_ptr[s], slice_len[s], rowcnt, colv, tidv, cf, diag, uc, 1, lane, hr, hi);
-    if (active) {
+    sell_row<1, NW, PACKED>(A, s, lane, tv, tp, uc, 1, hr, hi);
+    const int64_t r = s * 64 + lane;
+    if (r < A.nrows) {
       const double2 u = uc[r];
       dot += hr[0] * u.x + hi[0] * u.y;
       const double2 p = up[r];
@@ -155,6 +155,33 @@ k_lz1_asm(int step, int nb2, LzState S, int64_t nrows, int64_t nslices, const ui
   }
   const double t = block_sum_values(dot, red);
   if (threadIdx.x == 0) S.pd[blockIdx.x] = t;
+}
+
+// Kernel, dynamic LDS and grid for K1 on an assembled operator.
+struct Lz1Launch {
+  const void *kern;
+  size_t shm;
+  int blocks;
+};
+
+template <int NW, bool PACKED>
+static Lz1Launch lz1_choose(qs_ctx *ctx, const SellD &A) {
+  Lz1Launch L;
+  L.shm = term_table_bytes(A.nterms, NW);
+  if (L.shm <= QS_TERM_LDS_BYTES) {
+    L.kern = (const void *)k_lz1_asm<NW, PACKED, true>;
+  } else {
+    L.kern = (const void *)k_lz1_asm<NW, PACKED, false>;
+    L.shm = 0;
+  }
+  // grid stride over slices with no more workgroups than stay resident
+  L.blocks = (int)std::max<int64_t>(1, std::min<int64_t>((A.nslices + 3) / 4, qs_resident_blocks(ctx, L.kern, L.shm)));
+  return L;
+}
+
+static Lz1Launch lz1_launch_for(qs_ctx *ctx, const SellD &A) {
+  if (A.nw == 1) return A.packed ? lz1_choose<1, true>(ctx, A) : lz1_choose<1, false>(ctx, A);
+  return A.packed ? lz1_choose<2, true>(ctx, A) : lz1_choose<2, false>(ctx, A);
 }
 
 // K1 epilogue for the matrix-free operator (h already in hbuf)
@@ -205,13 +232,10 @@ extern "C" qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, 
   if (n == 0) return qs_fail(ctx, QS_EINVAL, "empty basis");
   const int nb2 = red_blocks(ctx, n);
   int nb1 = nb2;
-  const int nterms = h->op->nterms;
-  const bool lds = (size_t)nterms * 16 <= QS_COEF_LDS_BYTES;
-  const size_t shm = lds ? (size_t)nterms * 16 : 0;
+  Lz1Launch l1{nullptr, 0, 0};
   if (h->assembled) {
-    // grid-stride over slices with no more workgroups than stay resident
-    const void *kern = lds ? (const void *)k_lz1_asm<true> : (const void *)k_lz1_asm<false>;
-    nb1 = (int)std::max<int64_t>(1, std::min<int64_t>((h->nslices + 3) / 4, qs_resident_blocks(ctx, kern, shm)));
+    l1 = lz1_launch_for(ctx, h->view);
+    nb1 = l1.blocks;
   }
   const size_t vbytes = (size_t)n * 16;
   const size_t sbytes = ((size_t)(nsteps + 1) + 2 * (size_t)nsteps + nb1 + nb2 + 64) * 8;
@@ -238,16 +262,9 @@ extern "C" qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, 
   double2 *uc = u0, *up = u1;
   for (int i = 0; i < nsteps && e == hipSuccess && st == QS_OK; ++i) {
     if (h->assembled) {
-      if (lds)
-        hipLaunchKernelGGL(k_lz1_asm<true>, dim3(nb1), dim3(QS_BLOCK), shm, ctx->stream, i, nb2, S, n, h->nslices,
-                           h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,
-                           (const double2 *)h->op->d_coef, nterms, (const double2 *)h->d_diag, (const double2 *)uc,
-                           up);
-      else
-        hipLaunchKernelGGL(k_lz1_asm<false>, dim3(nb1), dim3(QS_BLOCK), 0, ctx->stream, i, nb2, S, n, h->nslices,
-                           h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid,
-                           (const double2 *)h->op->d_coef, nterms, (const double2 *)h->d_diag, (const double2 *)uc,
-                           up);
+      void *args[] = {&i, (void *)&nb2, &S, (void *)&h->view, &uc, &up};
+      e = hipLaunchKernel(l1.kern, dim3(nb1), dim3(QS_BLOCK), args, l1.shm, ctx->stream);
+      if (e != hipSuccess) break;
     } else {
       // matrix-free: h = H u_c into hb, then the epilogue
       st = qs_launch_mf_ext(ctx, h, (const double *)uc, (double *)hb);

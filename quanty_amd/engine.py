@@ -151,8 +151,15 @@ class Hop:
         a, b, c, d, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int(), C.c_int64()
         self.ctx.check(self.lib.qs_hop_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d), C.byref(e)),
                        "qs_hop_stats")
-        return {"nnz_eff": a.value, "nnz_stored": b.value, "stream_bytes": c.value, "assembled": bool(d.value),
-                "nnz_offdiag": e.value}
+        out = {"nnz_eff": a.value, "nnz_stored": b.value, "stream_bytes": c.value, "assembled": bool(d.value),
+               "nnz_offdiag": e.value}
+        if out["assembled"]:
+            nb, bh, es, pk, sm = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int(), C.c_int()
+            self.ctx.check(self.lib.qs_hop_layout(self.h, C.byref(nb), C.byref(bh), C.byref(es), C.byref(pk),
+                                                  C.byref(sm)), "qs_hop_layout")
+            out.update(nblocks=nb.value, block_hits=bh.value, ell_slots=es.value, packed=bool(pk.value),
+                       shift_min=sm.value)
+        return out
 
     def apply(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
         assert x.dtype == torch.complex128 and x.is_contiguous() and x.numel() == self.cols.n

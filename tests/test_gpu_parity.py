@@ -12,6 +12,8 @@ from models import chain, chain_states, golden, str_to_bytes
 from quanty_amd import det as D
 from quanty_amd import operators as O
 from quanty_amd import problems as P
+from quanty_amd import default_context
+from quanty_amd._lib import QS_PARAM_ELL_PACK, QS_PARAM_SHIFT_MIN
 from quanty_amd.engine import Basis, Hop, Op, apply, cf_spectrum, dot, expect
 
 pytestmark = pytest.mark.gpu
@@ -90,6 +92,57 @@ def test_hpsi_matches_oracle(kw, mode):
     y = hop.apply(torch.from_numpy(x).cuda()).cpu().numpy()
     want = oracle_apply_dense(nf, pr["H"], b.words(), x, pr["restrictions"], b)
     assert np.max(np.abs(y - want)) <= 1e-13 * np.max(np.abs(want))
+
+
+class assembled_layout:
+    """Assemble operators with the given shift-block threshold and ELL entry
+    format inside the block; the defaults come back afterwards."""
+
+    def __init__(self, shift_min, pack=1):
+        self.v = {QS_PARAM_SHIFT_MIN: shift_min, QS_PARAM_ELL_PACK: pack}
+
+    def __enter__(self):
+        c = default_context()
+        self.old = {k: c.get_param(k) for k in self.v}
+        for k, v in self.v.items():
+            c.set_param(k, v)
+
+    def __exit__(self, *exc):
+        c = default_context()
+        for k, v in self.old.items():
+            c.set_param(k, v)
+
+
+@pytest.mark.parametrize("shift_min,pack", [(1, 1), (8, 1), (8, 0), (65, 1), (65, 0), (3, 0)])
+def test_hpsi_layouts_match_oracle(shift_min, pack):
+    """Every shift-block threshold and both ELL entry formats give the oracle's
+    H*psi (square and rectangular bindings); the layout counts add up."""
+    pr = P.anderson(**SMALL[2])
+    nf = pr["NF"]
+    b = Basis.enumerate(nf, pr["restrictions"])
+    with assembled_layout(shift_min, pack):
+        hop = Hop(Op(nf, pr["H"]), b, mode="assembled")
+    st = hop.stats()
+    assert st["shift_min"] == shift_min and st["packed"] == bool(pack)
+    if shift_min == 65:
+        assert st["nblocks"] == 0 and st["block_hits"] == 0
+    else:
+        assert st["nblocks"] > 0 and st["block_hits"] >= shift_min * st["nblocks"]
+    assert st["ell_slots"] >= st["nnz_offdiag"] - st["block_hits"]
+    x = cvec(b.n, 11)
+    y = hop.apply(torch.from_numpy(x).cuda()).cpu().numpy()
+    want = oracle_apply_dense(nf, pr["H"], b.words(), x, pr["restrictions"], b)
+    assert np.max(np.abs(y - want)) <= 1e-13 * np.max(np.abs(want))
+    # rectangular: the Ni L-edge transition between two sectors
+    ni = P.ni_xas()
+    gs = Basis.enumerate(26, ni["gs_restrictions"])
+    fs = Basis.enumerate(26, ni["fs_restrictions"])
+    xg = cvec(gs.n, 2)
+    with assembled_layout(shift_min, pack):
+        th = Hop(Op(26, ni["T"][0]), fs, gs, mode="assembled")
+    yt = th.apply(torch.from_numpy(xg).cuda()).cpu().numpy()
+    wt = oracle_apply_dense(26, ni["T"][0], gs.words(), xg, ni["fs_restrictions"], fs)
+    assert np.max(np.abs(yt - wt)) <= 1e-13 * np.max(np.abs(wt))
 
 
 def test_optimespsi_known_answer_gpu():
