@@ -203,6 +203,16 @@ qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out);
  * over the same start. */
 qs_status qs_batch_add(qs_batch *b, const qs_hop *h, const double *v0, int depth, int max_depth,
                        const double *ycoef, int *slot);
+/* qs_batch_add with flags. QS_BATCH_KEEP_BASIS: the slot also keeps its
+ * Lanczos vectors on the device ((max_depth + 1) * n complex), so that
+ * qs_batch_ritz forms a Ritz vector without a second pass. */
+enum { QS_BATCH_KEEP_BASIS = 1 };
+qs_status qs_batch_add_ex(qs_batch *b, const qs_hop *h, const double *v0, int depth, int max_depth, int flags,
+                          const double *ycoef, int *slot);
+/* y = sum_{i<m} z[i] v_i over the kept normalized Lanczos vectors v_i of a
+ * QS_BATCH_KEEP_BASIS slot (m <= completed steps; z host, y device, n complex).
+ * The same sum, in the same order, as a second pass with ycoef = z. */
+qs_status qs_batch_ritz(qs_batch *b, int slot, int m, const double *z, double *y);
 /* Change the depth of a coefficient-only slot (<= its max_depth); a finished
  * slot resumes from its last step when the depth grows. */
 qs_status qs_batch_set_depth(qs_batch *b, int slot, int depth);

@@ -30,10 +30,13 @@ struct SlotDev {
   double2 *uc;
   double2 *up;
   double2 *y;            // Ritz-vector accumulator or null
+  double2 *V;            // kept Lanczos vectors u_0..u_pad (n each) or null
   const double *ycoef;   // z_i, length depth
   double *a;             // depth + 1
   double *b;             // depth + 1
   double *s;             // depth + 1
+  double sc;             // s_i of the current step i
+  double cp;             // b_i s_{i-1} of the current step (0 at i = 0)
   int step;              // completed steps
   int depth;
   int active;
@@ -42,8 +45,10 @@ struct SlotDev {
 
 struct WorkItem {
   int slot;
-  int idx;  // slice (K1) or row chunk (K2)
+  int idx;  // group of QS_K1_GROUP slices (K1) or row chunk (K2)
 };
+
+#define QS_K1_GROUP 4  // slices per K1 work item: one per wavefront of a workgroup
 
 struct qs_batch {
   qs_ctx *ctx = nullptr;
@@ -52,13 +57,20 @@ struct qs_batch {
   std::vector<SlotDev> h_slots;
   std::vector<int> used;
   std::vector<char *> mem;     // per-slot allocation (vectors + scalars)
+  std::vector<double2 *> kept; // per-slot kept Lanczos vectors or null
+  std::vector<size_t> kept_bytes;
+  // kept-vector buffers of removed slots, reused by later slots (large device
+  // allocations and frees are slow and synchronous)
+  std::vector<std::pair<size_t, double2 *>> pool;
   std::vector<const qs_hop *> hops;
   // worklists (rebuilt when the set of slots changes)
   bool dirty = true;
   WorkItem *d_w1 = nullptr, *d_w2 = nullptr;
   int64_t n1 = 0, n2 = 0, cap1 = 0, cap2 = 0;
   int *d_r1 = nullptr, *d_r2 = nullptr;  // per slot: [begin, end) of its entries, 2 ints each
-  double *d_p1 = nullptr, *d_p2 = nullptr;  // partials per work item
+  double *d_p1 = nullptr, *d_p2 = nullptr;  // partials per slice (K1) and per work item (K2)
+  size_t lds1 = 0;                          // K1 dynamic LDS: the largest term table that is staged
+  bool fast1 = true;                        // every slot: one-word determinants, packed entries
 };
 
 // ---------------------------------------------------------------------------
@@ -72,6 +84,7 @@ k_b_init(SlotDev *slots, int slot, const double2 *__restrict__ v0, double *__res
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < S.n; i += (int64_t)gridDim.x * blockDim.x) {
     const double2 x = v0[i];
     S.uc[i] = x;
+    if (S.V) S.V[i] = x;
     S.up[i] = make_double2(0.0, 0.0);
     if (S.y) S.y[i] = make_double2(0.0, 0.0);
     sum += x.x * x.x + x.y * x.y;
@@ -90,60 +103,119 @@ k_b_init_finish(SlotDev *slots, int slot, const double *__restrict__ part, int n
     S.a[0] = 0.0;
     S.b[0] = b0;
     S.s[0] = b0 > 0.0 ? 1.0 / b0 : 0.0;
+    S.sc = S.s[0];
+    S.cp = 0.0;
     S.step = 0;
   }
 }
 
-__global__ void __launch_bounds__(QS_BLOCK)
-k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64_t nwork, double *__restrict__ part) {
+// K1 over the work items of all active slots. A workgroup takes one item (4
+// consecutive slices of one slot, one per wavefront) per iteration and keeps
+// the slot's term tables in LDS while consecutive items stay in that slot;
+// tables larger than the dynamic LDS are read from global memory. The items
+// of XCD x (workgroups b with b % 8 == x) are a contiguous range, so a
+// workgroup meets few slot changes.
+template <int NW, bool PACKED>
+__device__ __forceinline__ void b1_row(const SellD &A, int64_t sl, int lane, bool lds, const char *smem,
+                                       const double2 *uc, double *hr, double *hi) {
+  if (lds)
+    sell_row<1, NW, PACKED>(A, sl, lane, reinterpret_cast<const double2 *>(smem),
+                            reinterpret_cast<const uint64_t *>(smem + (size_t)A.nterms * 16), uc, 1, hr, hi);
+  else
+    sell_row<1, NW, PACKED>(A, sl, lane, A.val, A.par, uc, 1, hr, hi);
+}
+
+// FAST: every active slot has one-word determinants and packed entries (the
+// common case), which keeps the kernel within 64 VGPRs (8 waves per SIMD).
+template <bool FAST>
+__global__ void __launch_bounds__(QS_BLOCK, FAST ? 8 : 1)
+k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64_t nwork, double *__restrict__ part,
+     int lds_bytes) {
+  extern __shared__ char smem[];
   const int lane = threadIdx.x & 63;
-  int64_t w0, w1, dw;
-  xcd_items(nwork, w0, w1, dw);
-  w0 = wave_uniform(w0);
-  for (int64_t w = w0; w < w1; w += dw) {
-    const WorkItem it = work[w];
+  const int wv = threadIdx.x >> 6;
+  constexpr int NX = 8;
+  const int64_t nblk = gridDim.x;
+  int64_t i0, i1, di;
+  if (nblk < NX) {
+    i0 = blockIdx.x;
+    i1 = nwork;
+    di = nblk;
+  } else {
+    const int xcd = blockIdx.x % NX;
+    i0 = nwork * xcd / NX + blockIdx.x / NX;
+    i1 = nwork * (xcd + 1) / NX;
+    di = (nblk - xcd + NX - 1) / NX;
+  }
+  int staged = -1;
+  for (int64_t i = i0; i < i1; i += di) {
+    const WorkItem it = work[i];
     const SlotDev &S = slots[it.slot];
-    if (!S.active || S.step >= S.depth) continue;  // uniform across the wave
-    const int i = S.step;
-    const double sc = S.s[i];
-    const double cp = i > 0 ? S.b[i] * S.s[i - 1] : 0.0;
-    const int64_t sl = it.idx;
-    const int64_t r = sl * 64 + lane;
-    const bool act = r < S.n;
-    double hr[1], hi[1];
-    // the slot's layout picks the row kernel (uniform across the wave)
+    if (!S.active || S.step >= S.depth) continue;  // uniform across the workgroup
     const SellD &A = S.A;
-    if (A.nw == 1) {
-      if (A.packed) sell_row<1, 1, true>(A, sl, lane, A.val, A.par, S.uc, 1, hr, hi);
-      else sell_row<1, 1, false>(A, sl, lane, A.val, A.par, S.uc, 1, hr, hi);
-    } else {
-      if (A.packed) sell_row<1, 2, true>(A, sl, lane, A.val, A.par, S.uc, 1, hr, hi);
-      else sell_row<1, 2, false>(A, sl, lane, A.val, A.par, S.uc, 1, hr, hi);
+    const bool lds = term_table_bytes(A.nterms, A.nw) <= (size_t)lds_bytes;
+    if (lds && staged != it.slot) {
+      __syncthreads();  // the previous table is no longer read
+      if (FAST || A.nw == 1) stage_term_tables<1>(A, smem);
+      else stage_term_tables<2>(A, smem);
+      __syncthreads();
+      staged = it.slot;
     }
+    const int64_t sl = (int64_t)it.idx * QS_K1_GROUP + wv;
     double dot = 0.0;
-    if (act) {
-      const double2 u = S.uc[r];
-      dot = hr[0] * u.x + hi[0] * u.y;
-      const double2 p = S.up[r];
-      S.up[r] = make_double2(sc * hr[0] - cp * p.x, sc * hi[0] - cp * p.y);
+    if (sl < S.nslices) {  // uniform across the wavefront
+      const double sc = S.sc, cp = S.cp;
+      double hr[1], hi[1];
+      if (FAST) {
+        b1_row<1, true>(A, sl, lane, lds, smem, S.uc, hr, hi);
+      } else if (A.nw == 1) {
+        if (A.packed) b1_row<1, true>(A, sl, lane, lds, smem, S.uc, hr, hi);
+        else b1_row<1, false>(A, sl, lane, lds, smem, S.uc, hr, hi);
+      } else {
+        if (A.packed) b1_row<2, true>(A, sl, lane, lds, smem, S.uc, hr, hi);
+        else b1_row<2, false>(A, sl, lane, lds, smem, S.uc, hr, hi);
+      }
+      const int64_t r = sl * 64 + lane;
+      if (r < S.n) {
+        const double2 u = S.uc[r];
+        dot = hr[0] * u.x + hi[0] * u.y;
+        const double2 p = S.up[r];
+        S.up[r] = make_double2(sc * hr[0] - cp * p.x, sc * hi[0] - cp * p.y);
+      }
     }
-    for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);  // fixed butterfly: same order every call
-    if (lane == 0) part[w] = dot;
+    dot = wave_sum(dot);  // fixed butterfly: same order every call
+    if (lane == 0) part[i * QS_K1_GROUP + wv] = dot;
   }
 }
 
-// one wavefront per slot: fixed-order sum of the slot's work-item partials
-__global__ void k_b_r1(SlotDev *slots, int nslots, const int *__restrict__ range, const double *__restrict__ part) {
+// Sum of a slot's work-item partials part[b..e) by one 256-thread block, in
+// an order fixed by e - b alone (thread t takes b+t, b+t+256, ... into four
+// interleaved accumulators), so a slot's result does not depend on the other
+// slots of the batch.
+__device__ __forceinline__ double slot_sum(const double *__restrict__ part, int b, int e, double *red) {
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int k = b + threadIdx.x;
+  const int st = blockDim.x;
+  for (; k + 3 * st < e; k += 4 * st) {
+    s0 += part[k];
+    s1 += part[k + st];
+    s2 += part[k + 2 * st];
+    s3 += part[k + 3 * st];
+  }
+  for (; k < e; k += st) s0 += part[k];
+  return block_sum_values((s0 + s1) + (s2 + s3), red);
+}
+
+// one block per slot: a_{i+1} = s_i^2 Re<H u_i|u_i>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b_r1(SlotDev *slots, int nslots, const int *__restrict__ range, const double *__restrict__ part) {
+  __shared__ double red[QS_BLOCK / 64];
   const int slot = blockIdx.x;
   if (slot >= nslots) return;
   SlotDev &S = slots[slot];
-  if (!S.active || S.step >= S.depth) return;
-  const int lane = threadIdx.x;
-  const int b = range[2 * slot], e = range[2 * slot + 1];
-  double sum = 0.0;
-  for (int k = b + lane; k < e; k += 64) sum += part[k];
-  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-  if (lane == 0) {
+  if (!S.active || S.step >= S.depth) return;  // uniform across the block
+  const double sum = slot_sum(part, range[2 * slot], range[2 * slot + 1], red);
+  if (threadIdx.x == 0) {
     const double sc = S.s[S.step];
     S.a[S.step + 1] = sc * sc * sum;
   }
@@ -168,10 +240,11 @@ k_b2(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64
       const double2 u = S.uc[r];
       if (S.y) {
         const double2 yv = S.y[r];
-        S.y[r] = make_double2(yv.x + zc * u.x, yv.y + zc * u.y);
+        S.y[r] = make_double2(fma(zc, u.x, yv.x), fma(zc, u.y, yv.y));
       }
       const double2 un = make_double2(wv.x - f * u.x, wv.y - f * u.y);
       S.up[r] = un;
+      if (S.V) S.V[(size_t)(i + 1) * S.n + r] = un;
       nrm += un.x * un.x + un.y * un.y;
     }
     const double t = block_sum_values(nrm, red);
@@ -179,25 +252,41 @@ k_b2(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64
   }
 }
 
-__global__ void k_b_r2(SlotDev *slots, int nslots, const int *__restrict__ range, const double *__restrict__ part) {
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b_r2(SlotDev *slots, int nslots, const int *__restrict__ range, const double *__restrict__ part) {
+  __shared__ double red[QS_BLOCK / 64];
   const int slot = blockIdx.x;
   if (slot >= nslots) return;
   SlotDev &S = slots[slot];
-  if (!S.active || S.step >= S.depth) return;
-  const int lane = threadIdx.x;
-  const int b = range[2 * slot], e = range[2 * slot + 1];
-  double sum = 0.0;
-  for (int k = b + lane; k < e; k += 64) sum += part[k];
-  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-  if (lane == 0) {
+  if (!S.active || S.step >= S.depth) return;  // uniform across the block
+  const double sum = slot_sum(part, range[2 * slot], range[2 * slot + 1], red);
+  if (threadIdx.x == 0) {
     const int i = S.step + 1;
     const double bn = sqrt(sum);
     S.b[i] = bn;
     S.s[i] = bn > 0.0 ? 1.0 / bn : 0.0;
+    S.sc = S.s[i];
+    S.cp = bn * S.s[i - 1];
     double2 *t = S.uc;
     S.uc = S.up;
     S.up = t;
     S.step = i;
+  }
+}
+
+// y = sum_{j<m} z_j s_j u_j over the kept vectors, j ascending per row (the
+// order of the second-pass accumulation in k_b2)
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b_ritz(const SlotDev *__restrict__ slots, int slot, int m, const double *__restrict__ z, double2 *__restrict__ y) {
+  const SlotDev &S = slots[slot];
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < S.n; r += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = 0; j < m; ++j) {
+      const double c = z[j] * S.s[j];
+      const double2 u = S.V[(size_t)j * S.n + r];
+      acc = make_double2(fma(c, u.x, acc.x), fma(c, u.y, acc.y));
+    }
+    y[r] = acc;
   }
 }
 
@@ -213,12 +302,26 @@ static qs_status rebuild(qs_batch *B) {
     r2[2 * s] = (int)w2.size();
     if (B->used[s]) {
       const SlotDev &S = B->h_slots[s];
-      for (int64_t k = 0; k < S.nslices; ++k) w1.push_back({s, (int)k});
+      for (int64_t k = 0; k * QS_K1_GROUP < S.nslices; ++k) w1.push_back({s, (int)k});
       for (int64_t k = 0; k * QS_K2_CHUNK < S.n; ++k) w2.push_back({s, (int)k});
     }
     r1[2 * s + 1] = (int)w1.size();
     r2[2 * s + 1] = (int)w2.size();
+    // K1 partials are kept per slice position of an item
+    r1[2 * s] *= QS_K1_GROUP;
+    r1[2 * s + 1] *= QS_K1_GROUP;
   }
+  size_t lds = 0;
+  bool fast = true;
+  for (int s = 0; s < B->cap; ++s)
+    if (B->used[s]) {
+      const SellD &A = B->h_slots[s].A;
+      const size_t tb = term_table_bytes(A.nterms, A.nw);
+      if (tb <= QS_TERM_LDS_BYTES) lds = std::max(lds, tb);
+      fast = fast && A.nw == 1 && A.packed;
+    }
+  B->lds1 = lds;
+  B->fast1 = fast;
   auto grow = [&](WorkItem **p, int64_t *cap, int64_t need) -> qs_status {
     if (need > *cap) {
       if (*p) (void)hipFree(*p);
@@ -234,7 +337,7 @@ static qs_status rebuild(qs_batch *B) {
   QS_TRY(grow(&B->d_w2, &B->cap2, std::max<size_t>(1, w2.size())));
   if (B->cap1 != c1 || !B->d_p1) {
     if (B->d_p1) (void)hipFree(B->d_p1);
-    QS_TRY(qs_malloc(ctx, (void **)&B->d_p1, (size_t)B->cap1 * 8));
+    QS_TRY(qs_malloc(ctx, (void **)&B->d_p1, (size_t)B->cap1 * QS_K1_GROUP * 8));
   }
   if (B->cap2 != c2 || !B->d_p2) {
     if (B->d_p2) (void)hipFree(B->d_p2);
@@ -263,6 +366,8 @@ extern "C" qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out)
   B->h_slots.assign(max_slots, SlotDev{});
   B->used.assign(max_slots, 0);
   B->mem.assign(max_slots, nullptr);
+  B->kept.assign(max_slots, nullptr);
+  B->kept_bytes.assign(max_slots, 0);
   B->hops.assign(max_slots, nullptr);
   qs_status st = qs_malloc(ctx, (void **)&B->d_slots, (size_t)max_slots * sizeof(SlotDev));
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r1, (size_t)max_slots * 8);
@@ -282,6 +387,11 @@ extern "C" qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out)
 
 extern "C" qs_status qs_batch_add(qs_batch *B, const qs_hop *h, const double *v0, int depth, int max_depth,
                                   const double *ycoef, int *slot_out) {
+  return qs_batch_add_ex(B, h, v0, depth, max_depth, 0, ycoef, slot_out);
+}
+
+extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double *v0, int depth, int max_depth,
+                                     int flags, const double *ycoef, int *slot_out) {
   if (!B || !h || !v0 || depth < 1 || !slot_out) return QS_EINVAL;
   if (max_depth < depth) max_depth = depth;
   qs_ctx *ctx = B->ctx;
@@ -298,9 +408,41 @@ extern "C" qs_status qs_batch_add(qs_batch *B, const qs_hop *h, const double *v0
   const size_t vb = (size_t)std::max<int64_t>(n, 1) * 16;
   const bool want_y = ycoef != nullptr;
   const size_t sc = (size_t)(max_depth + 1) * 8;
+  if (flags & ~QS_BATCH_KEEP_BASIS) return qs_fail(ctx, QS_EINVAL, "unknown flags 0x%x", flags);
+  const bool keep = (flags & QS_BATCH_KEEP_BASIS) != 0;
   char *m = nullptr;
   QS_TRY(qs_malloc(ctx, (void **)&m, (want_y ? 3 : 2) * vb + 4 * sc + 256));
+  double2 *V = nullptr;
+  size_t vbytes = 0;
+  if (keep) {
+    const size_t need = (size_t)(max_depth + 1) * vb;
+    // the smallest pooled buffer that fits and wastes at most half of itself
+    int best = -1;
+    for (int i = 0; i < (int)B->pool.size(); ++i)
+      if (B->pool[i].first >= need && B->pool[i].first <= 2 * need &&
+          (best < 0 || B->pool[i].first < B->pool[best].first))
+        best = i;
+    if (best >= 0) {
+      vbytes = B->pool[best].first;
+      V = B->pool[best].second;
+      B->pool.erase(B->pool.begin() + best);
+    } else {
+      qs_status sv = qs_malloc(ctx, (void **)&V, need);
+      if (sv == QS_ENOMEM_DEVICE && !B->pool.empty()) {  // give the pool back and retry
+        (void)hipGetLastError();
+        for (auto &pb : B->pool) (void)hipFree(pb.second);
+        B->pool.clear();
+        sv = qs_malloc(ctx, (void **)&V, need);
+      }
+      if (sv != QS_OK) {
+        (void)hipFree(m);
+        return sv;
+      }
+      vbytes = need;
+    }
+  }
   SlotDev S{};
+  S.V = V;
   S.n = n;
   S.nslices = h->nslices;
   S.A = h->view;
@@ -334,11 +476,14 @@ extern "C" qs_status qs_batch_add(qs_batch *B, const qs_hop *h, const double *v0
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (st != QS_OK || e != hipSuccess) {
     (void)hipFree(m);
+    if (V) B->pool.push_back({vbytes, V});
     return st != QS_OK ? st : qs_hip_fail(ctx, e, "qs_batch_add", __FILE__, __LINE__);
   }
   B->h_slots[slot] = S;
   B->used[slot] = 1;
   B->mem[slot] = m;
+  B->kept[slot] = V;
+  B->kept_bytes[slot] = vbytes;
   B->hops[slot] = h;
   B->dirty = true;
   *slot_out = slot;
@@ -351,13 +496,20 @@ extern "C" qs_status qs_batch_run(qs_batch *B, int steps) {
   QS_HIP(ctx, hipSetDevice(ctx->device));
   if (B->dirty) QS_TRY(rebuild(B));
   if (B->n1 == 0) return QS_OK;
-  const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>((B->n1 + 3) / 4, (int64_t)ctx->num_cu * 8));
+  const int lds1 = (int)B->lds1;
+  const void *k1 = B->fast1 ? (const void *)k_b1<true> : (const void *)k_b1<false>;
+  const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>(B->n1, qs_resident_blocks(ctx, k1, (size_t)lds1)));
   const int g2 = (int)std::max<int64_t>(1, std::min<int64_t>(B->n2, (int64_t)ctx->num_cu * 4));
   for (int k = 0; k < steps; ++k) {
-    hipLaunchKernelGGL(k_b1, dim3(g1), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_w1, B->n1, B->d_p1);
-    hipLaunchKernelGGL(k_b_r1, dim3(B->cap), dim3(64), 0, ctx->stream, B->d_slots, B->cap, B->d_r1, B->d_p1);
+    if (B->fast1)
+      hipLaunchKernelGGL(k_b1<true>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1, B->n1,
+                         B->d_p1, lds1);
+    else
+      hipLaunchKernelGGL(k_b1<false>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1, B->n1,
+                         B->d_p1, lds1);
+    hipLaunchKernelGGL(k_b_r1, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r1, B->d_p1);
     hipLaunchKernelGGL(k_b2, dim3(g2), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_w2, B->n2, B->d_p2);
-    hipLaunchKernelGGL(k_b_r2, dim3(B->cap), dim3(64), 0, ctx->stream, B->d_slots, B->cap, B->d_r2, B->d_p2);
+    hipLaunchKernelGGL(k_b_r2, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r2, B->d_p2);
   }
   QS_HIP(ctx, hipGetLastError());
   if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -399,6 +551,27 @@ extern "C" qs_status qs_batch_poll(qs_batch *B, int slot, int *step, double *a, 
   return QS_OK;
 }
 
+extern "C" qs_status qs_batch_ritz(qs_batch *B, int slot, int m, const double *z, double *y) {
+  if (!B || slot < 0 || slot >= B->cap || !B->used[slot] || m < 0 || (m > 0 && (!z || !y))) return QS_EINVAL;
+  qs_ctx *ctx = B->ctx;
+  const SlotDev &H = B->h_slots[slot];
+  if (!H.V) return qs_fail(ctx, QS_EINVAL, "slot %d keeps no Lanczos vectors", slot);
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  SlotDev S;
+  QS_HIP(ctx, hipMemcpyAsync(&S, B->d_slots + slot, sizeof(SlotDev), hipMemcpyDeviceToHost, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (m > S.step) return qs_fail(ctx, QS_EINVAL, "Ritz vector over %d vectors, slot %d has %d", m, slot, S.step);
+  void *scr;
+  QS_TRY(qs_scratch(ctx, (size_t)std::max(m, 1) * 8, &scr));
+  if (m > 0) QS_HIP(ctx, hipMemcpyAsync(scr, z, (size_t)m * 8, hipMemcpyHostToDevice, ctx->stream));
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((H.n + QS_BLOCK - 1) / QS_BLOCK, (int64_t)ctx->num_cu * 8));
+  hipLaunchKernelGGL(k_b_ritz, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, m, (const double *)scr,
+                     (double2 *)y);
+  QS_HIP(ctx, hipGetLastError());
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
 extern "C" qs_status qs_batch_remove(qs_batch *B, int slot) {
   if (!B || slot < 0 || slot >= B->cap || !B->used[slot]) return QS_EINVAL;
   qs_ctx *ctx = B->ctx;
@@ -408,6 +581,17 @@ extern "C" qs_status qs_batch_remove(qs_batch *B, int slot) {
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   (void)hipFree(B->mem[slot]);
   B->mem[slot] = nullptr;
+  if (B->kept[slot]) {
+    // keep the buffer for later slots; at most 4 buffers stay pooled
+    B->pool.push_back({B->kept_bytes[slot], B->kept[slot]});
+    if (B->pool.size() > 4) {
+      auto smallest = std::min_element(B->pool.begin(), B->pool.end());
+      (void)hipFree(smallest->second);
+      B->pool.erase(smallest);
+    }
+  }
+  B->kept[slot] = nullptr;
+  B->kept_bytes[slot] = 0;
   B->used[slot] = 0;
   B->hops[slot] = nullptr;
   B->h_slots[slot] = SlotDev{};
@@ -445,6 +629,9 @@ extern "C" qs_status qs_batch_destroy(qs_batch *B) {
   if (B->ctx) (void)hipStreamSynchronize(B->ctx->stream);
   for (char *m : B->mem)
     if (m) (void)hipFree(m);
+  for (double2 *v : B->kept)
+    if (v) (void)hipFree(v);
+  for (auto &pb : B->pool) (void)hipFree(pb.second);
   if (B->d_slots) (void)hipFree(B->d_slots);
   if (B->d_w1) (void)hipFree(B->d_w1);
   if (B->d_w2) (void)hipFree(B->d_w2);

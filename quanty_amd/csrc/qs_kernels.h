@@ -11,7 +11,9 @@
 
 // Term tables (V_t and P_t) up to this size are staged in LDS by the
 // assembled kernels.
+#ifndef QS_TERM_LDS_BYTES
 #define QS_TERM_LDS_BYTES (24 * 1024)
+#endif
 
 // Wave-uniform copy of a value the compiler cannot prove uniform.
 __device__ __forceinline__ int64_t wave_uniform(int64_t v) {
@@ -77,22 +79,28 @@ __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, co
     }
   }
   // shift blocks: up to 64 headers are loaded at once (lane i holds header i)
-  // and broadcast one by one; the hit lanes of a block read x[base + lane]
+  // and broadcast one by one; the hit lanes of a block read x[base + lane].
+  // Groups of 4 headers with a fixed trip count let the compiler overlap their
+  // gathers (a loop of readlanes is not unrolled at a runtime trip count);
+  // lanes past the last header hold an empty mask.
   const uint64_t b0 = A.blk_ptr[s], b1 = A.blk_ptr[s + 1];
   for (uint64_t c0 = b0; c0 < b1; c0 += 64) {
     const int nb = (int)((b1 - c0) < 64 ? (b1 - c0) : 64);
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     u32x4 hv = {0u, 0u, 0u, 0u};
     if (lane < nb) hv = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(A.blk + c0 + lane));
-#pragma unroll 4
-    for (int k = 0; k < nb; ++k) {
-      const int32_t base = (int32_t)__builtin_amdgcn_readlane(hv.x, k);
-      const uint32_t t = __builtin_amdgcn_readlane(hv.y, k);
-      const uint64_t mask = ((uint64_t)__builtin_amdgcn_readlane(hv.w, k) << 32) |
-                            (uint32_t)__builtin_amdgcn_readlane(hv.z, k);
-      const bool hit = (mask >> lane) & 1u;
-      const int64_t c = hit ? (int64_t)base + lane : 0;
-      sell_acc<KC, NW>(hit, c, tv[t], tp + (size_t)t * NW, d, x, ldx, ar, ai);
+    for (int k0 = 0; k0 < nb; k0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + u;
+        const int32_t base = (int32_t)__builtin_amdgcn_readlane(hv.x, k);
+        const uint32_t t = __builtin_amdgcn_readlane(hv.y, k);
+        const uint64_t mask = ((uint64_t)__builtin_amdgcn_readlane(hv.w, k) << 32) |
+                              (uint32_t)__builtin_amdgcn_readlane(hv.z, k);
+        const bool hit = (mask >> lane) & 1u;
+        const int64_t c = hit ? (int64_t)base + lane : 0;
+        sell_acc<KC, NW>(hit, c, tv[t], tp + (size_t)t * NW, d, x, ldx, ar, ai);
+      }
     }
   }
   // ELL remainder, U positions per batch, software-pipelined: the stream

@@ -290,13 +290,16 @@ class Batch:
         self.n = {}
 
     def add(self, hop: Hop, v0: torch.Tensor, depth: int, ycoef: np.ndarray | None = None,
-            max_depth: int | None = None) -> int:
+            max_depth: int | None = None, keep_basis: bool = False) -> int:
+        """keep_basis: the slot keeps its Lanczos vectors ((max_depth+1) * n
+        complex on the device) for `ritz`."""
         slot = C.c_int()
         yc = None if ycoef is None else np.ascontiguousarray(ycoef, dtype=np.float64)
         md = depth if max_depth is None else max(depth, max_depth)
         torch.cuda.synchronize(v0.device)
-        self.ctx.check(self.lib.qs_batch_add(self.h, hop.h, ptr(v0), depth, md, ptr(yc), C.byref(slot)),
-                       "qs_batch_add")
+        flags = 1 if keep_basis else 0
+        self.ctx.check(self.lib.qs_batch_add_ex(self.h, hop.h, ptr(v0), depth, md, flags, ptr(yc), C.byref(slot)),
+                       "qs_batch_add_ex")
         self.depth[slot.value] = depth
         self.maxdepth = getattr(self, "maxdepth", {})
         self.maxdepth[slot.value] = md
@@ -317,6 +320,14 @@ class Batch:
         b = np.zeros(d + 1) if want_coef else None
         self.ctx.check(self.lib.qs_batch_poll(self.h, slot, C.byref(st), ptr(a), ptr(b), ptr(y)), "qs_batch_poll")
         return st.value, a, b
+
+    def ritz(self, slot: int, z: np.ndarray, y: torch.Tensor) -> torch.Tensor:
+        """y = sum_i z[i] v_i over the slot's kept Lanczos vectors."""
+        zz = np.ascontiguousarray(z, dtype=np.float64)
+        assert y.dtype == torch.complex128 and y.is_contiguous() and y.numel() == self.n[slot]
+        torch.cuda.synchronize(y.device)
+        self.ctx.check(self.lib.qs_batch_ritz(self.h, slot, len(zz), ptr(zz), ptr(y)), "qs_batch_ritz")
+        return y
 
     def remove(self, slot: int):
         self.ctx.check(self.lib.qs_batch_remove(self.h, slot), "qs_batch_remove")

@@ -2,11 +2,12 @@
 
 A request is one impurity spectrum: a seeded Anderson impurity (d shell plus
 bath shells, sizes from 10^3 to a few 10^6 determinants), its ground state
-(two-pass Lanczos: coefficients extended in steps until the lowest Ritz pair
-has |H psi - E psi|^2 <= tol max(1, E^2), then the Ritz vector rebuilt from the
-same start), T|psi0> for a random one-body d<->bath transition T, a Lanczos
-tridiagonalization of the requested depth, and the continued fraction on the
-grid. Requests are admitted as slots and HBM budget free up; every admitted
+(Lanczos coefficients extended in steps until the lowest Ritz pair has
+|H psi - E psi|^2 <= tol max(1, E^2); the Ritz vector is formed from the
+Lanczos vectors kept on the device, or, with keep_basis=False, rebuilt by a
+second pass from the same start), T|psi0> for a random one-body d<->bath
+transition T, a Lanczos tridiagonalization of the requested depth, and the
+continued fraction on the grid. Requests are admitted as slots and HBM budget free up; every admitted
 request is one slot of a `Batch`, so all of them advance together, one set of
 launches per Lanczos step. Bases, operators and vectors stay on the device
 from admission to completion. The plain baseline runs the same requests with
@@ -132,14 +133,16 @@ class _Job:
         self.phase = "gs1"
         self.slot = -1
         self.bytes = self.hop.stats()["stream_bytes"] + 6 * 16 * self.basis.n
+        self.kept_bytes = 0  # Lanczos vectors kept during the ground-state phase
 
     def release(self):
-        for k in ("hop", "thop", "H", "T", "basis", "v0"):
+        for k in ("hop", "thop", "H", "T", "basis", "v0", "gs_vector"):
             setattr(self, k, None)
 
 
-def estimate_bytes(dim: int) -> int:
-    return dim * (6 * 16 + 8 * 110)
+def estimate_bytes(dim: int, kept_vectors: int = 0) -> int:
+    """Device bytes of a request: vectors, assembled stream, kept Lanczos vectors."""
+    return dim * (6 * 16 + 8 * 110 + 16 * kept_vectors)
 
 
 class SpectrumServer:
@@ -148,7 +151,7 @@ class SpectrumServer:
 
     def __init__(self, ctx, max_slots: int = 64, hbm_budget: float = 200e9, gs_depth: int = 60,
                  gs_max: int = 500, gs_step: int = 40, gs_tol: float = 1e-12, chunk: int = 8, ne: int = 2000,
-                 emin: float = -10.0, emax: float = 20.0):
+                 emin: float = -10.0, emax: float = 20.0, keep_basis: bool = True):
         self.ctx = ctx
         self.max_slots = max_slots
         self.budget = hbm_budget
@@ -158,6 +161,7 @@ class SpectrumServer:
         self.gs_tol = gs_tol
         self.chunk = chunk
         self.ne, self.emin, self.emax = ne, emin, emax
+        self.keep_basis = keep_basis
         self.batch = Batch(max_slots, ctx)
         self.jobs: dict[int, _Job] = {}
         self.used = 0
@@ -174,19 +178,26 @@ class SpectrumServer:
             if res ** 2 > self.gs_tol * max(1.0, E ** 2) and m < job.gs_max:
                 self.batch.set_depth(job.slot, min(job.gs_max, m + self.gs_step))
                 return False
-            self.batch.remove(job.slot)
             r.E0 = float(w[0])
             r.gs_residual = res
             r.gs_steps = m
-            job.slot = self.batch.add(job.hop, job.v0, m, ycoef=z)
-            job.gs_depth = m
-            job.phase = "gs2"
-            return False
-        if job.phase == "gs2":
-            y = job.basis.zeros()
-            self.batch.poll(job.slot, want_coef=False, y=y)
+            if not self.keep_basis:
+                self.batch.remove(job.slot)
+                job.slot = self.batch.add(job.hop, job.v0, m, ycoef=z)
+                job.gs_depth = m
+                job.phase = "gs2"
+                return False
+            y = self.batch.ritz(job.slot, z, job.basis.zeros())
             self.batch.remove(job.slot)
+            self.used -= job.kept_bytes
+            job.kept_bytes = 0
+        if job.phase in ("gs1", "gs2"):
+            if job.phase == "gs2":
+                y = job.basis.zeros()
+                self.batch.poll(job.slot, want_coef=False, y=y)
+                self.batch.remove(job.slot)
             y /= torch.linalg.vector_norm(y)
+            job.gs_vector = y  # the normalized ground state (device)
             tpsi = job.thop.apply(y.contiguous())
             job.slot = self.batch.add(job.hop, tpsi, r.depth)
             job.phase = "spec"
@@ -213,15 +224,19 @@ class SpectrumServer:
             # admission: FIFO, by free slots and HBM budget
             while waiting and len(self.jobs) < (1 if plain else self.max_slots):
                 r = waiting[0]
-                if self.jobs and self.used + estimate_bytes(r.dim) > self.budget:
+                kept = self.gs_max + 1 if self.keep_basis else 0
+                if self.jobs and self.used + estimate_bytes(r.dim, kept) > self.budget:
                     break
                 waiting.pop(0)
                 r.t_admit = time.perf_counter() - t0
                 job = _Job(r, self.ctx, self.gs_depth)
                 job.gs_max = min(self.gs_max, job.basis.n)
-                job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max)
+                job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max,
+                                          keep_basis=self.keep_basis)
+                if self.keep_basis:
+                    job.kept_bytes = 16 * (job.gs_max + 1) * job.basis.n
                 self.jobs[r.rid] = job
-                self.used += job.bytes
+                self.used += job.bytes + job.kept_bytes
             if not self.jobs:
                 if i < len(pending) and realtime:
                     time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))

@@ -56,12 +56,63 @@ def test_batch_matches_single_and_is_mix_invariant():
         assert np.allclose(a, oa, rtol=1e-10, atol=1e-12) and np.allclose(bb[:d], ob, rtol=1e-10)
 
 
+def test_batch_mixed_layouts():
+    """Slots with unpacked entries and with two-word (76-mode) determinants
+    next to ordinary slots: every slot matches its single-problem Lanczos and
+    the ordinary slots are bit-identical to a run without the others."""
+    from quanty_amd import default_context
+    from quanty_amd._lib import QS_PARAM_ELL_PACK, QS_PARAM_SHIFT_MIN
+
+    ctx = default_context()
+    plain = [_setup(kw) for kw in SHAPES[:2]]
+    ctx.set_param(QS_PARAM_ELL_PACK, 0)
+    ctx.set_param(QS_PARAM_SHIFT_MIN, 8)
+    try:
+        unpacked = _setup(SHAPES[3])
+    finally:
+        ctx.set_param(QS_PARAM_ELL_PACK, 1)
+        ctx.set_param(QS_PARAM_SHIFT_MIN, 56)
+    assert not unpacked[2].stats()["packed"]
+    # 76 modes: two-word determinants, a p-d hopping operator on a small sector
+    nf = 76
+    restr = [("1" * nf, 3, 3), ("1" * 20 + "0" * 56, 1, 2), ("0" * 10 + "1" * 60 + "0" * 6, 1, 2)]
+    b76 = Basis.enumerate(nf, restr)
+    assert b76.nw == 2 and b76.n == 26520
+    rng = np.random.default_rng(3)
+    terms = []
+    for _ in range(60):
+        i, j = (int(q) for q in rng.integers(0, nf, 2))
+        if i != j:
+            v = complex(*rng.normal(size=2))
+            terms += [((0x8000 | i, j)[::-1], v), ((0x8000 | j, i)[::-1], v.conjugate())]
+    h76 = Hop(Op(nf, terms), b76, mode="assembled")
+    g = np.random.default_rng(76)
+    v76 = torch.from_numpy(g.normal(size=b76.n) + 1j * g.normal(size=b76.n)).cuda()
+    probs = plain + [unpacked, (None, b76, h76, v76)]
+    depths = [30, 25, 35, 20]
+    mixed = Batch(8)
+    slots = [mixed.add(hop, v0, d) for (_, _, hop, v0), d in zip(probs, depths)]
+    mixed.run(40)
+    for k, ((_, b, hop, v0), d, s) in enumerate(zip(probs, depths, slots)):
+        st, a, bb = mixed.poll(s)
+        assert st == d
+        ra, rb = hop.lanczos(v0, d)
+        assert np.allclose(a, ra, rtol=1e-10, atol=1e-12)
+        assert np.allclose(bb[:d], rb, rtol=1e-10, atol=0)
+        if k < 2:
+            alone = Batch(2)
+            s1 = alone.add(hop, v0, d)
+            alone.run(d)
+            _, a1, b1 = alone.poll(s1)
+            assert np.array_equal(a, a1) and np.array_equal(bb, b1)
+
+
 def test_two_pass_ritz_vector():
     """Coefficients, then the Ritz vector rebuilt from the same start: the
     lowest eigenpair of the tridiagonal matrix becomes an eigenvector of H."""
     pr, b, hop, v0 = _setup(SHAPES[0])
     m = 300
-    bt = Batch(2)
+    bt = Batch(3)
     s = bt.add(hop, v0, m)
     bt.run(m)
     _, a, bb = bt.poll(s)
@@ -74,6 +125,13 @@ def test_two_pass_ritz_vector():
     bt.run(m)
     y = b.zeros()
     bt.poll(s2, want_coef=False, y=y)
+    # the same Ritz vector from the kept Lanczos vectors of a one-pass slot
+    s3 = bt.add(hop, v0, m, keep_basis=True)
+    bt.run(m)
+    _, a3, b3 = bt.poll(s3)
+    assert np.array_equal(a3, a) and np.array_equal(b3, bb)
+    yk = bt.ritz(s3, z, b.zeros())
+    assert torch.equal(yk, y)
     y /= torch.linalg.vector_norm(y)
     hy = hop.apply(y.contiguous())
     res = torch.linalg.vector_norm(hy - w[0] * y).item()
@@ -108,14 +166,13 @@ def test_server_spectrum_matches_oracle():
     orig = srv._advance
 
     def spy(job):
-        if job.phase == "gs2":
+        out = orig(job)
+        if job.phase == "spec" and "psi" not in keep:
             keep["basis"] = job.basis.words()
-            y = job.basis.zeros()
-            srv.batch.poll(job.slot, want_coef=False, y=y)
-            keep["psi"] = (y / torch.linalg.vector_norm(y)).cpu().numpy()
+            keep["psi"] = job.gs_vector.cpu().numpy()
             keep["nf"] = job.nf
             keep["req"] = job.req
-        return orig(job)
+        return out
 
     srv._advance = spy
     (r,) = srv.serve(reqs, realtime=False)
@@ -130,3 +187,20 @@ def test_server_spectrum_matches_oracle():
     want = oracle.cf(a, b, r.E0, srv.emin, srv.emax, srv.ne, r.gamma)
     assert np.max(np.abs(r.spectrum - want)) <= 1e-8 * np.max(np.abs(want))
     assert r.gs_residual ** 2 <= 1e-12 * max(1.0, r.E0 ** 2)
+
+
+def test_server_kept_basis_equals_second_pass():
+    """The ground state formed from the kept Lanczos vectors gives the same
+    spectra as the two-pass rebuild."""
+    from quanty_amd import default_context
+    reqs = serve.request_stream(6, rate=0.0, seed=5, dim_lo=1e2, dim_hi=2e4)
+    kept = {r.rid: r for r in serve.SpectrumServer(default_context(), max_slots=8, gs_depth=40,
+                                                   chunk=5).serve(reqs, realtime=False)}
+    reqs2 = serve.request_stream(6, rate=0.0, seed=5, dim_lo=1e2, dim_hi=2e4)
+    two = {r.rid: r for r in serve.SpectrumServer(default_context(), max_slots=8, gs_depth=40, chunk=5,
+                                                  keep_basis=False).serve(reqs2, realtime=False)}
+    assert len(kept) == len(two) == 6
+    for rid, r in kept.items():
+        assert r.E0 == two[rid].E0 and r.gs_steps == two[rid].gs_steps
+        assert np.array_equal(r.spectrum, two[rid].spectrum), rid
+
