@@ -161,12 +161,15 @@ class Hop:
                        shift_min=sm.value)
         return out
 
-    def apply(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+    def apply(self, x: torch.Tensor, y: torch.Tensor | None = None, ctx: Context | None = None) -> torch.Tensor:
+        """y = O x. ctx: the context (stream) to run on, when not the one the
+        binding was made with (a context serves one host thread at a time)."""
         assert x.dtype == torch.complex128 and x.is_contiguous() and x.numel() == self.cols.n
         if y is None:
             y = self.rows.zeros()
+        c = ctx or self.ctx
         torch.cuda.synchronize(x.device)
-        self.ctx.check(self.lib.qs_hop_apply(self.ctx.h, self.h, ptr(x), ptr(y)), "qs_hop_apply")
+        c.check(self.lib.qs_hop_apply(c.h, self.h, ptr(x), ptr(y)), "qs_hop_apply")
         return y
 
     def lanczos(self, v0: torch.Tensor, nsteps: int):

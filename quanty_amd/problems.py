@@ -189,3 +189,64 @@ def random_one_body_transition(rng: SplitMix64, dmodes, bath_modes, sigma=1.0):
     M1 = np.array([[rng.cgauss(sigma) for _ in bath_modes] for _ in dmodes])
     M2 = np.array([[rng.cgauss(sigma) for _ in dmodes] for _ in bath_modes])
     return O.add(O.one_body(dmodes, bath_modes, M1), O.one_body(bath_modes, dmodes, M2))
+
+
+# ---------------------------------------------------------------------------
+# 76 modes (two-word determinants): 2p core + d shell + 6 bath shells
+# ---------------------------------------------------------------------------
+def large76(seed: int = 1, params=None, nd: int = 8):
+    """L2,3 edge of a d impurity with six bath shells: 2p core (modes 0-5), 3d
+    (6-15), B1-B3 nominally full (16-45), B4-B6 nominally empty (46-75), N = 44.
+    Sectors: 2p6 (ground state, 10,347,855 dets) and 2p5 (15,330,840 dets), both
+    with d occupation 6-10, at most 2 holes in B1-B3 and at most 2 electrons in
+    B4-B6. H_i: dd Coulomb, 3d spin-orbit and 10Dq of the Ni2+ parameters, bath
+    energies U(-5, 5), d<->bath hybridizations r e^{i phi} (r U(0.5, 2)) diagonal
+    in the orbital, complex Gaussian intra-bath hoppings (sigma 0.3). H_f adds
+    the 2p-3d Coulomb (F0, F2; G1, G3), 2p spin-orbit and a core energy. T is
+    the z-polarized 2p -> 3d dipole."""
+    p = dict(NI_PARAMS)
+    p.update(params or {})
+    rng = SplitMix64(seed)
+    NF = 76
+    up_p, dn_p = shell(0, 1)
+    up_d, dn_d = shell(6, 2)
+    pmodes = O.shell_modes(up_p, dn_p)
+    dmodes = O.shell_modes(up_d, dn_d)
+    F0dd = p["Udd"] + (p["F2dd"] + p["F4dd"]) * 2 / 63
+    F0pd = p["Upd"] + p["G1pd"] / 15 + 3 * p["G3pd"] / 70
+    eg, t2g = O.oh_d_projectors()
+    terms = [O.coulomb(up_d, dn_d, [F0dd, p["F2dd"], p["F4dd"]]),
+             O.shell_operator(up_d, dn_d, "ldots", p["zeta_3d"]),
+             O.one_body(dmodes, dmodes, p["tenDq"] * O.d_shell_matrix_to_spin(0.6 * eg - 0.4 * t2g)),
+             O.number(dmodes, [-nd * p["Udd"] / 2] * 10)]
+    baths = []
+    for b in range(6):
+        up_b, dn_b = shell(16 + 10 * b, 2)
+        bm = O.shell_modes(up_b, dn_b)
+        baths.append(bm)
+        terms.append(O.number(bm, [rng.uniform(-5.0, 5.0) for _ in range(10)]))
+        V = np.zeros((10, 10), dtype=complex)
+        for i in range(10):
+            r = rng.uniform(0.5, 2.0)
+            ph = rng.uniform(0.0, 2 * math.pi)
+            V[i, i] = r * complex(math.cos(ph), math.sin(ph))
+        terms.append(O.one_body(dmodes, bm, V))
+        terms.append(O.one_body(bm, dmodes, V.conj().T))
+        T = np.zeros((10, 10), dtype=complex)
+        for i in range(10):
+            for j in range(i + 1, 10):
+                z = rng.cgauss(0.3)
+                T[i, j] = z
+                T[j, i] = z.conjugate()
+        terms.append(O.one_body(bm, bm, T))
+    Hi = O.add(*terms)
+    Hf = O.add(Hi, O.shell_operator(up_p, dn_p, "ldots", p["zeta_2p"]), O.number(pmodes, [-p["Upd"] * 2] * 6),
+               O.coulomb_two_shells(up_p, dn_p, up_d, dn_d, [F0pd, p["F2pd"]], [p["G1pd"], p["G3pd"]]))
+    Tz = O.dipole_p_to_d(up_d, dn_d, up_p, dn_p, [(1, 0, 1.0)])
+    full = [m for bm in baths[:3] for m in bm]
+    empty = [m for bm in baths[3:] for m in bm]
+    rest = [(mask(NF, dmodes), 6, 10), (mask(NF, full), 28, 30), (mask(NF, empty), 0, 2)]
+    gs = [(mask(NF, range(NF)), 44, 44), (mask(NF, pmodes), 6, 6)] + rest
+    fs = [(mask(NF, range(NF)), 44, 44), (mask(NF, pmodes), 5, 5)] + rest
+    return dict(NF=NF, Hi=Hi, Hf=Hf, T=Tz, gs_restrictions=gs, fs_restrictions=fs, dmodes=dmodes, baths=baths,
+                pmodes=pmodes, seed=seed)

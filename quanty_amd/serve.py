@@ -18,6 +18,8 @@ from __future__ import annotations
 import json
 import math
 import os
+import queue
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -68,8 +70,10 @@ def impurity_family():
     return out
 
 
-def pick_shape(target_dim: float):
-    fam = impurity_family()
+def pick_shape(target_dim: float, nbath_max: int = 4):
+    """The impurity shape (at most nbath_max bath shells) whose sector size is
+    closest to target_dim on a log scale."""
+    fam = [t for t in impurity_family() if t[1]["nbath"] <= nbath_max]
     best = min(fam, key=lambda t: abs(math.log(t[0]) - math.log(target_dim)))
     return best
 
@@ -95,15 +99,16 @@ class Request:
 
 
 def request_stream(n: int, rate: float, seed: int = 7, dim_lo=1e3, dim_hi=3e6, depth=(100, 400),
-                   gamma=(0.1, 0.5)) -> list[Request]:
-    """Poisson arrivals at `rate` req/s; dim log-uniform; depth U{lo..hi}; Gamma U(lo, hi)."""
+                   gamma=(0.1, 0.5), nbath_max: int = 4) -> list[Request]:
+    """Poisson arrivals at `rate` req/s (all at t = 0 when rate is 0); dim
+    log-uniform; depth U{lo..hi}; Gamma U(lo, hi); at most nbath_max bath shells."""
     rng = P.SplitMix64(seed)
     t = 0.0
     out = []
     for i in range(n):
         t += -math.log(max(rng.uniform(), 1e-300)) / rate if rate > 0 else 0.0
         target = math.exp(rng.uniform(math.log(dim_lo), math.log(dim_hi)))
-        dim, shape = pick_shape(target)
+        dim, shape = pick_shape(target, nbath_max)
         d = depth[0] + int(rng.uniform() * (depth[1] - depth[0] + 1))
         g = rng.uniform(*gamma)
         out.append(Request(arrival=t, rid=i, seed=int(rng.next() & 0x7FFFFFFF), shape=shape, dim=dim, depth=d, gamma=g))
@@ -140,6 +145,42 @@ class _Job:
             setattr(self, k, None)
 
 
+class _Prep:
+    """Builds the state of admitted requests (basis, operators, assembled
+    stream, start vector) on worker threads, each with its own library context,
+    so that the preparation overlaps the batch's GPU steps."""
+
+    def __init__(self, device: int, gs_depth: int, workers: int = 1):
+        from ._lib import Context
+        self.gs_depth = gs_depth
+        self.todo: queue.Queue = queue.Queue()
+        self.ready: queue.Queue = queue.Queue()
+        self.ctxs = [Context(device) for _ in range(max(1, workers))]
+        self.threads = [threading.Thread(target=self._run, args=(c,), daemon=True) for c in self.ctxs]
+        for t in self.threads:
+            t.start()
+
+    def _run(self, ctx):
+        while True:
+            r = self.todo.get()
+            if r is None:
+                return
+            try:
+                job = _Job(r, ctx, self.gs_depth)
+                torch.cuda.synchronize()
+                self.ready.put(job)
+            except BaseException as e:  # handed to the serving loop, which raises it
+                self.ready.put(e)
+
+    def close(self):
+        for _ in self.threads:
+            self.todo.put(None)
+        for t in self.threads:
+            t.join()
+        for c in self.ctxs:
+            c.close()
+
+
 def estimate_bytes(dim: int, kept_vectors: int = 0) -> int:
     """Device bytes of a request: vectors, assembled stream, kept Lanczos vectors."""
     return dim * (6 * 16 + 8 * 110 + 16 * kept_vectors)
@@ -151,7 +192,8 @@ class SpectrumServer:
 
     def __init__(self, ctx, max_slots: int = 64, hbm_budget: float = 200e9, gs_depth: int = 60,
                  gs_max: int = 500, gs_step: int = 40, gs_tol: float = 1e-12, chunk: int = 8, ne: int = 2000,
-                 emin: float = -10.0, emax: float = 20.0, keep_basis: bool = True):
+                 emin: float = -10.0, emax: float = 20.0, keep_basis: bool = True, prepare_ahead: bool = True,
+                 prep_workers: int = 2):
         self.ctx = ctx
         self.max_slots = max_slots
         self.budget = hbm_budget
@@ -162,6 +204,8 @@ class SpectrumServer:
         self.chunk = chunk
         self.ne, self.emin, self.emax = ne, emin, emax
         self.keep_basis = keep_basis
+        self.prepare_ahead = prepare_ahead
+        self.prep_workers = prep_workers
         self.batch = Batch(max_slots, ctx)
         self.jobs: dict[int, _Job] = {}
         self.used = 0
@@ -198,7 +242,7 @@ class SpectrumServer:
                 self.batch.remove(job.slot)
             y /= torch.linalg.vector_norm(y)
             job.gs_vector = y  # the normalized ground state (device)
-            tpsi = job.thop.apply(y.contiguous())
+            tpsi = job.thop.apply(y.contiguous(), ctx=self.ctx)
             job.slot = self.batch.add(job.hop, tpsi, r.depth)
             job.phase = "spec"
             return False
@@ -208,51 +252,79 @@ class SpectrumServer:
         r.spectrum = cf_spectrum(a, b[:r.depth], r.E0, self.emin, self.emax, self.ne, r.gamma, self.ctx)[0]
         return True
 
+    def _start(self, job: _Job):
+        """Put a prepared job into the batch."""
+        job.gs_max = min(self.gs_max, job.basis.n)
+        job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max, keep_basis=self.keep_basis)
+        if self.keep_basis:
+            job.kept_bytes = 16 * (job.gs_max + 1) * job.basis.n
+        self.jobs[job.req.rid] = job
+        self.used += job.bytes + job.kept_bytes
+
     def serve(self, requests: list[Request], realtime: bool = True, plain: bool = False) -> list[Request]:
         """Run the stream. realtime: arrivals are honoured on the wall clock;
-        plain: one request at a time (the baseline)."""
+        plain: one request at a time, prepared when its turn comes (the
+        baseline). Otherwise admitted requests are prepared on a worker thread
+        while the batch steps."""
         pending = sorted(requests)
         waiting: list[Request] = []
         done: list[Request] = []
+        prep = _Prep(self.ctx.device, self.gs_depth, self.prep_workers) if (self.prepare_ahead and not plain) \
+            else None
+        inflight: dict[int, int] = {}  # rid -> reserved bytes of requests being prepared
+        kept = self.gs_max + 1 if self.keep_basis else 0
+        slots = 1 if plain else self.max_slots
         t0 = time.perf_counter()
         i = 0
-        while i < len(pending) or waiting or self.jobs:
-            now = time.perf_counter() - t0
-            while i < len(pending) and (not realtime or pending[i].arrival <= now):
-                waiting.append(pending[i])
-                i += 1
-            # admission: FIFO, by free slots and HBM budget
-            while waiting and len(self.jobs) < (1 if plain else self.max_slots):
-                r = waiting[0]
-                kept = self.gs_max + 1 if self.keep_basis else 0
-                if self.jobs and self.used + estimate_bytes(r.dim, kept) > self.budget:
-                    break
-                waiting.pop(0)
-                r.t_admit = time.perf_counter() - t0
-                job = _Job(r, self.ctx, self.gs_depth)
-                job.gs_max = min(self.gs_max, job.basis.n)
-                job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max,
-                                          keep_basis=self.keep_basis)
-                if self.keep_basis:
-                    job.kept_bytes = 16 * (job.gs_max + 1) * job.basis.n
-                self.jobs[r.rid] = job
-                self.used += job.bytes + job.kept_bytes
-            if not self.jobs:
-                if i < len(pending) and realtime:
-                    time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
-                continue
-            self.batch.run(self.chunk)
-            steps = self.batch.steps(self.max_slots)
-            for rid in list(self.jobs):
-                job = self.jobs[rid]
-                depth = job.gs_depth if job.phase in ("gs1", "gs2") else job.req.depth
-                if steps[job.slot] >= depth:
-                    if self._advance(job):
-                        job.req.t_done = time.perf_counter() - t0
-                        done.append(job.req)
-                        self.used -= job.bytes
-                        job.release()
-                        del self.jobs[rid]
+        try:
+            while i < len(pending) or waiting or self.jobs or inflight:
+                now = time.perf_counter() - t0
+                while i < len(pending) and (not realtime or pending[i].arrival <= now):
+                    waiting.append(pending[i])
+                    i += 1
+                # admission: FIFO, by free slots and HBM budget
+                while waiting and len(self.jobs) + len(inflight) < slots:
+                    r = waiting[0]
+                    est = estimate_bytes(r.dim, kept)
+                    if (self.jobs or inflight) and self.used + est > self.budget:
+                        break
+                    waiting.pop(0)
+                    r.t_admit = time.perf_counter() - t0
+                    if prep is None:
+                        self._start(_Job(r, self.ctx, self.gs_depth))
+                    else:
+                        inflight[r.rid] = est
+                        self.used += est
+                        prep.todo.put(r)
+                # prepared jobs join the batch; wait for one when nothing else can run
+                while prep is not None and inflight:
+                    try:
+                        job = prep.ready.get(block=not self.jobs, timeout=None if not self.jobs else 0)
+                    except queue.Empty:
+                        break
+                    if isinstance(job, BaseException):
+                        raise job
+                    self.used -= inflight.pop(job.req.rid)
+                    self._start(job)
+                if not self.jobs:
+                    if i < len(pending) and realtime and not inflight:
+                        time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
+                    continue
+                self.batch.run(self.chunk)
+                steps = self.batch.steps(self.max_slots)
+                for rid in list(self.jobs):
+                    job = self.jobs[rid]
+                    depth = job.gs_depth if job.phase in ("gs1", "gs2") else job.req.depth
+                    if steps[job.slot] >= depth:
+                        if self._advance(job):
+                            job.req.t_done = time.perf_counter() - t0
+                            done.append(job.req)
+                            self.used -= job.bytes
+                            job.release()
+                            del self.jobs[rid]
+        finally:
+            if prep is not None:
+                prep.close()
         return done
 
 

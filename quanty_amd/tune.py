@@ -1,0 +1,99 @@
+"""Autotuner for the assembled operator layout.
+
+The assembled H*psi stores, per 64-row slice, shift blocks (one 16-byte header
+for a term whose rows read consecutive columns) and per-entry ELL slots. Which
+groups become blocks is set by QS_PARAM_SHIFT_MIN; the best value depends on
+the operator's structure. `tune_shift_min` assembles the operator for each
+candidate, times H*psi with HIP events on the library's stream and returns the
+fastest (optionally leaving it set on the context).
+
+    python3 -m quanty_amd.tune            # the 50-mode Anderson impurity, 2,179,235 dets
+"""
+from __future__ import annotations
+
+import argparse
+import time
+
+import numpy as np
+import torch
+
+from ._lib import QS_PARAM_SHIFT_MIN
+from .engine import Basis, Hop, Op
+
+CANDIDATES = (65, 64, 56, 48, 32, 24, 16, 8)
+
+
+def time_apply(hop: Hop, x: torch.Tensor, y: torch.Tensor, reps: int) -> float:
+    """Average H*psi time in microseconds over `reps` back-to-back launches."""
+    ctx = hop.ctx
+    stream = torch.cuda.ExternalStream(ctx.lib.qs_ctx_stream(ctx.h), device=x.device)
+    torch.cuda.synchronize()
+    ctx.lib.qs_ctx_set_async(ctx.h, 1)
+    try:
+        ctx.check(ctx.lib.qs_hop_apply(ctx.h, hop.h, x.data_ptr(), y.data_ptr()), "qs_hop_apply")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.check(ctx.lib.qs_hop_apply(ctx.h, hop.h, x.data_ptr(), y.data_ptr()), "qs_hop_apply")
+        e1.record(stream)
+        e1.synchronize()
+    finally:
+        ctx.lib.qs_ctx_set_async(ctx.h, 0)
+    return e0.elapsed_time(e1) / reps * 1e3
+
+
+def tune_shift_min(op: Op, basis: Basis, candidates=CANDIDATES, reps: int = 10, apply: bool = True,
+                   seed: int = 1):
+    """Time H*psi of `op` on `basis` for each shift-block threshold.
+
+    Returns (best threshold, rows) with one dict per candidate: threshold,
+    microseconds, blocks, entries in blocks, ELL slots, stream bytes. With
+    apply=True the best threshold stays set on the operator's context."""
+    ctx = op.ctx
+    old = ctx.get_param(QS_PARAM_SHIFT_MIN)
+    g = np.random.default_rng(seed)
+    x = torch.from_numpy(g.normal(size=basis.n) + 1j * g.normal(size=basis.n)).cuda()
+    y = basis.zeros()
+    rows = []
+    try:
+        for sm in candidates:
+            ctx.set_param(QS_PARAM_SHIFT_MIN, sm)
+            hop = Hop(op, basis, mode="assembled")
+            st = hop.stats()
+            us = time_apply(hop, x, y, reps)
+            rows.append({"shift_min": sm, "us": us, "nblocks": st["nblocks"], "block_hits": st["block_hits"],
+                         "ell_slots": st["ell_slots"], "stream_bytes": st["stream_bytes"]})
+            del hop
+    finally:
+        ctx.set_param(QS_PARAM_SHIFT_MIN, old)
+    best = min(rows, key=lambda r: r["us"])["shift_min"]
+    if apply:
+        ctx.set_param(QS_PARAM_SHIFT_MIN, best)
+    return best, rows
+
+
+def main():
+    from . import default_context
+    from . import problems as P
+
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--reps", type=int, default=20)
+    args = ap.parse_args()
+    ctx = default_context()
+    pr = P.anderson(seed=args.seed)
+    t0 = time.perf_counter()
+    basis = Basis.enumerate(pr["NF"], pr["restrictions"], ctx)
+    op = Op(pr["NF"], pr["H"], ctx)
+    best, rows = tune_shift_min(op, basis, reps=args.reps, apply=False, seed=args.seed)
+    print(f"50-mode Anderson impurity, {basis.n} determinants, {len(pr['H'])} terms "
+          f"({time.perf_counter() - t0:.1f} s); fastest threshold {best}")
+    print("| threshold | H*psi us | blocks | entries in blocks | ELL slots | stream MB |")
+    print("|---|---|---|---|---|---|")
+    for r in rows:
+        print(f"| {r['shift_min']} | {r['us']:.1f} | {r['nblocks']:,} | {r['block_hits']:,} | {r['ell_slots']:,} | "
+              f"{r['stream_bytes'] / 1e6:.1f} |")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,174 @@
+"""Timed runs of the benchmark configurations on one GPU.
+
+    python3 -m quanty_amd.workloads --config 0    # Ni2+ L2,3 XAS: 3 states x 3 dipoles, 200-step CF
+    python3 -m quanty_amd.workloads --config 1    # 50-mode impurity: 3 lowest states, tol 1e-12
+    python3 -m quanty_amd.workloads --config 2a   # 9 right-hand sides, 300-step block Lanczos
+    python3 -m quanty_amd.workloads --config 2b   # 512 small impurity problems, batched vs one at a time
+    python3 -m quanty_amd.workloads --config 4    # 76-mode (two-word) L2,3 Green's function
+
+Each run prints one JSON line: what was run, its sizes, wall times (HIP work
+synchronized on both sides) and the derived rates. Accuracy against the oracle
+is checked by the test suite, not here.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import default_context
+from . import problems as P
+from .engine import Basis, Hop, Op, block_cf_spectrum, cf_spectrum
+
+
+def _log(msg):
+    print(f"[workloads] {msg}", file=sys.stderr, flush=True)
+
+
+def _sync_time(f, what: str = ""):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = f()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    if what:
+        _log(f"{what}: {t:.3f} s")
+    return out, t
+
+
+def config0(reps: int = 5):
+    """Ni2+ Oh charge-transfer L2,3 XAS through the CreateSpectra mirror: the 3
+    lowest states of H_i (190 dets), the x/y/z dipoles into the 2p5 sector (120
+    dets), 200-step Lanczos and the continued fraction on 2001 points."""
+    from .api import CreateSpectra, Eigensystem, Operator
+
+    ni = P.ni_xas()
+    Hi, Hf = Operator(26, ni["Hi"]), Operator(26, ni["Hf"])
+    Ts = [Operator(26, t) for t in ni["T"]]
+    start = [26, 0, ["000000 1111111111 0000000000", 8, 8], ["111111 0000000000 1111111111", 16, 16]]
+    opts = [["Emin", -10.0], ["Emax", 20.0], ["NE", 2000], ["Gamma", 0.4], ["NTri", 200], ["DenseBorder", 0]]
+
+    def run():
+        psis = Eigensystem(Hi, start, 3, [["Zero", 1e-14]])
+        return CreateSpectra(Hf, Ts, psis, opts)
+
+    run()
+    _, t = _sync_time(lambda: [run() for _ in range(reps)])
+    t /= reps
+    return {"config": "0", "what": "Ni2+ L2,3 XAS: Eigensystem (3 states, 190 dets) + CreateSpectra (3 dipoles x "
+                                   "3 states, 120-det final sector, NTri 200, 2001 points, Gamma 0.4)",
+            "seconds": t, "spectra": 9, "spectra_per_s": 9 / t}
+
+
+def config1(seed: int = 1, nstates: int = 3, tol: float = 1e-12, max_krylov: int = 500):
+    """The 3 lowest states of the 50-mode Anderson impurity (2,179,235 dets)."""
+    ctx = default_context()
+    pr = P.anderson(seed=seed)
+    (b, hop), t_setup = _sync_time(lambda: (lambda b: (b, Hop(Op(pr["NF"], pr["H"], ctx), b, mode="assembled")))(
+        Basis.enumerate(pr["NF"], pr["restrictions"], ctx)))
+    (E, vecs, var, it), t = _sync_time(lambda: hop.ground_states(nstates, tol=tol, max_krylov=max_krylov),
+                                       f"{nstates} lowest states")
+    return {"config": "1", "what": f"{nstates} lowest states, tol {tol} on the variance, Krylov <= {max_krylov}",
+            "dim": b.n, "setup_s": t_setup, "seconds": t, "E": [float(e) for e in E],
+            "variance": [float(v) for v in var], "iterations": int(it)}, (pr, b, hop, vecs)
+
+
+def config2a(seed: int = 1, nsteps: int = 300, state=None):
+    """9 right-hand sides (3 random one-body d<->bath transitions x the 3 lowest
+    states of the 50-mode impurity) in one 300-step block Lanczos, then the
+    9x9 tensor continued fraction on 2001 points."""
+    ctx = default_context()
+    if state is None:
+        _, state = config1(seed)
+    pr, b, hop, vecs = state
+    rng = P.SplitMix64(seed + 2000)
+    bath = [m for bb in pr["baths"] for m in bb]
+    Ts = [Hop(Op(pr["NF"], P.random_one_body_transition(rng, pr["dmodes"], bath), ctx), b, mode="matfree")
+          for _ in range(3)]
+    X = torch.stack([T.apply(vecs[j].contiguous()) for T in Ts for j in range(3)])
+    (A, B, kept), t = _sync_time(lambda: hop.block_lanczos(X, nsteps))
+    _, t_cf = _sync_time(lambda: block_cf_spectrum(A, B, kept, 0.0, -10.0, 20.0, 2000, 0.4, ctx))
+    return {"config": "2a", "what": f"block Lanczos, block size 9 (3 T x 3 states), {nsteps} steps, then the "
+                                    "tensor CF on 2001 points", "dim": b.n, "seconds_lanczos": t,
+            "seconds_cf": t_cf, "kept_min": int(kept[1:].min()), "spectra": 9, "spectra_per_s": 9 / (t + t_cf)}
+
+
+def config2b(n: int = 512, seed: int = 21, slots: int = 64):
+    """512 impurity problems (d + 1..3 bath shells, dims log-uniform 10^2..10^5),
+    each a ground state plus one 200-step spectrum: all submitted at once to the
+    continuously batched server, and the same problems one at a time."""
+    from . import serve
+
+    ctx = default_context()
+    warm = serve.request_stream(4, 0.0, seed=seed + 1, dim_lo=1e2, dim_hi=1e3, depth=(200, 200), nbath_max=3)
+    serve.SpectrumServer(ctx, max_slots=4).serve(warm, realtime=False)
+    reqs = serve.request_stream(n, 0.0, seed=seed, dim_lo=1e2, dim_hi=1e5, depth=(200, 200), nbath_max=3)
+    t0 = time.perf_counter()
+    served = serve.SpectrumServer(ctx, max_slots=slots, prep_workers=4).serve(reqs, realtime=False)
+    t_served = time.perf_counter() - t0
+    reqs2 = serve.request_stream(n, 0.0, seed=seed, dim_lo=1e2, dim_hi=1e5, depth=(200, 200), nbath_max=3)
+    t0 = time.perf_counter()
+    plain = serve.SpectrumServer(ctx, max_slots=1).serve(reqs2, realtime=False, plain=True)
+    t_plain = time.perf_counter() - t0
+    by = {r.rid: r for r in plain}
+    same = all(np.array_equal(r.spectrum, by[r.rid].spectrum) for r in served)
+    dims = np.array([r.dim for r in reqs])
+    return {"config": "2b", "what": f"{n} problems, GS + 200-step spectrum each, batched ({slots} slots) vs one at "
+                                    "a time", "dim_min": int(dims.min()), "dim_max": int(dims.max()),
+            "dim_median": float(np.median(dims)), "seconds_batched": t_served, "seconds_plain": t_plain,
+            "spectra_per_s_batched": n / t_served, "spectra_per_s_plain": n / t_plain,
+            "speedup": t_plain / t_served, "identical_spectra": bool(same),
+            "gs_steps_mean": float(np.mean([r.gs_steps for r in served]))}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--config", required=True, choices=["0", "1", "2a", "2b", "4"])
+    args = ap.parse_args()
+    if args.config == "0":
+        out = config0()
+    elif args.config == "1":
+        out, _ = config1()
+    elif args.config == "2a":
+        out = config2a()
+    elif args.config == "2b":
+        out = config2b()
+    else:
+        out = config4()
+    print(json.dumps(out), flush=True)
+
+
+def config4(seed: int = 1, nsteps: int = 300):
+    """The 76-mode problem held as two-word determinants: ground state of H_i
+    in the 2p6 sector, the L-edge dipole into the 2p5 sector, a 300-step Lanczos
+    on H_f and the continued fraction."""
+    ctx = default_context()
+    pr = P.large76(seed=seed)
+    nf = pr["NF"]
+    (gs, t_gs_basis) = _sync_time(lambda: Basis.enumerate(nf, pr["gs_restrictions"], ctx), "enumerate 2p6")
+    (fs, t_fs_basis) = _sync_time(lambda: Basis.enumerate(nf, pr["fs_restrictions"], ctx), "enumerate 2p5")
+    (hi, t_hi) = _sync_time(lambda: Hop(Op(nf, pr["Hi"], ctx), gs, mode="assembled"), "assemble H_i")
+    (E, vecs, var, it), t_gs = _sync_time(lambda: hi.ground_states(1, tol=1e-12, max_krylov=200), "ground state")
+    hi_stats = hi.stats()
+    del hi
+    (hf, t_hf) = _sync_time(lambda: Hop(Op(nf, pr["Hf"], ctx), fs, mode="assembled"), "assemble H_f")
+    T = Hop(Op(nf, pr["T"], ctx), fs, gs, mode="matfree")
+    tpsi, t_t = _sync_time(lambda: T.apply(vecs[0].contiguous()), "T psi")
+    (a, b), t_lz = _sync_time(lambda: hf.lanczos(tpsi, nsteps), f"{nsteps}-step Lanczos")
+    _, t_cf = _sync_time(lambda: cf_spectrum(a, b, float(E[0]), -10.0, 20.0, 2000, 0.4, ctx), "continued fraction")
+    st = hf.stats()
+    return {"config": "4", "what": "76 modes (two-word determinants): GS of H_i, dipole into 2p5, "
+                                   f"{nsteps}-step Lanczos on H_f, CF on 2001 points",
+            "gs_dim": gs.n, "fs_dim": fs.n, "E0": float(E[0]), "gs_variance": float(var[0]), "gs_iterations": int(it),
+            "seconds": {"enumerate_gs": t_gs_basis, "enumerate_fs": t_fs_basis, "assemble_hi": t_hi,
+                        "ground_state": t_gs, "assemble_hf": t_hf, "T_psi": t_t, "lanczos": t_lz, "cf": t_cf},
+            "hf_nnz_eff": st["nnz_eff"], "hf_stream_bytes": st["stream_bytes"], "hf_packed": st["packed"],
+            "hi_nnz_eff": hi_stats["nnz_eff"], "lanczos_ms_per_step": 1e3 * t_lz / nsteps}
+
+
+if __name__ == "__main__":
+    main()

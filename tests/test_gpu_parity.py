@@ -145,6 +145,33 @@ def test_hpsi_layouts_match_oracle(shift_min, pack):
     assert np.max(np.abs(yt - wt)) <= 1e-13 * np.max(np.abs(wt))
 
 
+@pytest.mark.parametrize("pack", [1, 0])
+def test_hpsi_uint128_matches_oracle(pack):
+    """Two-word determinants (76 modes): H_i and H_f of the 76-mode L-edge model
+    on small sectors, and the dipole between them, against the oracle; packed
+    and per-field entries."""
+    pr = P.large76(seed=2)
+    nf = pr["NF"]
+    full = [x for b in pr["baths"][:3] for x in b]
+    empty = [x for b in pr["baths"][3:] for x in b]
+    common = [(P.mask(nf, full), 30, 30), (P.mask(nf, empty), 0, 1), (P.mask(nf, range(nf)), 44, 44)]
+    gsr = [(P.mask(nf, pr["pmodes"]), 6, 6), (P.mask(nf, pr["dmodes"]), 7, 8)] + common
+    fsr = [(P.mask(nf, pr["pmodes"]), 5, 5), (P.mask(nf, pr["dmodes"]), 8, 9)] + common
+    gs, fs = Basis.enumerate(nf, gsr), Basis.enumerate(nf, fsr)
+    assert gs.nw == 2 and (gs.n, fs.n) == (3645, 8160)
+    with assembled_layout(8, pack):
+        cases = [(pr["Hi"], gs, gs, gsr), (pr["Hf"], fs, fs, fsr), (pr["T"], fs, gs, fsr)]
+        hops = [(Hop(Op(nf, t), r, c, mode=mode), t, r, c, rr) for t, r, c, rr in cases
+                for mode in ("assembled", "matfree")]
+    for hop, terms, rows, cols, rr in hops:
+        if hop.stats()["assembled"]:
+            assert hop.stats()["packed"] == bool(pack)
+        x = cvec(cols.n, 5)
+        y = hop.apply(torch.from_numpy(x).cuda()).cpu().numpy()
+        want = oracle_apply_dense(nf, terms, cols.words(), x, rr, rows)
+        assert np.max(np.abs(y - want)) <= 1e-13 * np.max(np.abs(want))
+
+
 def test_optimespsi_known_answer_gpu():
     terms = [O.written([O.cr(0), O.an(0)], 1j), O.written([O.cr(0), O.cr(1), O.an(0), O.an(1)], 1.0)]
     b = Basis.enumerate(2, [("11", 2, 2)])
