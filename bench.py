@@ -171,6 +171,8 @@ def serve_section(args, ctx):
     s1, s2 = serve.latency_stats(served), serve.latency_stats(plain)
     dims = np.array([r.dim for r in reqs])
     work = [(r.dim, r.gs_steps, r.depth) for r in served]
+    rs = np.array([dim * (gs + depth) for dim, gs, depth in work], dtype=float)
+    big = np.array([dim > 3e5 for dim, _, _ in work])
     return {"served": s1, "plain": s2, "speedup": s1["spectra_per_s"] / s2["spectra_per_s"],
             "wall_s": {"served": t_served, "plain": t_plain}, "identical_spectra": bool(same),
             "stream": {"requests": args.steps, "rate_per_s": args.rate, "dim_min": int(dims.min()),
@@ -178,7 +180,9 @@ def serve_section(args, ctx):
                        "gamma": "U(0.1,0.5)", "ne_points": 2001, "slots": args.slots, "hbm_budget_GB": 200,
                        "gs": "Lanczos to |H psi - E psi|^2 <= 1e-12 max(1, E^2), Ritz vector from the kept "
                              "Lanczos vectors",
-                       "gs_steps_mean": float(np.mean([w[1] for w in work]))}}, work
+                       "gs_steps_mean": float(np.mean([w[1] for w in work])),
+                       "row_steps": float(rs.sum()), "row_steps_share_dim_gt_3e5": float(rs[big].sum() / rs.sum())}}, \
+        work
 
 
 def cpu_baseline(args, pr, basis, work):
