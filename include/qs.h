@@ -224,6 +224,12 @@ qs_status qs_batch_run(qs_batch *b, int steps);
 qs_status qs_batch_poll(qs_batch *b, int slot, int *step, double *a, double *bb, double *y);
 qs_status qs_batch_remove(qs_batch *b, int slot);
 int qs_batch_active(const qs_batch *b);
+/* Coefficients of n slots in one transfer: steps[n] completed steps, a and b
+ * (host, n x ld, row i = slots[i]) receive a[0..ld) and b[0..ld) of each slot
+ * (zero past its max_depth). */
+qs_status qs_batch_coeffs(qs_batch *b, int n, const int *slots, int ld, int *steps, double *a, double *bb);
+/* qs_batch_set_depth for n slots with one launch. */
+qs_status qs_batch_set_depths(qs_batch *b, int n, const int *slots, const int *depths);
 /* Completed steps of every slot in one transfer: steps[max_slots], -1 for free slots. */
 qs_status qs_batch_steps(qs_batch *b, int *steps);
 qs_status qs_batch_destroy(qs_batch *b);

@@ -82,6 +82,8 @@ _SIGS = [
     ("qs_batch_remove", C.c_int, [_P, C.c_int]),
     ("qs_batch_active", C.c_int, [_P]),
     ("qs_batch_steps", C.c_int, [_P, _P]),
+    ("qs_batch_coeffs", C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P, _P]),
+    ("qs_batch_set_depths", C.c_int, [_P, C.c_int, _P, _P]),
     ("qs_batch_destroy", C.c_int, [_P]),
 ]
 EXPORTED = [s[0] for s in _SIGS]

@@ -57,11 +57,12 @@ struct qs_batch {
   std::vector<SlotDev> h_slots;
   std::vector<int> used;
   std::vector<char *> mem;     // per-slot allocation (vectors + scalars)
+  std::vector<size_t> mem_bytes;
   std::vector<double2 *> kept; // per-slot kept Lanczos vectors or null
   std::vector<size_t> kept_bytes;
-  // kept-vector buffers of removed slots, reused by later slots (large device
-  // allocations and frees are slow and synchronous)
-  std::vector<std::pair<size_t, double2 *>> pool;
+  // device buffers of removed slots, reused by later slots (device
+  // allocations and frees are slow and synchronize the device)
+  std::vector<std::pair<size_t, void *>> pool;
   std::vector<const qs_hop *> hops;
   // worklists (rebuilt when the set of slots changes)
   bool dirty = true;
@@ -293,6 +294,55 @@ k_b_ritz(const SlotDev *__restrict__ slots, int slot, int m, const double *__res
 // ---------------------------------------------------------------------------
 // host
 // ---------------------------------------------------------------------------
+#define QS_POOL_MAX_BUFFERS 128
+#define QS_POOL_MAX_BYTES ((size_t)64 << 30)
+
+static void pool_trim(qs_batch *B) {
+  size_t total = 0;
+  for (auto &pb : B->pool) total += pb.first;
+  while (!B->pool.empty() && (B->pool.size() > QS_POOL_MAX_BUFFERS || total > QS_POOL_MAX_BYTES)) {
+    // too many buffers: drop the smallest; too many bytes: drop the largest
+    auto it = B->pool.size() > QS_POOL_MAX_BUFFERS ? std::min_element(B->pool.begin(), B->pool.end())
+                                                   : std::max_element(B->pool.begin(), B->pool.end());
+    total -= it->first;
+    (void)hipFree(it->second);
+    B->pool.erase(it);
+  }
+}
+
+// A device buffer of at least `need` bytes: the smallest pooled one that wastes
+// at most half of itself, else a new allocation (after giving the pool back
+// if the device is full).
+static qs_status pool_get(qs_batch *B, size_t need, void **p, size_t *bytes) {
+  int best = -1;
+  for (int i = 0; i < (int)B->pool.size(); ++i)
+    if (B->pool[i].first >= need && B->pool[i].first <= 2 * need &&
+        (best < 0 || B->pool[i].first < B->pool[best].first))
+      best = i;
+  if (best >= 0) {
+    *bytes = B->pool[best].first;
+    *p = B->pool[best].second;
+    B->pool.erase(B->pool.begin() + best);
+    return QS_OK;
+  }
+  qs_status st = qs_malloc(B->ctx, p, need);
+  if (st == QS_ENOMEM_DEVICE && !B->pool.empty()) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(B->ctx->stream);
+    for (auto &pb : B->pool) (void)hipFree(pb.second);
+    B->pool.clear();
+    st = qs_malloc(B->ctx, p, need);
+  }
+  if (st == QS_OK) *bytes = need;
+  return st;
+}
+
+static void pool_put(qs_batch *B, void *p, size_t bytes) {
+  if (!p) return;
+  B->pool.push_back({bytes, p});
+  pool_trim(B);
+}
+
 static qs_status rebuild(qs_batch *B) {
   qs_ctx *ctx = B->ctx;
   std::vector<WorkItem> w1, w2;
@@ -366,6 +416,7 @@ extern "C" qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out)
   B->h_slots.assign(max_slots, SlotDev{});
   B->used.assign(max_slots, 0);
   B->mem.assign(max_slots, nullptr);
+  B->mem_bytes.assign(max_slots, 0);
   B->kept.assign(max_slots, nullptr);
   B->kept_bytes.assign(max_slots, 0);
   B->hops.assign(max_slots, nullptr);
@@ -411,34 +462,15 @@ extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double 
   if (flags & ~QS_BATCH_KEEP_BASIS) return qs_fail(ctx, QS_EINVAL, "unknown flags 0x%x", flags);
   const bool keep = (flags & QS_BATCH_KEEP_BASIS) != 0;
   char *m = nullptr;
-  QS_TRY(qs_malloc(ctx, (void **)&m, (want_y ? 3 : 2) * vb + 4 * sc + 256));
+  size_t mbytes = 0;
+  QS_TRY(pool_get(B, (want_y ? 3 : 2) * vb + 4 * sc + 256, (void **)&m, &mbytes));
   double2 *V = nullptr;
   size_t vbytes = 0;
   if (keep) {
-    const size_t need = (size_t)(max_depth + 1) * vb;
-    // the smallest pooled buffer that fits and wastes at most half of itself
-    int best = -1;
-    for (int i = 0; i < (int)B->pool.size(); ++i)
-      if (B->pool[i].first >= need && B->pool[i].first <= 2 * need &&
-          (best < 0 || B->pool[i].first < B->pool[best].first))
-        best = i;
-    if (best >= 0) {
-      vbytes = B->pool[best].first;
-      V = B->pool[best].second;
-      B->pool.erase(B->pool.begin() + best);
-    } else {
-      qs_status sv = qs_malloc(ctx, (void **)&V, need);
-      if (sv == QS_ENOMEM_DEVICE && !B->pool.empty()) {  // give the pool back and retry
-        (void)hipGetLastError();
-        for (auto &pb : B->pool) (void)hipFree(pb.second);
-        B->pool.clear();
-        sv = qs_malloc(ctx, (void **)&V, need);
-      }
-      if (sv != QS_OK) {
-        (void)hipFree(m);
-        return sv;
-      }
-      vbytes = need;
+    qs_status sv = pool_get(B, (size_t)(max_depth + 1) * vb, (void **)&V, &vbytes);
+    if (sv != QS_OK) {
+      pool_put(B, m, mbytes);
+      return sv;
     }
   }
   SlotDev S{};
@@ -475,13 +507,15 @@ extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double 
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (st != QS_OK || e != hipSuccess) {
-    (void)hipFree(m);
-    if (V) B->pool.push_back({vbytes, V});
+    (void)hipStreamSynchronize(ctx->stream);
+    pool_put(B, m, mbytes);
+    pool_put(B, V, vbytes);
     return st != QS_OK ? st : qs_hip_fail(ctx, e, "qs_batch_add", __FILE__, __LINE__);
   }
   B->h_slots[slot] = S;
   B->used[slot] = 1;
   B->mem[slot] = m;
+  B->mem_bytes[slot] = mbytes;
   B->kept[slot] = V;
   B->kept_bytes[slot] = vbytes;
   B->hops[slot] = h;
@@ -579,23 +613,97 @@ extern "C" qs_status qs_batch_remove(qs_batch *B, int slot) {
   SlotDev z{};
   QS_HIP(ctx, hipMemcpyAsync(B->d_slots + slot, &z, sizeof(SlotDev), hipMemcpyHostToDevice, ctx->stream));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  (void)hipFree(B->mem[slot]);
+  pool_put(B, B->mem[slot], B->mem_bytes[slot]);
   B->mem[slot] = nullptr;
-  if (B->kept[slot]) {
-    // keep the buffer for later slots; at most 4 buffers stay pooled
-    B->pool.push_back({B->kept_bytes[slot], B->kept[slot]});
-    if (B->pool.size() > 4) {
-      auto smallest = std::min_element(B->pool.begin(), B->pool.end());
-      (void)hipFree(smallest->second);
-      B->pool.erase(smallest);
-    }
-  }
+  B->mem_bytes[slot] = 0;
+  pool_put(B, B->kept[slot], B->kept_bytes[slot]);
   B->kept[slot] = nullptr;
   B->kept_bytes[slot] = 0;
   B->used[slot] = 0;
   B->hops[slot] = nullptr;
   B->h_slots[slot] = SlotDev{};
   B->dirty = true;
+  return QS_OK;
+}
+
+// one workgroup per listed slot: its step count and a[0..ld), b[0..ld) (zero
+// past the slot's capacity) packed into out = [steps (n ints, padded to 8 B) |
+// a (n x ld) | b (n x ld)]
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b_gather(const SlotDev *__restrict__ slots, int n, const int *__restrict__ list, int ld, char *__restrict__ out) {
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const SlotDev &S = slots[list[i]];
+  int *steps = reinterpret_cast<int *>(out);
+  double *a = reinterpret_cast<double *>(out + (size_t)((n + 1) / 2) * 8);
+  double *b = a + (size_t)n * ld;
+  if (threadIdx.x == 0) steps[i] = S.active ? S.step : -1;
+  const int cap = S.pad + 1;
+  for (int k = threadIdx.x; k < ld; k += blockDim.x) {
+    a[(size_t)i * ld + k] = k < cap ? S.a[k] : 0.0;
+    b[(size_t)i * ld + k] = k < cap ? S.b[k] : 0.0;
+  }
+}
+
+extern "C" qs_status qs_batch_coeffs(qs_batch *B, int n, const int *slots, int ld, int *steps, double *a,
+                                     double *b) {
+  if (!B || n < 0 || ld < 1 || (n > 0 && (!slots || !steps || !a || !b))) return QS_EINVAL;
+  qs_ctx *ctx = B->ctx;
+  for (int i = 0; i < n; ++i)
+    if (slots[i] < 0 || slots[i] >= B->cap || !B->used[slots[i]])
+      return qs_fail(ctx, QS_EINVAL, "slot %d is not in use", slots[i]);
+  if (n == 0) return QS_OK;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t head = (size_t)((n + 1) / 2) * 8;
+  const size_t body = head + (size_t)2 * n * ld * 8;
+  void *scr;
+  QS_TRY(qs_scratch(ctx, (size_t)n * 4 + 64 + body, &scr));
+  int *d_list = (int *)scr;
+  char *d_out = (char *)scr + (((size_t)n * 4 + 63) / 64) * 64;
+  QS_HIP(ctx, hipMemcpyAsync(d_list, slots, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_b_gather, dim3(n), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, n, d_list, ld, d_out);
+  QS_HIP(ctx, hipGetLastError());
+  std::vector<char> h(body);
+  QS_HIP(ctx, hipMemcpyAsync(h.data(), d_out, body, hipMemcpyDeviceToHost, ctx->stream));
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  memcpy(steps, h.data(), (size_t)n * 4);
+  memcpy(a, h.data() + head, (size_t)n * ld * 8);
+  memcpy(b, h.data() + head + (size_t)n * ld * 8, (size_t)n * ld * 8);
+  return QS_OK;
+}
+
+__global__ void k_b_set_depths(SlotDev *slots, int n, const int *__restrict__ list) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) slots[list[2 * i]].depth = list[2 * i + 1];
+}
+
+extern "C" qs_status qs_batch_set_depths(qs_batch *B, int n, const int *slots, const int *depths) {
+  if (!B || n < 0 || (n > 0 && (!slots || !depths))) return QS_EINVAL;
+  qs_ctx *ctx = B->ctx;
+  for (int i = 0; i < n; ++i) {
+    if (slots[i] < 0 || slots[i] >= B->cap || !B->used[slots[i]])
+      return qs_fail(ctx, QS_EINVAL, "slot %d is not in use", slots[i]);
+    const SlotDev &H = B->h_slots[slots[i]];
+    if (depths[i] < 1 || depths[i] > H.pad)
+      return qs_fail(ctx, QS_EINVAL, "depth %d outside 1..%d", depths[i], H.pad);
+    if (H.y) return qs_fail(ctx, QS_EINVAL, "the depth of a vector-building slot is fixed");
+  }
+  if (n == 0) return QS_OK;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  std::vector<int> pairs((size_t)2 * n);
+  for (int i = 0; i < n; ++i) {
+    pairs[2 * i] = slots[i];
+    pairs[2 * i + 1] = depths[i];
+  }
+  void *scr;
+  QS_TRY(qs_scratch(ctx, pairs.size() * 4, &scr));
+  QS_HIP(ctx, hipMemcpyAsync(scr, pairs.data(), pairs.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_b_set_depths, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, B->d_slots, n,
+                     (const int *)scr);
+  QS_HIP(ctx, hipGetLastError());
+  // the host copy of `pairs` must outlive the transfer
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < n; ++i) B->h_slots[slots[i]].depth = depths[i];
   return QS_OK;
 }
 

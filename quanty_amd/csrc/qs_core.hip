@@ -183,10 +183,158 @@ struct StateKey {
   bool operator<(const StateKey &o) const { return c < o.c; }
 };
 
+// The same table for up to 8 restrictions with bounds <= 255: a state (the
+// occupation count inside each mask) is packed into one byte per restriction
+// of a uint64, and the byte values of a chunk are grouped by the counts they
+// add, so each state meets a handful of distinct successors instead of 256.
+// State ids follow the packed keys in ascending order.
+bool rank_table_packed(int nf, int nres, const uint8_t *masks, const int *lo, const int *hi, RankTable &rt,
+                       std::string &err) {
+  const int nbytes = (nf + 7) / 8;
+  const int nch = nbytes;
+  struct Group {
+    uint64_t inc;
+    std::vector<int> vs;  // ascending byte values with this increment
+  };
+  std::vector<std::vector<Group>> groups(nch);
+  std::vector<std::vector<int>> gidx(nch, std::vector<int>(256, -1));
+  for (int c = 0; c < nch; ++c) {
+    const int rem = nf - 8 * c;
+    const uint32_t forbid = rem >= 8 ? 0u : ((1u << (8 - rem)) - 1u);
+    std::map<uint64_t, int> where;
+    for (int v = 0; v < 256; ++v) {
+      if (v & forbid) continue;
+      uint64_t inc = 0;
+      for (int r = 0; r < nres; ++r)
+        inc |= (uint64_t)__builtin_popcount((unsigned)(masks[r * nbytes + c] & v)) << (8 * r);
+      auto it = where.find(inc);
+      int g;
+      if (it == where.end()) {
+        g = (int)groups[c].size();
+        where[inc] = g;
+        groups[c].push_back({inc, {}});
+      } else {
+        g = it->second;
+      }
+      groups[c][g].vs.push_back(v);
+      gidx[c][v] = g;
+    }
+  }
+  // successor of a packed state, or false when a count passes its upper bound
+  auto step = [&](uint64_t s, uint64_t inc, uint64_t &t) {
+    t = s + inc;  // bytes never carry: every count stays <= 255
+    for (int r = 0; r < nres; ++r)
+      if ((int)((t >> (8 * r)) & 0xFF) > hi[r]) return false;
+    return true;
+  };
+  std::vector<std::vector<uint64_t>> keys(nch + 1);
+  keys[0].push_back(0);
+  for (int c = 0; c < nch; ++c) {
+    std::vector<uint64_t> nxt;
+    for (uint64_t st : keys[c])
+      for (const Group &g : groups[c]) {
+        uint64_t t;
+        if (step(st, g.inc, t)) nxt.push_back(t);
+      }
+    std::sort(nxt.begin(), nxt.end());
+    nxt.erase(std::unique(nxt.begin(), nxt.end()), nxt.end());
+    if (nxt.size() > 2000000) {
+      err = "restriction state space too large";
+      return false;
+    }
+    keys[c + 1] = std::move(nxt);
+  }
+  // completions per state, backward
+  std::vector<std::vector<int64_t>> cnt(nch + 1);
+  auto find = [&](int c, uint64_t t) -> int64_t {
+    auto it = std::lower_bound(keys[c].begin(), keys[c].end(), t);
+    return (it != keys[c].end() && *it == t) ? (int64_t)(it - keys[c].begin()) : -1;
+  };
+  cnt[nch].resize(keys[nch].size());
+  for (size_t i = 0; i < keys[nch].size(); ++i) {
+    bool acc = true;
+    for (int r = 0; r < nres; ++r) {
+      const int v = (int)((keys[nch][i] >> (8 * r)) & 0xFF);
+      if (v < lo[r] || v > hi[r]) acc = false;
+    }
+    cnt[nch][i] = acc ? 1 : 0;
+  }
+  for (int c = nch - 1; c >= 0; --c) {
+    cnt[c].assign(keys[c].size(), 0);
+    for (size_t i = 0; i < keys[c].size(); ++i) {
+      int64_t tot = 0;
+      for (const Group &g : groups[c]) {
+        uint64_t t;
+        if (!step(keys[c][i], g.inc, t)) continue;
+        const int64_t j = find(c + 1, t);
+        if (j >= 0) tot += cnt[c + 1][j] * (int64_t)g.vs.size();
+      }
+      cnt[c][i] = tot;
+    }
+  }
+  rt.total = cnt[0][0];
+  if (rt.total >= ((int64_t)1 << 32) - 1) {
+    err = "basis larger than 2^32-2 determinants";
+    return false;
+  }
+  // ids of viable states (nonzero completions), ascending keys
+  std::vector<std::vector<int64_t>> id(nch + 1);
+  uint32_t smax = 1;
+  for (int c = 0; c <= nch; ++c) {
+    id[c].assign(keys[c].size(), -1);
+    uint32_t k = 0;
+    for (size_t i = 0; i < keys[c].size(); ++i)
+      if (cnt[c][i] > 0) id[c][i] = k++;
+    if (c < nch) smax = std::max(smax, k);
+  }
+  rt.nchunks = nch;
+  rt.smax = smax;
+  rt.next.assign((size_t)nch * smax * 256, QS_DEAD);
+  rt.less.assign((size_t)nch * smax * 256, 0);
+  for (int c = 0; c < nch; ++c) {
+    const size_t ng = groups[c].size();
+    std::vector<int64_t> gc(ng);
+    std::vector<uint32_t> gn(ng);
+    for (size_t i = 0; i < keys[c].size(); ++i) {
+      if (id[c][i] < 0) continue;
+      for (size_t g = 0; g < ng; ++g) {
+        gc[g] = 0;
+        gn[g] = QS_DEAD;
+        uint64_t t;
+        if (!step(keys[c][i], groups[c][g].inc, t)) continue;
+        const int64_t j = find(c + 1, t);
+        if (j < 0 || cnt[c + 1][j] == 0) continue;
+        gc[g] = cnt[c + 1][j];
+        gn[g] = c + 1 == nch ? 0u : (uint32_t)id[c + 1][j];  // 0 = accepting at the last chunk
+      }
+      const size_t base = ((size_t)c * smax + (size_t)id[c][i]) * 256;
+      int64_t run = 0;
+      for (int v = 0; v < 256; ++v) {
+        rt.less[base + v] = (uint32_t)run;
+        const int g = gidx[c][v];
+        if (g < 0 || gc[g] == 0) continue;
+        rt.next[base + v] = gn[g];
+        run += gc[g];
+      }
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int *lo,
                               const int *hi, RankTable &rt, std::string &err) {
+  if (nres >= 0 && nres <= 8 && nf > 0 && nf <= 128) {
+    // no count exceeds 128 (nf <= 128), so bounds are clamped into a byte
+    int lo8[8], hi8[8];
+    for (int r = 0; r < nres; ++r) {
+      lo8[r] = std::max(0, std::min(lo[r], 129));
+      hi8[r] = std::min(hi[r], 128);
+    }
+    if (!rank_table_packed(nf, nres, masks, lo8, hi8, rt, err)) return QS_EINVAL;
+    return QS_OK;
+  }
   if (nf <= 0 || nf > 128) {
     err = "nf must be in 1..128";
     return QS_EINVAL;

@@ -340,6 +340,23 @@ class Batch:
     def active(self) -> int:
         return int(self.lib.qs_batch_active(self.h))
 
+    def coeffs(self, slots, ld: int):
+        """Completed steps (n,), a (n, ld) and b (n, ld) of several slots in one transfer."""
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        n = len(sl)
+        st = np.zeros(n, dtype=np.int32)
+        a = np.zeros((n, ld))
+        b = np.zeros((n, ld))
+        self.ctx.check(self.lib.qs_batch_coeffs(self.h, n, ptr(sl), ld, ptr(st), ptr(a), ptr(b)), "qs_batch_coeffs")
+        return st, a, b
+
+    def set_depths(self, slots, depths):
+        sl = np.ascontiguousarray(slots, dtype=np.int32)
+        dp = np.ascontiguousarray(depths, dtype=np.int32)
+        self.ctx.check(self.lib.qs_batch_set_depths(self.h, len(sl), ptr(sl), ptr(dp)), "qs_batch_set_depths")
+        for s, d in zip(sl, dp):
+            self.depth[int(s)] = int(d)
+
     def steps(self, cap: int) -> np.ndarray:
         out = np.zeros(cap, dtype=np.int32)
         self.ctx.check(self.lib.qs_batch_steps(self.h, ptr(out)), "qs_batch_steps")

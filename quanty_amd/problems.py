@@ -48,6 +48,25 @@ class SplitMix64:
         """Complex Gaussian with E|z|^2 = sigma^2."""
         return complex(self.gauss(sigma / math.sqrt(2)), self.gauss(sigma / math.sqrt(2)))
 
+    def uniforms(self, n: int, a=0.0, b=1.0) -> np.ndarray:
+        """The next n values of `uniform(a, b)` in one vectorized draw."""
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.s) + k * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        self.s = (self.s + n * 0x9E3779B97F4A7C15) & MASK64
+        return a + (b - a) * ((z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0))
+
+    def cgauss_array(self, n: int, sigma=1.0) -> np.ndarray:
+        """The next n values of `cgauss(sigma)` (4 uniforms each, in the same order)."""
+        u = self.uniforms(4 * n).reshape(n, 4)
+        s = sigma / math.sqrt(2)
+        re = s * np.sqrt(-2.0 * np.log(np.maximum(u[:, 0], 1e-300))) * np.cos(2.0 * math.pi * u[:, 1])
+        im = s * np.sqrt(-2.0 * np.log(np.maximum(u[:, 2], 1e-300))) * np.cos(2.0 * math.pi * u[:, 3])
+        return re + 1j * im
+
 
 def shell(first: int, l: int = 2):
     """(IndexUp, IndexDn) of a shell starting at mode `first`, dn/up interleaved."""
@@ -145,6 +164,26 @@ def _impurity_d_terms(coulomb, zeta, tenDq):
                  + O.one_body(dmodes, dmodes, tenDq * O.d_shell_matrix_to_spin(0.6 * eg - 0.4 * t2g)))
 
 
+_IU = np.triu_indices(10, 1)
+
+
+def _bath_couplings(rng: SplitMix64):
+    """One bath shell's random parameters, drawn in this order: 10 energies
+    U(-5, 5); for each orbital a hybridization r e^{i phi}, r U(0.5, 2),
+    phi U(0, 2 pi); the intra-bath hoppings T_ij (i < j) complex Gaussian of
+    width 0.3, row by row, with T_ji = conj(T_ij)."""
+    eps = rng.uniforms(10, -5.0, 5.0)
+    u = rng.uniforms(20)
+    r = 0.5 + (2.0 - 0.5) * u[0::2]
+    ph = 0.0 + (2 * math.pi - 0.0) * u[1::2]
+    V = np.diag(r * (np.cos(ph) + 1j * np.sin(ph)))
+    T = np.zeros((10, 10), dtype=complex)
+    z = rng.cgauss_array(45, 0.3)
+    T[_IU] = z
+    T[(_IU[1], _IU[0])] = z.conj()
+    return eps, V, T
+
+
 def anderson(seed: int = 1, nbath: int = 4, nelec: int = 28, full_baths: int = 2, d_range=(6, 10),
              max_holes: int = 2, max_elec: int = 2, coulomb=(7.3, 9.79, 6.08), zeta=0.083, tenDq=1.1):
     """Anderson impurity with `nbath` bath shells; the first `full_baths` are
@@ -160,23 +199,12 @@ def anderson(seed: int = 1, nbath: int = 4, nelec: int = 28, full_baths: int = 2
         up_b, dn_b = shell(10 + 10 * b, 2)
         bm = O.shell_modes(up_b, dn_b)
         baths.append(bm)
-        eps = [rng.uniform(-5.0, 5.0) for _ in range(10)]
-        terms.append(O.number(bm, eps))
+        eps, V, T = _bath_couplings(rng)
+        terms.append(O.number(bm, list(eps)))
         # d <-> bath, orbital diagonal (same m and spin)
-        V = np.zeros((10, 10), dtype=complex)
-        for i in range(10):
-            r = rng.uniform(0.5, 2.0)
-            ph = rng.uniform(0.0, 2 * math.pi)
-            V[i, i] = r * complex(math.cos(ph), math.sin(ph))
         terms.append(O.one_body(dmodes, bm, V))
         terms.append(O.one_body(bm, dmodes, V.conj().T))
         # intra-bath hopping, Hermitian, off-diagonal
-        T = np.zeros((10, 10), dtype=complex)
-        for i in range(10):
-            for j in range(i + 1, 10):
-                z = rng.cgauss(0.3)
-                T[i, j] = z
-                T[j, i] = z.conjugate()
         terms.append(O.one_body(bm, bm, T))
     H = O.add(*terms)
     restr = anderson_restrictions(nbath, nelec, full_baths, d_range, max_holes, max_elec)
@@ -186,8 +214,8 @@ def anderson(seed: int = 1, nbath: int = 4, nelec: int = 28, full_baths: int = 2
 def random_one_body_transition(rng: SplitMix64, dmodes, bath_modes, sigma=1.0):
     """Random one-body d<->bath transition: sum t_ij c+_i c_j over d->bath and
     bath->d pairs, every t_ij an independent complex Gaussian of width sigma."""
-    M1 = np.array([[rng.cgauss(sigma) for _ in bath_modes] for _ in dmodes])
-    M2 = np.array([[rng.cgauss(sigma) for _ in dmodes] for _ in bath_modes])
+    M1 = rng.cgauss_array(len(dmodes) * len(bath_modes), sigma).reshape(len(dmodes), len(bath_modes))
+    M2 = rng.cgauss_array(len(bath_modes) * len(dmodes), sigma).reshape(len(bath_modes), len(dmodes))
     return O.add(O.one_body(dmodes, bath_modes, M1), O.one_body(bath_modes, dmodes, M2))
 
 
@@ -224,20 +252,10 @@ def large76(seed: int = 1, params=None, nd: int = 8):
         up_b, dn_b = shell(16 + 10 * b, 2)
         bm = O.shell_modes(up_b, dn_b)
         baths.append(bm)
-        terms.append(O.number(bm, [rng.uniform(-5.0, 5.0) for _ in range(10)]))
-        V = np.zeros((10, 10), dtype=complex)
-        for i in range(10):
-            r = rng.uniform(0.5, 2.0)
-            ph = rng.uniform(0.0, 2 * math.pi)
-            V[i, i] = r * complex(math.cos(ph), math.sin(ph))
+        eps, V, T = _bath_couplings(rng)
+        terms.append(O.number(bm, list(eps)))
         terms.append(O.one_body(dmodes, bm, V))
         terms.append(O.one_body(bm, dmodes, V.conj().T))
-        T = np.zeros((10, 10), dtype=complex)
-        for i in range(10):
-            for j in range(i + 1, 10):
-                z = rng.cgauss(0.3)
-                T[i, j] = z
-                T[j, i] = z.conjugate()
         terms.append(O.one_body(bm, bm, T))
     Hi = O.add(*terms)
     Hf = O.add(Hi, O.shell_operator(up_p, dn_p, "ldots", p["zeta_2p"]), O.number(pmodes, [-p["Upd"] * 2] * 6),

@@ -191,7 +191,7 @@ class SpectrumServer:
     admitted requests together."""
 
     def __init__(self, ctx, max_slots: int = 64, hbm_budget: float = 200e9, gs_depth: int = 60,
-                 gs_max: int = 500, gs_step: int = 40, gs_tol: float = 1e-12, chunk: int = 8, ne: int = 2000,
+                 gs_max: int = 500, gs_step: int = 16, gs_tol: float = 1e-12, chunk: int = 8, ne: int = 2000,
                  emin: float = -10.0, emax: float = 20.0, keep_basis: bool = True, prepare_ahead: bool = True,
                  prep_workers: int = 2):
         self.ctx = ctx
@@ -210,19 +210,20 @@ class SpectrumServer:
         self.jobs: dict[int, _Job] = {}
         self.used = 0
 
-    def _advance(self, job: _Job):
-        """Move a job whose slot finished to its next phase."""
+    def _advance(self, job: _Job, coeffs=None):
+        """Move a job whose slot reached its depth to its next phase. coeffs:
+        (completed steps, a, b) of its slot when already fetched. Returns True
+        when the request is done."""
         r = job.req
         if job.phase == "gs1":
-            st, a, b = self.batch.poll(job.slot)
-            m = st
+            m, a, b = coeffs if coeffs is not None else self.batch.poll(job.slot)
             E, z = tridiag_lowest(a[1:m + 1], b[1:m])
-            w = [E]
             res = float(abs(b[m] * z[m - 1]))
             if res ** 2 > self.gs_tol * max(1.0, E ** 2) and m < job.gs_max:
-                self.batch.set_depth(job.slot, min(job.gs_max, m + self.gs_step))
+                job.gs_depth = min(job.gs_max, m + self.gs_step)
+                job.extend = True  # the caller sets the new depth (one launch for all such jobs)
                 return False
-            r.E0 = float(w[0])
+            r.E0 = float(E)
             r.gs_residual = res
             r.gs_steps = m
             if not self.keep_basis:
@@ -246,11 +247,42 @@ class SpectrumServer:
             job.slot = self.batch.add(job.hop, tpsi, r.depth)
             job.phase = "spec"
             return False
-        st, a, b = self.batch.poll(job.slot)
+        _, a, b = coeffs if coeffs is not None else self.batch.poll(job.slot)
         self.batch.remove(job.slot)
-        r.a, r.b = a, b[:r.depth]
-        r.spectrum = cf_spectrum(a, b[:r.depth], r.E0, self.emin, self.emax, self.ne, r.gamma, self.ctx)[0]
+        r.a, r.b = a[:r.depth + 1].copy(), b[:r.depth].copy()
+        r.spectrum = cf_spectrum(r.a, r.b, r.E0, self.emin, self.emax, self.ne, r.gamma, self.ctx)[0]
         return True
+
+    def _step_jobs(self, t0: float, done: list):
+        """After a run: advance every job whose slot reached its depth. The
+        coefficients of all of them come in one transfer and the depths of the
+        ground states that need more steps are set with one launch."""
+        steps = self.batch.steps(self.max_slots)
+        ready = []
+        for job in self.jobs.values():
+            depth = job.gs_depth if job.phase in ("gs1", "gs2") else job.req.depth
+            if steps[job.slot] >= depth:
+                ready.append(job)
+        need = [j for j in ready if j.phase in ("gs1", "spec")]
+        fetched = {}
+        if need:
+            ld = max((j.gs_depth if j.phase == "gs1" else j.req.depth) for j in need) + 1
+            st, A, Bc = self.batch.coeffs([j.slot for j in need], ld)
+            for j, m, a, b in zip(need, st, A, Bc):
+                fetched[j.req.rid] = (int(m), a, b)
+        extend = []
+        for job in ready:
+            job.extend = False
+            if self._advance(job, fetched.get(job.req.rid)):
+                job.req.t_done = time.perf_counter() - t0
+                done.append(job.req)
+                self.used -= job.bytes
+                job.release()
+                del self.jobs[job.req.rid]
+            elif job.extend:
+                extend.append(job)
+        if extend:
+            self.batch.set_depths([j.slot for j in extend], [j.gs_depth for j in extend])
 
     def _start(self, job: _Job):
         """Put a prepared job into the batch."""
@@ -311,17 +343,7 @@ class SpectrumServer:
                         time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
                     continue
                 self.batch.run(self.chunk)
-                steps = self.batch.steps(self.max_slots)
-                for rid in list(self.jobs):
-                    job = self.jobs[rid]
-                    depth = job.gs_depth if job.phase in ("gs1", "gs2") else job.req.depth
-                    if steps[job.slot] >= depth:
-                        if self._advance(job):
-                            job.req.t_done = time.perf_counter() - t0
-                            done.append(job.req)
-                            self.used -= job.bytes
-                            job.release()
-                            del self.jobs[rid]
+                self._step_jobs(t0, done)
         finally:
             if prep is not None:
                 prep.close()

@@ -165,8 +165,8 @@ def test_server_spectrum_matches_oracle():
     keep = {}
     orig = srv._advance
 
-    def spy(job):
-        out = orig(job)
+    def spy(job, *args):
+        out = orig(job, *args)
         if job.phase == "spec" and "psi" not in keep:
             keep["basis"] = job.basis.words()
             keep["psi"] = job.gs_vector.cpu().numpy()

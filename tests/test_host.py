@@ -57,6 +57,32 @@ def test_rank_table_counts_match_oracle():
         assert Basis.count(nf, restr) == len(oracle.enumerate_sector(nf, restr))
 
 
+@pytest.mark.parametrize("nres", [1, 3, 8, 9, 11])
+def test_rank_table_random_restrictions(nres):
+    """Random overlapping masks: 8 restrictions or fewer take the packed-state
+    table builder, more take the general one; both against brute force over
+    2^nf determinants (and bounds beyond the mode count)."""
+    rng = np.random.default_rng(nres)
+    nf = 14
+    for trial in range(6):
+        restr = []
+        for _ in range(nres):
+            m = rng.integers(0, 2, nf)
+            if m.sum() == 0:
+                m[rng.integers(nf)] = 1
+            k = int(m.sum())
+            lo = int(rng.integers(0, k + 1))
+            hi = int(rng.integers(lo, k + 1)) if trial % 3 else 400  # an upper bound past every count
+            restr.append(("".join(map(str, m)), lo, hi))
+        dets = np.arange(1 << nf)
+        bits = (dets[:, None] >> (nf - 1 - np.arange(nf))) & 1
+        ok = np.ones(len(dets), dtype=bool)
+        for mstr, lo, hi in restr:
+            c = bits @ np.array([int(ch) for ch in mstr])
+            ok &= (c >= lo) & (c <= hi)
+        assert Basis.count(nf, restr) == int(ok.sum()), restr
+
+
 @pytest.mark.parametrize("m", [1, 2, 7, 60, 400])
 def test_tridiagonal_lowest_and_extremes(m):
     from quanty_amd.engine import tridiag_extremes, tridiag_lowest
