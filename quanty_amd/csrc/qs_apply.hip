@@ -429,7 +429,8 @@ k_sell_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *
 // ---------------------------------------------------------------------------
 template <int KC, int NW, bool PACKED, bool LDS>
 __global__ void __launch_bounds__(QS_BLOCK)
-k_spmv(SellD A, const double2 *__restrict__ x, int64_t ldx, double2 *__restrict__ y, int64_t ldy) {
+k_spmv(SellD A, const double2 *__restrict__ x, int64_t ldx, int64_t ldvx, double2 *__restrict__ y, int64_t ldy,
+       int64_t ldvy) {
   extern __shared__ char smem[];
   const double2 *tv = A.val;
   const uint64_t *tp = A.par;
@@ -445,11 +446,11 @@ k_spmv(SellD A, const double2 *__restrict__ x, int64_t ldx, double2 *__restrict_
   s0 = wave_uniform(s0);
   for (int64_t s = s0; s < s1; s += ds) {
     double ar[KC], ai[KC];
-    sell_row<KC, NW, PACKED>(A, s, lane, tv, tp, x, ldx, ar, ai);
+    sell_row<KC, NW, PACKED>(A, s, lane, tv, tp, x, ldx, ldvx, ar, ai);
     const int64_t r = s * 64 + lane;
     if (r < A.nrows) {
 #pragma unroll
-      for (int k = 0; k < KC; ++k) y[r * ldy + k] = make_double2(ar[k], ai[k]);
+      for (int k = 0; k < KC; ++k) y[r * ldy + k * ldvy] = make_double2(ar[k], ai[k]);
     }
   }
 }
@@ -679,7 +680,8 @@ extern "C" qs_status qs_hop_layout(const qs_hop *h, int64_t *nblocks, int64_t *b
 // Launch k_spmv for KC interleaved vectors with the term tables in LDS when
 // they fit, on no more workgroups than stay resident (grid stride over slices).
 template <int KC, int NW, bool PACKED>
-static void launch_spmv(qs_ctx *ctx, const qs_hop *h, const double2 *x, int64_t ldx, double2 *y, int64_t ldy) {
+static void launch_spmv(qs_ctx *ctx, const qs_hop *h, const double2 *x, int64_t ldx, int64_t ldvx, double2 *y,
+                        int64_t ldy, int64_t ldvy) {
   const SellD &A = h->view;
   const size_t shm = term_table_bytes(A.nterms, NW);
   const int64_t want = (A.nslices + 3) / 4;
@@ -687,43 +689,51 @@ static void launch_spmv(qs_ctx *ctx, const qs_hop *h, const double2 *x, int64_t 
     const void *kern = (const void *)k_spmv<KC, NW, PACKED, true>;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, qs_resident_blocks(ctx, kern, shm)));
     hipLaunchKernelGGL((k_spmv<KC, NW, PACKED, true>), dim3(blocks), dim3(QS_BLOCK), shm, ctx->stream, A, x, ldx,
-                       y, ldy);
+                       ldvx, y, ldy, ldvy);
   } else {
     const void *kern = (const void *)k_spmv<KC, NW, PACKED, false>;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, qs_resident_blocks(ctx, kern, 0)));
     hipLaunchKernelGGL((k_spmv<KC, NW, PACKED, false>), dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, A, x, ldx,
-                       y, ldy);
+                       ldvx, y, ldy, ldvy);
   }
 }
 
 template <int KC>
-static void launch_spmv_kc(qs_ctx *ctx, const qs_hop *h, const double2 *x, int64_t ldx, double2 *y, int64_t ldy) {
+static void launch_spmv_kc(qs_ctx *ctx, const qs_hop *h, const double2 *x, int64_t ldx, int64_t ldvx, double2 *y,
+                           int64_t ldy, int64_t ldvy) {
   const SellD &A = h->view;
   if (A.nw == 1) {
-    if (A.packed) launch_spmv<KC, 1, true>(ctx, h, x, ldx, y, ldy);
-    else launch_spmv<KC, 1, false>(ctx, h, x, ldx, y, ldy);
+    if (A.packed) launch_spmv<KC, 1, true>(ctx, h, x, ldx, ldvx, y, ldy, ldvy);
+    else launch_spmv<KC, 1, false>(ctx, h, x, ldx, ldvx, y, ldy, ldvy);
   } else {
-    if (A.packed) launch_spmv<KC, 2, true>(ctx, h, x, ldx, y, ldy);
-    else launch_spmv<KC, 2, false>(ctx, h, x, ldx, y, ldy);
+    if (A.packed) launch_spmv<KC, 2, true>(ctx, h, x, ldx, ldvx, y, ldy, ldvy);
+    else launch_spmv<KC, 2, false>(ctx, h, x, ldx, ldvx, y, ldy, ldvy);
   }
 }
 
-qs_status qs_hop_apply_strided(qs_ctx *ctx, const qs_hop *h, int k, const double *x, int64_t ldx,
-                               double *y, int64_t ldy) {
-  // vectors interleaved: entry j of vector v at (j*ld + v)
+// k vectors, entry j of vector v at x[j*ldx + v*ldvx] (y likewise), in
+// launches of 4, 2 or 1 vectors
+qs_status qs_hop_apply_vectors(qs_ctx *ctx, const qs_hop *h, int k, const double *x, int64_t ldx, int64_t ldvx,
+                               double *y, int64_t ldy, int64_t ldvy) {
   const int64_t n = h->rows->n;
   if (n == 0 || k == 0) return QS_OK;
   for (int v0 = 0; v0 < k;) {
     const int kc = k - v0 >= 4 ? 4 : (k - v0 >= 2 ? 2 : 1);
-    const double2 *xs = (const double2 *)x + v0;
-    double2 *ys = (double2 *)y + v0;
-    if (kc == 4) launch_spmv_kc<4>(ctx, h, xs, ldx, ys, ldy);
-    else if (kc == 2) launch_spmv_kc<2>(ctx, h, xs, ldx, ys, ldy);
-    else launch_spmv_kc<1>(ctx, h, xs, ldx, ys, ldy);
+    const double2 *xs = (const double2 *)x + v0 * ldvx;
+    double2 *ys = (double2 *)y + v0 * ldvy;
+    if (kc == 4) launch_spmv_kc<4>(ctx, h, xs, ldx, ldvx, ys, ldy, ldvy);
+    else if (kc == 2) launch_spmv_kc<2>(ctx, h, xs, ldx, ldvx, ys, ldy, ldvy);
+    else launch_spmv_kc<1>(ctx, h, xs, ldx, ldvx, ys, ldy, ldvy);
     QS_HIP(ctx, hipGetLastError());
     v0 += kc;
   }
   return QS_OK;
+}
+
+// vectors interleaved: entry j of vector v at (j*ld + v)
+qs_status qs_hop_apply_strided(qs_ctx *ctx, const qs_hop *h, int k, const double *x, int64_t ldx,
+                               double *y, int64_t ldy) {
+  return qs_hop_apply_vectors(ctx, h, k, x, ldx, 1, y, ldy, 1);
 }
 
 extern "C" qs_status qs_hop_apply(qs_ctx *ctx, const qs_hop *h, const double *x, double *y) {
@@ -742,11 +752,10 @@ extern "C" qs_status qs_hop_apply_multi(qs_ctx *ctx, const qs_hop *h, int k, con
   if (!ctx || !h || k < 0) return QS_EINVAL;
   QS_HIP(ctx, hipSetDevice(ctx->device));
   const int64_t nc = h->cols->n, nr = h->rows->n;
-  for (int v = 0; v < k; ++v) {
-    if (h->assembled)
-      QS_TRY(qs_hop_apply_strided(ctx, h, 1, x + 2 * v * nc, 1, y + 2 * v * nr, 1));
-    else
-      QS_TRY(launch_mf(ctx, h->op, h->rows, h->cols, h->same, x + 2 * v * nc, y + 2 * v * nr));
+  if (h->assembled) {
+    QS_TRY(qs_hop_apply_vectors(ctx, h, k, x, 1, nc, y, 1, nr));
+  } else {
+    for (int v = 0; v < k; ++v) QS_TRY(launch_mf(ctx, h->op, h->rows, h->cols, h->same, x + 2 * v * nc, y + 2 * v * nr));
   }
   if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QS_OK;

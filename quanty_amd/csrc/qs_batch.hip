@@ -121,9 +121,9 @@ __device__ __forceinline__ void b1_row(const SellD &A, int64_t sl, int lane, boo
                                        const double2 *uc, double *hr, double *hi) {
   if (lds)
     sell_row<1, NW, PACKED>(A, sl, lane, reinterpret_cast<const double2 *>(smem),
-                            reinterpret_cast<const uint64_t *>(smem + (size_t)A.nterms * 16), uc, 1, hr, hi);
+                            reinterpret_cast<const uint64_t *>(smem + (size_t)A.nterms * 16), uc, 1, 0, hr, hi);
   else
-    sell_row<1, NW, PACKED>(A, sl, lane, A.val, A.par, uc, 1, hr, hi);
+    sell_row<1, NW, PACKED>(A, sl, lane, A.val, A.par, uc, 1, 0, hr, hi);
 }
 
 // FAST: every active slot has one-word determinants and packed entries (the

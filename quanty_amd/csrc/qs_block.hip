@@ -11,6 +11,7 @@
 // (per-block partials, then one block per output element), so results are
 // reproducible run to run.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <numeric>
@@ -23,10 +24,9 @@
 using cd = std::complex<double>;
 
 #define QS_KMAX 16
-#define QS_BDOT_BLOCKS 512
 
-qs_status qs_hop_apply_strided(qs_ctx *ctx, const qs_hop *h, int k, const double *x, int64_t ldx, double *y,
-                               int64_t ldy);
+qs_status qs_hop_apply_vectors(qs_ctx *ctx, const qs_hop *h, int k, const double *x, int64_t ldx, int64_t ldvx,
+                               double *y, int64_t ldy, int64_t ldvy);
 qs_status qs_launch_mf_ext(qs_ctx *ctx, const qs_hop *h, const double *x, double *y);
 
 // ---------------------------------------------------------------------------
@@ -49,53 +49,6 @@ __global__ void k_rand(int64_t n, uint64_t seed, double2 *__restrict__ out) {
   }
 }
 
-// vector-major (k vectors of n) <-> interleaved [n][k]
-__global__ void k_pack(int64_t n, int k, const double2 *__restrict__ src, double2 *__restrict__ dst, int to_interleaved) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n * k; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i / k;
-    const int v = (int)(i % k);
-    if (to_interleaved) dst[i] = src[(int64_t)v * n + r];
-    else dst[(int64_t)v * n + r] = src[i];
-  }
-}
-
-// column v of an interleaved block <-> contiguous vector
-__global__ void k_column(int64_t n, const double2 *__restrict__ src, int64_t lds, double2 *__restrict__ dst,
-                         int64_t ldd) {
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
-    dst[r * ldd] = src[r * lds];
-}
-
-// partial block inner products C[i][j] = sum_r conj(X[r][i]) Y[r][j]
-__global__ void __launch_bounds__(QS_BLOCK)
-k_bdot(int64_t n, int kx, int ky, const double2 *__restrict__ X, int64_t ldx, const double2 *__restrict__ Y,
-       int64_t ldy, double2 *__restrict__ part) {
-  __shared__ double2 red[QS_BLOCK];
-  const int np = kx * ky;
-  const int groups = blockDim.x / np;
-  const int t = threadIdx.x;
-  const int pair = t % np, grp = t / np;
-  double sr = 0.0, si = 0.0;
-  if (grp < groups) {
-    const int i = pair / ky, j = pair % ky;
-    for (int64_t r = (int64_t)blockIdx.x * groups + grp; r < n; r += (int64_t)gridDim.x * groups) {
-      const double2 a = X[r * ldx + i], b = Y[r * ldy + j];
-      sr += a.x * b.x + a.y * b.y;
-      si += a.x * b.y - a.y * b.x;
-    }
-  }
-  red[t] = make_double2(sr, si);
-  __syncthreads();
-  if (t < np) {
-    double2 s = red[t];
-    for (int g = 1; g < groups; ++g) {
-      s.x += red[g * np + t].x;
-      s.y += red[g * np + t].y;
-    }
-    part[(size_t)blockIdx.x * np + t] = s;
-  }
-}
-
 // one block per output element: fixed-order sum of the per-block partials
 __global__ void __launch_bounds__(QS_BLOCK)
 k_bdot_reduce(int nparts, int np, const double2 *__restrict__ part, double2 *__restrict__ out) {
@@ -111,37 +64,174 @@ k_bdot_reduce(int nparts, int np, const double2 *__restrict__ part, double2 *__r
   if (threadIdx.x == 0) out[e] = make_double2(r, i);
 }
 
-// Y = beta*Y + X C   (X: n x kx, C: kx x ky row-major, Y: n x ky)
+// Tall-skinny products against a set of basis vectors V (vector i of length n
+// at vset_ptr(V, i): the first na vectors at a + i*ld, the rest at
+// b + (i - na)*ld, so two separately stored blocks act as one set) and a block
+// W of w <= QS_KMAX columns (entry (r, j) at W[r*ldw + j*ldj]): C = V^+ W and
+// W <- beta W + V C. Each reads V once, which is what bounds a full
+// re-orthogonalization.
+struct VSet {
+  const double2 *a;
+  const double2 *b;
+  int na;
+  int64_t ld;
+};
+
+__device__ __forceinline__ const double2 *vset_ptr(const VSet &s, int i) {
+  return i < s.na ? s.a + (int64_t)i * s.ld : s.b + (int64_t)(i - s.na) * s.ld;
+}
+
+typedef double qs_d4 __attribute__((ext_vector_type(4)));
+
+// part[c*m*w + i*w + j] = sum over the rows of wave chunk c (rows
+// [c*rw, (c+1)*rw)) of conj(V_i[r]) W_j[r].
+//
+// Each wave owns one chunk and T tiles of 16 vectors; the FP64 matrix core
+// (v_mfma_f64_16x16x4: A 16x4 with A[i][k] in lane i + 16k, B 4x16 with
+// B[k][j] in lane j + 16k, D 16x16 with D[i][j] in lane j + 16(i%4), element
+// i/4) sums over 4 rows per instruction, so the rows need no cross-lane
+// reduction until the chunk is done. k = lane/16 selects row rg + 4u + k at
+// sub-step u, which gives each vector 64 contiguous bytes per load.
+// The complex product is split into two real ones, A = Re V then A = Im V,
+// against B columns in one of two forms:
+//   (Re W, Im W)  -> Re C = sum Re V Re W + Im V Im W
+//   (Im W, -Re W) -> Im C = sum Re V Im W - Im V Re W
+// WT = 1 (w <= 8): columns j < w hold the first form of W_j, w <= j < 2w the
+// second form of W_{j-w}. WT = 2 (w <= 16): tile 0 holds the first form,
+// tile 1 the second, column j for W_j.
+template <int WT, int T>
 __global__ void __launch_bounds__(QS_BLOCK)
-k_bupd(int64_t n, int kx, int ky, const double2 *__restrict__ X, int64_t ldx, const double2 *__restrict__ C,
-       double2 *__restrict__ Y, int64_t ldy, double beta) {
-  __shared__ double2 sc[QS_KMAX * QS_KMAX];
-  for (int i = threadIdx.x; i < kx * ky; i += blockDim.x) sc[i] = C[i];
-  __syncthreads();
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    double2 x[QS_KMAX];
+k_vtw(int64_t n, int m, VSet V, const double2 *__restrict__ Wm, int64_t ldw, int64_t ldj, int w, int rw,
+      double2 *__restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.y * (QS_BLOCK / 64) + (threadIdx.x >> 6);
+  const int64_t r0 = c * rw;
+  if (r0 >= n) return;
+  const int64_t r1 = min<int64_t>(n, r0 + rw);
+  const int q = lane >> 4, col = lane & 15;
+  const int t0 = blockIdx.x * T;
+  const int ntiles = (m + 15) >> 4;
+  bool bim[WT];
+  const bool bok = WT == 1 ? col < 2 * w : col < w;
+  const int bj = WT == 1 ? (col >= w ? col - w : col) : col;
+  if (WT == 1) bim[0] = col >= w;
+  else {
+    bim[0] = false;
+    bim[WT - 1] = true;
+  }
+  const double2 *wp = Wm + (int64_t)(bok ? bj : 0) * ldj;
+  const double2 *vp[T];
+  bool vok[T];
 #pragma unroll
-    for (int i = 0; i < QS_KMAX; ++i)
-      if (i < kx) x[i] = X[r * ldx + i];
-    for (int j = 0; j < ky; ++j) {
-      double ar = 0.0, ai = 0.0;
-      if (beta != 0.0) {
-        const double2 y = Y[r * ldy + j];
-        ar = beta * y.x;
-        ai = beta * y.y;
+  for (int t = 0; t < T; ++t) {
+    const int i = (t0 + t) * 16 + col;
+    vok[t] = i < m;
+    vp[t] = vset_ptr(V, vok[t] ? i : 0);
+  }
+  qs_d4 acc[T][WT];
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+#pragma unroll
+    for (int bt = 0; bt < WT; ++bt) acc[t][bt] = qs_d4{0.0, 0.0, 0.0, 0.0};
+  for (int64_t rg = r0; rg < r1; rg += 16) {
+    double2 wv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t r = rg + 4 * u + q;
+      wv[u] = (bok && r < r1) ? wp[r * ldw] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      if (t0 + t >= ntiles) break;
+      double2 av[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t r = rg + 4 * u + q;
+        av[u] = (vok[t] && r < r1) ? vp[t][r] : make_double2(0.0, 0.0);
       }
 #pragma unroll
-      for (int i = 0; i < QS_KMAX; ++i) {
-        if (i < kx) {
-          const double2 c = sc[i * ky + j];
-          ar = fma(x[i].x, c.x, fma(-x[i].y, c.y, ar));
-          ai = fma(x[i].x, c.y, fma(x[i].y, c.x, ai));
+      for (int u = 0; u < 4; ++u) {
+#pragma unroll
+        for (int bt = 0; bt < WT; ++bt) {
+          const double b0 = bim[bt] ? wv[u].y : wv[u].x;
+          const double b1 = bim[bt] ? -wv[u].x : wv[u].y;
+          acc[t][bt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].x, b0, acc[t][bt], 0, 0, 0);
+          acc[t][bt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u].y, b1, acc[t][bt], 0, 0, 0);
         }
       }
-      Y[r * ldy + j] = make_double2(ar, ai);
+    }
+  }
+  if (!bok) return;
+  double *out = (double *)part;
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    if (t0 + t >= ntiles) break;
+#pragma unroll
+    for (int bt = 0; bt < WT; ++bt) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int i = (t0 + t) * 16 + q + 4 * v;
+        if (i < m) out[2 * (((size_t)c * m + i) * w + bj) + (bim[bt] ? 1 : 0)] = acc[t][bt][v];
+      }
     }
   }
 }
+
+// W[r][j] = beta W[r][j] + sum_i V_i[r] C[i][j]  (C: m x WC row-major, device)
+template <int WC>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_vupd(int64_t n, int m, VSet V, const double2 *__restrict__ C, double2 *__restrict__ Wm, int64_t ldw, int64_t ldj,
+       double beta) {
+  constexpr int TI = 64;
+  __shared__ double2 sc[TI * WC];
+  for (int64_t rb = (int64_t)blockIdx.x * blockDim.x; rb < n; rb += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = rb + threadIdx.x;
+    const bool act = r < n;
+    double2 acc[WC];
+#pragma unroll
+    for (int j = 0; j < WC; ++j) {
+      acc[j] = make_double2(0.0, 0.0);
+      if (act && beta != 0.0) {
+        const double2 y = Wm[r * ldw + j * ldj];
+        acc[j] = make_double2(beta * y.x, beta * y.y);
+      }
+    }
+    for (int i0 = 0; i0 < m; i0 += TI) {
+      const int tn = min(TI, m - i0);
+      __syncthreads();
+      for (int e = threadIdx.x; e < tn * WC; e += blockDim.x) sc[e] = C[(size_t)i0 * WC + e];
+      __syncthreads();
+      if (act) {
+        for (int i = 0; i < tn; ++i) {
+          const double2 v = vset_ptr(V, i0 + i)[r];
+#pragma unroll
+          for (int j = 0; j < WC; ++j) {
+            const double2 c = sc[i * WC + j];
+            acc[j].x = fma(v.x, c.x, fma(-v.y, c.y, acc[j].x));
+            acc[j].y = fma(v.x, c.y, fma(v.y, c.x, acc[j].y));
+          }
+        }
+      }
+    }
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < WC; ++j) Wm[r * ldw + j * ldj] = acc[j];
+    }
+  }
+}
+
+// Runtime width -> template instantiation (1..QS_KMAX).
+#define QS_WIDTH_SWITCH(w, CALL)                                         \
+  switch (w) {                                                           \
+    case 1: CALL(1); break;   case 2: CALL(2); break;                     \
+    case 3: CALL(3); break;   case 4: CALL(4); break;                     \
+    case 5: CALL(5); break;   case 6: CALL(6); break;                     \
+    case 7: CALL(7); break;   case 8: CALL(8); break;                     \
+    case 9: CALL(9); break;   case 10: CALL(10); break;                   \
+    case 11: CALL(11); break; case 12: CALL(12); break;                   \
+    case 13: CALL(13); break; case 14: CALL(14); break;                   \
+    case 15: CALL(15); break; default: CALL(16); break;                   \
+  }
 
 // Tensor continued fraction: one workgroup per energy point. LDS holds the
 // running p x p block, a product buffer and the Gauss-Jordan work matrix.
@@ -281,70 +371,91 @@ int grid_for(qs_ctx *ctx, int64_t n) {
 struct Work {
   qs_ctx *ctx;
   int64_t n;
-  double2 *part = nullptr;  // QS_BDOT_BLOCKS * 256
-  double2 *dC = nullptr;    // 256 complex results / coefficients
   qs_status init(qs_ctx *c, int64_t nn) {
     ctx = c;
     n = nn;
-    QS_TRY(qs_malloc(ctx, (void **)&part, (size_t)QS_BDOT_BLOCKS * 256 * 16));
-    QS_TRY(qs_malloc(ctx, (void **)&dC, 512 * 16));
     return QS_OK;
   }
+  // buffers of the basis products (grown on demand)
+  double2 *vpart = nullptr;
+  size_t vpart_bytes = 0;
+  double2 *vC = nullptr;
+  size_t vC_bytes = 0;
   ~Work() {
-    if (part) (void)hipFree(part);
-    if (dC) (void)hipFree(dC);
+    if (vpart) (void)hipFree(vpart);
+    if (vC) (void)hipFree(vC);
   }
-  // C[i*ky + j] = <X_i|Y_j>
-  qs_status bdot(const double2 *X, int kx, int64_t ldx, const double2 *Y, int ky, int64_t ldy, std::vector<cd> &C) {
-    C.assign((size_t)kx * ky, cd(0));
-    if (kx == 0 || ky == 0) return QS_OK;
-    const int np = kx * ky;
-    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(QS_BDOT_BLOCKS, (n + 63) / 64));
-    hipLaunchKernelGGL(k_bdot, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, n, kx, ky, X, ldx, Y, ldy, part);
-    hipLaunchKernelGGL(k_bdot_reduce, dim3(np), dim3(QS_BLOCK), 0, ctx->stream, nb, np, part, dC);
+  qs_status grow(double2 **p, size_t *have, size_t need) {
+    if (need <= *have) return QS_OK;
+    QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    QS_TRY(qs_malloc(ctx, (void **)p, need));
+    *have = need;
+    return QS_OK;
+  }
+  // vector-major sets: one block (vectors at a + i*n) or two blocks, the
+  // first holding na vectors
+  VSet vs(const double2 *a) const { return VSet{a, a, INT_MAX, n}; }
+  VSet vs2(const double2 *a, int na, const double2 *b) const { return VSet{a, b, na, n}; }
+  // C[i*w + j] = <V_i|W_j> for the m vectors of V and the w vectors of W
+  // (vector-major, W_j at W + j*n)
+  qs_status vtw(const VSet &V, int m, const double2 *W, int w, std::vector<cd> &C) {
+    C.assign((size_t)m * w, cd(0));
+    if (m == 0 || w == 0) return QS_OK;
+    if (w > QS_KMAX) return qs_fail(ctx, QS_EINVAL, "block wider than %d", QS_KMAX);
+    const int wt = w <= 8 ? 1 : 2;
+    const int tiles_per_wave = wt == 1 ? 8 : 4;
+    const int ntiles = (m + 15) / 16;
+    const int xg = (ntiles + tiles_per_wave - 1) / tiles_per_wave;
+    // rows per wave: about 6144 waves in all, at least 32 rows per column so
+    // the partial sums stay a small fraction of the basis
+    int64_t rw = std::max<int64_t>({(n * xg + 6143) / 6144, (int64_t)32 * w, 64});
+    rw = (rw + 15) / 16 * 16;
+    const int64_t nch = (n + rw - 1) / rw;
+    const unsigned gy = (unsigned)((nch + QS_BLOCK / 64 - 1) / (QS_BLOCK / 64));
+    QS_TRY(grow(&vpart, &vpart_bytes, (size_t)nch * m * w * 16));
+    QS_TRY(grow(&vC, &vC_bytes, (size_t)m * w * 16));
+    if (wt == 1)
+      hipLaunchKernelGGL((k_vtw<1, 8>), dim3((unsigned)xg, gy), dim3(QS_BLOCK), 0, ctx->stream, n, m, V, W,
+                         (int64_t)1, n, w, (int)rw, vpart);
+    else
+      hipLaunchKernelGGL((k_vtw<2, 4>), dim3((unsigned)xg, gy), dim3(QS_BLOCK), 0, ctx->stream, n, m, V, W,
+                         (int64_t)1, n, w, (int)rw, vpart);
+    hipLaunchKernelGGL(k_bdot_reduce, dim3(m * w), dim3(QS_BLOCK), 0, ctx->stream, (int)nch, m * w,
+                       (const double2 *)vpart, vC);
     QS_HIP(ctx, hipGetLastError());
-    QS_HIP(ctx, hipMemcpyAsync(C.data(), dC, (size_t)np * 16, hipMemcpyDeviceToHost, ctx->stream));
+    QS_HIP(ctx, hipMemcpyAsync(C.data(), vC, (size_t)m * w * 16, hipMemcpyDeviceToHost, ctx->stream));
     QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return QS_OK;
   }
-  // Y = beta*Y + X C
-  qs_status bupd(const double2 *X, int kx, int64_t ldx, const std::vector<cd> &C, double2 *Y, int ky, int64_t ldy,
-                 double beta) {
-    if (ky == 0) return QS_OK;
-    if (kx == 0 && beta == 1.0) return QS_OK;
-    // the coefficient buffer is reused by the next call: wait for prior kernels
-    QS_HIP(ctx, hipMemcpyAsync(dC, C.data(), (size_t)kx * ky * 16, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_bupd, dim3(grid_for(ctx, n)), dim3(QS_BLOCK), 0, ctx->stream, n, kx, ky, X, ldx,
-                       (const double2 *)dC, Y, ldy, beta);
+  // W = beta W + V C (C host, m x w row-major; W vector-major)
+  qs_status vupd(const VSet &V, int m, const std::vector<cd> &C, double2 *W, int w, double beta) {
+    if (w == 0) return QS_OK;
+    if (w > QS_KMAX) return qs_fail(ctx, QS_EINVAL, "block wider than %d", QS_KMAX);
+    QS_TRY(grow(&vC, &vC_bytes, (size_t)std::max(m, 1) * w * 16));
+    if (m > 0) QS_HIP(ctx, hipMemcpyAsync(vC, C.data(), (size_t)m * w * 16, hipMemcpyHostToDevice, ctx->stream));
+#define QS_VUPD(WC)                                                                                         \
+  hipLaunchKernelGGL(k_vupd<WC>, dim3(grid_for(ctx, n)), dim3(QS_BLOCK), 0, ctx->stream, n, m, V,             \
+                     (const double2 *)vC, W, (int64_t)1, n, beta)
+    QS_WIDTH_SWITCH(w, QS_VUPD)
+#undef QS_VUPD
     QS_HIP(ctx, hipGetLastError());
-    QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    QS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // vC is reused by the next call
     return QS_OK;
   }
 };
-
-// y (interleaved, k vectors) = H x
-qs_status block_apply(qs_ctx *ctx, const qs_hop *h, const double2 *x, int k, int64_t ld, double2 *y, double2 *tmp) {
-  if (k == 0) return QS_OK;
-  if (h->assembled) return qs_hop_apply_strided(ctx, h, k, (const double *)x, ld, (double *)y, ld);
-  const int64_t n = h->rows->n;
-  for (int v = 0; v < k; ++v) {
-    hipLaunchKernelGGL(k_column, dim3(grid_for(ctx, n)), dim3(QS_BLOCK), 0, ctx->stream, n, x + v, ld, tmp, 1);
-    QS_TRY(qs_launch_mf_ext(ctx, h, (const double *)tmp, (double *)(tmp + n)));
-    hipLaunchKernelGGL(k_column, dim3(grid_for(ctx, n)), dim3(QS_BLOCK), 0, ctx->stream, n, tmp + n, 1, y + v, ld);
-  }
-  QS_HIP(ctx, hipGetLastError());
-  return QS_OK;
-}
 
 // Orthonormalize the k columns of W (in place into V): overlap S = W^+ W,
 // S = U diag(l) U^+, keep directions with sqrt(l) > max(sv_cut, 3 sqrt(eps) s_max)
 // (the singular-value cut of ComplexWaveFunctionListSVDOrthogonalizeSameBasisOnce,
 // Core/ComplexWaveFunctionArithmetic.cpp:1356-1550). Without reduction the
 // symmetric transform S^-1/2 is used, otherwise U_keep diag(l^-1/2).
-qs_status lowdin(Work &wk, const double2 *W, int k, double2 *V, double sv_cut, int *kept) {
-  std::vector<cd> S;
-  QS_TRY(wk.bdot(W, k, k, W, k, k, S));
-  // S[i*k+j] = <W_i|W_j>, column-major for heev: element (i,j) at i + j*k -> transpose = conj
+// Loewdin transform from the overlap S (S[i*k+j] = <W_i|W_j>): k x kept
+// row-major, V = W Tm.
+std::vector<cd> lowdin_transform(const std::vector<cd> &S, int k, double sv_cut, int *kept) {
+  // column-major for heev: element (i,j) at i + j*k -> transpose = conj
   std::vector<cd> A((size_t)k * k);
   for (int i = 0; i < k; ++i)
     for (int j = 0; j < k; ++j) A[i + (size_t)j * k] = S[(size_t)i * k + j];
@@ -360,15 +471,15 @@ qs_status lowdin(Work &wk, const double2 *W, int k, double2 *V, double sv_cut, i
     if (l[i] > 0 && std::sqrt(l[i]) > cut) keep.push_back(i);
   const int kp = (int)keep.size();
   *kept = kp;
-  if (kp == 0) return QS_OK;
-  std::vector<cd> Tm((size_t)k * kp, cd(0));  // row-major k x kp
+  std::vector<cd> Tm((size_t)k * std::max(kp, 1), cd(0));  // row-major k x kp
+  if (kp == 0) return Tm;
   if (kp == k) {
     // S^-1/2 = U diag(l^-1/2) U^+
     for (int i = 0; i < k; ++i)
       for (int j = 0; j < k; ++j) {
-        cd s = 0;
-        for (int m = 0; m < k; ++m) s += U[i + (size_t)m * k] * (1.0 / std::sqrt(l[m])) * std::conj(U[j + (size_t)m * k]);
-        Tm[(size_t)i * k + j] = s;
+        cd sum = 0;
+        for (int m = 0; m < k; ++m) sum += U[i + (size_t)m * k] * (1.0 / std::sqrt(l[m])) * std::conj(U[j + (size_t)m * k]);
+        Tm[(size_t)i * k + j] = sum;
       }
   } else {
     for (int i = 0; i < k; ++i)
@@ -377,19 +488,36 @@ qs_status lowdin(Work &wk, const double2 *W, int k, double2 *V, double sv_cut, i
         Tm[(size_t)i * kp + jj] = U[i + (size_t)m * k] / std::sqrt(l[m]);
       }
   }
-  QS_TRY(wk.bupd(W, k, k, Tm, V, kp, kp, 0.0));
-  return QS_OK;
+  return Tm;
 }
 
-// W -= X (X^+ W)
-qs_status project_out(Work &wk, const double2 *X, int kx, double2 *W, int kw, std::vector<cd> *accum) {
+// vector-major blocks (vector v at + v*n)
+qs_status lowdin_v(Work &wk, const double2 *W, int k, double2 *V, double sv_cut, int *kept) {
+  std::vector<cd> S;
+  QS_TRY(wk.vtw(wk.vs(W), k, W, k, S));
+  const std::vector<cd> Tm = lowdin_transform(S, k, sv_cut, kept);
+  if (*kept == 0) return QS_OK;
+  return wk.vupd(wk.vs(W), k, Tm, V, *kept, 0.0);
+}
+
+// W -= X (X^+ W) for the kx vectors of X; the coefficients X^+ W are added to
+// *accum (kx x kw row-major) when given
+qs_status project_out_v(Work &wk, const VSet &X, int kx, double2 *W, int kw, std::vector<cd> *accum = nullptr) {
   if (kx == 0 || kw == 0) return QS_OK;
   std::vector<cd> C;
-  QS_TRY(wk.bdot(X, kx, kx, W, kw, kw, C));
+  QS_TRY(wk.vtw(X, kx, W, kw, C));
   if (accum)
     for (size_t i = 0; i < C.size(); ++i) (*accum)[i] += C[i];
   for (auto &c : C) c = -c;
-  QS_TRY(wk.bupd(X, kx, kx, C, W, kw, kw, 1.0));
+  return wk.vupd(X, kx, C, W, kw, 1.0);
+}
+
+// y = H x for k vectors laid out one after another
+qs_status block_apply_v(qs_ctx *ctx, const qs_hop *h, const double2 *x, int k, double2 *y) {
+  if (k == 0) return QS_OK;
+  const int64_t n = h->rows->n;
+  if (h->assembled) return qs_hop_apply_vectors(ctx, h, k, (const double *)x, 1, n, (double *)y, 1, n);
+  for (int v = 0; v < k; ++v) QS_TRY(qs_launch_mf_ext(ctx, h, (const double *)(x + (size_t)v * n), (double *)(y + (size_t)v * n)));
   return QS_OK;
 }
 
@@ -406,26 +534,26 @@ extern "C" qs_status qs_block_lanczos(qs_ctx *ctx, const qs_hop *h, int k, const
   const int64_t n = h->rows->n;
   const size_t bb = (size_t)n * k * 16;
   char *mem = nullptr;
-  QS_TRY(qs_malloc(ctx, (void **)&mem, 5 * bb + (size_t)n * 32 + 256));
+  QS_TRY(qs_malloc(ctx, (void **)&mem, 5 * bb + 256));
   double2 *X0 = (double2 *)mem, *B1 = (double2 *)(mem + bb), *B2 = (double2 *)(mem + 2 * bb);
-  double2 *B3 = (double2 *)(mem + 3 * bb), *Wb = (double2 *)(mem + 4 * bb), *tmp = (double2 *)(mem + 5 * bb);
+  double2 *B3 = (double2 *)(mem + 3 * bb), *Wb = (double2 *)(mem + 4 * bb);
   Work wk;
   qs_status st = wk.init(ctx, n);
   const size_t kk = (size_t)k * k;
   std::vector<cd> Aall((size_t)(nsteps + 1) * kk, cd(0)), Ball((size_t)nsteps * kk, cd(0));
   std::vector<int> kept(nsteps + 1, 0);
   kept[0] = k;
+  // blocks are stored one vector after another (vector v of a block at + v*n)
   auto run = [&]() -> qs_status {
     QS_TRY(st);
-    hipLaunchKernelGGL(k_pack, dim3(grid_for(ctx, n * k)), dim3(QS_BLOCK), 0, ctx->stream, n, k, (const double2 *)V0, X0, 1);
-    QS_HIP(ctx, hipGetLastError());
+    QS_HIP(ctx, hipMemcpyAsync(X0, V0, bb, hipMemcpyDeviceToDevice, ctx->stream));
     int p = 0;
-    QS_TRY(lowdin(wk, X0, k, B1, sv_cut, &p));
+    QS_TRY(lowdin_v(wk, X0, k, B1, sv_cut, &p));
     kept[1] = p;
     if (p == 0) return QS_OK;
     {  // B0[i][j] = <V1_j|X0_i>
       std::vector<cd> C;
-      QS_TRY(wk.bdot(B1, p, p, X0, k, k, C));
+      QS_TRY(wk.vtw(wk.vs(B1), p, X0, k, C));
       for (int i = 0; i < k; ++i)
         for (int j = 0; j < p; ++j) Ball[(size_t)i * k + j] = C[(size_t)j * k + i];
     }
@@ -434,32 +562,30 @@ extern "C" qs_status qs_block_lanczos(qs_ctx *ctx, const qs_hop *h, int k, const
     double2 *Vprev = nullptr, *Vcur = B1, *Vnext = B2;
     int pprev = 0;
     for (int s = 1; s <= nsteps; ++s) {
-      QS_TRY(block_apply(ctx, h, Vcur, p, p, Wb, tmp));
+      QS_TRY(block_apply_v(ctx, h, Vcur, p, Wb));
+      // X = [V_{s-1}, V_s]; X^+ H V_s holds B_{s-1} (first pprev rows) and A_s
+      const VSet X = pprev > 0 ? wk.vs2(Vprev, pprev, Vcur) : wk.vs(Vcur);
+      const int kx = pprev + p;
       std::vector<cd> C;
-      QS_TRY(wk.bdot(Vcur, p, p, Wb, p, p, C));  // A_s
+      QS_TRY(wk.vtw(X, kx, Wb, p, C));
       for (int i = 0; i < p; ++i)
-        for (int j = 0; j < p; ++j) Aall[(size_t)s * kk + (size_t)i * k + j] = C[(size_t)i * p + j];
-      if (s >= 2) {
-        QS_TRY(wk.bdot(Vprev, pprev, pprev, Wb, p, p, C));  // B_{s-1} = V_{s-1}^+ H V_s
-        for (int i = 0; i < pprev; ++i)
-          for (int j = 0; j < p; ++j) Ball[(size_t)(s - 1) * kk + (size_t)i * k + j] = C[(size_t)i * p + j];
-      }
+        for (int j = 0; j < p; ++j) Aall[(size_t)s * kk + (size_t)i * k + j] = C[(size_t)(pprev + i) * p + j];
+      for (int i = 0; i < pprev; ++i)
+        for (int j = 0; j < p; ++j) Ball[(size_t)(s - 1) * kk + (size_t)i * k + j] = C[(size_t)i * p + j];
       if (s == nsteps) break;
-      // three rounds of projection against the current and previous blocks, then
-      // Loewdin; repeated while directions are dropped
+      // two rounds of classical Gram-Schmidt against the previous and current
+      // blocks together (the first with the coefficients above), then Loewdin;
+      // repeated on the reduced block while directions are dropped
+      for (auto &c : C) c = -c;
+      QS_TRY(wk.vupd(X, kx, C, Wb, p, 1.0));
       int pw = p, pn = 0;
       while (true) {
-        QS_TRY(project_out(wk, Vcur, p, Wb, pw, nullptr));
-        QS_TRY(project_out(wk, Vcur, p, Wb, pw, nullptr));
-        QS_TRY(project_out(wk, Vprev, pprev, Wb, pw, nullptr));
-        QS_TRY(project_out(wk, Vprev, pprev, Wb, pw, nullptr));
-        QS_TRY(project_out(wk, Vcur, p, Wb, pw, nullptr));
-        QS_TRY(project_out(wk, Vprev, pprev, Wb, pw, nullptr));
-        QS_TRY(lowdin(wk, Wb, pw, Vnext, sv_cut, &pn));
+        QS_TRY(project_out_v(wk, X, kx, Wb, pw));
+        QS_TRY(lowdin_v(wk, Wb, pw, Vnext, sv_cut, &pn));
         if (pn == pw || pn == 0) break;
-        // re-run on the reduced block: copy Vnext (pn wide) back into Wb layout
         QS_HIP(ctx, hipMemcpyAsync(Wb, Vnext, (size_t)n * pn * 16, hipMemcpyDeviceToDevice, ctx->stream));
         pw = pn;
+        QS_TRY(project_out_v(wk, X, kx, Wb, pw));
       }
       kept[s + 1] = pn;
       if (pn == 0) break;  // invariant subspace: remaining blocks stay zero
@@ -537,43 +663,48 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
   if (tol <= 0) tol = 100.0 * std::sqrt(2.220446049250313e-16);
   const int keep_max = std::min(QS_KMAX, std::max(2 * p, p + 6));
   int mmax = (int)std::min<int64_t>(n, std::max(max_krylov, keep_max + 3 * p));
-  // blocks of the Krylov basis live in one buffer, each block at a fixed slot
-  const size_t slot_bytes = (size_t)n * QS_KMAX * 16;
-  // storage: mmax + 2*QS_KMAX vectors of basis + W + Y + tmp
+  // the Krylov basis and the working blocks are stored one vector after
+  // another (vector i at Q + i*n), so one pass over the basis projects a new
+  // block against every basis vector
   const size_t vbytes = (size_t)n * 16;
+  const size_t slot_bytes = (size_t)n * QS_KMAX * 16;
   char *mem = nullptr;
-  QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)(mmax + QS_KMAX) * vbytes + 2 * slot_bytes + 2 * vbytes + 256));
-  double2 *Qbuf = (double2 *)mem;
+  QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)(mmax + QS_KMAX) * vbytes + 3 * slot_bytes + 256));
+  double2 *Q = (double2 *)mem;
   double2 *Wb = (double2 *)(mem + (size_t)(mmax + QS_KMAX) * vbytes);
   double2 *Yb = (double2 *)((char *)Wb + slot_bytes);
-  double2 *tmp = (double2 *)((char *)Yb + slot_bytes);
+  double2 *Cb = (double2 *)((char *)Yb + slot_bytes);
   Work wk;
   qs_status st = wk.init(ctx, n);
   struct Blk {
-    double2 *ptr;
+    int off;  // first basis vector (and column of the projected matrix)
     int w;
-    int off;  // column offset in the projected matrix
   };
   std::vector<Blk> blocks;
   std::vector<double> theta;
   std::vector<cd> S;
+  // block of w vectors -> basis vectors [off, off + w)
+  auto store = [&](const double2 *blk, int w, int off) -> qs_status {
+    QS_HIP(ctx, hipMemcpyAsync(Q + (size_t)off * n, blk, (size_t)w * n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    return QS_OK;
+  };
   int iters = 0;
   auto run = [&]() -> qs_status {
     QS_TRY(st);
     // start block
     if (start) {
-      hipLaunchKernelGGL(k_pack, dim3(grid_for(ctx, n * p)), dim3(QS_BLOCK), 0, ctx->stream, n, p,
-                         (const double2 *)start, Wb, 1);
+      QS_HIP(ctx, hipMemcpyAsync(Wb, start, (size_t)n * p * 16, hipMemcpyDeviceToDevice, ctx->stream));
     } else {
       hipLaunchKernelGGL(k_rand, dim3(grid_for(ctx, n * p)), dim3(QS_BLOCK), 0, ctx->stream, n * p, seed, Wb);
     }
     QS_HIP(ctx, hipGetLastError());
     int w0 = 0;
-    QS_TRY(lowdin(wk, Wb, p, Qbuf, 0.0, &w0));
+    QS_TRY(lowdin_v(wk, Wb, p, Cb, 0.0, &w0));
     if (w0 == 0) return qs_fail(ctx, QS_EINVAL, "start vectors are linearly dependent");
-    blocks.push_back({Qbuf, w0, 0});
+    QS_TRY(store(Cb, w0, 0));
+    blocks.push_back({0, w0});
     int m = w0;               // basis vectors so far
-    std::vector<cd> T;        // projected matrix, column-major, ld = mmax + QS_KMAX
+    std::vector<cd> T;        // projected matrix, column-major
     const int ldT = mmax + 2 * QS_KMAX;
     T.assign((size_t)ldT * ldT, cd(0));
     auto Tat = [&](int i, int j) -> cd & { return T[(size_t)i + (size_t)j * ldT]; };
@@ -582,31 +713,53 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
     while (true) {
       ++iters;
       const Blk cur = blocks.back();
-      QS_TRY(block_apply(ctx, h, cur.ptr, cur.w, cur.w, Wb, tmp));
-      // full re-orthogonalization, two passes; coefficients accumulate the
-      // projected matrix column block of `cur`
-      for (int pass = 0; pass < 2; ++pass) {
-        for (const Blk &b : blocks) {
-          std::vector<cd> C((size_t)b.w * cur.w, cd(0));
-          QS_TRY(project_out(wk, b.ptr, b.w, Wb, cur.w, &C));
-          // first pass overwrites (the coupling written when the block was
-          // appended is recomputed here), the second adds its correction
-          for (int i = 0; i < b.w; ++i)
-            for (int j = 0; j < cur.w; ++j) {
-              cd &tv = Tat(b.off + i, cur.off + j);
-              tv = (pass == 0 ? cd(0) : tv) + C[(size_t)i * cur.w + j];
-            }
-        }
+      QS_TRY(block_apply_v(ctx, h, Q + (size_t)cur.off * n, cur.w, Wb));
+      // Local step: project W = H Q_cur against the current block and the one
+      // before it (basis range [lo, m)), which hold all of its components in
+      // exact arithmetic; the same product gives W^+ W for the norms below.
+      const int lo = blocks.size() >= 2 ? blocks[blocks.size() - 2].off : cur.off;
+      const int nl = m - lo;
+      std::vector<cd> C;
+      QS_TRY(wk.vtw(wk.vs2(Q + (size_t)lo * n, nl, Wb), nl + cur.w, Wb, cur.w, C));
+      std::vector<double> wnorm2(cur.w);
+      for (int j = 0; j < cur.w; ++j) {
+        double r2 = C[(size_t)(nl + j) * cur.w + j].real();
+        for (int i = 0; i < nl; ++i) r2 -= std::norm(C[(size_t)i * cur.w + j]);
+        wnorm2[j] = r2;
       }
-      for (const Blk &b : blocks)
-        for (int i = 0; i < b.w; ++i)
-          for (int j = 0; j < cur.w; ++j) Tat(cur.off + j, b.off + i) = std::conj(Tat(b.off + i, cur.off + j));
-      // new block
+      C.resize((size_t)nl * cur.w);
+      for (int i = 0; i < nl; ++i)
+        for (int j = 0; j < cur.w; ++j) Tat(lo + i, cur.off + j) = C[(size_t)i * cur.w + j];
+      for (auto &c : C) c = -c;
+      QS_TRY(wk.vupd(wk.vs(Q + (size_t)lo * n), nl, C, Wb, cur.w, 1.0));
+      // Full re-orthogonalization against all m basis vectors: one pass, and a
+      // second when the first removed more than half of a column's norm
+      // (Daniel-Gragg-Kaufman-Stewart). Its coefficients on the local range
+      // correct the projected matrix; those on older vectors are rounding-level
+      // and are dropped, which keeps the projected matrix block tridiagonal
+      // (plus the restart's arrow).
+      for (int pass = 0; pass < 2; ++pass) {
+        QS_TRY(wk.vtw(wk.vs(Q), m, Wb, cur.w, C));
+        bool again = false;
+        for (int j = 0; j < cur.w; ++j) {
+          double rem = 0.0;
+          for (int i = 0; i < m; ++i) rem += std::norm(C[(size_t)i * cur.w + j]);
+          if (rem > 0.5 * wnorm2[j]) again = true;
+          wnorm2[j] -= rem;
+        }
+        for (int i = lo; i < m; ++i)
+          for (int j = 0; j < cur.w; ++j) Tat(i, cur.off + j) += C[(size_t)i * cur.w + j];
+        for (auto &c : C) c = -c;
+        QS_TRY(wk.vupd(wk.vs(Q), m, C, Wb, cur.w, 1.0));
+        if (!again) break;
+      }
+      for (int i = 0; i < m; ++i)
+        for (int j = 0; j < cur.w; ++j) Tat(cur.off + j, i) = std::conj(Tat(i, cur.off + j));
+      // new block (in Cb; the current block is already stored)
       int wn = 0;
-      double2 *nptr = Qbuf + (size_t)m * n;
-      QS_TRY(lowdin(wk, Wb, cur.w, nptr, 0.0, &wn));
+      QS_TRY(lowdin_v(wk, Wb, cur.w, Cb, 0.0, &wn));
       std::vector<cd> R;  // R[i][j] = <Qnew_i|W_j>
-      if (wn > 0) QS_TRY(wk.bdot(nptr, wn, wn, Wb, cur.w, cur.w, R));
+      if (wn > 0) QS_TRY(wk.vtw(wk.vs(Cb), wn, Wb, cur.w, R));
       const bool exhausted = wn == 0 || m + wn > n;
       const bool full = mmax < n && m + wn + p > mmax;
       if (exhausted || full || m >= next_check) {
@@ -632,19 +785,16 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
           var[i] = exhausted ? 0.0 : r2;
           if (var[i] > tol * std::max(1.0, sumE2) / p) conv = false;
         }
+        // the first `cnt` Ritz vectors into Yb: Y = Q S[:, :cnt]
+        auto ritz = [&](int cnt) -> qs_status {
+          std::vector<cd> Cm((size_t)m * cnt);
+          for (int i = 0; i < m; ++i)
+            for (int j = 0; j < cnt; ++j) Cm[(size_t)i * cnt + j] = S[(size_t)i + (size_t)j * m];
+          return wk.vupd(wk.vs(Q), m, Cm, Yb, cnt, 0.0);
+        };
         if (conv || exhausted || restarts >= max_restarts) {
-          // Ritz vectors of the wanted states
-          std::vector<cd> Y((size_t)m * p);
-          for (int b = 0; b < (int)blocks.size(); ++b) {
-            const Blk &bl = blocks[b];
-            std::vector<cd> Cb((size_t)bl.w * p);
-            for (int i = 0; i < bl.w; ++i)
-              for (int j = 0; j < p; ++j) Cb[(size_t)i * p + j] = S[(size_t)(bl.off + i) + (size_t)j * m];
-            QS_TRY(wk.bupd(bl.ptr, bl.w, bl.w, Cb, Yb, p, p, b == 0 ? 0.0 : 1.0));
-          }
-          hipLaunchKernelGGL(k_pack, dim3(grid_for(ctx, n * p)), dim3(QS_BLOCK), 0, ctx->stream, n, p, Yb,
-                             (double2 *)vecs, 0);
-          QS_HIP(ctx, hipGetLastError());
+          QS_TRY(ritz(p));
+          QS_HIP(ctx, hipMemcpyAsync(vecs, Yb, (size_t)n * p * 16, hipMemcpyDeviceToDevice, ctx->stream));
           for (int i = 0; i < p; ++i) {
             E_out[i] = theta[i];
             if (variance_out) variance_out[i] = var[i];
@@ -656,13 +806,7 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
           // thick restart: keep r Ritz vectors + the new residual block
           ++restarts;
           const int r = std::min(keep_max, m - 1);
-          for (int b = 0; b < (int)blocks.size(); ++b) {
-            const Blk &bl = blocks[b];
-            std::vector<cd> Cb((size_t)bl.w * r);
-            for (int i = 0; i < bl.w; ++i)
-              for (int j = 0; j < r; ++j) Cb[(size_t)i * r + j] = S[(size_t)(bl.off + i) + (size_t)j * m];
-            QS_TRY(wk.bupd(bl.ptr, bl.w, bl.w, Cb, Yb, r, r, b == 0 ? 0.0 : 1.0));
-          }
+          QS_TRY(ritz(r));
           // coupling of the residual block to the kept Ritz vectors: <Qnew|H y_i> = R s_last,i
           std::vector<cd> Cnew((size_t)wn * r, cd(0));
           for (int a = 0; a < wn; ++a)
@@ -671,14 +815,12 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
               for (int j = 0; j < cur.w; ++j) s += R[(size_t)a * cur.w + j] * S[(size_t)(cur.off + j) + (size_t)i * m];
               Cnew[(size_t)a * r + i] = s;
             }
-          // move: Y -> slot 0 (r vectors), Qnew -> right after
-          QS_HIP(ctx, hipMemcpyAsync(Qbuf, Yb, (size_t)n * r * 16, hipMemcpyDeviceToDevice, ctx->stream));
-          QS_HIP(ctx, hipMemcpyAsync(Qbuf + (size_t)r * n, nptr, (size_t)n * wn * 16, hipMemcpyDeviceToDevice,
-                                     ctx->stream));
-          QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+          // basis: the r Ritz vectors, then the residual block
+          QS_TRY(store(Yb, r, 0));
+          QS_TRY(store(Cb, wn, r));
           blocks.clear();
-          blocks.push_back({Qbuf, r, 0});
-          blocks.push_back({Qbuf + (size_t)r * n, wn, r});
+          blocks.push_back({0, r});
+          blocks.push_back({r, wn});
           std::fill(T.begin(), T.end(), cd(0));
           for (int i = 0; i < r; ++i) Tat(i, i) = theta[i];
           for (int a = 0; a < wn; ++a)
@@ -698,7 +840,8 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
           Tat(m + a, cur.off + j) = R[(size_t)a * cur.w + j];
           Tat(cur.off + j, m + a) = std::conj(R[(size_t)a * cur.w + j]);
         }
-      blocks.push_back({nptr, wn, m});
+      QS_TRY(store(Cb, wn, m));
+      blocks.push_back({m, wn});
       m += wn;
     }
   };

@@ -26,7 +26,8 @@ __device__ __forceinline__ int64_t wave_uniform(int64_t v) {
 // (-1)^popcount(d & P_t) V_t x[c] for KC interleaved vectors when `hit`.
 template <int KC, int NW>
 __device__ __forceinline__ void sell_acc(bool hit, int64_t c, double2 v, const uint64_t *P, const uint64_t *d,
-                                         const double2 *__restrict__ x, int64_t ldx, double *ar, double *ai) {
+                                         const double2 *__restrict__ x, int64_t ldx, int64_t ldv, double *ar,
+                                         double *ai) {
   uint32_t p = 0;
 #pragma unroll
   for (int w = 0; w < NW; ++w) p += __popcll(d[w] & P[w]);
@@ -35,7 +36,7 @@ __device__ __forceinline__ void sell_acc(bool hit, int64_t c, double2 v, const u
   v.y = __longlong_as_double(__double_as_longlong(v.y) ^ sb);
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
-    const double2 xv = x[c * ldx + k];
+    const double2 xv = x[c * ldx + k * ldv];
     const double tr = fma(v.x, xv.x, fma(-v.y, xv.y, ar[k]));
     const double ti = fma(v.x, xv.y, fma(v.y, xv.x, ai[k]));
     ar[k] = hit ? tr : ar[k];
@@ -43,15 +44,16 @@ __device__ __forceinline__ void sell_acc(bool hit, int64_t c, double2 v, const u
   }
 }
 
-// Row s*64 + lane of the assembled operator A applied to KC interleaved
-// vectors (entry j of vector k at x[j*ldx + k]). `s` must be wave-uniform.
+// Row s*64 + lane of the assembled operator A applied to KC vectors, entry j
+// of vector k at x[j*ldx + k*ldv] (interleaved: ldx = KC, ldv = 1; one vector
+// after another: ldx = 1, ldv = n). `s` must be wave-uniform.
 // tv / tp: the term values and parity masks (LDS copies or A.val / A.par).
 // Both loops are branch-free so that many gathers are in flight per wave:
 // lanes without an entry read x[0] and keep their sums.
 template <int KC, int NW, bool PACKED>
 __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, const double2 *tv,
                                          const uint64_t *tp, const double2 *__restrict__ x, int64_t ldx,
-                                         double *ar, double *ai) {
+                                         int64_t ldv, double *ar, double *ai) {
   constexpr int U = KC == 1 ? 8 : 4;  // ELL positions per batch
 #pragma unroll
   for (int k = 0; k < KC; ++k) {
@@ -72,7 +74,7 @@ __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, co
       const double2 dg = A.diag[r];
 #pragma unroll
       for (int k = 0; k < KC; ++k) {
-        const double2 xv = x[r * ldx + k];
+        const double2 xv = x[r * ldx + k * ldv];
         ar[k] = dg.x * xv.x - dg.y * xv.y;
         ai[k] = dg.x * xv.y + dg.y * xv.x;
       }
@@ -99,7 +101,7 @@ __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, co
                               (uint32_t)__builtin_amdgcn_readlane(hv.z, k);
         const bool hit = (mask >> lane) & 1u;
         const int64_t c = hit ? (int64_t)base + lane : 0;
-        sell_acc<KC, NW>(hit, c, tv[t], tp + (size_t)t * NW, d, x, ldx, ar, ai);
+        sell_acc<KC, NW>(hit, c, tv[t], tp + (size_t)t * NW, d, x, ldx, ldv, ar, ai);
       }
     }
   }
@@ -133,7 +135,7 @@ __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, co
       const bool ok = j0 + u < cnt;
       const uint32_t c = ok ? (e[u] & cmask) : 0u;
       const uint32_t t = ok ? (PACKED ? e[u] >> cbits : tt[u]) : 0u;
-      sell_acc<KC, NW>(ok, (int64_t)c, tv[t], tp + (size_t)t * NW, d, x, ldx, ar, ai);
+      sell_acc<KC, NW>(ok, (int64_t)c, tv[t], tp + (size_t)t * NW, d, x, ldx, ldv, ar, ai);
     }
     if (more) {
 #pragma unroll

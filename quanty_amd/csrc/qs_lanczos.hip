@@ -19,8 +19,6 @@
 
 #define QS_RED_BLOCKS 1024
 
-qs_status qs_hop_apply_strided(qs_ctx *ctx, const qs_hop *h, int k, const double *x, int64_t ldx,
-                               double *y, int64_t ldy);
 qs_status qs_launch_mf_ext(qs_ctx *ctx, const qs_hop *h, const double *x, double *y);
 
 // ---------------------------------------------------------------------------
@@ -144,7 +142,7 @@ k_lz1_asm(int step, int nb2, LzState S, SellD A, const double2 *__restrict__ uc,
   double dot = 0.0;
   for (int64_t s = s0; s < s1; s += ds) {
     double hr[1], hi[1];
-    sell_row<1, NW, PACKED>(A, s, lane, tv, tp, uc, 1, hr, hi);
+    sell_row<1, NW, PACKED>(A, s, lane, tv, tp, uc, 1, 0, hr, hi);
     const int64_t r = s * 64 + lane;
     if (r < A.nrows) {
       const double2 u = uc[r];
