@@ -244,6 +244,17 @@ qs_status qs_batch_destroy(qs_batch *b);
 qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int max_krylov,
                            int max_restarts, uint64_t seed, const double *start, double *E_out,
                            double *vecs, double *variance_out, int *iterations_out);
+/* qs_ground_states with the Krylov schedule spelled out: the basis holds at
+ * most nstates + krylov_start vectors before the first thick restart and grows
+ * by krylov_step + nstates per restart while that stays below max_krylov — the
+ * NKrylovStart / NKrylovStep / NKrylovMax settings (defaults 100 / 10 / 500,
+ * Core/StructDef.cpp:298-300) as BlockLanczosGroundStateConserveBasisKrylov
+ * uses them (Core/BlockLanczos.cpp:5130, 5311-5325). qs_ground_states is this
+ * with krylov_start 100 and krylov_step 10. */
+qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int krylov_start,
+                              int krylov_step, int max_krylov, int max_restarts, uint64_t seed,
+                              const double *start, double *E_out, double *vecs, double *variance_out,
+                              int *iterations_out);
 
 /* Tensor Green's function from qs_block_lanczos output, evaluated on the GPU:
  * G(w) = B0^* G_1 B0^T, G_j = (z - A_j - B_j G_{j+1} B_j^+)^-1 bottom-up,

@@ -470,7 +470,8 @@ def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
         raise QSError(f"the space has {basis.n} determinants, fewer than Npsi = {Npsi}")
     hop = Hop(H.device(ctx), basis, mode="auto")
     E, vecs, var, it = hop.ground_states(Npsi, tol=float(o["Zero"]), max_krylov=int(o["NKrylovMax"]),
-                                         max_restarts=int(o["MaxRestarts"]), seed=int(o["Seed"]))
+                                         max_restarts=int(o["MaxRestarts"]), seed=int(o["Seed"]),
+                                         krylov_start=int(o["NKrylovStart"]), krylov_step=int(o["NKrylovStep"]))
     return [Wavefunction(nf, basis=basis, vec=vecs[i].clone()) for i in range(Npsi)]
 
 

@@ -649,9 +649,10 @@ extern "C" qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const doubl
 // ---------------------------------------------------------------------------
 // lowest eigenstates
 // ---------------------------------------------------------------------------
-extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int max_krylov,
-                                      int max_restarts, uint64_t seed, const double *start, double *E_out,
-                                      double *vecs, double *variance_out, int *iterations_out) {
+extern "C" qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int krylov_start,
+                                         int krylov_step, int max_krylov, int max_restarts, uint64_t seed,
+                                         const double *start, double *E_out, double *vecs, double *variance_out,
+                                         int *iterations_out) {
   if (!ctx || !h || nstates < 1 || nstates > QS_KMAX || !E_out || !vecs) return QS_EINVAL;
   if (h->rows != h->cols) return qs_fail(ctx, QS_EINVAL, "qs_ground_states needs rows == cols");
   QS_HIP(ctx, hipSetDevice(ctx->device));
@@ -662,7 +663,12 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
   if (max_restarts <= 0) max_restarts = 100;
   if (tol <= 0) tol = 100.0 * std::sqrt(2.220446049250313e-16);
   const int keep_max = std::min(QS_KMAX, std::max(2 * p, p + 6));
-  int mmax = (int)std::min<int64_t>(n, std::max(max_krylov, keep_max + 3 * p));
+  if (krylov_start <= 0) krylov_start = 100;
+  if (krylov_step < 0) krylov_step = 10;
+  const int mmax = (int)std::min<int64_t>(n, std::max(max_krylov, keep_max + 3 * p));
+  // basis size before a thick restart: nstates + krylov_start at first, grown
+  // by krylov_step + nstates per restart while below max_krylov
+  int cap = std::min(mmax, std::max(p + krylov_start, keep_max + 3 * p));
   // the Krylov basis and the working blocks are stored one vector after
   // another (vector i at Q + i*n), so one pass over the basis projects a new
   // block against every basis vector
@@ -708,7 +714,7 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
     const int ldT = mmax + 2 * QS_KMAX;
     T.assign((size_t)ldT * ldT, cd(0));
     auto Tat = [&](int i, int j) -> cd & { return T[(size_t)i + (size_t)j * ldT]; };
-    int next_check = std::min(mmax, std::max(4 * p, 24));
+    int next_check = std::min(cap, std::max(4 * p, 24));
     int restarts = 0;
     while (true) {
       ++iters;
@@ -761,15 +767,18 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
       std::vector<cd> R;  // R[i][j] = <Qnew_i|W_j>
       if (wn > 0) QS_TRY(wk.vtw(wk.vs(Cb), wn, Wb, cur.w, R));
       const bool exhausted = wn == 0 || m + wn > n;
-      const bool full = mmax < n && m + wn + p > mmax;
+      const bool full = cap < n && m + wn + p > cap;
       if (exhausted || full || m >= next_check) {
         // Rayleigh-Ritz on the current basis (m vectors)
         std::vector<cd> Tm((size_t)m * m);
         for (int j = 0; j < m; ++j)
           for (int i = 0; i < m; ++i) Tm[i + (size_t)j * m] = 0.5 * (Tat(i, j) + std::conj(Tat(j, i)));
+        // all Ritz values; vectors of the lowest max(p, keep_max) only
+        const int nvec = std::min(m, std::max(p, keep_max));
         theta.assign(m, 0.0);
-        S.assign((size_t)m * m, cd(0));
-        qs_heev(m, (const double *)Tm.data(), theta.data(), (double *)S.data());
+        S.assign((size_t)m * nvec, cd(0));
+        if (!qs_heev_lowest(m, (const double *)Tm.data(), theta.data(), (double *)S.data(), nvec))
+          return qs_fail(ctx, QS_ENOTCONVERGED, "projected eigenproblem (%d x %d) did not converge", m, m);
         // residual norms |R s_last|
         std::vector<double> var(p, 0.0);
         double sumE2 = 0.0;
@@ -829,10 +838,11 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
               Tat(i, r + a) = std::conj(Cnew[(size_t)a * r + i]);
             }
           m = r + wn;
-          next_check = std::min(mmax, m + std::max(4 * p, 24));
+          if (cap + krylov_step + p < max_krylov) cap = std::min(mmax, cap + krylov_step + p);
+          next_check = std::min(cap, m + std::max(4 * p, 24));
           continue;
         }
-        next_check = std::min(mmax, (int)(next_check * 1.5) + p);
+        next_check = std::min(cap, (int)(next_check * 1.5) + p);
       }
       // append the new block
       for (int a = 0; a < wn; ++a)
@@ -850,6 +860,13 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
   (void)hipFree(mem);
   if (iterations_out) *iterations_out = iters;
   return st;
+}
+
+extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int max_krylov,
+                                      int max_restarts, uint64_t seed, const double *start, double *E_out,
+                                      double *vecs, double *variance_out, int *iterations_out) {
+  return qs_ground_states_ex(ctx, h, nstates, tol, 100, 10, max_krylov, max_restarts, seed, start, E_out, vecs,
+                             variance_out, iterations_out);
 }
 
 extern "C" qs_status qs_basis_closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,

@@ -2,10 +2,13 @@
 // projected (Krylov / Loewdin) matrices, which are at most a few hundred wide.
 //
 // Method: Householder reduction of the complex Hermitian matrix to a complex
-// Hermitian tridiagonal one, a diagonal unitary that makes the off-diagonal
-// real, then the implicit-shift QL iteration on the real symmetric tridiagonal
-// matrix with accumulated rotations (Wilkinson & Reinsch, Handbook for Automatic
-// Computation vol. II, procedures tred/tql2).
+// Hermitian tridiagonal one (lower triangle only, column access), a diagonal
+// unitary that makes the off-diagonal real, then the implicit-shift QL
+// iteration on the real symmetric tridiagonal matrix (Wilkinson & Reinsch,
+// Handbook for Automatic Computation vol. II, procedures tred/tql2). When only
+// a few eigenvectors are wanted, the QL rotations are recorded and replayed
+// on those unit vectors, and only they are transformed back; otherwise the
+// rotations are accumulated into a column-major matrix.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -18,72 +21,102 @@ using cd = std::complex<double>;
 
 namespace {
 
-// A: column-major n*n (A[i + j*n] = row i, col j), Hermitian; only used as input.
-// Returns w ascending and Z (column-major, column j = eigenvector j).
-bool heev_impl(int n, const cd *Ain, double *w, cd *Z) {
+// A: column-major n*n (A[i + j*n] = row i, col j), Hermitian; only its lower
+// triangle is read. Returns all eigenvalues ascending in w and the eigenvectors
+// of the nvec lowest in Z (column-major n x nvec).
+bool heev_impl(int n, const cd *Ain, double *w, cd *Z, int nvec) {
   if (n <= 0) return true;
-  std::vector<cd> A(Ain, Ain + (size_t)n * n);
-  auto at = [&](int i, int j) -> cd & { return A[i + (size_t)j * n]; };
-  std::vector<cd> Q((size_t)n * n, cd(0));
-  for (int i = 0; i < n; ++i) Q[i + (size_t)i * n] = 1.0;
-  auto q = [&](int i, int j) -> cd & { return Q[i + (size_t)j * n]; };
-  std::vector<cd> v(n), p(n), wv(n);
-  // Householder: zero column k below the subdiagonal
+  nvec = std::max(0, std::min(nvec, n));
+  const size_t nn = (size_t)n * n;
+  std::vector<double> ar(nn), ai(nn);
+  for (size_t t = 0; t < nn; ++t) {
+    ar[t] = Ain[t].real();
+    ai[t] = Ain[t].imag();
+  }
+  std::vector<double> bet(n, 0.0), er(n, 0.0), ei(n, 0.0);
+  std::vector<double> pr(n), pi(n), wr(n), wi(n);
+  // Householder step k: v (stored over column k, rows k+1..n-1) maps the column
+  // below the diagonal onto -phase*alpha e1; A22 <- H A22 H = A22 - v w^+ - w v^+
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;
-    double alpha = 0.0;
-    for (int i = 0; i < m; ++i) alpha += std::norm(at(k + 1 + i, k));
-    alpha = std::sqrt(alpha);
-    if (alpha == 0.0) continue;
-    cd x0 = at(k + 1, k);
-    cd phase = std::abs(x0) > 0 ? x0 / std::abs(x0) : cd(1.0);
-    for (int i = 0; i < m; ++i) v[i] = at(k + 1 + i, k);
-    v[0] += phase * alpha;
-    double vnorm2 = 0.0;
-    for (int i = 0; i < m; ++i) vnorm2 += std::norm(v[i]);
-    if (vnorm2 == 0.0) continue;
-    const double beta = 2.0 / vnorm2;
-    // p = beta * A22 v
+    double *xr = &ar[(size_t)(k + 1) + (size_t)k * n], *xi = &ai[(size_t)(k + 1) + (size_t)k * n];
+    double a2 = 0.0;
+    for (int i = 0; i < m; ++i) a2 += xr[i] * xr[i] + xi[i] * xi[i];
+    if (a2 == 0.0) continue;  // already reduced: e_k = 0, H_k = I
+    const double alpha = std::sqrt(a2);
+    const double ax0 = std::hypot(xr[0], xi[0]);
+    const double phr = ax0 > 0 ? xr[0] / ax0 : 1.0, phi = ax0 > 0 ? xi[0] / ax0 : 0.0;
+    er[k] = -phr * alpha;
+    ei[k] = -phi * alpha;
+    xr[0] += phr * alpha;
+    xi[0] += phi * alpha;
+    const double b = 1.0 / (a2 + alpha * ax0);  // 2 / |v|^2
+    bet[k] = b;
+    // p = b A22 v from the lower triangle: column j adds A_ij v_j to p_i (i > j)
+    // and conj(A_ij) v_i to p_j
+    std::fill(pr.begin(), pr.begin() + m, 0.0);
+    std::fill(pi.begin(), pi.begin() + m, 0.0);
+    for (int jj = 0; jj < m; ++jj) {
+      const size_t cj = (size_t)(k + 1 + jj) * n + (k + 1);
+      const double *cr = &ar[cj], *ci = &ai[cj];
+      const double vr = xr[jj], vi = xi[jj];
+      double sr = cr[jj] * vr, si = cr[jj] * vi;
+      for (int ii = jj + 1; ii < m; ++ii) {
+        const double r = cr[ii], im = ci[ii];
+        pr[ii] += r * vr - im * vi;
+        pi[ii] += r * vi + im * vr;
+        sr += r * xr[ii] + im * xi[ii];
+        si += r * xi[ii] - im * xr[ii];
+      }
+      pr[jj] += sr;
+      pi[jj] += si;
+    }
+    double kr = 0.0, ki = 0.0;  // v^+ p (real up to rounding)
     for (int i = 0; i < m; ++i) {
-      cd s = 0;
-      for (int j = 0; j < m; ++j) s += at(k + 1 + i, k + 1 + j) * v[j];
-      p[i] = beta * s;
+      pr[i] *= b;
+      pi[i] *= b;
+      kr += xr[i] * pr[i] + xi[i] * pi[i];
+      ki += xr[i] * pi[i] - xi[i] * pr[i];
     }
-    cd K = 0;
-    for (int i = 0; i < m; ++i) K += std::conj(v[i]) * p[i];
-    K *= 0.5 * beta;
-    for (int i = 0; i < m; ++i) wv[i] = p[i] - K * v[i];
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < m; ++i)
-        at(k + 1 + i, k + 1 + j) -= v[i] * std::conj(wv[j]) + wv[i] * std::conj(v[j]);
-    const cd sub = -phase * alpha;
-    at(k + 1, k) = sub;
-    at(k, k + 1) = std::conj(sub);
-    for (int i = 1; i < m; ++i) {
-      at(k + 1 + i, k) = 0;
-      at(k, k + 1 + i) = 0;
+    kr *= 0.5 * b;
+    ki *= 0.5 * b;
+    for (int i = 0; i < m; ++i) {
+      wr[i] = pr[i] - (kr * xr[i] - ki * xi[i]);
+      wi[i] = pi[i] - (kr * xi[i] + ki * xr[i]);
     }
-    // Q <- Q H on columns k+1..n-1
-    for (int r = 0; r < n; ++r) {
-      cd s = 0;
-      for (int j = 0; j < m; ++j) s += q(r, k + 1 + j) * v[j];
-      s *= beta;
-      for (int j = 0; j < m; ++j) q(r, k + 1 + j) -= s * std::conj(v[j]);
+    for (int jj = 0; jj < m; ++jj) {
+      const size_t cj = (size_t)(k + 1 + jj) * n + (k + 1);
+      double *cr = &ar[cj], *ci = &ai[cj];
+      const double vr = xr[jj], vi = xi[jj], w_r = wr[jj], w_i = wi[jj];
+      for (int ii = jj; ii < m; ++ii) {
+        cr[ii] -= (xr[ii] * w_r + xi[ii] * w_i) + (wr[ii] * vr + wi[ii] * vi);
+        ci[ii] -= (xi[ii] * w_r - xr[ii] * w_i) + (wi[ii] * vr - wr[ii] * vi);
+      }
+      ci[jj] = 0.0;
     }
   }
-  // real tridiagonal via diagonal phases
+  if (n >= 2) {
+    er[n - 2] = ar[(size_t)(n - 1) + (size_t)(n - 2) * n];
+    ei[n - 2] = ai[(size_t)(n - 1) + (size_t)(n - 2) * n];
+  }
+  // real tridiagonal D^+ T D with D[i+1] = D[i] s_i / |s_i|
   std::vector<double> d(n), e(n, 0.0);
   std::vector<cd> D(n, cd(1.0));
-  for (int i = 0; i < n; ++i) d[i] = at(i, i).real();
+  for (int i = 0; i < n; ++i) d[i] = ar[(size_t)i + (size_t)i * n];
   for (int i = 0; i + 1 < n; ++i) {
-    const cd s = at(i + 1, i);
-    const double a = std::abs(s);
+    const double a = std::hypot(er[i], ei[i]);
     e[i] = a;
-    D[i + 1] = a > 0 ? D[i] * (s / a) : D[i];
+    D[i + 1] = a > 0 ? D[i] * cd(er[i] / a, ei[i] / a) : D[i];
   }
-  // implicit QL with shifts; Zr accumulates rotations (row-major n*n)
-  std::vector<double> Zr((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) Zr[(size_t)i * n + i] = 1.0;
+  // implicit QL with shifts; rotations recorded (few vectors) or accumulated
+  const bool accumulate = nvec > 0 && (size_t)nvec * 4 > (size_t)n;
+  std::vector<double> Zc;
+  if (accumulate) {
+    Zc.assign(nn, 0.0);
+    for (int i = 0; i < n; ++i) Zc[(size_t)i * n + i] = 1.0;
+  }
+  std::vector<int> rot_i;
+  std::vector<double> rot_c, rot_s;
   const double eps = std::numeric_limits<double>::epsilon();
   for (int l = 0; l < n; ++l) {
     int iter = 0;
@@ -118,10 +151,17 @@ bool heev_impl(int n, const cd *Ain, double *w, cd *Z) {
         pp = s * r;
         d[i + 1] = g + pp;
         g = c * r - b;
-        for (int k2 = 0; k2 < n; ++k2) {
-          f = Zr[(size_t)k2 * n + i + 1];
-          Zr[(size_t)k2 * n + i + 1] = s * Zr[(size_t)k2 * n + i] + c * f;
-          Zr[(size_t)k2 * n + i] = c * Zr[(size_t)k2 * n + i] - s * f;
+        if (accumulate) {
+          double *z0 = &Zc[(size_t)i * n], *z1 = &Zc[(size_t)(i + 1) * n];
+          for (int k2 = 0; k2 < n; ++k2) {
+            const double f1 = z1[k2];
+            z1[k2] = s * z0[k2] + c * f1;
+            z0[k2] = c * z0[k2] - s * f1;
+          }
+        } else if (nvec > 0) {
+          rot_i.push_back(i);
+          rot_c.push_back(c);
+          rot_s.push_back(s);
         }
       }
       if (early) continue;
@@ -130,18 +170,50 @@ bool heev_impl(int n, const cd *Ain, double *w, cd *Z) {
       e[m] = 0.0;
     }
   }
-  // sort ascending, eigenvectors = (Q D) Zr
   std::vector<int> ord(n);
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
-  for (int jj = 0; jj < n; ++jj) {
+  for (int jj = 0; jj < n; ++jj) w[jj] = d[ord[jj]];
+  // eigenvectors: tridiagonal vector x, y = D x, then H_{n-3} .. H_0 applied
+  std::vector<double> x(n);
+  std::vector<double> yr(n), yi(n);
+  for (int jj = 0; jj < nvec; ++jj) {
     const int j = ord[jj];
-    w[jj] = d[j];
-    for (int r = 0; r < n; ++r) {
-      cd s = 0;
-      for (int k2 = 0; k2 < n; ++k2) s += q(r, k2) * D[k2] * Zr[(size_t)k2 * n + j];
-      Z[r + (size_t)jj * n] = s;
+    if (accumulate) {
+      std::copy(&Zc[(size_t)j * n], &Zc[(size_t)j * n] + n, x.begin());
+    } else {
+      // Z = G_1 G_2 ... G_K, so Z e_j applies G_K first
+      std::fill(x.begin(), x.end(), 0.0);
+      x[j] = 1.0;
+      for (size_t t = rot_i.size(); t-- > 0;) {
+        const int i = rot_i[t];
+        const double c = rot_c[t], s = rot_s[t];
+        const double x0 = x[i], x1 = x[i + 1];
+        x[i] = c * x0 + s * x1;
+        x[i + 1] = -s * x0 + c * x1;
+      }
     }
+    for (int i = 0; i < n; ++i) {
+      yr[i] = D[i].real() * x[i];
+      yi[i] = D[i].imag() * x[i];
+    }
+    for (int k = n - 3; k >= 0; --k) {
+      if (bet[k] == 0.0) continue;
+      const int m = n - k - 1;
+      const double *vr = &ar[(size_t)(k + 1) + (size_t)k * n], *vi = &ai[(size_t)(k + 1) + (size_t)k * n];
+      double sr = 0.0, si = 0.0;  // v^+ y
+      for (int i = 0; i < m; ++i) {
+        sr += vr[i] * yr[k + 1 + i] + vi[i] * yi[k + 1 + i];
+        si += vr[i] * yi[k + 1 + i] - vi[i] * yr[k + 1 + i];
+      }
+      sr *= bet[k];
+      si *= bet[k];
+      for (int i = 0; i < m; ++i) {
+        yr[k + 1 + i] -= vr[i] * sr - vi[i] * si;
+        yi[k + 1 + i] -= vr[i] * si + vi[i] * sr;
+      }
+    }
+    for (int i = 0; i < n; ++i) Z[(size_t)i + (size_t)jj * n] = cd(yr[i], yi[i]);
   }
   return true;
 }
@@ -149,7 +221,11 @@ bool heev_impl(int n, const cd *Ain, double *w, cd *Z) {
 }  // namespace
 
 void qs_heev(int n, const double *A, double *w, double *Z) {
-  heev_impl(n, reinterpret_cast<const cd *>(A), w, reinterpret_cast<cd *>(Z));
+  heev_impl(n, reinterpret_cast<const cd *>(A), w, reinterpret_cast<cd *>(Z), n);
+}
+
+bool qs_heev_lowest(int n, const double *A, double *w, double *Z, int nvec) {
+  return heev_impl(n, reinterpret_cast<const cd *>(A), w, reinterpret_cast<cd *>(Z), nvec);
 }
 
 // ---------------------------------------------------------------------------
@@ -275,6 +351,6 @@ extern "C" qs_status qs_tridiag_extremes_host(int m, const double *d, const doub
 
 extern "C" qs_status qs_heev_host(int n, const double *A, double *w, double *Z) {
   if (n < 0 || (n > 0 && (!A || !w || !Z))) return QS_EINVAL;
-  if (!heev_impl(n, reinterpret_cast<const cd *>(A), w, reinterpret_cast<cd *>(Z))) return QS_ENOTCONVERGED;
+  if (!heev_impl(n, reinterpret_cast<const cd *>(A), w, reinterpret_cast<cd *>(Z), n)) return QS_ENOTCONVERGED;
   return QS_OK;
 }

@@ -183,6 +183,9 @@ qs_status qs_dot_internal(qs_ctx *ctx, int64_t n, const double *a, const double 
 // A: n*n complex (overwritten), w: n eigenvalues ascending, Z: n*n eigenvectors
 // (column j = eigenvector j).
 void qs_heev(int n, const double *A, double *w, double *Z);
+// all eigenvalues ascending, eigenvectors of the nvec lowest (Z: n x nvec
+// column-major); false if the QL iteration does not converge
+bool qs_heev_lowest(int n, const double *A, double *w, double *Z, int nvec);
 
 // ---- device helpers -------------------------------------------------------
 __device__ __forceinline__ uint32_t qs_chunk_byte1(uint64_t w, int c) {

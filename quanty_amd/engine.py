@@ -192,8 +192,13 @@ class Hop:
         return A, B, kept
 
     def ground_states(self, nstates: int, tol: float = 1e-12, max_krylov: int = 500, max_restarts: int = 100,
-                      seed: int = 1, start: torch.Tensor | None = None):
-        """Lowest eigenpairs. Returns (E (nstates,), vecs (nstates, n) tensor, variances, iterations)."""
+                      seed: int = 1, start: torch.Tensor | None = None, krylov_start: int = 100,
+                      krylov_step: int = 10):
+        """Lowest eigenpairs. Returns (E (nstates,), vecs (nstates, n) tensor, variances, iterations).
+
+        The Krylov basis holds nstates + krylov_start vectors before the first
+        thick restart and grows by krylov_step + nstates per restart up to
+        max_krylov (NKrylovStart / NKrylovStep / NKrylovMax)."""
         E = np.zeros(nstates)
         var = np.zeros(nstates)
         it = C.c_int()
@@ -201,9 +206,9 @@ class Hop:
         if start is not None:
             start = start.contiguous()
         torch.cuda.synchronize()
-        self.ctx.check(self.lib.qs_ground_states(self.ctx.h, self.h, nstates, tol, max_krylov, max_restarts, seed,
-                                                 ptr(start), ptr(E), ptr(vecs), ptr(var), C.byref(it)),
-                       "qs_ground_states")
+        self.ctx.check(self.lib.qs_ground_states_ex(self.ctx.h, self.h, nstates, tol, krylov_start, krylov_step,
+                                                    max_krylov, max_restarts, seed, ptr(start), ptr(E), ptr(vecs),
+                                                    ptr(var), C.byref(it)), "qs_ground_states_ex")
         return E, vecs, var, it.value
 
     def __del__(self):
