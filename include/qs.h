@@ -71,8 +71,11 @@ qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
  * shift of the row positions are stored as one block header when at least
  * this many rows share the shift; 65 stores every entry individually.
  * QS_PARAM_ELL_PACK (default 1): 1 stores an individual entry as one uint32
- * (column and term id) when both fit, 0 always as uint32 column + uint16 term. */
-enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2 };
+ * (column and term id) when both fit, 0 always as uint32 column + uint16 term.
+ * QS_PARAM_BATCH_GRAPH (default 1): 1 runs qs_batch_run's steps as one HIP
+ * graph launch (a chain of kernel nodes per batch and step count, kept across
+ * slot changes); 0 launches the kernels one by one. Results are identical. */
+enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3 };
 qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value);
 qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
 
@@ -272,11 +275,18 @@ qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const double *A, const
  * w: n eigenvalues ascending; Z: n*n complex column-major eigenvectors. */
 qs_status qs_heev_host(int n, const double *A, double *w, double *Z);
 /* Real symmetric tridiagonal T (diagonal d[0..m-1], off-diagonal e[0..m-2]) on
- * the host: the lowest eigenpair (Sturm bisection + inverse iteration; z has
+ * the host: the lowest eigenpair (Newton on the Sturm sequence, safeguarded by
+ * bisection, + inverse iteration; z has
  * unit norm, largest component positive), and the smallest/largest
  * eigenvalues — the pole range of a Lanczos table
  * (TriDiagonalMatrixToEigenvalues, BasicMath_MatrixTypeConversion.cpp:1538). */
 qs_status qs_tridiag_lowest_host(int m, const double *d, const double *e, double *E, double *z);
+/* qs_tridiag_lowest_host with a bracket [lower, upper] expected to hold the
+ * lowest eigenvalue (e.g. while a Lanczos run is extended: the previous
+ * estimate, an upper bound, and that estimate minus a multiple of its
+ * residual); each end is checked and ignored when wrong; NaN for none. */
+qs_status qs_tridiag_lowest_from(int m, const double *d, const double *e, double lower, double upper, double *E,
+                                 double *z);
 qs_status qs_tridiag_extremes_host(int m, const double *d, const double *e, double *lo, double *hi);
 
 /* ---- continued-fraction Green's function -------------------------------- */

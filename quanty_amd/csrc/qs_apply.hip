@@ -296,6 +296,60 @@ k_apply_mf(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__
   if (active) y[row] = make_double2(ar, ai);
 }
 
+// Small bases: g lanes per row (g a power of two up to 64), lane k of a row's
+// group taking the terms j = k (mod g) of each tile, then an xor butterfly
+// over the group. Keeps enough threads in flight when rows are few and every
+// row walks hundreds of terms.
+template <int NW>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_apply_mf_split(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms,
+                 int nterms, Lookup L, int same, int g, const double2 *__restrict__ x, double2 *__restrict__ y) {
+  __shared__ TermD<NW> sh[QS_TILE];
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / g;
+  const int k = threadIdx.x & (g - 1);
+  const bool active = row < nrows;
+  uint64_t d[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) d[w] = active ? rows[row * NW + w] : 0;
+  double ar = 0.0, ai = 0.0;
+  for (int base = 0; base < nterms; base += QS_TILE) {
+    const int cnt = min(QS_TILE, nterms - base);
+    __syncthreads();
+    stage_terms<NW>(sh, terms, base, cnt);
+    __syncthreads();
+    if (!active) continue;
+    for (int j = k; j < cnt; j += g) {
+      const TermD<NW> &t = sh[j];
+      bool hit = true;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) hit = hit && ((d[w] & t.C[w]) == t.V[w]);
+      if (!hit) continue;
+      int64_t col;
+      if (t.diag && same) {
+        col = row;
+      } else {
+        uint64_t sv[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sv[w] = d[w] ^ t.F[w];
+        col = col_lookup<NW>(sv, L);
+      }
+      if (col < 0) continue;
+      uint32_t par = t.sgn;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) par ^= (uint32_t)__popcll(d[w] & t.P[w]) & 1u;
+      const double2 xv = x[col];
+      const double cr = par ? -t.re : t.re, ci = par ? -t.im : t.im;
+      ar = fma(cr, xv.x, fma(-ci, xv.y, ar));
+      ai = fma(cr, xv.y, fma(ci, xv.x, ai));
+    }
+  }
+  for (int o = g >> 1; o > 0; o >>= 1) {
+    ar += __shfl_xor(ar, o);
+    ai += __shfl_xor(ai, o);
+  }
+  if (active && k == 0) y[row] = make_double2(ar, ai);
+}
+
 // ---------------------------------------------------------------------------
 // assembly: one wavefront per 64-row slice, count pass then fill pass
 // ---------------------------------------------------------------------------
@@ -480,6 +534,22 @@ static qs_status launch_mf(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, c
                            bool same, const double *x, double *y) {
   if (rows->n == 0) return QS_OK;
   Lookup L = make_lookup(cols);
+  // lanes per row: enough threads for the device when rows are few
+  int g = 1;
+  while (g < 64 && rows->n * g < ((int64_t)1 << 19) && g < op->nterms) g *= 2;
+  if (g > 1) {
+    const unsigned blocks = (unsigned)((rows->n * g + QS_BLOCK - 1) / QS_BLOCK);
+    if (op->nw == 1)
+      hipLaunchKernelGGL(k_apply_mf_split<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, rows->n,
+                         (const TermD<1> *)op->d_terms, op->nterms, L, (int)same, g, (const double2 *)x,
+                         (double2 *)y);
+    else
+      hipLaunchKernelGGL(k_apply_mf_split<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, rows->n,
+                         (const TermD<2> *)op->d_terms, op->nterms, L, (int)same, g, (const double2 *)x,
+                         (double2 *)y);
+    QS_HIP(ctx, hipGetLastError());
+    return QS_OK;
+  }
   const unsigned blocks = (unsigned)((rows->n + QS_BLOCK - 1) / QS_BLOCK);
   if (op->nw == 1)
     hipLaunchKernelGGL(k_apply_mf<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows->d_words, rows->n,

@@ -18,6 +18,9 @@
 #include <cstring>
 #include <vector>
 
+#include <map>
+#include <tuple>
+
 #include "qs_internal.h"
 #include "qs_kernels.h"
 
@@ -72,7 +75,16 @@ struct qs_batch {
   double *d_p1 = nullptr, *d_p2 = nullptr;  // partials per slice (K1) and per work item (K2)
   size_t lds1 = 0;                          // K1 dynamic LDS: the largest term table that is staged
   bool fast1 = true;                        // every slot: one-word determinants, packed entries
+  int64_t *d_n = nullptr;                   // work item counts of K1 and K2 (device, read by the kernels)
+  // graphs of `steps` steps by (K1 variant, K1 dynamic LDS, steps);
+  // valid while the worklist and partial buffers keep their addresses
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
 };
+
+static void drop_graphs(qs_batch *B) {
+  for (auto &g : B->graphs) (void)hipGraphExecDestroy(g.second);
+  B->graphs.clear();
+}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -130,9 +142,10 @@ __device__ __forceinline__ void b1_row(const SellD &A, int64_t sl, int lane, boo
 // common case), which keeps the kernel within 64 VGPRs (8 waves per SIMD).
 template <bool FAST>
 __global__ void __launch_bounds__(QS_BLOCK, FAST ? 8 : 1)
-k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64_t nwork, double *__restrict__ part,
-     int lds_bytes) {
+k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, const int64_t *__restrict__ counts,
+     double *__restrict__ part, int lds_bytes) {
   extern __shared__ char smem[];
+  const int64_t nwork = counts[0];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   constexpr int NX = 8;
@@ -223,8 +236,10 @@ k_b_r1(SlotDev *slots, int nslots, const int *__restrict__ range, const double *
 }
 
 __global__ void __launch_bounds__(QS_BLOCK)
-k_b2(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, int64_t nwork, double *__restrict__ part) {
+k_b2(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, const int64_t *__restrict__ counts,
+     double *__restrict__ part) {
   __shared__ double red[QS_BLOCK];
+  const int64_t nwork = counts[1];
   for (int64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
     const WorkItem it = work[w];
     const SlotDev &S = slots[it.slot];
@@ -383,6 +398,7 @@ static qs_status rebuild(qs_batch *B) {
     return QS_OK;
   };
   const int64_t c1 = B->cap1, c2 = B->cap2;
+  const void *old_ptrs[4] = {B->d_w1, B->d_w2, B->d_p1, B->d_p2};
   QS_TRY(grow(&B->d_w1, &B->cap1, std::max<size_t>(1, w1.size())));
   QS_TRY(grow(&B->d_w2, &B->cap2, std::max<size_t>(1, w2.size())));
   if (B->cap1 != c1 || !B->d_p1) {
@@ -399,9 +415,13 @@ static qs_status rebuild(qs_batch *B) {
     QS_HIP(ctx, hipMemcpyAsync(B->d_w2, w2.data(), w2.size() * sizeof(WorkItem), hipMemcpyHostToDevice, ctx->stream));
   QS_HIP(ctx, hipMemcpyAsync(B->d_r1, r1.data(), r1.size() * 4, hipMemcpyHostToDevice, ctx->stream));
   QS_HIP(ctx, hipMemcpyAsync(B->d_r2, r2.data(), r2.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  const int64_t counts[2] = {(int64_t)w1.size(), (int64_t)w2.size()};
+  QS_HIP(ctx, hipMemcpyAsync(B->d_n, counts, sizeof counts, hipMemcpyHostToDevice, ctx->stream));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  B->n1 = (int64_t)w1.size();
-  B->n2 = (int64_t)w2.size();
+  if (old_ptrs[0] != B->d_w1 || old_ptrs[1] != B->d_w2 || old_ptrs[2] != B->d_p1 || old_ptrs[3] != B->d_p2)
+    drop_graphs(B);
+  B->n1 = counts[0];
+  B->n2 = counts[1];
   B->dirty = false;
   return QS_OK;
 }
@@ -423,6 +443,7 @@ extern "C" qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out)
   qs_status st = qs_malloc(ctx, (void **)&B->d_slots, (size_t)max_slots * sizeof(SlotDev));
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r1, (size_t)max_slots * 8);
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r2, (size_t)max_slots * 8);
+  if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_n, 2 * sizeof(int64_t));
   if (st == QS_OK) {
     hipError_t e = hipMemsetAsync(B->d_slots, 0, (size_t)max_slots * sizeof(SlotDev), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -524,6 +545,56 @@ extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double 
   return QS_OK;
 }
 
+// `steps` steps as a graph of kernel nodes in a chain (the same launches as
+// qs_batch_run's kernel-by-kernel path, with grids that fit any worklist).
+static qs_status build_graph(qs_batch *B, int steps, int g1, int g2, int lds1, hipGraphExec_t *out) {
+  qs_ctx *ctx = B->ctx;
+  hipGraph_t g = nullptr;
+  QS_HIP(ctx, hipGraphCreate(&g, 0));
+  const SlotDev *cs = B->d_slots;
+  SlotDev *ms = B->d_slots;
+  const WorkItem *w1 = B->d_w1, *w2 = B->d_w2;
+  const int64_t *cn = B->d_n;
+  double *p1 = B->d_p1, *p2 = B->d_p2;
+  const double *cp1 = B->d_p1, *cp2 = B->d_p2;
+  const int *r1 = B->d_r1, *r2 = B->d_r2;
+  int cap = B->cap, lds = lds1;
+  void *a1[] = {&cs, &w1, &cn, &p1, &lds};
+  void *ar1[] = {&ms, &cap, &r1, &cp1};
+  void *a2[] = {&cs, &w2, &cn, &p2};
+  void *ar2[] = {&ms, &cap, &r2, &cp2};
+  void *k1 = B->fast1 ? reinterpret_cast<void *>(&k_b1<true>) : reinterpret_cast<void *>(&k_b1<false>);
+  struct Node {
+    void *f;
+    int grid;
+    int shm;
+    void **args;
+  } const nodes[4] = {{k1, g1, lds1, a1},
+                      {reinterpret_cast<void *>(&k_b_r1), B->cap, 0, ar1},
+                      {reinterpret_cast<void *>(&k_b2), g2, 0, a2},
+                      {reinterpret_cast<void *>(&k_b_r2), B->cap, 0, ar2}};
+  hipGraphNode_t prev = nullptr;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < steps && e == hipSuccess; ++k)
+    for (const Node &nd : nodes) {
+      hipKernelNodeParams p{};
+      p.func = nd.f;
+      p.gridDim = dim3((unsigned)nd.grid);
+      p.blockDim = dim3(QS_BLOCK);
+      p.sharedMemBytes = (unsigned)nd.shm;
+      p.kernelParams = nd.args;
+      p.extra = nullptr;
+      hipGraphNode_t node = nullptr;
+      e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+      if (e != hipSuccess) break;
+      prev = node;
+    }
+  if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_batch_run (graph)", __FILE__, __LINE__);
+  return QS_OK;
+}
+
 extern "C" qs_status qs_batch_run(qs_batch *B, int steps) {
   if (!B || steps < 0) return QS_EINVAL;
   qs_ctx *ctx = B->ctx;
@@ -532,20 +603,41 @@ extern "C" qs_status qs_batch_run(qs_batch *B, int steps) {
   if (B->n1 == 0) return QS_OK;
   const int lds1 = (int)B->lds1;
   const void *k1 = B->fast1 ? (const void *)k_b1<true> : (const void *)k_b1<false>;
-  const int g1 = (int)std::max<int64_t>(1, std::min<int64_t>(B->n1, qs_resident_blocks(ctx, k1, (size_t)lds1)));
-  const int g2 = (int)std::max<int64_t>(1, std::min<int64_t>(B->n2, (int64_t)ctx->num_cu * 4));
-  for (int k = 0; k < steps; ++k) {
-    if (B->fast1)
-      hipLaunchKernelGGL(k_b1<true>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1, B->n1,
-                         B->d_p1, lds1);
-    else
-      hipLaunchKernelGGL(k_b1<false>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1, B->n1,
-                         B->d_p1, lds1);
-    hipLaunchKernelGGL(k_b_r1, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r1, B->d_p1);
-    hipLaunchKernelGGL(k_b2, dim3(g2), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_w2, B->n2, B->d_p2);
-    hipLaunchKernelGGL(k_b_r2, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r2, B->d_p2);
+  const int64_t res1 = qs_resident_blocks(ctx, k1, (size_t)lds1);
+  const int64_t res2 = (int64_t)ctx->num_cu * 4;
+  // the kernels read the work item counts from the device, so any grid
+  // covers the worklist; a graph uses the resident grids, which fit every
+  // later worklist
+  auto enqueue = [&](int g1, int g2, int n) {
+    for (int k = 0; k < n; ++k) {
+      if (B->fast1)
+        hipLaunchKernelGGL(k_b1<true>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1,
+                           B->d_n, B->d_p1, lds1);
+      else
+        hipLaunchKernelGGL(k_b1<false>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1,
+                           B->d_n, B->d_p1, lds1);
+      hipLaunchKernelGGL(k_b_r1, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r1, B->d_p1);
+      hipLaunchKernelGGL(k_b2, dim3(g2), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_w2, B->d_n, B->d_p2);
+      hipLaunchKernelGGL(k_b_r2, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r2, B->d_p2);
+    }
+  };
+  if (ctx->batch_graph && steps > 0) {
+    const auto key = std::make_tuple(B->fast1 ? 1 : 0, lds1, steps);
+    auto it = B->graphs.find(key);
+    if (it == B->graphs.end()) {
+      // The graph is assembled node by node rather than captured from the
+      // stream: other host threads may synchronize the device meanwhile,
+      // which would invalidate a capture.
+      hipGraphExec_t ex = nullptr;
+      QS_TRY(build_graph(B, steps, (int)std::max<int64_t>(1, res1), (int)std::max<int64_t>(1, res2), lds1, &ex));
+      it = B->graphs.emplace(key, ex).first;
+    }
+    QS_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
+  } else {
+    enqueue((int)std::max<int64_t>(1, std::min(B->n1, res1)), (int)std::max<int64_t>(1, std::min(B->n2, res2)),
+            steps);
+    QS_HIP(ctx, hipGetLastError());
   }
-  QS_HIP(ctx, hipGetLastError());
   if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QS_OK;
 }
@@ -743,6 +835,8 @@ extern "C" qs_status qs_batch_destroy(qs_batch *B) {
   if (B->d_slots) (void)hipFree(B->d_slots);
   if (B->d_w1) (void)hipFree(B->d_w1);
   if (B->d_w2) (void)hipFree(B->d_w2);
+  drop_graphs(B);
+  if (B->d_n) (void)hipFree(B->d_n);
   if (B->d_r1) (void)hipFree(B->d_r1);
   if (B->d_r2) (void)hipFree(B->d_r2);
   if (B->d_p1) (void)hipFree(B->d_p1);

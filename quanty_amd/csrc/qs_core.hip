@@ -148,6 +148,10 @@ extern "C" qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value) {
       if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_ELL_PACK must be 0 or 1");
       ctx->ell_pack = (int)value;
       return QS_OK;
+    case QS_PARAM_BATCH_GRAPH:
+      if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_BATCH_GRAPH must be 0 or 1");
+      ctx->batch_graph = (int)value;
+      return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
   }
@@ -161,6 +165,9 @@ extern "C" qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value) {
       return QS_OK;
     case QS_PARAM_ELL_PACK:
       *value = ctx->ell_pack;
+      return QS_OK;
+    case QS_PARAM_BATCH_GRAPH:
+      *value = ctx->batch_graph;
       return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);

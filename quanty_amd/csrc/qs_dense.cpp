@@ -265,21 +265,95 @@ double kth_eigenvalue(int m, const double *d, const double *e, int k) {
   return 0.5 * (lo + hi);
 }
 
+// Number of eigenvalues below x and S(x) = sum_j 1/(lambda_j - x) = -p'(x)/p(x)
+// for p(x) = det(T - x) = prod_i q_i(x), the LDL^T pivots
+// q_i = d_i - x - e_{i-1}^2 / q_{i-1}, with q_i' = -1 + e_{i-1}^2 q_{i-1}' / q_{i-1}^2.
+int sturm_newton(int m, const double *d, const double *e, double x, double *S) {
+  const double tiny = std::numeric_limits<double>::min();
+  double q = d[0] - x, dq = -1.0;
+  int cnt = q < 0;
+  double s = -dq / (q != 0.0 ? q : tiny);
+  for (int i = 1; i < m; ++i) {
+    const double qq = q != 0.0 ? q : std::numeric_limits<double>::epsilon() * (std::fabs(e[i - 1]) + 1e-300);
+    const double t = e[i - 1] * e[i - 1] / qq;
+    dq = -1.0 + t * dq / qq;
+    q = d[i] - x - t;
+    if (q < 0) ++cnt;
+    s -= dq / (q != 0.0 ? q : tiny);
+  }
+  *S = s;
+  return cnt;
+}
+
+// Lowest eigenvalue by Newton's iteration on det(T - x) from the left of the
+// spectrum, where it rises monotonically to lambda_min. A bracket [lo, hi]
+// (no eigenvalue below lo, at least one below hi) is kept; Newton runs from
+// lo (from hi once the left iterate overshoots, as rounding makes it do next
+// to lambda_min), and a bisection step is taken whenever the bracket has not
+// halved over the last two steps.
+// lower / upper: values expected below / above lambda_min (checked), or NaN.
+// Stops once the bracket is narrower than tol (absolute).
+double lowest_eigenvalue(int m, const double *d, const double *e, double lower, double upper, double tol) {
+  double glo = d[0], ghi = d[0], dmin = d[0];
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(e[i]) : 0.0);
+    glo = std::min(glo, d[i] - r);
+    ghi = std::max(ghi, d[i] + r);
+    dmin = std::min(dmin, d[i]);
+  }
+  const double span = std::max(ghi - glo, 1e-300);
+  double lo = glo - 1e-12 * span, hi = ghi + 1e-12 * span, S_lo = 0.0, S;
+  const double h0 = dmin + 1e-12 * span;  // lambda_min <= min d_i
+  if (std::isfinite(upper) && upper < h0) {
+    if (upper > lo && sturm_newton(m, d, e, upper, &S) > 0) hi = upper;
+  }
+  if (hi > h0 && h0 < hi && sturm_newton(m, d, e, h0, &S) > 0) hi = h0;
+  if (std::isfinite(lower) && lower > lo && lower < hi && sturm_newton(m, d, e, lower, &S) == 0) lo = lower;
+  (void)sturm_newton(m, d, e, lo, &S_lo);
+  double S_hi = NAN, w1 = INFINITY, w2 = INFINITY;  // bracket widths one and two steps back
+  for (int it = 0; it < 400; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (hi - lo <= tol || mid <= lo || mid >= hi) return mid;  // bracket at the tolerance
+    double x = mid;
+    // Newton unless the bracket failed to halve over the last two steps;
+    // steps of at least tol/2, so that a converged iterate closes the bracket
+    if (hi - lo <= 0.5 * w2) {
+      double cand = NAN;
+      if (S_lo > 0 && std::isfinite(S_lo)) cand = lo + std::max(1.0 / S_lo, 0.5 * tol);
+      if (!(cand < hi) && S_hi < 0 && std::isfinite(S_hi)) cand = hi + std::min(1.0 / S_hi, -0.5 * tol);
+      if (cand > lo && cand < hi) x = cand;
+    }
+    w2 = w1;
+    w1 = hi - lo;
+    if (sturm_newton(m, d, e, x, &S) > 0) {
+      hi = x;
+      S_hi = S;
+    } else {
+      lo = x;
+      S_lo = S;
+    }
+  }
+  return lo;
+}
+
 }  // namespace
 
-// Lowest eigenpair of T: eigenvalue by Sturm bisection, eigenvector by inverse
-// iteration with the tridiagonal LU (Thomas algorithm, partial pivoting).
-extern "C" qs_status qs_tridiag_lowest_host(int m, const double *d, const double *e, double *E, double *z) {
+// Lowest eigenpair of T: eigenvalue by safeguarded Newton on the Sturm
+// sequence, eigenvector by inverse iteration with the tridiagonal LU (Thomas
+// algorithm, partial pivoting).
+static qs_status tridiag_lowest(int m, const double *d, const double *e, double lower, double upper, double *E,
+                                double *z) {
   if (m < 1 || !d || !E || !z || (m > 1 && !e)) return QS_EINVAL;
-  const double lam = kth_eigenvalue(m, d, e, 0);
-  *E = lam;
   if (m == 1) {
+    *E = d[0];
     z[0] = 1.0;
     return QS_OK;
   }
   double scale = 0.0;
   for (int i = 0; i < m; ++i) scale = std::max(scale, std::fabs(d[i]) + (i + 1 < m ? std::fabs(e[i]) : 0.0));
-  const double shift = lam - 1e-14 * std::max(scale, 1e-300);
+  scale = std::max(scale, 1e-300);
+  const double lam = lowest_eigenvalue(m, d, e, lower, upper, 4 * std::numeric_limits<double>::epsilon() * scale);
+  const double shift = lam - 1e-14 * scale;
   // LU of (T - shift) with partial pivoting: rows i, i+1 may swap; U has 3 diagonals
   std::vector<double> u0(m), u1(m, 0.0), u2(m, 0.0), l(m, 0.0);
   std::vector<int> swp(m, 0);
@@ -313,7 +387,7 @@ extern "C" qs_status qs_tridiag_lowest_host(int m, const double *d, const double
   }
   u0[m - 1] = a != 0.0 ? a : 1e-300;
   std::vector<double> x(m, 1.0), y(m);
-  for (int iter = 0; iter < 4; ++iter) {
+  for (int iter = 0; iter < 3; ++iter) {
     // forward: apply the row swaps and L
     for (int i = 0; i < m; ++i) y[i] = x[i];
     for (int i = 0; i < m - 1; ++i) {
@@ -332,6 +406,7 @@ extern "C" qs_status qs_tridiag_lowest_host(int m, const double *d, const double
     nrm = std::sqrt(nrm);
     for (int i = 0; i < m; ++i) x[i] /= nrm;
   }
+  *E = lam;
   // fix the sign: largest component positive
   int im = 0;
   for (int i = 1; i < m; ++i)
@@ -339,6 +414,16 @@ extern "C" qs_status qs_tridiag_lowest_host(int m, const double *d, const double
   const double sg = x[im] < 0 ? -1.0 : 1.0;
   for (int i = 0; i < m; ++i) z[i] = sg * x[i];
   return QS_OK;
+}
+
+extern "C" qs_status qs_tridiag_lowest_host(int m, const double *d, const double *e, double *E, double *z) {
+  return tridiag_lowest(m, d, e, std::numeric_limits<double>::quiet_NaN(), std::numeric_limits<double>::quiet_NaN(),
+                        E, z);
+}
+
+extern "C" qs_status qs_tridiag_lowest_from(int m, const double *d, const double *e, double lower, double upper,
+                                            double *E, double *z) {
+  return tridiag_lowest(m, d, e, lower, upper, E, z);
 }
 
 // Smallest and largest eigenvalue of T (pole range of a Lanczos table).

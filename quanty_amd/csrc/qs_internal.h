@@ -29,6 +29,7 @@ struct qs_ctx {
   std::map<std::pair<const void *, size_t>, int> resident;
   int shift_min = 56;              // QS_PARAM_SHIFT_MIN for operators assembled from now on
   int ell_pack = 1;                // QS_PARAM_ELL_PACK
+  int batch_graph = 1;             // QS_PARAM_BATCH_GRAPH
 };
 
 // Exact rank of a determinant among all determinants satisfying the

@@ -376,15 +376,22 @@ class Batch:
             pass
 
 
-def tridiag_lowest(d: np.ndarray, e: np.ndarray):
-    """Lowest eigenpair of the real symmetric tridiagonal matrix (d, e) (host, libqs)."""
+def tridiag_lowest(d: np.ndarray, e: np.ndarray, lower: float | None = None, upper: float | None = None):
+    """Lowest eigenpair of the real symmetric tridiagonal matrix (d, e) (host,
+    libqs). lower / upper: values expected below / above the lowest eigenvalue,
+    which shorten the search (checked; a wrong one is ignored)."""
     lib = load()
     d = np.ascontiguousarray(d, dtype=np.float64)
     e = np.ascontiguousarray(e, dtype=np.float64)
     m = len(d)
     E = C.c_double()
     z = np.zeros(m)
-    st = lib.qs_tridiag_lowest_host(m, ptr(d), ptr(e) if m > 1 else None, C.byref(E), ptr(z))
+    if lower is None and upper is None:
+        st = lib.qs_tridiag_lowest_host(m, ptr(d), ptr(e) if m > 1 else None, C.byref(E), ptr(z))
+    else:
+        nan = float("nan")
+        st = lib.qs_tridiag_lowest_from(m, ptr(d), ptr(e) if m > 1 else None, nan if lower is None else float(lower),
+                                        nan if upper is None else float(upper), C.byref(E), ptr(z))
     if st != 0:
         raise QSError(f"qs_tridiag_lowest_host failed with status {st}")
     return E.value, z

@@ -136,6 +136,7 @@ class _Job:
         self.v0 = v.to(torch.complex128).contiguous()
         self.gs_depth = min(gs_depth, self.basis.n)
         self.phase = "gs1"
+        self.gs_prev = None  # (lowest Ritz value, residual) of the last ground-state check
         self.slot = -1
         self.bytes = self.hop.stats()["stream_bytes"] + 6 * 16 * self.basis.n
         self.kept_bytes = 0  # Lanczos vectors kept during the ground-state phase
@@ -217,8 +218,16 @@ class SpectrumServer:
         r = job.req
         if job.phase == "gs1":
             m, a, b = coeffs if coeffs is not None else self.batch.poll(job.slot)
-            E, z = tridiag_lowest(a[1:m + 1], b[1:m])
+            # the previous check's Ritz value bounds the new lowest one from
+            # above, and lies within its residual of an eigenvalue of H
+            lo = hi = None
+            if job.gs_prev is not None:
+                E0, r0 = job.gs_prev
+                hi = E0 + 1e-12 * max(1.0, abs(E0))
+                lo = E0 - 1.01 * r0 - 1e-12 * max(1.0, abs(E0))
+            E, z = tridiag_lowest(a[1:m + 1], b[1:m], lo, hi)
             res = float(abs(b[m] * z[m - 1]))
+            job.gs_prev = (E, res)
             if res ** 2 > self.gs_tol * max(1.0, E ** 2) and m < job.gs_max:
                 job.gs_depth = min(job.gs_max, m + self.gs_step)
                 job.extend = True  # the caller sets the new depth (one launch for all such jobs)

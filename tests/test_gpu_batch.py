@@ -204,3 +204,49 @@ def test_server_kept_basis_equals_second_pass():
         assert r.E0 == two[rid].E0 and r.gs_steps == two[rid].gs_steps
         assert np.array_equal(r.spectrum, two[rid].spectrum), rid
 
+
+
+def _graph_sequence(graph: int):
+    """Slots joining and leaving between runs of 6 steps, with other device
+    work in between; returns every finished slot's coefficients."""
+    from quanty_amd import default_context
+    from quanty_amd._lib import QS_PARAM_BATCH_GRAPH
+
+    ctx = default_context()
+    old = ctx.get_param(QS_PARAM_BATCH_GRAPH)
+    ctx.set_param(QS_PARAM_BATCH_GRAPH, graph)
+    try:
+        probs = [_setup(kw) for kw in SHAPES]
+        B = Batch(4, ctx)
+        out = {}
+        live = {B.add(probs[0][2], probs[0][3], 30): 0, B.add(probs[1][2], probs[1][3], 24): 1}
+        joins = [(2, 36), (3, 18), (0, 12)]
+        for rnd in range(12):
+            B.run(6)
+            # unrelated device work between replays
+            y = probs[1][2].apply(probs[1][3])
+            torch.cuda.synchronize()
+            assert torch.isfinite(y.real).all()
+            for s in list(live):
+                st, a, b = B.poll(s)
+                if st >= B.maxdepth[s]:
+                    out[(rnd, live[s])] = (a.copy(), b.copy())
+                    B.remove(s)
+                    del live[s]
+            if joins and len(live) < 3:
+                i, d = joins.pop(0)
+                live[B.add(probs[i][2], probs[i][3], d)] = i
+        assert not live and not joins
+        return out
+    finally:
+        ctx.set_param(QS_PARAM_BATCH_GRAPH, old)
+
+
+def test_batch_graph_replays_match_eager():
+    """Graph launches (replayed across slot changes) give the coefficients of
+    kernel-by-kernel launches, bit for bit."""
+    eager = _graph_sequence(0)
+    graph = _graph_sequence(1)
+    assert eager.keys() == graph.keys() and len(eager) == 5
+    for k in eager:
+        assert np.array_equal(eager[k][0], graph[k][0]) and np.array_equal(eager[k][1], graph[k][1]), k

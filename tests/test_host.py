@@ -99,3 +99,41 @@ def test_tridiagonal_lowest_and_extremes(m):
     assert abs(np.linalg.norm(z) - 1) < 1e-12
     lo, hi = tridiag_extremes(d, e)
     assert abs(lo - w[0]) < 1e-12 * max(1, abs(w[0])) and abs(hi - w[-1]) < 1e-12 * max(1, abs(w[-1]))
+
+
+def test_tridiagonal_lowest_with_bracket_hints():
+    """The bracketed search returns the eigenvalue numpy finds whether the
+    hints are right, loose or wrong (wrong ones are detected and ignored),
+    including Lanczos tables with ghost copies of the lowest eigenvalue."""
+    from quanty_amd.engine import tridiag_lowest
+    g = np.random.default_rng(7)
+    for trial in range(300):
+        m = int(g.integers(2, 160))
+        if trial % 2:
+            d = g.normal(size=m) * 5 + 50
+            e = np.abs(g.normal(size=m - 1)) * 2
+            e[g.random(m - 1) < 0.1] = 0.0
+        else:  # Lanczos without re-orthogonalization on a small matrix
+            n = max(4, m // 4)
+            A = g.normal(size=(n, n))
+            A = A + A.T
+            v = g.normal(size=n)
+            v /= np.linalg.norm(v)
+            vp, bprev = np.zeros(n), 0.0
+            d, e = np.zeros(m), np.zeros(m - 1)
+            for k in range(m):
+                w = A @ v - bprev * vp
+                d[k] = v @ w
+                w -= d[k] * v
+                if k < m - 1:
+                    bprev = max(np.linalg.norm(w), 1e-300)
+                    e[k] = bprev
+                    vp, v = v, w / bprev
+        T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        w0 = np.linalg.eigvalsh(T)[0]
+        s = max(1.0, np.abs(np.linalg.eigvalsh(T)).max())
+        for lo, hi in [(None, None), (w0 - 1e-4 * s, w0 + 1e-9 * s), (w0 + 0.3 * s, w0 - 0.3 * s),
+                       (w0 - 10 * s, None), (None, w0 + 1e-14 * s)]:
+            E, z = tridiag_lowest(d, e, lo, hi)
+            assert abs(E - w0) < 1e-13 * s, (trial, lo, hi, E, w0)
+            assert np.linalg.norm(T @ z - E * z) < 1e-11 * s
