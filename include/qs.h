@@ -184,8 +184,9 @@ qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, int nsteps,
 /* Block Lanczos for k start vectors V0 (k vectors back to back, device):
  * block recurrence of LanczosBlockTriDiagonalizeCC (Core/BlockLanczos.cpp:2131-2371):
  * Loewdin-orthonormalised start block, B0 = <v|V0>, per step A_k = V_k^+ H V_k,
- * local re-orthogonalization against the two previous blocks, singular values
- * below sv_cut dropped. A_out: (nsteps+1)*k*k complex, B_out: nsteps*k*k complex
+ * local re-orthogonalization against the two previous blocks (two classical
+ * Gram-Schmidt rounds against both together), singular values below sv_cut
+ * dropped. A_out: (nsteps+1)*k*k complex, B_out: nsteps*k*k complex
  * (row-major blocks, host); *kept receives the block size of each step
  * (nsteps+1 ints, may be NULL). */
 qs_status qs_block_lanczos(qs_ctx *ctx, const qs_hop *h, int k, const double *V0, int nsteps,
