@@ -50,9 +50,20 @@ typedef struct qs_hop qs_hop;     /* an operator bound to (row basis, column bas
 qs_status qs_ctx_create(int device, qs_ctx **ctx);
 qs_status qs_ctx_destroy(qs_ctx *ctx);
 const char *qs_last_error(const qs_ctx *ctx);
-/* Device memory helpers so C callers need no HIP headers. */
+/* Device memory helpers so C callers need no HIP headers. All device memory
+ * of the library (these, and what bases, operators, batches and solvers hold)
+ * comes from a process-wide cache: freed blocks stay allocated and are reused
+ * for later requests of a similar size, since memory handed back to the driver
+ * is cleared when it is allocated again (up to a second for the multi-GB
+ * blocks of large problems). Free or destroy only after the work that uses the
+ * memory has completed (qs_sync for an async context). */
 qs_status qs_dev_alloc(qs_ctx *ctx, size_t bytes, void **ptr);
 qs_status qs_dev_free(qs_ctx *ctx, void *ptr);
+/* Return the cached blocks of ctx's device to the driver. The cache is also
+ * emptied by itself when an allocation would otherwise fail. */
+qs_status qs_trim_memory(qs_ctx *ctx);
+/* Bytes in use and bytes cached on ctx's device. */
+qs_status qs_memory_stats(qs_ctx *ctx, size_t *live_bytes, size_t *cached_bytes);
 qs_status qs_memcpy_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes);
 qs_status qs_memcpy_d2h(qs_ctx *ctx, void *dst, const void *src, size_t bytes);
 qs_status qs_sync(qs_ctx *ctx);
@@ -74,8 +85,12 @@ qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
  * (column and term id) when both fit, 0 always as uint32 column + uint16 term.
  * QS_PARAM_BATCH_GRAPH (default 1): 1 runs qs_batch_run's steps as one HIP
  * graph launch (a chain of kernel nodes per batch and step count, kept across
- * slot changes); 0 launches the kernels one by one. Results are identical. */
-enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3 };
+ * slot changes); 0 launches the kernels one by one. Results are identical.
+ * QS_PARAM_TERM_LDS (default 24576, range 0..65536): assembled H*psi kernels
+ * stage an operator's term tables (value and parity mask per term) in LDS when
+ * they take at most this many bytes, and read them from global memory
+ * otherwise. Larger tables in LDS leave fewer workgroups per CU. */
+enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3, QS_PARAM_TERM_LDS = 4 };
 qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value);
 qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
 

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libqs.so")
 
 QS_OK = 0
 QS_HOP_MATFREE, QS_HOP_ASSEMBLED, QS_HOP_AUTO = 0, 1, 2
-QS_PARAM_SHIFT_MIN, QS_PARAM_ELL_PACK, QS_PARAM_BATCH_GRAPH = 1, 2, 3
+QS_PARAM_SHIFT_MIN, QS_PARAM_ELL_PACK, QS_PARAM_BATCH_GRAPH, QS_PARAM_TERM_LDS = 1, 2, 3, 4
 QS_BATCH_KEEP_BASIS = 1
 STATUS = {0: "QS_OK", 1: "QS_ENOMEM_DEVICE", 2: "QS_ENOMEM_HOST", 3: "QS_EINVAL",
           4: "QS_EHIP", 5: "QS_ENOTCONVERGED", 6: "QS_EUNSUPPORTED"}
@@ -31,6 +31,8 @@ _SIGS = [
     ("qs_last_error", C.c_char_p, [_P]),
     ("qs_dev_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     ("qs_dev_free", C.c_int, [_P, _P]),
+    ("qs_trim_memory", C.c_int, [_P]),
+    ("qs_memory_stats", C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     ("qs_memcpy_h2d", C.c_int, [_P, _P, _P, C.c_size_t]),
     ("qs_memcpy_d2h", C.c_int, [_P, _P, _P, C.c_size_t]),
     ("qs_sync", C.c_int, [_P]),
@@ -147,6 +149,16 @@ class Context:
         v = C.c_int64()
         self.check(self.lib.qs_ctx_get_param(self.h, param, C.byref(v)), "qs_ctx_get_param")
         return v.value
+
+    def memory_stats(self) -> tuple[int, int]:
+        """(bytes in use, bytes cached) of the library's device memory on this device."""
+        live, cached = C.c_size_t(), C.c_size_t()
+        self.check(self.lib.qs_memory_stats(self.h, C.byref(live), C.byref(cached)), "qs_memory_stats")
+        return live.value, cached.value
+
+    def trim_memory(self):
+        """Return the cached device memory to the driver."""
+        self.check(self.lib.qs_trim_memory(self.h), "qs_trim_memory")
 
     def close(self):
         if getattr(self, "h", None):

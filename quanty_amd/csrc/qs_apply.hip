@@ -176,9 +176,9 @@ extern "C" int qs_op_nterms(const qs_op *op) { return op ? op->nterms : -1; }
 
 extern "C" qs_status qs_op_destroy(qs_op *op) {
   if (!op) return QS_OK;
-  if (op->d_terms) (void)hipFree(op->d_terms);
-  if (op->d_val) (void)hipFree(op->d_val);
-  if (op->d_par) (void)hipFree(op->d_par);
+  qs_free(op->d_terms);
+  qs_free(op->d_val);
+  qs_free(op->d_par);
   delete op;
   return QS_OK;
 }
@@ -570,7 +570,7 @@ qs_status qs_launch_mf_ext(qs_ctx *ctx, const qs_hop *h, const double *x, double
 static void free_stream(qs_hop *h) {
   void *bufs[] = {h->d_blk_ptr, h->d_blk, h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid, h->d_diag};
   for (void *b : bufs)
-    if (b) (void)hipFree(b);
+    qs_free(b);
   h->d_blk_ptr = nullptr;
   h->d_blk = nullptr;
   h->d_slice_ptr = nullptr;
@@ -755,7 +755,7 @@ static void launch_spmv(qs_ctx *ctx, const qs_hop *h, const double2 *x, int64_t 
   const SellD &A = h->view;
   const size_t shm = term_table_bytes(A.nterms, NW);
   const int64_t want = (A.nslices + 3) / 4;
-  if (shm <= QS_TERM_LDS_BYTES) {
+  if (shm <= (size_t)ctx->term_lds) {
     const void *kern = (const void *)k_spmv<KC, NW, PACKED, true>;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, qs_resident_blocks(ctx, kern, shm)));
     hipLaunchKernelGGL((k_spmv<KC, NW, PACKED, true>), dim3(blocks), dim3(QS_BLOCK), shm, ctx->stream, A, x, ldx,
@@ -858,6 +858,6 @@ extern "C" qs_status qs_expect(qs_ctx *ctx, const qs_op *op, const qs_basis *b, 
   qs_status st = launch_mf(ctx, op, b, b, true, x, y);
   // <x|O x>
   if (st == QS_OK) st = qs_dot_internal(ctx, b->n, x, y, re, im);
-  (void)hipFree(y);
+  qs_free(y);
   return st;
 }

@@ -320,7 +320,7 @@ static void pool_trim(qs_batch *B) {
     auto it = B->pool.size() > QS_POOL_MAX_BUFFERS ? std::min_element(B->pool.begin(), B->pool.end())
                                                    : std::max_element(B->pool.begin(), B->pool.end());
     total -= it->first;
-    (void)hipFree(it->second);
+    qs_free(it->second);
     B->pool.erase(it);
   }
 }
@@ -344,7 +344,7 @@ static qs_status pool_get(qs_batch *B, size_t need, void **p, size_t *bytes) {
   if (st == QS_ENOMEM_DEVICE && !B->pool.empty()) {
     (void)hipGetLastError();
     (void)hipStreamSynchronize(B->ctx->stream);
-    for (auto &pb : B->pool) (void)hipFree(pb.second);
+    for (auto &pb : B->pool) qs_free(pb.second);
     B->pool.clear();
     st = qs_malloc(B->ctx, p, need);
   }
@@ -382,14 +382,14 @@ static qs_status rebuild(qs_batch *B) {
     if (B->used[s]) {
       const SellD &A = B->h_slots[s].A;
       const size_t tb = term_table_bytes(A.nterms, A.nw);
-      if (tb <= QS_TERM_LDS_BYTES) lds = std::max(lds, tb);
+      if (tb <= (size_t)ctx->term_lds) lds = std::max(lds, tb);
       fast = fast && A.nw == 1 && A.packed;
     }
   B->lds1 = lds;
   B->fast1 = fast;
   auto grow = [&](WorkItem **p, int64_t *cap, int64_t need) -> qs_status {
     if (need > *cap) {
-      if (*p) (void)hipFree(*p);
+      qs_free(*p);
       *p = nullptr;
       int64_t c = std::max<int64_t>(need, 2 * *cap);
       QS_TRY(qs_malloc(ctx, (void **)p, (size_t)c * sizeof(WorkItem)));
@@ -402,11 +402,11 @@ static qs_status rebuild(qs_batch *B) {
   QS_TRY(grow(&B->d_w1, &B->cap1, std::max<size_t>(1, w1.size())));
   QS_TRY(grow(&B->d_w2, &B->cap2, std::max<size_t>(1, w2.size())));
   if (B->cap1 != c1 || !B->d_p1) {
-    if (B->d_p1) (void)hipFree(B->d_p1);
+    qs_free(B->d_p1);
     QS_TRY(qs_malloc(ctx, (void **)&B->d_p1, (size_t)B->cap1 * QS_K1_GROUP * 8));
   }
   if (B->cap2 != c2 || !B->d_p2) {
-    if (B->d_p2) (void)hipFree(B->d_p2);
+    qs_free(B->d_p2);
     QS_TRY(qs_malloc(ctx, (void **)&B->d_p2, (size_t)B->cap2 * 8));
   }
   if (!w1.empty())
@@ -828,19 +828,19 @@ extern "C" qs_status qs_batch_destroy(qs_batch *B) {
   if (!B) return QS_OK;
   if (B->ctx) (void)hipStreamSynchronize(B->ctx->stream);
   for (char *m : B->mem)
-    if (m) (void)hipFree(m);
+    qs_free(m);
   for (double2 *v : B->kept)
-    if (v) (void)hipFree(v);
-  for (auto &pb : B->pool) (void)hipFree(pb.second);
-  if (B->d_slots) (void)hipFree(B->d_slots);
-  if (B->d_w1) (void)hipFree(B->d_w1);
-  if (B->d_w2) (void)hipFree(B->d_w2);
+    qs_free(v);
+  for (auto &pb : B->pool) qs_free(pb.second);
+  qs_free(B->d_slots);
+  qs_free(B->d_w1);
+  qs_free(B->d_w2);
   drop_graphs(B);
-  if (B->d_n) (void)hipFree(B->d_n);
-  if (B->d_r1) (void)hipFree(B->d_r1);
-  if (B->d_r2) (void)hipFree(B->d_r2);
-  if (B->d_p1) (void)hipFree(B->d_p1);
-  if (B->d_p2) (void)hipFree(B->d_p2);
+  qs_free(B->d_n);
+  qs_free(B->d_r1);
+  qs_free(B->d_r2);
+  qs_free(B->d_p1);
+  qs_free(B->d_p2);
   delete B;
   return QS_OK;
 }

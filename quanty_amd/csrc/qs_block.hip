@@ -382,13 +382,13 @@ struct Work {
   double2 *vC = nullptr;
   size_t vC_bytes = 0;
   ~Work() {
-    if (vpart) (void)hipFree(vpart);
-    if (vC) (void)hipFree(vC);
+    qs_free(vpart);
+    qs_free(vC);
   }
   qs_status grow(double2 **p, size_t *have, size_t need) {
     if (need <= *have) return QS_OK;
     QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (*p) (void)hipFree(*p);
+    qs_free(*p);
     *p = nullptr;
     *have = 0;
     QS_TRY(qs_malloc(ctx, (void **)p, need));
@@ -602,7 +602,7 @@ extern "C" qs_status qs_block_lanczos(qs_ctx *ctx, const qs_hop *h, int k, const
   };
   st = run();
   (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(mem);
+  qs_free(mem);
   if (st != QS_OK) return st;
   memcpy(A_out, Aall.data(), Aall.size() * 16);
   memcpy(B_out, Ball.data(), Ball.size() * 16);
@@ -641,7 +641,7 @@ extern "C" qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const doubl
   }
   if (e == hipSuccess) e = hipMemcpyAsync(out, dO, no * 16, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(mem);
+  qs_free(mem);
   if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_block_cf_spectrum", __FILE__, __LINE__);
   return QS_OK;
 }
@@ -857,7 +857,7 @@ extern "C" qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstat
   };
   st = run();
   (void)hipStreamSynchronize(ctx->stream);
-  (void)hipFree(mem);
+  qs_free(mem);
   if (iterations_out) *iterations_out = iters;
   return st;
 }

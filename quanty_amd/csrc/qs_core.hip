@@ -3,8 +3,11 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstring>
+#include <iterator>
 #include <map>
 
+#include <mutex>
+#include <unordered_map>
 #include "qs_internal.h"
 
 // ---------------------------------------------------------------------------
@@ -26,16 +29,99 @@ qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *f
   return qs_fail(ctx, QS_EHIP, "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
 }
 
+// Device memory cache. Blocks freed by the library stay allocated and are
+// handed to later requests of a similar size: device memory the driver takes
+// back is cleared when it is allocated again, which costs up to a second for
+// the multi-GB Krylov bases and assembled operators of a serving stream and
+// stalls every other allocating thread meanwhile. Cached blocks are returned to
+// the driver only when an allocation would otherwise fail (largest first, as
+// many as it takes), or by qs_trim_memory.
+namespace {
+struct DevBlock {
+  int dev;
+  size_t bytes;
+};
+std::mutex g_alloc_mu;
+std::unordered_map<void *, DevBlock> g_live;
+std::map<int, std::multimap<size_t, void *>> g_cached;  // device -> (bytes, block)
+
+size_t alloc_size(size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  const size_t g = bytes >= ((size_t)1 << 20) ? ((size_t)1 << 20) : 256;
+  return (bytes + g - 1) / g * g;
+}
+
+// largest waste accepted when reusing a cached block for `bytes`
+size_t reuse_slack(size_t bytes) { return bytes < ((size_t)1 << 20) ? bytes : std::max(bytes / 16, (size_t)1 << 20); }
+
+void release_cached(int dev) {
+  auto it = g_cached.find(dev);
+  if (it == g_cached.end()) return;
+  for (auto &b : it->second) (void)hipFree(b.second);
+  it->second.clear();
+}
+}  // namespace
+
 qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes) {
   *p = nullptr;
-  if (bytes == 0) bytes = 256;
-  hipError_t e = hipMalloc(p, bytes);
+  const size_t want = alloc_size(bytes);
+  std::lock_guard<std::mutex> lock(g_alloc_mu);
+  auto &cache = g_cached[ctx->device];
+  auto it = cache.lower_bound(want);
+  if (it != cache.end() && it->first <= want + reuse_slack(want)) {
+    *p = it->second;
+    g_live[*p] = DevBlock{ctx->device, it->first};
+    cache.erase(it);
+    return QS_OK;
+  }
+  hipError_t e = hipMalloc(p, want);
+  // out of memory: give cached blocks back to the driver, largest first, until
+  // the allocation fits
+  while (e != hipSuccess && !cache.empty()) {
+    (void)hipGetLastError();
+    auto big = std::prev(cache.end());
+    (void)hipFree(big->second);
+    cache.erase(big);
+    e = hipMalloc(p, want);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();
     *p = nullptr;
-    return qs_fail(ctx, QS_ENOMEM_DEVICE, "hipMalloc of %zu bytes failed: %s", bytes,
-                   hipGetErrorString(e));
+    return qs_fail(ctx, QS_ENOMEM_DEVICE, "hipMalloc of %zu bytes failed: %s", want, hipGetErrorString(e));
   }
+  g_live[*p] = DevBlock{ctx->device, want};
+  return QS_OK;
+}
+
+void qs_free(void *p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lock(g_alloc_mu);
+  auto it = g_live.find(p);
+  if (it == g_live.end()) {
+    (void)hipFree(p);  // not from qs_malloc
+    return;
+  }
+  g_cached[it->second.dev].insert({it->second.bytes, p});
+  g_live.erase(it);
+}
+
+extern "C" qs_status qs_trim_memory(qs_ctx *ctx) {
+  if (!ctx) return QS_EINVAL;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  std::lock_guard<std::mutex> lock(g_alloc_mu);
+  release_cached(ctx->device);
+  return QS_OK;
+}
+
+extern "C" qs_status qs_memory_stats(qs_ctx *ctx, size_t *live_bytes, size_t *cached_bytes) {
+  if (!ctx) return QS_EINVAL;
+  std::lock_guard<std::mutex> lock(g_alloc_mu);
+  size_t l = 0, c = 0;
+  for (auto &b : g_live)
+    if (b.second.dev == ctx->device) l += b.second.bytes;
+  for (auto &b : g_cached[ctx->device]) c += b.first;
+  if (live_bytes) *live_bytes = l;
+  if (cached_bytes) *cached_bytes = c;
   return QS_OK;
 }
 
@@ -43,7 +129,7 @@ qs_status qs_scratch(qs_ctx *ctx, size_t bytes, void **p) {
   if (bytes > ctx->scratch_bytes) {
     if (ctx->scratch) {
       QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      QS_HIP(ctx, hipFree(ctx->scratch));
+      qs_free(ctx->scratch);
       ctx->scratch = nullptr;
       ctx->scratch_bytes = 0;
     }
@@ -100,7 +186,7 @@ extern "C" qs_status qs_ctx_destroy(qs_ctx *ctx) {
   if (!ctx) return QS_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->scratch) qs_free(ctx->scratch);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return QS_OK;
@@ -114,7 +200,7 @@ extern "C" qs_status qs_dev_alloc(qs_ctx *ctx, size_t bytes, void **ptr) {
 }
 extern "C" qs_status qs_dev_free(qs_ctx *ctx, void *ptr) {
   if (!ctx) return QS_EINVAL;
-  if (ptr) QS_HIP(ctx, hipFree(ptr));
+  qs_free(ptr);
   return QS_OK;
 }
 extern "C" qs_status qs_memcpy_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes) {
@@ -152,6 +238,11 @@ extern "C" qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value) {
       if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_BATCH_GRAPH must be 0 or 1");
       ctx->batch_graph = (int)value;
       return QS_OK;
+    case QS_PARAM_TERM_LDS:
+      if (value < 0 || value > QS_TERM_LDS_MAX)
+        return qs_fail(ctx, QS_EINVAL, "QS_PARAM_TERM_LDS must be in 0..%d", QS_TERM_LDS_MAX);
+      ctx->term_lds = (int)value;
+      return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
   }
@@ -168,6 +259,9 @@ extern "C" qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value) {
       return QS_OK;
     case QS_PARAM_BATCH_GRAPH:
       *value = ctx->batch_graph;
+      return QS_OK;
+    case QS_PARAM_TERM_LDS:
+      *value = ctx->term_lds;
       return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
@@ -686,8 +780,8 @@ extern "C" qs_status qs_basis_index(qs_ctx *ctx, const qs_basis *b, const uint64
 
 extern "C" qs_status qs_basis_destroy(qs_basis *b) {
   if (!b) return QS_OK;
-  if (b->d_words) (void)hipFree(b->d_words);
-  if (b->d_tbl) (void)hipFree(b->d_tbl);
+  qs_free(b->d_words);
+  qs_free(b->d_tbl);
   delete b;
   return QS_OK;
 }

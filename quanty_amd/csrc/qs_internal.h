@@ -16,6 +16,11 @@
 #define QS_MAX_CHUNKS 16          // 128 modes / 8 modes per chunk
 #define QS_DEAD 0xFFFFFFFFu        // rank-table successor of an invalid prefix
 
+// Term tables (V_t and P_t) up to this many bytes are staged in LDS by the
+// assembled kernels unless the context sets another limit.
+#define QS_TERM_LDS_DEFAULT (24 * 1024)
+#define QS_TERM_LDS_MAX (64 * 1024)
+
 struct qs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -30,6 +35,7 @@ struct qs_ctx {
   int shift_min = 56;              // QS_PARAM_SHIFT_MIN for operators assembled from now on
   int ell_pack = 1;                // QS_PARAM_ELL_PACK
   int batch_graph = 1;             // QS_PARAM_BATCH_GRAPH
+  int term_lds = QS_TERM_LDS_DEFAULT;  // QS_PARAM_TERM_LDS
 };
 
 // Exact rank of a determinant among all determinants satisfying the
@@ -166,6 +172,9 @@ qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *f
   } while (0)
 
 qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes);
+// Return a qs_malloc block to the library's device memory cache (null is a
+// no-op). The block must no longer be in use by queued work.
+void qs_free(void *p);
 qs_status qs_scratch(qs_ctx *ctx, size_t bytes, void **p);
 
 // Workgroups of QS_BLOCK (256) threads of `kernel` with `shm` bytes of dynamic

@@ -9,10 +9,11 @@
 #define QS_BLOCK 256
 #endif
 
-// Term tables (V_t and P_t) up to this size are staged in LDS by the
-// assembled kernels.
-#ifndef QS_TERM_LDS_BYTES
-#define QS_TERM_LDS_BYTES (24 * 1024)
+
+// Minimum workgroups per CU asked of the register allocator for the fused
+// Lanczos kernel's wide variants (two-word determinants or unpacked entries).
+#ifndef QS_MINB_WIDE
+#define QS_MINB_WIDE 1
 #endif
 
 // Wave-uniform copy of a value the compiler cannot prove uniform.

@@ -122,7 +122,7 @@ __device__ __forceinline__ void lz_k1_prologue(int step, int nb2, LzState S, dou
 // K1, assembled operator (fused SpMV + epilogue)
 template <int NW, bool PACKED, bool LDS>
 // the packed one-word variant fits 64 VGPRs (8 waves per SIMD) without spills
-__global__ void __launch_bounds__(QS_BLOCK, (NW == 1 && PACKED) ? 8 : 1)
+__global__ void __launch_bounds__(QS_BLOCK, (NW == 1 && PACKED) ? 8 : QS_MINB_WIDE)
 k_lz1_asm(int step, int nb2, LzState S, SellD A, const double2 *__restrict__ uc, double2 *__restrict__ up) {
   __shared__ double red[QS_BLOCK / 64];
   extern __shared__ char smem[];
@@ -166,7 +166,7 @@ template <int NW, bool PACKED>
 static Lz1Launch lz1_choose(qs_ctx *ctx, const SellD &A) {
   Lz1Launch L;
   L.shm = term_table_bytes(A.nterms, NW);
-  if (L.shm <= QS_TERM_LDS_BYTES) {
+  if (L.shm <= (size_t)ctx->term_lds) {
     L.kern = (const void *)k_lz1_asm<NW, PACKED, true>;
   } else {
     L.kern = (const void *)k_lz1_asm<NW, PACKED, false>;
@@ -279,7 +279,7 @@ extern "C" qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, 
   if (e == hipSuccess && st == QS_OK) e = hipMemcpyAsync(a_out, S.a, (size_t)(nsteps + 1) * 8, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess && st == QS_OK) e = hipMemcpyAsync(b_out, S.b, (size_t)nsteps * 8, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(mem);
+  qs_free(mem);
   if (st != QS_OK) return st;
   if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_lanczos", __FILE__, __LINE__);
   a_out[0] = 0.0;
@@ -337,7 +337,7 @@ extern "C" qs_status qs_cf_spectrum(qs_ctx *ctx, int nspec, int n, const double 
   if (e == hipSuccess) e = hipMemcpyAsync(re, dre, np * 8, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(im, dim, np * 8, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  (void)hipFree(mem);
+  qs_free(mem);
   if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_cf_spectrum", __FILE__, __LINE__);
   return QS_OK;
 }

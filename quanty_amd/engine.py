@@ -14,6 +14,13 @@ from . import det as D
 from ._lib import QS_HOP_ASSEMBLED, QS_HOP_AUTO, QS_HOP_MATFREE, Context, QSError, default_context, load, ptr
 
 
+def _torch_ready(device) -> None:
+    """Wait for torch's queued work on `device`'s current stream, where tensors
+    handed to the library were written. Only that stream: a device-wide
+    synchronize would also wait for other host threads' library streams."""
+    torch.cuda.current_stream(device).synchronize()
+
+
 def _device(ctx: Context):
     return torch.device("cuda", ctx.device)
 
@@ -168,14 +175,14 @@ class Hop:
         if y is None:
             y = self.rows.zeros()
         c = ctx or self.ctx
-        torch.cuda.synchronize(x.device)
+        _torch_ready(x.device)
         c.check(self.lib.qs_hop_apply(c.h, self.h, ptr(x), ptr(y)), "qs_hop_apply")
         return y
 
     def lanczos(self, v0: torch.Tensor, nsteps: int):
         a = np.zeros(nsteps + 1)
         b = np.zeros(nsteps)
-        torch.cuda.synchronize(v0.device)
+        _torch_ready(v0.device)
         self.ctx.check(self.lib.qs_lanczos(self.ctx.h, self.h, ptr(v0), nsteps, ptr(a), ptr(b)), "qs_lanczos")
         return a, b
 
@@ -186,7 +193,7 @@ class Hop:
         A = np.zeros((nsteps + 1, k, k), dtype=np.complex128)
         B = np.zeros((nsteps, k, k), dtype=np.complex128)
         kept = np.zeros(nsteps + 1, dtype=np.int32)
-        torch.cuda.synchronize(V0.device)
+        _torch_ready(V0.device)
         self.ctx.check(self.lib.qs_block_lanczos(self.ctx.h, self.h, k, ptr(V0), nsteps, sv_cut, ptr(A), ptr(B),
                                                  ptr(kept)), "qs_block_lanczos")
         return A, B, kept
@@ -205,7 +212,7 @@ class Hop:
         vecs = torch.zeros((nstates, self.rows.n), dtype=torch.complex128, device=_device(self.ctx))
         if start is not None:
             start = start.contiguous()
-        torch.cuda.synchronize()
+        _torch_ready(vecs.device)
         self.ctx.check(self.lib.qs_ground_states_ex(self.ctx.h, self.h, nstates, tol, krylov_start, krylov_step,
                                                     max_krylov, max_restarts, seed, ptr(start), ptr(E), ptr(vecs),
                                                     ptr(var), C.byref(it)), "qs_ground_states_ex")
@@ -222,14 +229,14 @@ class Hop:
 
 def apply(op: Op, rows: Basis, cols: Basis, x: torch.Tensor) -> torch.Tensor:
     y = rows.zeros()
-    torch.cuda.synchronize(x.device)
+    _torch_ready(x.device)
     op.ctx.check(op.lib.qs_apply(op.ctx.h, op.h, rows.h, cols.h, ptr(x), ptr(y)), "qs_apply")
     return y
 
 
 def expect(op: Op, basis: Basis, x: torch.Tensor) -> complex:
     re, im = C.c_double(), C.c_double()
-    torch.cuda.synchronize(x.device)
+    _torch_ready(x.device)
     op.ctx.check(op.lib.qs_expect(op.ctx.h, op.h, basis.h, ptr(x), C.byref(re), C.byref(im)), "qs_expect")
     return complex(re.value, im.value)
 
@@ -237,7 +244,7 @@ def expect(op: Op, basis: Basis, x: torch.Tensor) -> complex:
 def dot(a: torch.Tensor, b: torch.Tensor, ctx: Context | None = None) -> complex:
     ctx = ctx or default_context()
     re, im = C.c_double(), C.c_double()
-    torch.cuda.synchronize(a.device)
+    _torch_ready(a.device)
     ctx.check(ctx.lib.qs_dot(ctx.h, a.numel(), ptr(a), ptr(b), C.byref(re), C.byref(im)), "qs_dot")
     return complex(re.value, im.value)
 
@@ -304,7 +311,7 @@ class Batch:
         slot = C.c_int()
         yc = None if ycoef is None else np.ascontiguousarray(ycoef, dtype=np.float64)
         md = depth if max_depth is None else max(depth, max_depth)
-        torch.cuda.synchronize(v0.device)
+        _torch_ready(v0.device)
         flags = 1 if keep_basis else 0
         self.ctx.check(self.lib.qs_batch_add_ex(self.h, hop.h, ptr(v0), depth, md, flags, ptr(yc), C.byref(slot)),
                        "qs_batch_add_ex")
@@ -333,7 +340,7 @@ class Batch:
         """y = sum_i z[i] v_i over the slot's kept Lanczos vectors."""
         zz = np.ascontiguousarray(z, dtype=np.float64)
         assert y.dtype == torch.complex128 and y.is_contiguous() and y.numel() == self.n[slot]
-        torch.cuda.synchronize(y.device)
+        _torch_ready(y.device)
         self.ctx.check(self.lib.qs_batch_ritz(self.h, slot, len(zz), ptr(zz), ptr(y)), "qs_batch_ritz")
         return y
 

@@ -146,17 +146,33 @@ class _Job:
             setattr(self, k, None)
 
 
+# Library contexts of the preparation workers, kept for the life of the
+# process. Each context owns a HIP stream, and a process has few hardware
+# queues (4 by default): streams created anew for every server would come to
+# share the serving stream's queue and serialize behind its work.
+_WORKER_CTXS: dict[int, list] = {}
+_WORKER_LOCK = threading.Lock()
+
+
+def _worker_contexts(device: int, n: int) -> list:
+    from ._lib import Context
+    with _WORKER_LOCK:
+        pool = _WORKER_CTXS.setdefault(device, [])
+        while len(pool) < n:
+            pool.append(Context(device))
+        return pool[:n]
+
+
 class _Prep:
     """Builds the state of admitted requests (basis, operators, assembled
     stream, start vector) on worker threads, each with its own library context,
     so that the preparation overlaps the batch's GPU steps."""
 
     def __init__(self, device: int, gs_depth: int, workers: int = 1):
-        from ._lib import Context
         self.gs_depth = gs_depth
         self.todo: queue.Queue = queue.Queue()
         self.ready: queue.Queue = queue.Queue()
-        self.ctxs = [Context(device) for _ in range(max(1, workers))]
+        self.ctxs = _worker_contexts(device, max(1, workers))
         self.threads = [threading.Thread(target=self._run, args=(c,), daemon=True) for c in self.ctxs]
         for t in self.threads:
             t.start()
@@ -168,7 +184,7 @@ class _Prep:
                 return
             try:
                 job = _Job(r, ctx, self.gs_depth)
-                torch.cuda.synchronize()
+                torch.cuda.current_stream().synchronize()  # the start vector; library calls are synchronous
                 self.ready.put(job)
             except BaseException as e:  # handed to the serving loop, which raises it
                 self.ready.put(e)
@@ -178,8 +194,6 @@ class _Prep:
             self.todo.put(None)
         for t in self.threads:
             t.join()
-        for c in self.ctxs:
-            c.close()
 
 
 def estimate_bytes(dim: int, kept_vectors: int = 0) -> int:

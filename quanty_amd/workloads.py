@@ -4,6 +4,7 @@
     python3 -m quanty_amd.workloads --config 1    # 50-mode impurity: 3 lowest states, tol 1e-12
     python3 -m quanty_amd.workloads --config 2a   # 9 right-hand sides, 300-step block Lanczos
     python3 -m quanty_amd.workloads --config 2b   # 512 small impurity problems, batched vs one at a time
+    python3 -m quanty_amd.workloads --config 3    # the serving stream at 10, 50, 200 and 500 req/s
     python3 -m quanty_amd.workloads --config 4    # 76-mode (two-word) L2,3 Green's function
 
 Each run prints one JSON line: what was run, its sizes, wall times (HIP work
@@ -125,9 +126,40 @@ def config2b(n: int = 512, seed: int = 21, slots: int = 64):
             "gs_steps_mean": float(np.mean([r.gs_steps for r in served]))}
 
 
+def config3(requests: int = 64, rates=(10.0, 50.0, 200.0, 500.0), seed: int = 101, slots: int = 64):
+    """The serving stream at several arrival rates: `requests` Poisson arrivals
+    per rate (dims log-uniform 10^3..3*10^6, depth U{100..400}, Gamma
+    U(0.1,0.5)), served with continuous batching and one at a time; latency
+    from arrival to the spectrum on the host."""
+    from . import serve
+
+    ctx = default_context()
+    warm = serve.request_stream(4, 0.0, seed=seed + 100, dim_lo=1e3, dim_hi=1e4)
+    serve.SpectrumServer(ctx, max_slots=4).serve(warm, realtime=False)
+    rows = []
+    for rate in rates:
+        reqs = serve.request_stream(requests, rate, seed=seed)
+        served = serve.SpectrumServer(ctx, max_slots=slots).serve(reqs, realtime=True)
+        _log(f"rate {rate}: served")
+        plain = serve.SpectrumServer(ctx, max_slots=1).serve(serve.request_stream(requests, rate, seed=seed),
+                                                              realtime=True, plain=True)
+        _log(f"rate {rate}: plain")
+        by = {r.rid: r for r in plain}
+        s1, s2 = serve.latency_stats(served), serve.latency_stats(plain)
+        rows.append({"rate_per_s": rate, "served": s1, "plain": s2,
+                     "speedup": s1["spectra_per_s"] / s2["spectra_per_s"],
+                     "identical_spectra": all(np.array_equal(r.spectrum, by[r.rid].spectrum) for r in served)})
+    dims = np.array([r.dim for r in serve.request_stream(requests, 1.0, seed=seed)])
+    return {"config": "3", "what": f"{requests} Poisson requests per rate, dims log-uniform 1e3..3e6, depth "
+                                   f"U{{100..400}}, Gamma U(0.1,0.5), {slots} slots, 200 GB budget",
+            "dim_min": int(dims.min()), "dim_max": int(dims.max()), "dim_median": float(np.median(dims)),
+            "rates": rows}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--config", required=True, choices=["0", "1", "2a", "2b", "4"])
+    ap.add_argument("--config", required=True, choices=["0", "1", "2a", "2b", "3", "4"])
+    ap.add_argument("--requests", type=int, default=64, help="requests per rate (config 3)")
     args = ap.parse_args()
     if args.config == "0":
         out = config0()
@@ -137,6 +169,8 @@ def main():
         out = config2a()
     elif args.config == "2b":
         out = config2b()
+    elif args.config == "3":
+        out = config3(args.requests)
     else:
         out = config4()
     print(json.dumps(out), flush=True)
