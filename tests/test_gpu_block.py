@@ -139,3 +139,51 @@ def test_ground_states_full_size_anderson():
     hv = hop.apply(vecs[0].contiguous())
     e0 = dot(vecs[0].contiguous(), hv).real
     assert abs(e0 - E[0]) < 1e-9 * abs(E[0])
+
+
+def test_ground_states_krylov_schedules_agree():
+    """The Krylov growth schedule (start, step, maximum basis) changes how the
+    search proceeds, not its answer: every schedule reaches the dense
+    eigenvalues, including a fixed 40-vector basis that restarts every few
+    blocks."""
+    pr = P.anderson(**SMALL_GS)
+    nf = pr["NF"]
+    b = Basis.enumerate(nf, pr["restrictions"])
+    want = np.linalg.eigvalsh(dense_matrix(nf, pr["H"], b, pr["restrictions"]))[:3]
+    hop = Hop(Op(nf, pr["H"]), b, mode="assembled")
+    for start, step, mk in [(100, 10, 500), (20, 5, 60), (37, 0, 40), (300, 10, 500)]:
+        E, vecs, var, it = hop.ground_states(3, tol=1e-14, max_krylov=mk, krylov_start=start, krylov_step=step,
+                                             max_restarts=400)
+        assert np.allclose(E, want, rtol=1e-10, atol=1e-10), (start, step, mk, E, want)
+        for i in range(3):
+            v = vecs[i].contiguous()
+            assert abs(dot(v, v).real - 1.0) < 1e-10
+            r = hop.apply(v) - E[i] * v
+            assert dot(r, r).real < 1e-10 * max(1.0, float(np.sum(E ** 2)))
+
+
+def test_device_memory_cache_reuses_and_trims():
+    """Freed library memory is cached and reused (no growth over repeated
+    create/destroy cycles), and qs_trim_memory empties the cache."""
+    from quanty_amd import default_context
+    ctx = default_context()
+    pr = P.anderson(**SMALL_GS)
+    nf = pr["NF"]
+    b = Basis.enumerate(nf, pr["restrictions"])
+    op = Op(nf, pr["H"])
+    warm = Hop(op, b, mode="assembled")  # first use may grow the context's scratch space
+    del warm
+    live0, _ = ctx.memory_stats()
+    hop = Hop(op, b, mode="assembled")
+    live1, _ = ctx.memory_stats()
+    assert live1 > live0
+    del hop
+    live2, cached2 = ctx.memory_stats()
+    assert live2 == live0 and cached2 >= live1 - live0
+    for _ in range(3):  # the same sizes come back from the cache
+        hop = Hop(op, b, mode="assembled")
+        del hop
+        live, cached = ctx.memory_stats()
+        assert live == live0 and cached == cached2
+    ctx.trim_memory()
+    assert ctx.memory_stats() == (live0, 0)
