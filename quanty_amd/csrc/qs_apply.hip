@@ -364,7 +364,7 @@ template <int NW, bool FILL>
 __global__ void __launch_bounds__(QS_BLOCK)
 k_sell_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms, int nterms,
              Lookup L, int same, int hmin, uint16_t *__restrict__ rowcnt, uint32_t *__restrict__ slice_len,
-             uint32_t *__restrict__ blk_cnt, double2 *__restrict__ diag, unsigned long long *__restrict__ ctr,
+             uint32_t *__restrict__ blk_cnt, void *__restrict__ diag, int real_diag, unsigned long long *__restrict__ ctr,
              const uint64_t *__restrict__ blk_ptr, uint4 *__restrict__ blk, const uint64_t *__restrict__ slice_ptr,
              uint32_t *__restrict__ colv, uint16_t *__restrict__ tidv, int packed, uint32_t colbits) {
   __shared__ TermD<NW> sh[QS_TILE];
@@ -458,7 +458,10 @@ k_sell_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *
   }
   if (active) {
     rowcnt[row] = (uint16_t)nell;
-    if (diag) diag[row] = make_double2(dr, di);
+    if (diag) {
+      if (real_diag) static_cast<double *>(diag)[row] = dr;
+      else static_cast<double2 *>(diag)[row] = make_double2(dr, di);
+    }
   }
   uint32_t m = nell;
   unsigned long long e = eff;
@@ -594,7 +597,8 @@ static void launch_build(qs_ctx *ctx, qs_hop *h, unsigned blocks, const Lookup &
   const qs_op *op = h->op;
   hipLaunchKernelGGL((k_sell_build<NW, FILL>), dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, h->rows->d_words,
                      h->rows->n, (const TermD<NW> *)op->d_terms, op->nterms, L, (int)h->same, h->shift_min,
-                     h->d_rowcnt, h->d_slice_len, blk_cnt, (double2 *)h->d_diag, ctr, h->d_blk_ptr, h->d_blk,
+                     h->d_rowcnt, h->d_slice_len, blk_cnt, (void *)h->d_diag, h->view.real_diag, ctr, h->d_blk_ptr,
+                     h->d_blk,
                      h->d_slice_ptr, h->d_col, h->d_tid, h->view.packed, h->view.colbits);
 }
 
@@ -611,7 +615,13 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   QS_TRY(qs_malloc(ctx, (void **)&h->d_blk_ptr, (size_t)(nslices + 1) * 8));
   QS_TRY(qs_malloc(ctx, (void **)&h->d_slice_len, (size_t)std::max<int64_t>(nslices, 1) * 4));
   QS_TRY(qs_malloc(ctx, (void **)&h->d_rowcnt, (size_t)std::max<int64_t>(n, 1) * 2));
-  if (h->same) QS_TRY(qs_malloc(ctx, (void **)&h->d_diag, (size_t)std::max<int64_t>(n, 1) * 16));
+  // the diagonal is real when every diagonal term is (Hermitian operators)
+  bool real_diag = h->same;
+  for (const TermHost &t : op->h_terms)
+    if ((t.F[0] | t.F[1]) == 0 && t.im != 0.0) real_diag = false;
+  h->view.real_diag = real_diag ? 1 : 0;
+  const size_t diag_elem = real_diag ? 8 : 16;
+  if (h->same) QS_TRY(qs_malloc(ctx, (void **)&h->d_diag, (size_t)std::max<int64_t>(n, 1) * diag_elem));
   // scratch: 3 counters, then the per-slice block counts
   void *scr;
   QS_TRY(qs_scratch(ctx, 64 + (size_t)std::max<int64_t>(nslices, 1) * 4, &scr));
@@ -653,7 +663,7 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   h->nnz_stored = (int64_t)(ent + nblk);
   const uint64_t per_entry = h->view.packed ? 4 : 6;
   h->stream_bytes = (int64_t)(ent * per_entry + nblk * 16 + (uint64_t)n * (2 + 8 * op->nw) +
-                              (uint64_t)(nslices + 1) * 20 + (h->same ? n * 16 : 0));
+                              (uint64_t)(nslices + 1) * 20 + (h->same ? (uint64_t)n * diag_elem : 0));
   if (budget && (size_t)h->stream_bytes > budget)
     return qs_fail(ctx, QS_ENOMEM_DEVICE, "assembled stream needs %lld bytes, budget %zu",
                    (long long)h->stream_bytes, budget);
@@ -683,7 +693,7 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   A.tid = h->d_tid;
   A.val = (const double2 *)op->d_val;
   A.par = op->d_par;
-  A.diag = (const double2 *)h->d_diag;
+  A.diag = h->d_diag;
   h->assembled = 1;
   return QS_OK;
 }

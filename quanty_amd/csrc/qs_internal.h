@@ -116,6 +116,7 @@ struct SellD {
   int packed = 0;
   uint32_t colbits = 0;
   int nterms = 0;
+  int real_diag = 0;                   // diag holds doubles (every diagonal term is real)
   const uint64_t *rows = nullptr;      // row determinants (nw words each)
   const uint64_t *blk_ptr = nullptr;   // nslices + 1 block offsets
   const uint4 *blk = nullptr;          // shift-block headers
@@ -126,7 +127,7 @@ struct SellD {
   const uint16_t *tid = nullptr;       // ELL term when not packed
   const double2 *val = nullptr;        // V_t
   const uint64_t *par = nullptr;       // P_t
-  const double2 *diag = nullptr;       // pre-summed diagonal (rows == cols) or null
+  const void *diag = nullptr;          // pre-summed diagonal (rows == cols) or null: double2, or double if real_diag
 };
 
 struct qs_hop {

@@ -72,12 +72,22 @@ __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, co
     for (int w = 0; w < NW; ++w) d[w] = A.rows[r * NW + w];
     cnt = A.rowcnt[r];
     if (A.diag) {
-      const double2 dg = A.diag[r];
+      if (A.real_diag) {
+        const double dg = static_cast<const double *>(A.diag)[r];
 #pragma unroll
-      for (int k = 0; k < KC; ++k) {
-        const double2 xv = x[r * ldx + k * ldv];
-        ar[k] = dg.x * xv.x - dg.y * xv.y;
-        ai[k] = dg.x * xv.y + dg.y * xv.x;
+        for (int k = 0; k < KC; ++k) {
+          const double2 xv = x[r * ldx + k * ldv];
+          ar[k] = dg * xv.x;
+          ai[k] = dg * xv.y;
+        }
+      } else {
+        const double2 dg = static_cast<const double2 *>(A.diag)[r];
+#pragma unroll
+        for (int k = 0; k < KC; ++k) {
+          const double2 xv = x[r * ldx + k * ldv];
+          ar[k] = dg.x * xv.x - dg.y * xv.y;
+          ai[k] = dg.x * xv.y + dg.y * xv.x;
+        }
       }
     }
   }
