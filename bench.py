@@ -133,7 +133,8 @@ def hpsi_section(args, ctx, torch):
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         tr = json.load(open(TRAFFIC_FILE))
-        if tr.get("nnz_stored") == st["nnz_stored"] and tr.get("dim") == n:
+        if tr.get("nnz_stored") == st["nnz_stored"] and tr.get("dim") == n and \
+                tr.get("stream_bytes") == st["stream_bytes"]:
             traffic = tr["fetch_bytes"] + tr["write_bytes"]
     hp = {"dim": n, "ms": hpsi_ms, "nnz_eff": st["nnz_eff"], "nnz_offdiag": st["nnz_offdiag"],
           "nnz_stored": st["nnz_stored"], "nblocks": st["nblocks"], "block_hits": st["block_hits"],
