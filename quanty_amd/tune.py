@@ -5,7 +5,9 @@ for a term whose rows read consecutive columns) and per-entry ELL slots. Which
 groups become blocks is set by QS_PARAM_SHIFT_MIN; the best value depends on
 the operator's structure. `tune_shift_min` assembles the operator for each
 candidate, times H*psi with HIP events on the library's stream and returns the
-fastest (optionally leaving it set on the context).
+fastest (optionally leaving it set on the context). `tune_term_lds` does the
+same for the largest term table (values and parity masks) staged in LDS:
+staging saves global reads per entry but takes LDS from other workgroups.
 
     python3 -m quanty_amd.tune            # the 50-mode Anderson impurity, 2,179,235 dets
 """
@@ -17,10 +19,11 @@ import time
 import numpy as np
 import torch
 
-from ._lib import QS_PARAM_SHIFT_MIN
+from ._lib import QS_PARAM_SHIFT_MIN, QS_PARAM_TERM_LDS
 from .engine import Basis, Hop, Op
 
 CANDIDATES = (65, 64, 56, 48, 32, 24, 16, 8)
+LDS_CANDIDATES = (0, 24 * 1024, 64 * 1024)
 
 
 def time_apply(hop: Hop, x: torch.Tensor, y: torch.Tensor, reps: int) -> float:
@@ -72,6 +75,31 @@ def tune_shift_min(op: Op, basis: Basis, candidates=CANDIDATES, reps: int = 10, 
     return best, rows
 
 
+def tune_term_lds(op: Op, basis: Basis, candidates=LDS_CANDIDATES, reps: int = 10, apply: bool = True,
+                  seed: int = 1):
+    """Time H*psi of `op` on `basis` for each limit on the term tables staged
+    in LDS (0: always read from global memory). The operator is assembled once;
+    the limit takes effect at launch. Returns (best limit, rows); with
+    apply=True the best limit stays set on the operator's context."""
+    ctx = op.ctx
+    old = ctx.get_param(QS_PARAM_TERM_LDS)
+    g = np.random.default_rng(seed)
+    x = torch.from_numpy(g.normal(size=basis.n) + 1j * g.normal(size=basis.n)).cuda()
+    y = basis.zeros()
+    hop = Hop(op, basis, mode="assembled")
+    rows = []
+    try:
+        for lds in candidates:
+            ctx.set_param(QS_PARAM_TERM_LDS, lds)
+            rows.append({"term_lds": lds, "us": time_apply(hop, x, y, reps)})
+    finally:
+        ctx.set_param(QS_PARAM_TERM_LDS, old)
+    best = min(rows, key=lambda r: r["us"])["term_lds"]
+    if apply:
+        ctx.set_param(QS_PARAM_TERM_LDS, best)
+    return best, rows
+
+
 def main():
     from . import default_context
     from . import problems as P
@@ -93,6 +121,13 @@ def main():
     for r in rows:
         print(f"| {r['shift_min']} | {r['us']:.1f} | {r['nblocks']:,} | {r['block_hits']:,} | {r['ell_slots']:,} | "
               f"{r['stream_bytes'] / 1e6:.1f} |")
+    lbest, lrows = tune_term_lds(op, basis, reps=args.reps, apply=False, seed=args.seed)
+    print(f"\nlimit on the term tables staged in LDS, layout threshold {ctx.get_param(QS_PARAM_SHIFT_MIN)} "
+          f"({op.nterms} terms); fastest {lbest}")
+    print("| LDS limit (bytes) | H*psi us |")
+    print("|---|---|")
+    for r in lrows:
+        print(f"| {r['term_lds']} | {r['us']:.1f} |")
 
 
 if __name__ == "__main__":
