@@ -59,9 +59,10 @@ const char *qs_last_error(const qs_ctx *ctx);
  * memory has completed (qs_sync for an async context). */
 qs_status qs_dev_alloc(qs_ctx *ctx, size_t bytes, void **ptr);
 qs_status qs_dev_free(qs_ctx *ctx, void *ptr);
-/* Return the cached blocks of ctx's device to the driver. The cache is also
- * emptied by itself when an allocation would otherwise fail. */
-qs_status qs_trim_memory(qs_ctx *ctx);
+/* Return cached blocks of ctx's device to the driver, largest first, until at
+ * most keep_bytes stay cached (0: all). The cache also gives blocks back by
+ * itself when an allocation would otherwise fail. */
+qs_status qs_trim_memory(qs_ctx *ctx, size_t keep_bytes);
 /* Bytes in use and bytes cached on ctx's device. */
 qs_status qs_memory_stats(qs_ctx *ctx, size_t *live_bytes, size_t *cached_bytes);
 qs_status qs_memcpy_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes);

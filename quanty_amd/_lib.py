@@ -31,7 +31,7 @@ _SIGS = [
     ("qs_last_error", C.c_char_p, [_P]),
     ("qs_dev_alloc", C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     ("qs_dev_free", C.c_int, [_P, _P]),
-    ("qs_trim_memory", C.c_int, [_P]),
+    ("qs_trim_memory", C.c_int, [_P, C.c_size_t]),
     ("qs_memory_stats", C.c_int, [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     ("qs_memcpy_h2d", C.c_int, [_P, _P, _P, C.c_size_t]),
     ("qs_memcpy_d2h", C.c_int, [_P, _P, _P, C.c_size_t]),
@@ -156,9 +156,9 @@ class Context:
         self.check(self.lib.qs_memory_stats(self.h, C.byref(live), C.byref(cached)), "qs_memory_stats")
         return live.value, cached.value
 
-    def trim_memory(self):
-        """Return the cached device memory to the driver."""
-        self.check(self.lib.qs_trim_memory(self.h), "qs_trim_memory")
+    def trim_memory(self, keep: int = 0):
+        """Return cached device memory to the driver until at most `keep` bytes stay cached."""
+        self.check(self.lib.qs_trim_memory(self.h, int(keep)), "qs_trim_memory")
 
     def close(self):
         if getattr(self, "h", None):

@@ -63,9 +63,6 @@ struct qs_batch {
   std::vector<size_t> mem_bytes;
   std::vector<double2 *> kept; // per-slot kept Lanczos vectors or null
   std::vector<size_t> kept_bytes;
-  // device buffers of removed slots, reused by later slots (device
-  // allocations and frees are slow and synchronize the device)
-  std::vector<std::pair<size_t, void *>> pool;
   std::vector<const qs_hop *> hops;
   // worklists (rebuilt when the set of slots changes)
   bool dirty = true;
@@ -309,53 +306,18 @@ k_b_ritz(const SlotDev *__restrict__ slots, int slot, int m, const double *__res
 // ---------------------------------------------------------------------------
 // host
 // ---------------------------------------------------------------------------
-#define QS_POOL_MAX_BUFFERS 128
-#define QS_POOL_MAX_BYTES ((size_t)64 << 30)
-
-static void pool_trim(qs_batch *B) {
-  size_t total = 0;
-  for (auto &pb : B->pool) total += pb.first;
-  while (!B->pool.empty() && (B->pool.size() > QS_POOL_MAX_BUFFERS || total > QS_POOL_MAX_BYTES)) {
-    // too many buffers: drop the smallest; too many bytes: drop the largest
-    auto it = B->pool.size() > QS_POOL_MAX_BUFFERS ? std::min_element(B->pool.begin(), B->pool.end())
-                                                   : std::max_element(B->pool.begin(), B->pool.end());
-    total -= it->first;
-    qs_free(it->second);
-    B->pool.erase(it);
-  }
-}
-
-// A device buffer of at least `need` bytes: the smallest pooled one that wastes
-// at most half of itself, else a new allocation (after giving the pool back
-// if the device is full).
+// Slot memory comes from the library's device memory cache, which keeps
+// freed blocks for reuse across slots, batches and servers.
 static qs_status pool_get(qs_batch *B, size_t need, void **p, size_t *bytes) {
-  int best = -1;
-  for (int i = 0; i < (int)B->pool.size(); ++i)
-    if (B->pool[i].first >= need && B->pool[i].first <= 2 * need &&
-        (best < 0 || B->pool[i].first < B->pool[best].first))
-      best = i;
-  if (best >= 0) {
-    *bytes = B->pool[best].first;
-    *p = B->pool[best].second;
-    B->pool.erase(B->pool.begin() + best);
-    return QS_OK;
-  }
-  qs_status st = qs_malloc(B->ctx, p, need);
-  if (st == QS_ENOMEM_DEVICE && !B->pool.empty()) {
-    (void)hipGetLastError();
-    (void)hipStreamSynchronize(B->ctx->stream);
-    for (auto &pb : B->pool) qs_free(pb.second);
-    B->pool.clear();
-    st = qs_malloc(B->ctx, p, need);
-  }
-  if (st == QS_OK) *bytes = need;
-  return st;
+  QS_TRY(qs_malloc(B->ctx, p, need));
+  *bytes = need;
+  return QS_OK;
 }
 
 static void pool_put(qs_batch *B, void *p, size_t bytes) {
-  if (!p) return;
-  B->pool.push_back({bytes, p});
-  pool_trim(B);
+  (void)B;
+  (void)bytes;
+  qs_free(p);
 }
 
 static qs_status rebuild(qs_batch *B) {
@@ -831,7 +793,6 @@ extern "C" qs_status qs_batch_destroy(qs_batch *B) {
     qs_free(m);
   for (double2 *v : B->kept)
     qs_free(v);
-  for (auto &pb : B->pool) qs_free(pb.second);
   qs_free(B->d_slots);
   qs_free(B->d_w1);
   qs_free(B->d_w2);

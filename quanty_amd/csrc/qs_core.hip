@@ -30,7 +30,7 @@ qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *f
 }
 
 // Device memory cache. Blocks freed by the library stay allocated and are
-// handed to later requests of a similar size: device memory the driver takes
+// handed to later requests of the same size class: device memory the driver takes
 // back is cleared when it is allocated again, which costs up to a second for
 // the multi-GB Krylov bases and assembled operators of a serving stream and
 // stalls every other allocating thread meanwhile. Cached blocks are returned to
@@ -45,21 +45,21 @@ std::mutex g_alloc_mu;
 std::unordered_map<void *, DevBlock> g_live;
 std::map<int, std::multimap<size_t, void *>> g_cached;  // device -> (bytes, block)
 
+// Block sizes: multiples of 256 B below 1 MiB; above, eight classes per
+// octave (m * 2^(e-3), m = 8..15), at most 12.5% over the request, so that
+// requests of similar size share blocks.
 size_t alloc_size(size_t bytes) {
   if (bytes == 0) bytes = 256;
-  const size_t g = bytes >= ((size_t)1 << 20) ? ((size_t)1 << 20) : 256;
+  if (bytes < ((size_t)1 << 20)) return (bytes + 255) / 256 * 256;
+  int e = 63 - __builtin_clzll((unsigned long long)bytes);
+  const size_t g = (size_t)1 << (e - 3);
   return (bytes + g - 1) / g * g;
 }
 
-// largest waste accepted when reusing a cached block for `bytes`
-size_t reuse_slack(size_t bytes) { return bytes < ((size_t)1 << 20) ? bytes : std::max(bytes / 16, (size_t)1 << 20); }
+// largest waste accepted when reusing a cached block for `bytes` (a class-
+// rounded size): the next class up above 1 MiB, twice the size below
+size_t reuse_slack(size_t bytes) { return bytes < ((size_t)1 << 20) ? bytes : bytes / 7; }
 
-void release_cached(int dev) {
-  auto it = g_cached.find(dev);
-  if (it == g_cached.end()) return;
-  for (auto &b : it->second) (void)hipFree(b.second);
-  it->second.clear();
-}
 }  // namespace
 
 qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes) {
@@ -105,11 +105,19 @@ void qs_free(void *p) {
   g_live.erase(it);
 }
 
-extern "C" qs_status qs_trim_memory(qs_ctx *ctx) {
+extern "C" qs_status qs_trim_memory(qs_ctx *ctx, size_t keep_bytes) {
   if (!ctx) return QS_EINVAL;
   QS_HIP(ctx, hipSetDevice(ctx->device));
   std::lock_guard<std::mutex> lock(g_alloc_mu);
-  release_cached(ctx->device);
+  auto &cache = g_cached[ctx->device];
+  size_t total = 0;
+  for (auto &b : cache) total += b.first;
+  while (total > keep_bytes && !cache.empty()) {  // largest blocks first
+    auto big = std::prev(cache.end());
+    total -= big->first;
+    (void)hipFree(big->second);
+    cache.erase(big);
+  }
   return QS_OK;
 }
 

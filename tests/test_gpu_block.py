@@ -185,5 +185,9 @@ def test_device_memory_cache_reuses_and_trims():
         del hop
         live, cached = ctx.memory_stats()
         assert live == live0 and cached == cached2
+    keep = cached2 // 2
+    ctx.trim_memory(keep)
+    live, cached = ctx.memory_stats()
+    assert live == live0 and cached <= keep
     ctx.trim_memory()
     assert ctx.memory_stats() == (live0, 0)
