@@ -55,8 +55,10 @@ const char *qs_last_error(const qs_ctx *ctx);
  * comes from a process-wide cache: freed blocks stay allocated and are reused
  * for later requests of a similar size, since memory handed back to the driver
  * is cleared when it is allocated again (up to a second for the multi-GB
- * blocks of large problems). Free or destroy only after the work that uses the
- * memory has completed (qs_sync for an async context). */
+ * blocks of large problems). Cached blocks go back to the driver when a new
+ * block would leave less than 8 GiB of the device free for other allocators.
+ * Free or destroy only after the work that uses the memory has completed
+ * (qs_sync for an async context). */
 qs_status qs_dev_alloc(qs_ctx *ctx, size_t bytes, void **ptr);
 qs_status qs_dev_free(qs_ctx *ctx, void *ptr);
 /* Return cached blocks of ctx's device to the driver, largest first, until at

@@ -34,8 +34,12 @@ qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *f
 // back is cleared when it is allocated again, which costs up to a second for
 // the multi-GB Krylov bases and assembled operators of a serving stream and
 // stalls every other allocating thread meanwhile. Cached blocks are returned to
-// the driver only when an allocation would otherwise fail (largest first, as
-// many as it takes), or by qs_trim_memory.
+// the driver (largest first, as many as it takes) when a new block would leave
+// less than QS_FREE_RESERVE of the device free or would not fit, and by
+// qs_trim_memory.
+// device memory left free for other allocators when the cache takes a new block
+#define QS_FREE_RESERVE ((size_t)8 << 30)
+
 namespace {
 struct DevBlock {
   int dev;
@@ -74,6 +78,19 @@ qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes) {
     cache.erase(it);
     return QS_OK;
   }
+  // a new block: keep QS_FREE_RESERVE of device memory free for other
+  // allocators in the process (cached blocks go back to the driver, largest
+  // first, to make room)
+  size_t dev_free = 0, dev_total = 0;
+  if (!cache.empty() && hipMemGetInfo(&dev_free, &dev_total) == hipSuccess) {
+    while (!cache.empty() && dev_free < want + QS_FREE_RESERVE) {
+      auto big = std::prev(cache.end());
+      (void)hipFree(big->second);
+      dev_free += big->first;
+      cache.erase(big);
+    }
+  }
+  (void)hipGetLastError();
   hipError_t e = hipMalloc(p, want);
   // out of memory: give cached blocks back to the driver, largest first, until
   // the allocation fits
