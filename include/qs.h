@@ -168,6 +168,10 @@ qs_status qs_hop_stats(const qs_hop *h, int64_t *nnz_eff, int64_t *nnz_stored,
  * uint32, and the shift threshold used. Any pointer may be NULL. */
 qs_status qs_hop_layout(const qs_hop *h, int64_t *nblocks, int64_t *block_hits, int64_t *ell_slots,
                         int *packed, int *shift_min);
+/* The diagonal <d_i|O|d_i> of an assembled binding with rows == cols, as n
+ * complex numbers into device memory d (no counterpart in the reference; used
+ * to weight Lanczos start vectors towards low-energy determinants). */
+qs_status qs_hop_diagonal(qs_ctx *ctx, const qs_hop *h, double *d);
 /* y = O x (pull form: y_i = sum_t V_t <d_i| t |d_j> x_j). x over cols, y over
  * rows, device pointers. Equals OperatorPsiRestricted (Core/OperatorWaveFunction.cpp:2288)
  * with the row basis as restriction set, and OperatorPsiConserveBasis (:245) when

@@ -59,6 +59,7 @@ _SIGS = [
                                C.POINTER(_I64)]),
     ("qs_hop_layout", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
                                 C.POINTER(C.c_int)]),
+    ("qs_hop_diagonal", C.c_int, [_P, _P, _P]),
     ("qs_hop_apply", C.c_int, [_P, _P, _P, _P]),
     ("qs_hop_apply_multi", C.c_int, [_P, _P, C.c_int, _P, _P]),
     ("qs_hop_destroy", C.c_int, [_P]),

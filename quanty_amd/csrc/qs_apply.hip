@@ -746,6 +746,28 @@ extern "C" qs_status qs_hop_stats(const qs_hop *h, int64_t *nnz_eff, int64_t *nn
   return QS_OK;
 }
 
+__global__ void k_diag_out(int64_t n, const void *__restrict__ diag, int real_diag, double2 *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = real_diag ? make_double2(static_cast<const double *>(diag)[i], 0.0)
+                       : static_cast<const double2 *>(diag)[i];
+}
+
+extern "C" qs_status qs_hop_diagonal(qs_ctx *ctx, const qs_hop *h, double *d) {
+  if (!ctx || !h || !d) return QS_EINVAL;
+  if (!h->assembled || !h->same || !h->d_diag)
+    return qs_fail(ctx, QS_EINVAL, "qs_hop_diagonal needs an assembled binding with rows == cols");
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t n = h->rows->n;
+  if (n > 0) {
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + QS_BLOCK - 1) / QS_BLOCK, 4096));
+    hipLaunchKernelGGL(k_diag_out, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, n, (const void *)h->d_diag,
+                       h->view.real_diag, (double2 *)d);
+    QS_HIP(ctx, hipGetLastError());
+  }
+  if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
 extern "C" qs_status qs_hop_layout(const qs_hop *h, int64_t *nblocks, int64_t *block_hits, int64_t *ell_slots,
                                    int *packed, int *shift_min) {
   if (!h) return QS_EINVAL;

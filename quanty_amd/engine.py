@@ -168,6 +168,13 @@ class Hop:
                        shift_min=sm.value)
         return out
 
+    def diagonal(self) -> torch.Tensor:
+        """<d_i|O|d_i> for every row (assembled, rows == cols), a complex128 device vector."""
+        d = self.rows.zeros()
+        _torch_ready(d.device)
+        self.ctx.check(self.lib.qs_hop_diagonal(self.ctx.h, self.h, ptr(d)), "qs_hop_diagonal")
+        return d
+
     def apply(self, x: torch.Tensor, y: torch.Tensor | None = None, ctx: Context | None = None) -> torch.Tensor:
         """y = O x. ctx: the context (stream) to run on, when not the one the
         binding was made with (a context serves one host thread at a time)."""

@@ -2,7 +2,8 @@
 
 A request is one impurity spectrum: a seeded Anderson impurity (d shell plus
 bath shells, sizes from 10^3 to a few 10^6 determinants), its ground state
-(Lanczos coefficients extended in steps until the lowest Ritz pair has
+(Lanczos from a seeded random vector weighted towards low-energy determinants,
+extended in steps until the lowest Ritz pair has
 |H psi - E psi|^2 <= tol max(1, E^2); the Ritz vector is formed from the
 Lanczos vectors kept on the device, or, with keep_basis=False, rebuilt by a
 second pass from the same start), T|psi0> for a random one-body d<->bath
@@ -118,7 +119,8 @@ def request_stream(n: int, rate: float, seed: int = 7, dim_lo=1e3, dim_hi=3e6, d
 class _Job:
     """Device-resident state of an admitted request."""
 
-    def __init__(self, req: Request, ctx, gs_depth: int):
+    def __init__(self, req: Request, ctx, gs_depth: int, start_temperature: float | None = 0.3,
+                 start_floor: float = 1e-3):
         self.req = req
         pr = P.anderson(seed=req.seed, **req.shape)
         self.nf = pr["NF"]
@@ -133,6 +135,12 @@ class _Job:
         dev = torch.device("cuda", ctx.device)
         v = torch.randn(self.basis.n, dtype=torch.float64, device=dev, generator=g) + \
             1j * torch.randn(self.basis.n, dtype=torch.float64, device=dev, generator=g)
+        if start_temperature is not None:
+            # weight determinants by exp(-(H_ii - min H_ii)/T), at least start_floor:
+            # the ground state lives mostly on low-energy determinants, and the
+            # floor keeps every determinant (every symmetry sector) in the start
+            d = self.hop.diagonal().real
+            v = v * torch.clamp(torch.exp(-(d - d.min()) / start_temperature), min=start_floor)
         self.v0 = v.to(torch.complex128).contiguous()
         self.gs_depth = min(gs_depth, self.basis.n)
         self.phase = "gs1"
@@ -168,8 +176,9 @@ class _Prep:
     stream, start vector) on worker threads, each with its own library context,
     so that the preparation overlaps the batch's GPU steps."""
 
-    def __init__(self, device: int, gs_depth: int, workers: int = 1):
+    def __init__(self, device: int, gs_depth: int, workers: int = 1, start=(0.3, 1e-3)):
         self.gs_depth = gs_depth
+        self.start = start
         self.todo: queue.Queue = queue.Queue()
         self.ready: queue.Queue = queue.Queue()
         self.ctxs = _worker_contexts(device, max(1, workers))
@@ -183,7 +192,7 @@ class _Prep:
             if r is None:
                 return
             try:
-                job = _Job(r, ctx, self.gs_depth)
+                job = _Job(r, ctx, self.gs_depth, *self.start)
                 torch.cuda.current_stream().synchronize()  # the start vector; library calls are synchronous
                 self.ready.put(job)
             except BaseException as e:  # handed to the serving loop, which raises it
@@ -208,7 +217,7 @@ class SpectrumServer:
     def __init__(self, ctx, max_slots: int = 64, hbm_budget: float = 200e9, gs_depth: int = 60,
                  gs_max: int = 500, gs_step: int = 16, gs_tol: float = 1e-12, chunk: int = 8, ne: int = 2000,
                  emin: float = -10.0, emax: float = 20.0, keep_basis: bool = True, prepare_ahead: bool = True,
-                 prep_workers: int = 2):
+                 prep_workers: int = 2, start_temperature: float | None = 0.3, start_floor: float = 1e-3):
         self.ctx = ctx
         self.max_slots = max_slots
         self.budget = hbm_budget
@@ -221,6 +230,10 @@ class SpectrumServer:
         self.keep_basis = keep_basis
         self.prepare_ahead = prepare_ahead
         self.prep_workers = prep_workers
+        # ground-state start: random, weighted by exp(-(H_ii - min)/T) (at least
+        # start_floor) unless start_temperature is None
+        self.start_temperature = start_temperature
+        self.start_floor = start_floor
         self.batch = Batch(max_slots, ctx)
         self.jobs: dict[int, _Job] = {}
         self.used = 0
@@ -324,8 +337,8 @@ class SpectrumServer:
         pending = sorted(requests)
         waiting: list[Request] = []
         done: list[Request] = []
-        prep = _Prep(self.ctx.device, self.gs_depth, self.prep_workers) if (self.prepare_ahead and not plain) \
-            else None
+        prep = _Prep(self.ctx.device, self.gs_depth, self.prep_workers,
+                     (self.start_temperature, self.start_floor)) if (self.prepare_ahead and not plain) else None
         inflight: dict[int, int] = {}  # rid -> reserved bytes of requests being prepared
         kept = self.gs_max + 1 if self.keep_basis else 0
         slots = 1 if plain else self.max_slots
@@ -346,7 +359,7 @@ class SpectrumServer:
                     waiting.pop(0)
                     r.t_admit = time.perf_counter() - t0
                     if prep is None:
-                        self._start(_Job(r, self.ctx, self.gs_depth))
+                        self._start(_Job(r, self.ctx, self.gs_depth, self.start_temperature, self.start_floor))
                     else:
                         inflight[r.rid] = est
                         self.used += est

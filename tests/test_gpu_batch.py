@@ -250,3 +250,21 @@ def test_batch_graph_replays_match_eager():
     assert eager.keys() == graph.keys() and len(eager) == 5
     for k in eager:
         assert np.array_equal(eager[k][0], graph[k][0]) and np.array_equal(eager[k][1], graph[k][1]), k
+
+
+def test_server_weighted_start_reaches_the_same_ground_state():
+    """The ground-state start weighted towards low-energy determinants (with a
+    floor that keeps every determinant) converges to the energy of the plain
+    random start: both meet |H psi - E psi|^2 <= 1e-12 max(1, E^2), so their
+    Ritz values differ by at most residual^2 / gap, here below 1e-8 relative."""
+    from quanty_amd import default_context
+    ctx = default_context()
+    reqs = serve.request_stream(6, rate=0.0, seed=9, dim_lo=1e3, dim_hi=3e4)
+    a = {r.rid: r for r in serve.SpectrumServer(ctx, max_slots=8, start_temperature=None).serve(reqs, realtime=False)}
+    reqs2 = serve.request_stream(6, rate=0.0, seed=9, dim_lo=1e3, dim_hi=3e4)
+    b = {r.rid: r for r in serve.SpectrumServer(ctx, max_slots=8).serve(reqs2, realtime=False)}
+    assert a.keys() == b.keys()
+    for rid in a:
+        for r in (a[rid], b[rid]):
+            assert r.gs_residual ** 2 <= 1e-12 * max(1.0, r.E0 ** 2)
+        assert abs(a[rid].E0 - b[rid].E0) <= 1e-8 * max(1.0, abs(a[rid].E0)), (rid, a[rid].E0, b[rid].E0)
