@@ -217,10 +217,12 @@ __device__ __forceinline__ double slot_sum(const double *__restrict__ part, int 
   return block_sum_values((s0 + s1) + (s2 + s3), red);
 }
 
-// one block per slot: a_{i+1} = s_i^2 Re<H u_i|u_i>
-__global__ void __launch_bounds__(QS_BLOCK)
+// one block per slot: a_{i+1} = s_i^2 Re<H u_i|u_i>. A large slot has one
+// partial per slice (34k at 2.2 M rows): 1024 threads keep the sum short.
+#define QS_R1_BLOCK 1024
+__global__ void __launch_bounds__(QS_R1_BLOCK)
 k_b_r1(SlotDev *slots, int nslots, const int *__restrict__ range, const double *__restrict__ part) {
-  __shared__ double red[QS_BLOCK / 64];
+  __shared__ double red[QS_R1_BLOCK / 64];
   const int slot = blockIdx.x;
   if (slot >= nslots) return;
   SlotDev &S = slots[slot];
@@ -529,12 +531,13 @@ static qs_status build_graph(qs_batch *B, int steps, int g1, int g2, int lds1, h
   struct Node {
     void *f;
     int grid;
+    int block;
     int shm;
     void **args;
-  } const nodes[4] = {{k1, g1, lds1, a1},
-                      {reinterpret_cast<void *>(&k_b_r1), B->cap, 0, ar1},
-                      {reinterpret_cast<void *>(&k_b2), g2, 0, a2},
-                      {reinterpret_cast<void *>(&k_b_r2), B->cap, 0, ar2}};
+  } const nodes[4] = {{k1, g1, QS_BLOCK, lds1, a1},
+                      {reinterpret_cast<void *>(&k_b_r1), B->cap, QS_R1_BLOCK, 0, ar1},
+                      {reinterpret_cast<void *>(&k_b2), g2, QS_BLOCK, 0, a2},
+                      {reinterpret_cast<void *>(&k_b_r2), B->cap, QS_BLOCK, 0, ar2}};
   hipGraphNode_t prev = nullptr;
   hipError_t e = hipSuccess;
   for (int k = 0; k < steps && e == hipSuccess; ++k)
@@ -542,7 +545,7 @@ static qs_status build_graph(qs_batch *B, int steps, int g1, int g2, int lds1, h
       hipKernelNodeParams p{};
       p.func = nd.f;
       p.gridDim = dim3((unsigned)nd.grid);
-      p.blockDim = dim3(QS_BLOCK);
+      p.blockDim = dim3((unsigned)nd.block);
       p.sharedMemBytes = (unsigned)nd.shm;
       p.kernelParams = nd.args;
       p.extra = nullptr;
@@ -578,7 +581,8 @@ extern "C" qs_status qs_batch_run(qs_batch *B, int steps) {
       else
         hipLaunchKernelGGL(k_b1<false>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1,
                            B->d_n, B->d_p1, lds1);
-      hipLaunchKernelGGL(k_b_r1, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r1, B->d_p1);
+      hipLaunchKernelGGL(k_b_r1, dim3(B->cap), dim3(QS_R1_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r1,
+                         B->d_p1);
       hipLaunchKernelGGL(k_b2, dim3(g2), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_w2, B->d_n, B->d_p2);
       hipLaunchKernelGGL(k_b_r2, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r2, B->d_p2);
     }
