@@ -116,11 +116,13 @@ qs_status qs_basis_count(qs_ctx *ctx, int nf, int nres, const uint8_t *masks,
 qs_status qs_basis_from_words(qs_ctx *ctx, int nf, const uint64_t *words, int64_t n,
                               qs_basis **out);
 /* Closure of a determinant set under an operator: the smallest set containing
- * `seed` and every target t|d> != 0 of op's terms, optionally filtered by
- * restrictions (nres may be 0), iterated until nothing new appears or the set
- * exceeds max_dim (QS_EINVAL then). This is the reachable space the reference
- * grows implicitly by repeated H*psi (Core/BlockLanczos.cpp:5849-5858,
- * LuaInterface/ssp.cpp:2265-2316). */
+ * `seed` and every target t|d> != 0 of op's terms (a term counts when it acts
+ * on the determinant, whatever its amplitude), optionally filtered by at most
+ * 16 restrictions (nres may be 0), iterated until nothing new appears. When
+ * the set passes max_dim the call returns QS_ENOTCONVERGED and *out = NULL.
+ * This is the reachable space the reference grows by repeated H*psi
+ * (Core/BlockLanczos.cpp:5849-5858, LuaInterface/ssp.cpp:2265-2316). Built on
+ * the GPU; the result is sorted, lookups use binary search. */
 qs_status qs_basis_closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,
                            const uint8_t *masks, const int *lo, const int *hi,
                            int64_t max_dim, qs_basis **out);
@@ -292,6 +294,46 @@ qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstates, double 
 qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const double *A, const double *B,
                                const int *kept, double E0, double emin, double emax, int ne,
                                double gamma, double *out);
+
+/* Dense matrix of a binding on the device: M[c*nrows + r] = <r|O|c> (complex,
+ * column-major, nrows x ncols), the entries of every term added in term order
+ * — OperatorToDeterminantMatrix (Core/MatrixOperator.cpp:1149-1226). M: device,
+ * nrows*ncols complex. */
+qs_status qs_hop_dense(qs_ctx *ctx, const qs_hop *h, double *M);
+
+/* Eigen-decomposition of a dense Hermitian n x n matrix A (device, complex,
+ * column-major; its lower triangle is read and it is overwritten by the
+ * eigenvectors, column k = z_k) by rocSOLVER's zheevd, and the projections of
+ * nv vectors X (device, vector s at X + 2*s*n) on the eigenvectors:
+ * E[k] (host, ascending), P[k*nv + s] = <z_k|x_s> (host, complex).
+ * |P[k][s]|^2 are the pole weights of HermitianCompactMatrixToListOfPoles
+ * (Core/BasicMath/BasicMath_CompactMatrixVector.cpp:3990-4026), and
+ * conj(P[k][r]) P[k][c] the pole blocks of ...ToBlockListOfPoles (:4033-4113).
+ * QS_ENOTCONVERGED when zheevd reports failure. */
+qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const double *X, double *E, double *P);
+
+/* Spectra from poles E[0..npole) with projections P (npole*nv complex, as
+ * written by qs_dense_poles), on the GPU; host arrays.
+ * tensor = 0: out[s*(ne+1) + i] = sum_k |P[k][s]|^2 / (w_i + E0[s] - E_k + i gamma/2),
+ *   w_i = emin + i (emax-emin)/ne — ListOfPolesToSpectrum (Core/Spectral.cpp:1516-1591).
+ * tensor = 1: out[(i*nv + r)*nv + c] = sum_k conj(P[k][r]) P[k][c] / (w_i - (E_k - E0[0]) + i gamma/2),
+ *   w_i = (i/ne)(emax-emin) + emin — BlockListOfPolesToSpectraWithBroadening
+ *   (Core/BasicMathBlock/BasicMath_BlockTypeConversion.cpp:588-660).
+ * out: complex. */
+qs_status qs_pole_spectrum(qs_ctx *ctx, int npole, int nv, const double *E, const double *P, int tensor,
+                           const double *E0, double emin, double emax, int ne, double gamma, double *out);
+
+/* Broadening of nspec spectra of npts points on a grid of step dE, in place
+ * (re, im: nspec*npts host arrays): out(o) = sum_i in(i) K(o, i), K the
+ * Gaussian (kind 0) or Lorentzian (kind 1) line of full width at half maximum
+ * f_i, averaged over the 7 sub-points o - i + j/7, j = -3..3.
+ * per_point = 0: f_i = fwhm[0] for all i, and nothing changes when
+ * 2*fwhm[0] < dE — BroadenG / BroadenL (Core/Spectral.cpp:958-1044).
+ * per_point = 1: f_i = max(fwhm[i], dE/2), npts values — BroadenGvar /
+ * BroadenLvar (Core/Spectral.cpp:1046-1131). Evaluated on the GPU; the sum
+ * over i runs in a fixed order. */
+qs_status qs_broaden(qs_ctx *ctx, int nspec, int npts, double dE, int kind, int per_point, const double *fwhm,
+                     double *re, double *im);
 
 /* Dense Hermitian eigen-decomposition on the host (Householder + implicit QL),
  * used for the small projected problems. A: n*n complex column-major;

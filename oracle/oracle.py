@@ -29,6 +29,8 @@ def load():
         build()
     L = C.CDLL(LIB)
     P, I64, D = C.c_void_p, C.c_int64, C.c_double
+    L.or_broaden.argtypes = [C.c_int, D, C.c_int, C.c_int, P, P, P]
+    L.or_fwhm_table.argtypes = [C.c_int, P, P, D, D, C.c_int, P]
     L.or_ladder.restype = D
     L.or_ladder.argtypes = [P, P, C.c_int]
     L.or_restricted_ok.restype = C.c_int
@@ -185,6 +187,31 @@ def cf(a, b, E0, emin, emax, ne, gamma) -> np.ndarray:
     im = np.zeros(ne + 1)
     L.or_cf(_p(a), _p(b), len(b), E0, emin, emax, ne, gamma, _p(re), _p(im))
     return re + 1j * im
+
+
+def broaden(G, dE, kind, fwhm, per_point=False) -> np.ndarray:
+    """G: (nspec, npts) complex; kind "G" or "L"; fwhm: a number, or npts values
+    when per_point."""
+    L = load()
+    G = np.atleast_2d(np.asarray(G, dtype=np.complex128))
+    f = np.ascontiguousarray(np.atleast_1d(fwhm), dtype=np.float64)
+    out = np.empty_like(G)
+    for s in range(G.shape[0]):
+        re = np.ascontiguousarray(G[s].real)
+        im = np.ascontiguousarray(G[s].imag)
+        L.or_broaden(G.shape[1] - 1, float(dE), 0 if kind == "G" else 1, int(per_point), _p(f), _p(re), _p(im))
+        out[s] = re + 1j * im
+    return out
+
+
+def fwhm_table(table, emin, dE, npts) -> np.ndarray:
+    L = load()
+    t = np.asarray(table, dtype=np.float64).reshape(-1, 2)
+    E = np.ascontiguousarray(t[:, 0])
+    F = np.ascontiguousarray(t[:, 1])
+    out = np.zeros(npts)
+    L.or_fwhm_table(len(t), _p(E), _p(F), float(emin), float(dE), npts - 1, _p(out))
+    return out
 
 
 def max_threads() -> int:

@@ -15,6 +15,10 @@
  *   or_truncate          ComplexWaveFunctionTruncateBasis         Core/ComplexWaveFunctionArithmetic.cpp:1073
  *   or_lanczos           LanczosTriDiagonalize{,Restricted}CC     Core/Lanczos.cpp:2169-2264, 2501
  *   or_cf                TriDiagonalMatrixToSpectrum              Core/Spectral.cpp:1433-1480
+ *   or_broaden           BroadenG / BroadenL / BroadenGvar / BroadenLvar
+ *                                                                 Core/Spectral.cpp:958-1131
+ *   or_fwhm_table        the {{E, FWHM}, ...} interpolation of LuaSpectraBroaden
+ *                                                                 LuaInterface/Spectra.cpp:44-62
  *   or_enumerate         the restricted sector as a set (the reference grows it by
  *                        repeated H*psi, Core/BlockLanczos.cpp:5849-5858); restated
  *                        as a depth-first walk over modes in ascending numeric order.
@@ -469,4 +473,66 @@ int or_max_threads(void) {
 #else
   return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* broadening (Core/Spectral.cpp:958-1131)                                   */
+/* ------------------------------------------------------------------------ */
+/* One spectrum of N+1 points (re, im in place), step dE. kind 0 = Gaussian,
+ * 1 = Lorentzian. per_point = 0: one FWHM f[0]; nothing happens when
+ * 2 f[0] < dE (BroadenG :958, BroadenL :1003). per_point = 1: FWHM f[i] of
+ * source point i, raised to at least dE/2 (BroadenGvar :1046, BroadenLvar
+ * :1093). Output point Eout sums over source points Ein in ascending order;
+ * each weight averages the line shape over Eout - Ein + j/7, j = -3..3. */
+void or_broaden(int N, double dE, int kind, int per_point, const double *f, double *re, double *im) {
+  if (!per_point && 2 * f[0] < dE) return;
+  double *outr = (double *)calloc((size_t)N + 1, sizeof(double));
+  double *outi = (double *)calloc((size_t)N + 1, sizeof(double));
+#pragma omp parallel for
+  for (int eo = 0; eo <= N; ++eo) {
+    for (int ei = 0; ei <= N; ++ei) {
+      const double fw = per_point ? (f[ei] > dE / 2.0 ? f[ei] : dE / 2.0) : f[0];
+      double w = 0;
+      if (kind == 0) {
+        const double sig = fw / (2.0 * sqrt(2.0 * log(2.0)));
+        const double sa = 2.0 * sig * sig;
+        const double sb = 1 / (sig * sqrt(2.0 * M_PI) / dE);
+        for (int j = -3; j <= 3; ++j) {
+          const double x = (eo - ei + j / 7.0) * dE;
+          w += exp(-(x * x) / sa);
+        }
+        w *= sb / 7.0;
+      } else {
+        for (int j = -3; j <= 3; ++j) {
+          const double x = dE * (eo - ei + j / 7.0);
+          w += 1 / (fw * fw + 4 * (x * x));
+        }
+        w *= dE * 2 * fw / (7.0 * M_PI);
+      }
+      outr[eo] += re[ei] * w;
+      outi[eo] += im[ei] * w;
+    }
+  }
+  memcpy(re, outr, ((size_t)N + 1) * sizeof(double));
+  memcpy(im, outi, ((size_t)N + 1) * sizeof(double));
+  free(outr);
+  free(outi);
+}
+
+/* FWHM at the grid points emin + j dE, j = 0..N, from a table of nt
+ * (energy, FWHM) pairs with ascending energies: linear between the two
+ * entries around the point, the first (last) FWHM before (after) the table
+ * (LuaInterface/Spectra.cpp:44-62). */
+void or_fwhm_table(int nt, const double *E, const double *F, double emin, double dE, int N, double *out) {
+  int k = 0;
+  for (int j = 0; j <= N; ++j) {
+    const double x = emin + j * dE;
+    while (k < nt && x > E[k]) ++k;
+    if (k == 0)
+      out[j] = F[0];
+    else if (k == nt)
+      out[j] = F[nt - 1];
+    else
+      out[j] = F[k - 1] + (F[k] - F[k - 1]) * (x - E[k - 1]) / (E[k] - E[k - 1]);
+  }
 }

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libqs.so")
 
 QS_OK = 0
 QS_HOP_MATFREE, QS_HOP_ASSEMBLED, QS_HOP_AUTO = 0, 1, 2
+QS_ENOTCONVERGED = 5
 QS_PARAM_SHIFT_MIN, QS_PARAM_ELL_PACK, QS_PARAM_BATCH_GRAPH, QS_PARAM_TERM_LDS = 1, 2, 3, 4
 QS_BATCH_KEEP_BASIS = 1
 STATUS = {0: "QS_OK", 1: "QS_ENOMEM_DEVICE", 2: "QS_ENOMEM_HOST", 3: "QS_EINVAL",
@@ -74,6 +75,10 @@ _SIGS = [
                                       _P, _P, _P]),
     ("qs_cf_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _D, _D, C.c_int, _D, _P, _P]),
     ("qs_block_cf_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _D, _D, _D, C.c_int, _D, _P]),
+    ("qs_broaden", C.c_int, [_P, C.c_int, C.c_int, _D, C.c_int, C.c_int, _P, _P, _P]),
+    ("qs_hop_dense", C.c_int, [_P, _P, _P]),
+    ("qs_dense_poles", C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P, _P]),
+    ("qs_pole_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.c_int, _P, _D, _D, C.c_int, _D, _P]),
     ("qs_heev_host", C.c_int, [C.c_int, _P, _P, _P]),
     ("qs_tridiag_lowest_host", C.c_int, [C.c_int, _P, _P, C.POINTER(_D), _P]),
     ("qs_tridiag_lowest_from", C.c_int, [C.c_int, _P, _P, _D, _D, C.POINTER(_D), _P]),

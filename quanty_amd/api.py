@@ -25,7 +25,8 @@ import torch
 from . import det as D
 from . import operators as O
 from ._lib import QSError, default_context
-from .engine import Basis, Hop, Op, block_cf_spectrum, cf_spectrum, tridiag_extremes
+from .engine import (Basis, Hop, Op, block_cf_spectrum, broaden, cf_spectrum, dense_poles, pole_spectrum,
+                     tridiag_extremes)
 
 SQRT_EPS = math.sqrt(np.finfo(float).eps)
 
@@ -490,6 +491,22 @@ class TriResponse(dict):
         return complex(g[0, 0])
 
 
+class PolesResponse(dict):
+    """A response function as a list of poles: {a: [a0, E_1, ..., E_N], b: [w_1, ..., w_N],
+    mu, type="ListOfPoles", name} (LuaPushTable_ListOfPoles,
+    LuaInterface/lua_extensions/TableToArray.cpp:3640-3669), callable as
+    G(omega, gamma) = a0 + sum_k w_k / (omega - E_k + i gamma/2)."""
+
+    def __init__(self, a, b, mu=0.0, name=""):
+        super().__init__(a=np.asarray(a, dtype=float), b=np.asarray(b, dtype=float), mu=mu, type="ListOfPoles",
+                         name=name)
+
+    def __call__(self, omega, gamma):
+        P = np.sqrt(self["b"]).astype(np.complex128)[:, None]
+        g = pole_spectrum(self["a"][1:], P, 0.0, float(omega), float(omega) + 1.0, 1, float(gamma))
+        return complex(g[0, 0]) + float(self["a"][0])
+
+
 class Spectra:
     """Spectra on the grid w_i = Emin + i (Emax-Emin)/NE, i = 0..NE (SpectraTypeArray)."""
 
@@ -505,6 +522,49 @@ class Spectra:
     def Sum(self, weights):
         return Spectra(self.Emin, self.Emax, self.NE, self.Gamma,
                        [np.tensordot(np.asarray(weights), self.G.reshape(len(weights), -1), 1)])
+
+    def Copy(self):
+        return Spectra(self.Emin, self.Emax, self.NE, self.Gamma, self.G.copy(), self.EminPole, self.EmaxPole)
+
+    def _widths(self, w, which):
+        """A FWHM as one number, or per grid point from a table {{E_1, FWHM_1}, ...}
+        (ascending energies): linear between the two entries around a point,
+        the first (last) FWHM before (after) the table (LuaInterface/Spectra.cpp:37-62)."""
+        if isinstance(w, (int, float, np.floating, np.integer)):
+            return float(w)
+        t = np.asarray(w, dtype=object)
+        if t.ndim != 2 or t.shape[1] != 2 or any(isinstance(x, complex) for x in t.ravel()):
+            raise QSError(f"Spectra.Broaden: the {which} width should be a number or a table "
+                          "{{Energy_1, FWHM_1}, ...} of real numbers")
+        t = t.astype(float)
+        E, F = t[:, 0], t[:, 1]
+        if np.any(np.diff(E) < 0):
+            raise QSError(f"Spectra.Broaden: the energies of the {which} width table should ascend")
+        x = self.Emin + np.arange(self.NE + 1) * ((self.Emax - self.Emin) / self.NE)
+        k = np.searchsorted(E, x, side="left")  # table entries strictly below x
+        lo = np.clip(k - 1, 0, len(E) - 1)
+        hi = np.clip(k, 0, len(E) - 1)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            mid = F[lo] + (F[hi] - F[lo]) * (x - E[lo]) / (E[hi] - E[lo])
+        return np.where(k == 0, F[0], np.where(k == len(E), F[-1], mid))
+
+    def Broaden(self, gauss, lorentz):
+        """Gaussian, then Lorentzian broadening of every spectrum, in place
+        (the method form spectra.Broaden(G, L), LuaInterface/Spectra.cpp:118-215;
+        the function form Spectra.Broaden(s, G, L) broadens a copy: use
+        s.Copy().Broaden(G, L)). Each width is a FWHM or a table (see _widths);
+        a single width narrower than half the grid step leaves the spectra as
+        they are, table widths are raised to half the step
+        (Core/Spectral.cpp:958-1131). Computed on the GPU."""
+        dE = (self.Emax - self.Emin) / self.NE
+        g = self._widths(gauss, "Gaussian")
+        l_ = self._widths(lorentz, "Lorentzian")
+        shape = self.G.shape
+        G = self.G.reshape(-1, self.NE + 1)
+        G = broaden(G, dE, "G", g)
+        G = broaden(G, dE, "L", l_)
+        self.G = G.reshape(shape)
+        return self
 
     def Print(self, file=None):
         """Energy column then Re/Im of every spectrum, 17 significant digits."""
@@ -527,13 +587,78 @@ def _pole_range(a, b, E0):
     return lo - E0, hi - E0
 
 
+def _default_range(o, eminpole, emaxpole):
+    """Emin/Emax from the pole range when not given (LuaInterface/ssp.cpp:2405-2430)."""
+    emin, emax = o["Emin"], o["Emax"]
+    if emin is None or emax is None:
+        span = emaxpole - eminpole
+        if span == 0:
+            dmin, dmax = (-1.0, 1.0) if eminpole == 0 else ((2.125 * eminpole, -0.125 * eminpole) if eminpole < 0
+                                                             else (-0.125 * eminpole, 2.125 * eminpole))
+        else:
+            dmin, dmax = eminpole - 0.125 * span, emaxpole + 0.125 * span
+        emin = dmin if emin is None else emin
+        emax = dmax if emax is None else emax
+    return float(emin), float(emax)
+
+
+def _dense_spectra(H, cb, tpsi, psiEn, o, ctx):
+    """The dense method on the closure basis cb of T psi (LuaInterface/ssp.cpp:2344-2584):
+    the determinant matrix of H, its eigen-decomposition, the pole weights of every
+    T psi, and the spectra from the poles. Returns (Spectra, tables[, eigenstates])."""
+    n, nsp, nf = cb.n, len(tpsi), H.NF
+    Xh = np.zeros((nsp, n), dtype=np.complex128)
+    for i, tp in enumerate(tpsi):
+        w, c = tp.host()
+        if len(w):
+            idx = cb.index(w)
+            if np.any(idx < 0):
+                raise QSError("T psi has determinants outside the closure of its basis")
+            Xh[i, idx] = c
+    dev = torch.device("cuda", ctx.device)
+    X = torch.from_numpy(Xh).to(dev)
+    hop = Hop(H.device(ctx), cb, mode="matfree")
+    Hd = hop.dense()
+    E, P = dense_poles(Hd, X, ctx)
+    ne = int(o["NE"])
+    gamma = float(o["Gamma"])
+    if not o["Tensor"]:
+        eminpole = min(E[0] - e for e in psiEn)
+        emaxpole = max(E[-1] - e for e in psiEn)
+        emin, emax = _default_range(o, eminpole, emaxpole)
+        if gamma == 0:
+            gamma = 10 * (emax - emin) / ne
+        G = pole_spectrum(E, P, np.array(psiEn), emin, emax, ne, gamma, ctx=ctx)
+        tables = [PolesResponse(np.r_[0.0, E - psiEn[i]], P[:, i].real ** 2 + P[:, i].imag ** 2, 0.0,
+                                f"Spectrum {i}") for i in range(nsp)]
+        out = [Spectra(emin, emax, ne, gamma, G, eminpole, emaxpole), tables]
+    else:
+        E0 = psiEn[0]
+        eminpole, emaxpole = E[0] - E0, E[-1] - E0
+        emin, emax = _default_range(o, eminpole, emaxpole)
+        if gamma == 0:
+            gamma = 10 * (emax - emin) / ne
+        G = pole_spectrum(E, P, E0, emin, emax, ne, gamma, tensor=True, ctx=ctx)
+        tables = {"type": "BlockListOfPoles", "E": E - E0, "blocks": np.conj(P)[:, :, None] * P[:, None, :]}
+        out = [Spectra(emin, emax, ne, gamma, G.reshape(ne + 1, -1).T, eminpole, emaxpole), tables]
+    if o["KrylovBasis"]:
+        out.append([Wavefunction(nf, basis=cb, vec=Hd[:, k].contiguous()) for k in range(n)])
+    return tuple(out)
+
+
 def CreateSpectra(H: Operator, T, psi, options=None):
     """Spectra of transition operators T on wavefunctions psi (LuaCreateSpectra,
     LuaInterface/ssp.cpp:1906): E0 = <psi|H|psi> unless given, T psi (restricted
-    when Restrictions are given), NTri-step Lanczos per (T, psi) — or one block
-    Lanczos over all of them with Tensor = true — and the continued fraction on
-    NE+1 points with Lorentzian FWHM Gamma. Spectra of the dense method are
-    computed by Lanczos here (results agree; the Krylov space is exhausted)."""
+    when Restrictions are given), then one of two methods (option Method):
+      - dense ("Dense", or "Auto" when the closure of the T psi under H has at
+        most DenseBorder determinants, ssp.cpp:2259-2316): the determinant
+        matrix of H on that closure, its eigen-decomposition, and the spectra
+        from the poles; the tables are lists of poles (KrylovBasis = true adds
+        the eigenstates);
+      - Lanczos ("Lanczos", or "Auto" above DenseBorder): NTri-step Lanczos per
+        (T, psi) — or one block Lanczos over all of them with Tensor = true —
+        and the continued fraction; the tables are tridiagonal.
+    Spectra on NE+1 points with Lorentzian FWHM Gamma."""
     o = _options(options, {"Tensor": False, "KrylovBasis": False, "NTri": 200, "epsilon": SQRT_EPS / 100,
                            "Epsilon": SQRT_EPS / 100, "SingularValue": SQRT_EPS, "restrictions": None,
                            "Restrictions": None, "Emin": None, "Emax": None, "NE": 1000, "Gamma": 0.0,
@@ -546,11 +671,28 @@ def CreateSpectra(H: Operator, T, psi, options=None):
     nf = H.NF
     R = _restrictions(o["Restrictions"] if o["Restrictions"] is not None else o["restrictions"], nf)
     ntri = int(o["NTri"])
+    method = str(o["Method"] or "Auto").lower()
+    if method not in ("auto", "dense", "lanczos"):
+        raise QSError(f'CreateSpectra: unknown Method "{o["Method"]}", possible are "Auto", "Dense" and "Lanczos"')
+    if method == "lanczos" and o["ForceDenseMethod"]:
+        raise QSError('CreateSpectra: Method "Lanczos" and ForceDenseMethod true contradict each other')
+    forced = bool(o["ForceDenseMethod"]) or method == "dense"
+    border = 0 if method == "lanczos" else int(o["DenseBorder"])
     E0s = []
     for p in psis:
         E0s.append(float(o["E0"]) if o["E0"] is not None else _expect(H, p).real)
     # T psi for every (T, psi), T outer, psi inner
     tpsi = [operator_psi(t, p, restrictions=[nf, 0, *R] if R else None) for t in Ts for p in psis]
+    psiEn = [E0s[i % len(psis)] for i in range(len(tpsi))]
+    if forced or border > 0:
+        seeds = [tp.host()[0] for tp in tpsi]
+        seeds = np.unique(np.concatenate(seeds), axis=0) if any(len(w) for w in seeds) else np.zeros((0, 1))
+        if len(seeds) and (forced or len(seeds) <= border):
+            seed = Basis.from_words(nf, seeds, ctx)
+            cb = Basis.closure(H.device(ctx), seed, [(m, lo, hi) for m, lo, hi in R] if R else None,
+                               max_dim=(1 << 62) if forced else border)
+            if cb is not None:
+                return _dense_spectra(H, cb, tpsi, psiEn, o, ctx)
     # the space H explores from T psi: fix the occupation of every group of modes H conserves
     groups = conserved_groups(H)
     cv = set()
@@ -571,24 +713,15 @@ def CreateSpectra(H: Operator, T, psi, options=None):
         tpsi[i] = Wavefunction(nf, basis=rows, vec=torch.from_numpy(v).to(torch.device("cuda", ctx.device)))
     hop = Hop(H.device(ctx), rows, mode="auto")
     nsp = len(tpsi)
-    psiEn = [E0s[i % len(psis)] for i in range(nsp)]
     if not o["Tensor"]:
         A = np.zeros((nsp, ntri + 1))
         Bv = np.zeros((nsp, ntri))
         for i, tp in enumerate(tpsi):
             A[i], Bv[i] = hop.lanczos(tp._vec.contiguous(), ntri)
-        emin, emax, gamma, ne = o["Emin"], o["Emax"], float(o["Gamma"]), int(o["NE"])
+        gamma, ne = float(o["Gamma"]), int(o["NE"])
         lo, hi = zip(*[_pole_range(A[i], Bv[i], psiEn[i]) for i in range(nsp)])
         eminpole, emaxpole = min(lo), max(hi)
-        if emin is None or emax is None:
-            span = emaxpole - eminpole
-            if span == 0:
-                dmin, dmax = (-1.0, 1.0) if eminpole == 0 else ((2.125 * eminpole, -0.125 * eminpole) if eminpole < 0
-                                                                 else (-0.125 * eminpole, 2.125 * eminpole))
-            else:
-                dmin, dmax = eminpole - 0.125 * span, emaxpole + 0.125 * span
-            emin = dmin if emin is None else emin
-            emax = dmax if emax is None else emax
+        emin, emax = _default_range(o, eminpole, emaxpole)
         if gamma == 0:
             gamma = 10 * (emax - emin) / ne
         G = cf_spectrum(A, Bv, np.array(psiEn), float(emin), float(emax), ne, gamma, ctx)

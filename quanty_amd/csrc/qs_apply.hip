@@ -566,6 +566,66 @@ static qs_status launch_mf(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, c
   return QS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// dense matrix of a binding (OperatorToDeterminantMatrix, Core/MatrixOperator.cpp:1149-1226):
+// M[c * nrows + r] = <r|O|c>, column-major. One lane per row, so the terms of
+// a row add into its entries in term order without atomics.
+// ---------------------------------------------------------------------------
+__global__ void k_zero2(int64_t n, double2 *__restrict__ m) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) m[i] = make_double2(0.0, 0.0);
+}
+
+template <int NW>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_dense(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms, int nterms,
+        Lookup L, int same, double2 *__restrict__ M) {
+  __shared__ TermD<NW> sh[QS_TILE];
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = row < nrows;
+  uint64_t d[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) d[w] = active ? rows[row * NW + w] : 0;
+  for (int base = 0; base < nterms; base += QS_TILE) {
+    const int cnt = min(QS_TILE, nterms - base);
+    __syncthreads();
+    stage_terms<NW>(sh, terms, base, cnt);
+    __syncthreads();
+    if (active) {
+      visit_tile<NW>(d, row, sh, cnt, base, same != 0, L, [&](int tj, int64_t col, uint32_t par, bool) {
+        const TermD<NW> &t = sh[tj - base];
+        double2 &m = M[col * nrows + row];
+        m.x += par ? -t.re : t.re;
+        m.y += par ? -t.im : t.im;
+      });
+    }
+  }
+}
+
+extern "C" qs_status qs_hop_dense(qs_ctx *ctx, const qs_hop *h, double *M) {
+  if (!ctx || !h) return QS_EINVAL;
+  const int64_t nr = h->rows->n, nc = h->cols->n;
+  if (nr == 0 || nc == 0) return QS_OK;
+  if (!M) return QS_EINVAL;
+  if (nr > ((int64_t)1 << 31) / (nc > 0 ? nc : 1))
+    return qs_fail(ctx, QS_EINVAL, "qs_hop_dense: a %lld x %lld matrix is too large", (long long)nr, (long long)nc);
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t nel = nr * nc;
+  hipLaunchKernelGGL(k_zero2, dim3((unsigned)((nel + 255) / 256)), dim3(256), 0, ctx->stream, nel, (double2 *)M);
+  QS_HIP(ctx, hipGetLastError());
+  Lookup L = make_lookup(h->cols);
+  const unsigned blocks = (unsigned)((nr + QS_BLOCK - 1) / QS_BLOCK);
+  if (h->op->nw == 1)
+    hipLaunchKernelGGL(k_dense<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, h->rows->d_words, nr,
+                       (const TermD<1> *)h->op->d_terms, h->op->nterms, L, (int)h->same, (double2 *)M);
+  else
+    hipLaunchKernelGGL(k_dense<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, h->rows->d_words, nr,
+                       (const TermD<2> *)h->op->d_terms, h->op->nterms, L, (int)h->same, (double2 *)M);
+  QS_HIP(ctx, hipGetLastError());
+  if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
 qs_status qs_launch_mf_ext(qs_ctx *ctx, const qs_hop *h, const double *x, double *y) {
   return launch_mf(ctx, h->op, h->rows, h->cols, h->same, x, y);
 }

@@ -869,8 +869,4 @@ extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates,
                              variance_out, iterations_out);
 }
 
-extern "C" qs_status qs_basis_closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,
-                                      const uint8_t *masks, const int *lo, const int *hi, int64_t max_dim,
-                                      qs_basis **out) {
-  return qs_fail(ctx, QS_EUNSUPPORTED, "qs_basis_closure: not built yet");
-}
+

@@ -212,6 +212,7 @@ extern "C" qs_status qs_ctx_destroy(qs_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->scratch) qs_free(ctx->scratch);
+  qs_blas_release(ctx);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return QS_OK;
@@ -760,6 +761,27 @@ extern "C" qs_status qs_basis_from_words(qs_ctx *ctx, int nf, const uint64_t *wo
   if (e != hipSuccess) {
     qs_basis_destroy(b);
     return qs_hip_fail(ctx, e, "qs_basis_from_words", __FILE__, __LINE__);
+  }
+  *out = b;
+  return QS_OK;
+}
+
+qs_status qs_basis_from_device_words(qs_ctx *ctx, int nf, const uint64_t *d_words, int64_t n, qs_basis **out) {
+  qs_basis *b = new qs_basis();
+  b->nf = nf;
+  b->nw = nf <= 64 ? 1 : 2;
+  b->n = n;
+  qs_status st = basis_alloc_words(ctx, b);
+  if (st != QS_OK) {
+    delete b;
+    return st;
+  }
+  hipError_t e = n ? hipMemcpyAsync(b->d_words, d_words, (size_t)n * b->nw * 8, hipMemcpyDeviceToDevice, ctx->stream)
+                   : hipSuccess;
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    qs_basis_destroy(b);
+    return qs_hip_fail(ctx, e, "qs_basis_from_device_words", __FILE__, __LINE__);
   }
   *out = b;
   return QS_OK;

@@ -36,7 +36,14 @@ struct qs_ctx {
   int ell_pack = 1;                // QS_PARAM_ELL_PACK
   int batch_graph = 1;             // QS_PARAM_BATCH_GRAPH
   int term_lds = QS_TERM_LDS_DEFAULT;  // QS_PARAM_TERM_LDS
+  void *blas = nullptr;            // rocblas handle for the dense eigensolver, made on first use
 };
+
+// A basis holding n sorted, unique determinants copied from device memory.
+qs_status qs_basis_from_device_words(qs_ctx *ctx, int nf, const uint64_t *d_words, int64_t n, qs_basis **out);
+
+// Releases ctx->blas (qs_spectra.hip).
+void qs_blas_release(qs_ctx *ctx);
 
 // Exact rank of a determinant among all determinants satisfying the
 // restrictions, in ascending numeric order. The modes are cut into chunks of 8

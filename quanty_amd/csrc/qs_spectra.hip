@@ -1,0 +1,345 @@
+// Spectra on the GPU outside the Lanczos recurrence:
+//  - the dense small-problem path: eigen-decomposition of a dense Hermitian
+//    matrix (rocSOLVER zheevd, the counterpart of the reference's LAPACK
+//    zheev), projections of the transition vectors on the eigenvectors, and
+//    spectra from the resulting lists of poles (ListOfPolesToSpectrum,
+//    Core/Spectral.cpp:1516-1591; BlockListOfPolesToSpectraWithBroadening,
+//    Core/BasicMathBlock/BasicMath_BlockTypeConversion.cpp:588-660);
+//  - Gaussian and Lorentzian broadening of spectra on an equidistant energy
+//    grid (BroadenG / BroadenL / BroadenGvar / BroadenLvar,
+//    Core/Spectral.cpp:958-1131).
+//
+// Broadening:
+// out(o) = sum_i in(i) * K(o, i): the kernel K is the source point's line
+// shape averaged over 7 sub-points (o - i + j/7, j = -3..3) of the grid cell.
+// Grid: one thread per output point o, the source points i split into
+// BR_CHUNKS contiguous chunks (blockIdx.y), partial sums per chunk reduced in
+// chunk order by k_broaden_sum, so results do not depend on the launch.
+// When every source point has the same width, K depends on o - i only and is
+// tabulated once (2 npts - 1 values, same arithmetic as the direct form).
+#include "qs_internal.h"
+#include "qs_kernels.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+#include <vector>
+
+namespace {
+
+constexpr int BR_TILE = 8;     // spectra per thread pass
+constexpr int BR_CHUNKS = 16;  // source-point chunks
+
+// Line-shape weight of source point i seen at o - i = d: Gaussian (KIND 0,
+// q0 = 2 sigma^2, q1 = dE / (sigma sqrt(2 pi)) / 7) or Lorentzian (KIND 1,
+// q0 = FWHM, q1 = 2 dE FWHM / (7 pi)).
+template <int KIND>
+__device__ __forceinline__ double line_weight(int d, double de, double q0, double q1) {
+  double w = 0.0;
+#pragma unroll
+  for (int j = -3; j <= 3; ++j) {
+    const double x = ((double)d + (double)j / 7.0) * de;
+    if (KIND == 0)
+      w += exp(-(x * x) / q0);
+    else
+      w += 1.0 / (q0 * q0 + 4.0 * (x * x));
+  }
+  return w * q1;
+}
+
+template <int KIND>
+__global__ void k_broaden_table(int npts, double de, double q0, double q1, double *__restrict__ tab) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * npts - 1) return;
+  tab[t] = line_weight<KIND>(t - (npts - 1), de, q0, q1);
+}
+
+// TABLE: weights from tab[o - i + npts - 1]; else from per-source (q0, q1) in p.
+template <int KIND, bool TABLE>
+__global__ __launch_bounds__(64) void k_broaden(int nspec, int npts, int chunk, double de,
+                                                const double *__restrict__ p, const double *__restrict__ tab,
+                                                const double *__restrict__ re, const double *__restrict__ im,
+                                                double *__restrict__ part) {
+  const int o = blockIdx.x * 64 + threadIdx.x;
+  const int c = blockIdx.y;
+  const int s0 = blockIdx.z * BR_TILE;
+  const int ns = min(BR_TILE, nspec - s0);
+  const int i0 = c * chunk, i1 = min(npts, i0 + chunk);
+  if (o >= npts) return;
+  double ar[BR_TILE], ai[BR_TILE];
+#pragma unroll
+  for (int s = 0; s < BR_TILE; ++s) ar[s] = ai[s] = 0.0;
+  for (int i = i0; i < i1; ++i) {
+    const double w = TABLE ? tab[o - i + npts - 1] : line_weight<KIND>(o - i, de, p[2 * i], p[2 * i + 1]);
+#pragma unroll
+    for (int s = 0; s < BR_TILE; ++s) {
+      if (s < ns) {
+        ar[s] += re[(size_t)(s0 + s) * npts + i] * w;
+        ai[s] += im[(size_t)(s0 + s) * npts + i] * w;
+      }
+    }
+  }
+  const size_t plane = (size_t)nspec * npts;
+#pragma unroll
+  for (int s = 0; s < BR_TILE; ++s) {
+    if (s < ns) {
+      const size_t at = (size_t)c * 2 * plane + (size_t)(s0 + s) * npts + o;
+      part[at] = ar[s];
+      part[at + plane] = ai[s];
+    }
+  }
+}
+
+__global__ void k_broaden_sum(int nspec, int npts, int nchunks, const double *__restrict__ part,
+                              double *__restrict__ re, double *__restrict__ im) {
+  const size_t plane = (size_t)nspec * npts;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= plane) return;
+  double sr = 0.0, si = 0.0;
+  for (int c = 0; c < nchunks; ++c) {
+    sr += part[(size_t)c * 2 * plane + t];
+    si += part[(size_t)c * 2 * plane + plane + t];
+  }
+  re[t] = sr;
+  im[t] = si;
+}
+
+// (q0, q1) of a Gaussian or Lorentzian of full width f on a grid of step de.
+void line_params(int kind, double f, double de, double *q0, double *q1) {
+  if (kind == 0) {
+    const double sigma = f / (2.0 * std::sqrt(2.0 * std::log(2.0)));
+    *q0 = 2.0 * sigma * sigma;
+    *q1 = (1.0 / (sigma * std::sqrt(2.0 * M_PI) / de)) / 7.0;
+  } else {
+    *q0 = f;
+    *q1 = de * 2.0 * f / (7.0 * M_PI);
+  }
+}
+
+}  // namespace
+
+extern "C" qs_status qs_broaden(qs_ctx *ctx, int nspec, int npts, double dE, int kind, int per_point,
+                                const double *fwhm, double *re, double *im) {
+  if (!ctx || nspec < 1 || npts < 1 || !(dE > 0.0) || (kind != 0 && kind != 1) || !fwhm || !re || !im)
+    return QS_EINVAL;
+  if (npts > (1 << 24)) return QS_EINVAL;
+  for (int i = 0; i < (per_point ? npts : 1); ++i)
+    if (std::isnan(fwhm[i])) return QS_EINVAL;
+  if (!per_point && 2.0 * fwhm[0] < dE) return QS_OK;  // narrower than the grid: left as it is
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  const int nchunks = std::min(BR_CHUNKS, npts);
+  const int chunk = (npts + nchunks - 1) / nchunks;
+  const size_t plane = (size_t)nspec * npts;
+  const size_t np = per_point ? 2 * (size_t)npts : 2 * (size_t)npts - 1;
+  char *mem = nullptr;
+  QS_TRY(qs_malloc(ctx, (void **)&mem, (2 * plane + 2 * (size_t)nchunks * plane + np) * 8));
+  double *dre = (double *)mem, *dim = dre + plane, *part = dim + plane, *dp = part + 2 * (size_t)nchunks * plane;
+  std::vector<double> hp;
+  double q0 = 0.0, q1 = 0.0;
+  if (per_point) {
+    hp.resize(2 * (size_t)npts);
+    for (int i = 0; i < npts; ++i) line_params(kind, std::max(fwhm[i], dE / 2.0), dE, &hp[2 * i], &hp[2 * i + 1]);
+  } else {
+    line_params(kind, fwhm[0], dE, &q0, &q1);
+  }
+  hipError_t e = hipMemcpyAsync(dre, re, plane * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dim, im, plane * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && per_point) e = hipMemcpyAsync(dp, hp.data(), np * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess && !per_point) {
+    const unsigned g = (unsigned)((np + 255) / 256);
+    if (kind == 0)
+      hipLaunchKernelGGL(k_broaden_table<0>, dim3(g), dim3(256), 0, ctx->stream, npts, dE, q0, q1, dp);
+    else
+      hipLaunchKernelGGL(k_broaden_table<1>, dim3(g), dim3(256), 0, ctx->stream, npts, dE, q0, q1, dp);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    const dim3 grid((unsigned)((npts + 63) / 64), (unsigned)nchunks, (unsigned)((nspec + BR_TILE - 1) / BR_TILE));
+    const double *pp = per_point ? dp : nullptr, *tab = per_point ? nullptr : dp;
+    if (kind == 0 && per_point)
+      hipLaunchKernelGGL((k_broaden<0, false>), grid, dim3(64), 0, ctx->stream, nspec, npts, chunk, dE, pp, tab, dre,
+                         dim, part);
+    else if (kind == 0)
+      hipLaunchKernelGGL((k_broaden<0, true>), grid, dim3(64), 0, ctx->stream, nspec, npts, chunk, dE, pp, tab, dre,
+                         dim, part);
+    else if (per_point)
+      hipLaunchKernelGGL((k_broaden<1, false>), grid, dim3(64), 0, ctx->stream, nspec, npts, chunk, dE, pp, tab, dre,
+                         dim, part);
+    else
+      hipLaunchKernelGGL((k_broaden<1, true>), grid, dim3(64), 0, ctx->stream, nspec, npts, chunk, dE, pp, tab, dre,
+                         dim, part);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_broaden_sum, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, ctx->stream, nspec, npts,
+                       nchunks, part, dre, dim);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(re, dre, plane * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(im, dim, plane * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  qs_free(mem);
+  if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_broaden", __FILE__, __LINE__);
+  return QS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// dense small-problem path
+// ---------------------------------------------------------------------------
+void qs_blas_release(qs_ctx *ctx) {
+  if (ctx && ctx->blas) {
+    (void)rocblas_destroy_handle((rocblas_handle)ctx->blas);
+    ctx->blas = nullptr;
+  }
+}
+
+static qs_status blas_handle(qs_ctx *ctx, rocblas_handle *hb) {
+  if (!ctx->blas) {
+    rocblas_handle b = nullptr;
+    if (rocblas_create_handle(&b) != rocblas_status_success)
+      return qs_fail(ctx, QS_EHIP, "rocblas_create_handle failed");
+    ctx->blas = b;
+  }
+  *hb = (rocblas_handle)ctx->blas;
+  if (rocblas_set_stream(*hb, ctx->stream) != rocblas_status_success)
+    return qs_fail(ctx, QS_EHIP, "rocblas_set_stream failed");
+  return QS_OK;
+}
+
+namespace {
+
+// P[k * nv + s] = <z_k|x_s> = sum_r conj(Z[r, k]) X[r, s]; one wave per (k, s),
+// lanes over r with a fixed stride and a fixed butterfly.
+__global__ __launch_bounds__(64) void k_project(int n, const double2 *__restrict__ Z, const double2 *__restrict__ X,
+                                                int nv, double2 *__restrict__ P) {
+  const int k = blockIdx.x, sv = blockIdx.y;
+  const double2 *z = Z + (size_t)k * n;
+  const double2 *x = X + (size_t)sv * n;
+  double re = 0.0, im = 0.0;
+  for (int r = threadIdx.x; r < n; r += 64) {
+    const double2 a = z[r], b = x[r];
+    re += a.x * b.x + a.y * b.y;
+    im += a.x * b.y - a.y * b.x;
+  }
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (threadIdx.x == 0) P[(size_t)k * nv + sv] = make_double2(re, im);
+}
+
+// G_s(w_i) = sum_k |P[k][s]|^2 / (w_i + E0_s - E_k + i Gamma/2), w_i = i de + emin.
+__global__ void k_poles(int npole, int nv, const double *__restrict__ E, const double2 *__restrict__ P,
+                        const double *__restrict__ E0, double emin, double de, int ne, double gamma,
+                        double2 *__restrict__ out) {
+  const int64_t np = (int64_t)ne + 1;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= np * nv) return;
+  const int sv = (int)(idx / np);
+  const int i = (int)(idx % np);
+  const double om = (double)i * de + emin + E0[sv];
+  const double g2 = gamma / 2.0;
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < npole; ++k) {
+    const double2 p = P[(size_t)k * nv + sv];
+    const double b = p.x * p.x + p.y * p.y;
+    const double w = om - E[k];
+    const double den = g2 * g2 + w * w;
+    re += b * w / den;
+    im += b * (-g2) / den;
+  }
+  out[idx] = make_double2(re, im);
+}
+
+// G[i][r][c] = sum_k conj(P[k][r]) P[k][c] / (w_i - (E_k - E0) + i Gamma/2),
+// w_i = (i / ne) (emax - emin) + emin.
+__global__ void k_poles_tensor(int npole, int nv, const double *__restrict__ E, const double2 *__restrict__ P,
+                               double E0, double emin, double emax, int ne, double gamma,
+                               double2 *__restrict__ out) {
+  const int64_t nb = (int64_t)nv * nv;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ((int64_t)ne + 1) * nb) return;
+  const int i = (int)(idx / nb);
+  const int r = (int)((idx % nb) / nv);
+  const int c = (int)(idx % nv);
+  const double om = ((double)i / (double)ne) * (emax - emin) + emin;
+  const double g2 = gamma / 2.0;
+  double re = 0.0, im = 0.0;
+  for (int k = 0; k < npole; ++k) {
+    const double a = E[k] - E0;
+    const double den = g2 * g2 + (om - a) * (om - a);
+    const double lr = (om - a) / den, li = (-g2) / den;
+    const double2 pc = P[(size_t)k * nv + c], pr = P[(size_t)k * nv + r];
+    const double mr = pc.x * pr.x + pc.y * pr.y;
+    const double mi = -pc.x * pr.y + pc.y * pr.x;
+    re += mr * lr - mi * li;
+    im += mr * li + mi * lr;
+  }
+  out[idx] = make_double2(re, im);
+}
+
+}  // namespace
+
+extern "C" qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const double *X, double *E, double *P) {
+  if (!ctx || n < 1 || !A || nv < 0 || !E || (nv > 0 && (!X || !P))) return QS_EINVAL;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  rocblas_handle hb = nullptr;
+  QS_TRY(blas_handle(ctx, &hb));
+  char *mem = nullptr;
+  const size_t pbytes = (size_t)n * nv * 16;
+  QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)n * 16 + 256 + pbytes));
+  double *dw = (double *)mem, *de = dw + n;
+  rocblas_int *info = (rocblas_int *)(de + n);
+  double2 *dP = (double2 *)((char *)info + 256);
+  qs_status st = QS_OK;
+  const rocblas_status rs = rocsolver_zheevd(hb, rocblas_evect_original, rocblas_fill_lower, n,
+                                             (rocblas_double_complex *)A, n, dw, de, info);
+  if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zheevd failed (status %d)", (int)rs);
+  hipError_t e = hipSuccess;
+  if (st == QS_OK && nv > 0) {
+    hipLaunchKernelGGL(k_project, dim3((unsigned)n, (unsigned)nv), dim3(64), 0, ctx->stream, n, (const double2 *)A,
+                       (const double2 *)X, nv, dP);
+    e = hipGetLastError();
+  }
+  rocblas_int hinfo = 0;
+  if (st == QS_OK && e == hipSuccess) e = hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, ctx->stream);
+  if (st == QS_OK && e == hipSuccess) e = hipMemcpyAsync(E, dw, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (st == QS_OK && e == hipSuccess && nv > 0) e = hipMemcpyAsync(P, dP, pbytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (st == QS_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  else (void)hipStreamSynchronize(ctx->stream);
+  qs_free(mem);
+  if (st != QS_OK) return st;
+  if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_dense_poles", __FILE__, __LINE__);
+  if (hinfo != 0) return qs_fail(ctx, QS_ENOTCONVERGED, "zheevd did not converge (info %d)", (int)hinfo);
+  return QS_OK;
+}
+
+extern "C" qs_status qs_pole_spectrum(qs_ctx *ctx, int npole, int nv, const double *E, const double *P, int tensor,
+                                      const double *E0, double emin, double emax, int ne, double gamma, double *out) {
+  if (!ctx || npole < 1 || nv < 1 || ne < 1 || !E || !P || !E0 || !out) return QS_EINVAL;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t nout = tensor ? ((size_t)ne + 1) * nv * nv : ((size_t)ne + 1) * nv;
+  char *mem = nullptr;
+  QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)npole * 8 + (size_t)npole * nv * 16 + (size_t)nv * 8 + nout * 16));
+  double2 *dP = (double2 *)mem;
+  double2 *dout = dP + (size_t)npole * nv;
+  double *dE = (double *)(dout + nout);
+  double *dE0 = dE + npole;
+  hipError_t e = hipMemcpyAsync(dE, E, (size_t)npole * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dP, P, (size_t)npole * nv * 16, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dE0, E0, (size_t)nv * 8, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) {
+    const unsigned g = (unsigned)((nout + 255) / 256);
+    if (tensor)
+      hipLaunchKernelGGL(k_poles_tensor, dim3(g), dim3(256), 0, ctx->stream, npole, nv, dE, dP, E0[0], emin, emax, ne,
+                         gamma, dout);
+    else
+      hipLaunchKernelGGL(k_poles, dim3(g), dim3(256), 0, ctx->stream, npole, nv, dE, dP, dE0, emin,
+                         (emax - emin) / (double)ne, ne, gamma, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, nout * 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  qs_free(mem);
+  if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_pole_spectrum", __FILE__, __LINE__);
+  return QS_OK;
+}

@@ -11,7 +11,8 @@ import numpy as np
 import torch
 
 from . import det as D
-from ._lib import QS_HOP_ASSEMBLED, QS_HOP_AUTO, QS_HOP_MATFREE, Context, QSError, default_context, load, ptr
+from ._lib import (QS_ENOTCONVERGED, QS_HOP_ASSEMBLED, QS_HOP_AUTO, QS_HOP_MATFREE, Context, QSError,
+                   default_context, load, ptr)
 
 
 def _torch_ready(device) -> None:
@@ -77,6 +78,20 @@ class Basis:
         h = C.c_void_p()
         ctx.check(ctx.lib.qs_basis_from_words(ctx.h, nf, ptr(w), w.shape[0], C.byref(h)), "qs_basis_from_words")
         return cls(h, ctx)
+
+    @classmethod
+    def closure(cls, op: "Op", seed: "Basis", restrictions=None, max_dim: int = 1 << 62) -> "Basis | None":
+        """The smallest determinant set holding seed and every target of op's terms
+        (inside the restrictions), built on the GPU; None when it passes max_dim."""
+        ctx = seed.ctx
+        nres, masks, lo, hi = restriction_arrays(seed.nf, restrictions or [])
+        h = C.c_void_p()
+        st = ctx.lib.qs_basis_closure(ctx.h, op.h, seed.h, nres, ptr(masks), ptr(lo), ptr(hi), int(max_dim),
+                                      C.byref(h))
+        if st == QS_ENOTCONVERGED:
+            return None
+        ctx.check(st, "qs_basis_closure")
+        return cls(h, ctx, list(restrictions) if restrictions else None)
 
     def words(self) -> np.ndarray:
         out = np.zeros((self.n, self.nw), dtype=np.uint64)
@@ -174,6 +189,15 @@ class Hop:
         _torch_ready(d.device)
         self.ctx.check(self.lib.qs_hop_diagonal(self.ctx.h, self.h, ptr(d)), "qs_hop_diagonal")
         return d
+
+    def dense(self) -> torch.Tensor:
+        """The dense matrix <r|O|c> (rows x cols, complex128, on the device), stored
+        column-major: the returned tensor is the transpose view of a contiguous
+        (cols, rows) tensor, as qs_dense_poles takes it."""
+        M = torch.empty((self.cols.n, self.rows.n), dtype=torch.complex128, device=_device(self.ctx))
+        _torch_ready(M.device)
+        self.ctx.check(self.lib.qs_hop_dense(self.ctx.h, self.h, ptr(M)), "qs_hop_dense")
+        return M.t()
 
     def apply(self, x: torch.Tensor, y: torch.Tensor | None = None, ctx: Context | None = None) -> torch.Tensor:
         """y = O x. ctx: the context (stream) to run on, when not the one the
@@ -282,6 +306,60 @@ def block_cf_spectrum(A: np.ndarray, B: np.ndarray, kept: np.ndarray, E0: float,
     out = np.zeros((ne + 1, k, k), dtype=np.complex128)
     ctx.check(ctx.lib.qs_block_cf_spectrum(ctx.h, k, n, ptr(A), ptr(B), ptr(kept), E0, emin, emax, ne, gamma,
                                            ptr(out)), "qs_block_cf_spectrum")
+    return out
+
+
+def broaden(G: np.ndarray, dE: float, kind: str, fwhm, ctx: Context | None = None) -> np.ndarray:
+    """Gaussian (kind "G") or Lorentzian ("L") broadening on the GPU of spectra
+    G (nspec, npts) on a grid of step dE. fwhm: one width for every point, or
+    npts widths (one per source point, each at least dE/2). Returns a new array."""
+    ctx = ctx or default_context()
+    G = np.atleast_2d(np.asarray(G, dtype=np.complex128))
+    nspec, npts = G.shape
+    f = np.ascontiguousarray(np.atleast_1d(np.asarray(fwhm, dtype=np.float64)))
+    if f.size not in (1, npts):
+        raise QSError(f"broaden: {f.size} widths for {npts} points")
+    re = np.ascontiguousarray(G.real)
+    im = np.ascontiguousarray(G.imag)
+    ctx.check(ctx.lib.qs_broaden(ctx.h, nspec, npts, float(dE), 0 if kind == "G" else 1, int(f.size == npts and npts > 1),
+                                 ptr(f), ptr(re), ptr(im)), "qs_broaden")
+    return re + 1j * im
+
+
+def dense_poles(Hd: torch.Tensor, X: torch.Tensor | None = None, ctx: Context | None = None):
+    """Eigen-decomposition of the dense Hermitian matrix Hd (from Hop.dense(); it is
+    overwritten by the eigenvectors: afterwards Hd[:, k] = z_k) and projections of
+    the vectors X (nv, n) on them. Returns E (n,) ascending and P (n, nv) with
+    P[k, s] = <z_k|x_s>."""
+    ctx = ctx or default_context()
+    n = Hd.shape[0]
+    st = Hd.t()
+    assert Hd.shape == (n, n) and st.is_contiguous() and Hd.dtype == torch.complex128
+    nv = 0 if X is None else X.shape[0]
+    if nv:
+        X = X.contiguous()
+        assert X.shape == (nv, n) and X.dtype == torch.complex128
+    E = np.zeros(n)
+    P = np.zeros((n, max(nv, 1)), dtype=np.complex128)
+    _torch_ready(Hd.device)
+    ctx.check(ctx.lib.qs_dense_poles(ctx.h, n, ptr(st), nv, ptr(X) if nv else None, ptr(E), ptr(P)),
+              "qs_dense_poles")
+    return E, P[:, :nv]
+
+
+def pole_spectrum(E: np.ndarray, P: np.ndarray, E0, emin: float, emax: float, ne: int, gamma: float,
+                  tensor: bool = False, ctx: Context | None = None) -> np.ndarray:
+    """Spectra of pole lists on the GPU. E (npole,), P (npole, nv) = <z_k|x_s>.
+    tensor False: (nv, ne+1), G_s(w) = sum_k |P[k,s]|^2 / (w + E0_s - E_k + i gamma/2);
+    tensor True: (ne+1, nv, nv), G_rc(w) = sum_k conj(P[k,r]) P[k,c] / (w - (E_k - E0) + i gamma/2)."""
+    ctx = ctx or default_context()
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    P = np.ascontiguousarray(P, dtype=np.complex128)
+    npole, nv = P.shape
+    e0 = np.ascontiguousarray(np.broadcast_to(np.asarray(E0, dtype=np.float64), (nv,)))
+    out = np.zeros((ne + 1, nv, nv) if tensor else (nv, ne + 1), dtype=np.complex128)
+    ctx.check(ctx.lib.qs_pole_spectrum(ctx.h, npole, nv, ptr(E), ptr(P), int(tensor), ptr(e0), emin, emax, ne,
+                                       gamma, ptr(out)), "qs_pole_spectrum")
     return out
 
 

@@ -72,3 +72,23 @@ def test_conserved_groups():
     assert list(range(0, 6)) in gf and list(range(6, 26)) in gf
     ch = conserved_groups(NewOperator(4, 0, [[0, -1.0, 1], [1, -0.0, 1], [2, 3, -2.0, -3.0, 1]]))
     assert [0, 1] in ch and [2] in ch and [3] in ch
+
+
+def test_spectra_width_tables_match_oracle_interpolation():
+    """Spectra.Broaden's per-point widths from {{E, FWHM}, ...} tables
+    (LuaInterface/Spectra.cpp:37-62), against the oracle's restatement."""
+    import oracle
+    from quanty_amd.api import Spectra
+
+    S = Spectra(-13.5, 21.5, 3500, 0.1, np.zeros((1, 3501), dtype=complex))
+    dE = 35.0 / 3500
+    for t in ([[-13.5, 0.46], [3, 0.46], [3.01, 0.8], [21.5, 0.9]], [[0.0, 0.3]], [[-20, 1], [-15, 2]],
+              [[30, 1], [40, 2]], [[-1, 0.2], [-1, 0.4], [2, 0.1]]):
+        got = S._widths(t, "Gaussian")
+        ref = oracle.fwhm_table(t, -13.5, dE, 3501)
+        assert np.allclose(got, ref, rtol=0, atol=1e-15), t
+    assert S._widths(0.5, "Gaussian") == 0.5
+    with pytest.raises(QSError):
+        S._widths([[1, 0.2], [0, 0.3]], "Gaussian")
+    with pytest.raises(QSError):
+        S._widths([[1, 0.2, 3]], "Lorentzian")

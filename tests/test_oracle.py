@@ -245,3 +245,85 @@ def test_f6_mathematica_energies(cplx):
         # 1e-6 for the ground state (TestScripts/Eigensystem.Quanty:684,691)
         assert np.all(np.abs(e - want) < 1e-4)
         assert abs(e[0] - want[0]) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# broadening: or_broaden / or_fwhm_table
+# ---------------------------------------------------------------------------
+def _delta(npts, i0):
+    G = np.zeros(npts, dtype=complex)
+    G[i0] = 1.0 - 0.5j
+    return G
+
+
+def test_gaussian_broadening_conserves_weight_and_has_the_fwhm_variance():
+    """A Gaussian of FWHM f has sigma = f / (2 sqrt(2 ln 2)); averaging over the 7
+    sub-points j/7 of the cell adds dE^2 * mean(j^2)/49 = 4 dE^2 / 49 to the
+    variance. The Riemann sums of a well-resolved Gaussian are exact to
+    exp(-2 pi^2 sigma^2 / dE^2), so weight and moments are pinned to rounding."""
+    npts, dE, f = 4001, 0.01, 0.5
+    i0 = 2000
+    out = oracle.broaden(_delta(npts, i0), dE, "G", f)[0]
+    m = (np.arange(npts) - i0) * dE
+    sigma = f / (2 * math.sqrt(2 * math.log(2)))
+    w = out / (1.0 - 0.5j)
+    assert np.max(np.abs(w.imag)) < 1e-15
+    w = w.real
+    assert abs(w.sum() - 1) < 1e-12
+    assert abs((m * w).sum()) < 1e-14
+    assert abs((m * m * w).sum() - (sigma ** 2 + 4 * dE ** 2 / 49)) < 1e-12 * sigma ** 2
+    # a narrower FWHM than half the step leaves the spectrum unchanged
+    G = np.random.default_rng(1).standard_normal((1, 50)) + 0j
+    assert np.array_equal(oracle.broaden(G, 0.1, "G", 0.049), G)
+
+
+def test_lorentzian_broadening_adds_widths():
+    """Lorentzian (FWHM g2) convolved with the line of a pole broadened by g1 is
+    the pole's line with g1 + g2 (closed form; Re and Im alike). The grid
+    discretization and the cut tails leave a few 1e-4 near the pole."""
+    npts, emin, emax = 6001, -30.0, 30.0
+    dE = (emax - emin) / (npts - 1)
+    g1, g2, E1 = 0.3, 0.4, 0.7
+    a, b = np.array([0.0, E1]), np.array([1.0])
+    G1 = oracle.cf(a, b, 0.0, emin, emax, npts - 1, g1)
+    G12 = oracle.cf(a, b, 0.0, emin, emax, npts - 1, g1 + g2)
+    out = oracle.broaden(G1, dE, "L", g2)[0]
+    x = emin + np.arange(npts) * dE
+    near = np.abs(x - E1) < 3
+    scale = np.max(np.abs(G12))
+    assert np.max(np.abs(out[near] - G12[near])) < 3e-4 * scale
+    # doubling or halving the width would be far off
+    wrong = oracle.broaden(G1, dE, "L", 2 * g2)[0]
+    assert np.max(np.abs(wrong[near] - G12[near])) > 0.05 * scale
+
+
+@pytest.mark.parametrize("kind", ["G", "L"])
+def test_variable_width_uses_the_source_point_width(kind):
+    """Per-point widths: a delta at i0 spreads with the width of i0, whatever the
+    widths elsewhere; constant per-point widths equal the single width
+    bit for bit; widths below dE/2 are raised to dE/2."""
+    npts, dE, i0 = 801, 0.02, 300
+    rng = np.random.default_rng(3)
+    f = rng.uniform(0.1, 1.0, npts)
+    out = oracle.broaden(_delta(npts, i0), dE, kind, f, per_point=True)
+    ref = oracle.broaden(_delta(npts, i0), dE, kind, f[i0])
+    assert np.array_equal(out, ref)
+    G = rng.standard_normal((2, npts)) + 1j * rng.standard_normal((2, npts))
+    assert np.array_equal(oracle.broaden(G, dE, kind, np.full(npts, 0.3), per_point=True),
+                          oracle.broaden(G, dE, kind, 0.3))
+    assert np.array_equal(oracle.broaden(G, dE, kind, np.full(npts, 0.001), per_point=True),
+                          oracle.broaden(G, dE, kind, dE / 2))
+
+
+def test_fwhm_table_is_linear_interpolation():
+    rng = np.random.default_rng(5)
+    emin, npts = -3.0, 601
+    dE = 0.01
+    for nt in (1, 2, 5):
+        E = np.sort(rng.uniform(-4, 4, nt))
+        E[0] = emin + 17 * dE  # a table node on a grid point
+        E.sort()
+        F = rng.uniform(0.1, 1, nt)
+        got = oracle.fwhm_table(np.stack([E, F], 1), emin, dE, npts)
+        x = emin + np.arange(npts) * dE
+        assert np.allclose(got, np.interp(x, E, F), rtol=0, atol=1e-14)
