@@ -303,13 +303,13 @@ qs_status qs_hop_dense(qs_ctx *ctx, const qs_hop *h, double *M);
 
 /* Eigen-decomposition of a dense Hermitian n x n matrix A (device, complex,
  * column-major; its lower triangle is read and it is overwritten by the
- * eigenvectors, column k = z_k) by rocSOLVER's zheevd, and the projections of
+ * eigenvectors, column k = z_k) by rocSOLVER's Jacobi solver zheevj, and the projections of
  * nv vectors X (device, vector s at X + 2*s*n) on the eigenvectors:
  * E[k] (host, ascending), P[k*nv + s] = <z_k|x_s> (host, complex).
  * |P[k][s]|^2 are the pole weights of HermitianCompactMatrixToListOfPoles
  * (Core/BasicMath/BasicMath_CompactMatrixVector.cpp:3990-4026), and
  * conj(P[k][r]) P[k][c] the pole blocks of ...ToBlockListOfPoles (:4033-4113).
- * QS_ENOTCONVERGED when zheevd reports failure. */
+ * QS_ENOTCONVERGED when the Jacobi sweeps do not converge. */
 qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const double *X, double *E, double *P);
 
 /* Spectra from poles E[0..npole) with projections P (npole*nv complex, as

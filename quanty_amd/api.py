@@ -447,7 +447,9 @@ def _expect(op: Operator, psi: Wavefunction) -> complex:
 def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
     """Lowest Npsi eigenstates (LuaEigensystem, LuaInterface/ssp.cpp:4039).
     The basis is the particle sector of the start determinants, inside the
-    Restrictions when given (the closure of H from StartRestrictions stays in it)."""
+    Restrictions when given (the closure of H from StartRestrictions stays in it).
+    Up to max(DenseBorder, Npsi) determinants the determinant matrix is
+    diagonalized (rocSOLVER zheevj); above, thick-restart block Lanczos."""
     o = _options(options, {"Restrictions": None, "Zero": 100 * SQRT_EPS, "Epsilon": SQRT_EPS / 100,
                            "NKrylovMax": 500, "NKrylovStart": 100, "NKrylovStep": 10, "NKrylovMin": 50,
                            "DenseBorder": 1513, "NBitsKey": 16, "ShiftScheme": "Variance", "Seed": 1,
@@ -469,6 +471,12 @@ def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
     basis = _sector_of_counts(nf, groups, counts, _restrictions(o["Restrictions"], nf), ctx)
     if basis.n < Npsi:
         raise QSError(f"the space has {basis.n} determinants, fewer than Npsi = {Npsi}")
+    if basis.n <= max(int(o["DenseBorder"]), Npsi):
+        # dense: the determinant matrix and its eigenvectors
+        # (BlockLanczosGroundStateConserveBasisDense, Core/BlockLanczos.cpp:370-504, 5713)
+        Hd = Hop(H.device(ctx), basis, mode="matfree").dense()
+        dense_poles(Hd, None, ctx)
+        return [Wavefunction(nf, basis=basis, vec=Hd[:, k].contiguous()) for k in range(Npsi)]
     hop = Hop(H.device(ctx), basis, mode="auto")
     E, vecs, var, it = hop.ground_states(Npsi, tol=float(o["Zero"]), max_krylov=int(o["NKrylovMax"]),
                                          max_restarts=int(o["MaxRestarts"]), seed=int(o["Seed"]),

@@ -57,12 +57,54 @@ def config0(reps: int = 5):
         psis = Eigensystem(Hi, start, 3, [["Zero", 1e-14]])
         return CreateSpectra(Hf, Ts, psis, opts)
 
+    def run_defaults():
+        # the reference's defaults for spaces below DenseBorder: dense everywhere
+        psis = Eigensystem(Hi, start, 3)
+        return CreateSpectra(Hf, Ts, psis, opts[:-1])
+
     run()
     _, t = _sync_time(lambda: [run() for _ in range(reps)])
     t /= reps
+    run_defaults()
+    _, td = _sync_time(lambda: [run_defaults() for _ in range(reps)])
+    td /= reps
     return {"config": "0", "what": "Ni2+ L2,3 XAS: Eigensystem (3 states, 190 dets) + CreateSpectra (3 dipoles x "
                                    "3 states, 120-det final sector, NTri 200, 2001 points, Gamma 0.4)",
-            "seconds": t, "spectra": 9, "spectra_per_s": 9 / t}
+            "seconds": t, "spectra": 9, "spectra_per_s": 9 / t,
+            "seconds_dense_defaults": td, "spectra_per_s_dense_defaults": 9 / td}
+
+
+def dense_small(seed: int = 5, reps: int = 5):
+    """The dense small-problem path on a 1,410-determinant impurity (d + 1 bath
+    shell, 16 electrons): ground state, one d<->bath transition, CreateSpectra by
+    the dense method (closure, determinant matrix, zheevj, poles) against
+    200-step Lanczos, 2001 points."""
+    from .api import CreateSpectra, Operator, Wavefunction
+
+    ctx = default_context()
+    pr = P.anderson(seed=seed, nbath=1, nelec=16, full_baths=1, max_holes=1, max_elec=0, d_range=(0, 10))
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"], ctx)
+    hop = Hop(Op(nf, pr["H"], ctx), basis)
+    E, vecs, _, _ = hop.ground_states(1, tol=1e-14)
+    psi = Wavefunction(nf, basis=basis, vec=vecs[0].clone())
+    H = Operator(nf, dict(pr["H"]))
+    bath = [m for b in pr["baths"] for m in b]
+    T = Operator(nf, dict(P.random_one_body_transition(P.SplitMix64(seed), pr["dmodes"], bath)))
+    opts = [["Emin", -10.0], ["Emax", 20.0], ["NE", 2000], ["Gamma", 0.4], ["NTri", 200],
+            ["Restrictions", [nf, 0, *[[m, lo, hi] for m, lo, hi in pr["restrictions"]]]]]
+    sd, _ = CreateSpectra(H, T, psi, opts)
+    sl, _ = CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]])
+    _, td = _sync_time(lambda: [CreateSpectra(H, T, psi, opts) for _ in range(reps)])
+    _, tl = _sync_time(lambda: [CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]]) for _ in range(reps)])
+    # stages of the dense method
+    from .engine import dense_poles
+    Hd, t_build = _sync_time(lambda: Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense())
+    _, t_eig = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx))
+    diff = float(np.max(np.abs(sd.G - sl.G)) / np.max(np.abs(sd.G)))
+    return {"config": "dense", "dim": basis.n, "seconds_dense": td / reps, "seconds_lanczos": tl / reps,
+            "seconds_dense_matrix": t_build, "seconds_zheevj_and_projection": t_eig,
+            "lanczos200_vs_dense_max_rel_diff": diff}
 
 
 def config1(seed: int = 1, nstates: int = 3, tol: float = 1e-12, max_krylov: int = 500):
@@ -158,11 +200,13 @@ def config3(requests: int = 64, rates=(10.0, 50.0, 200.0, 500.0), seed: int = 10
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--config", required=True, choices=["0", "1", "2a", "2b", "3", "4"])
+    ap.add_argument("--config", required=True, choices=["0", "1", "2a", "2b", "3", "4", "dense"])
     ap.add_argument("--requests", type=int, default=64, help="requests per rate (config 3)")
     args = ap.parse_args()
     if args.config == "0":
         out = config0()
+    elif args.config == "dense":
+        out = dense_small()
     elif args.config == "1":
         out, _ = config1()
     elif args.config == "2a":

@@ -253,3 +253,41 @@ def test_dense_pole_range_is_the_closure_spectrum(ctx):
     assert abs(spec.EmaxPole - ev[-1]) < 1e-12 * np.max(np.abs(ev))
     full = np.linalg.eigvalsh(dense_from_terms(nf, terms, jw_matrices(nf)))
     assert full[0] < ev[0] - 1 and full[-1] > ev[-1] + 1  # the whole space reaches further
+
+
+def test_eigensystem_dense_equals_krylov(ctx):
+    """Eigensystem below DenseBorder diagonalizes the determinant matrix; above
+    it runs block Lanczos. Both give the same energies (1e-10) and eigenstates
+    with residuals at rounding level."""
+    from quanty_amd.api import Eigensystem, _expect
+    nf, terms = chain(4, 0)
+    H = Operator(nf, dict(terms))
+    start = [nf, 0, ["10" * (nf // 2), 2, 2], ["01" * (nf // 2), 2, 2]]
+    dense = Eigensystem(H, start, 4)
+    kry = Eigensystem(H, start, 4, [["DenseBorder", 0], ["Zero", 1e-14]])
+    assert dense[0]._basis.n == 100 and dense[0]._basis.n <= 1513
+    Ed = [_expect(H, p).real for p in dense]
+    Ek = [_expect(H, p).real for p in kry]
+    assert np.allclose(Ed, Ek, rtol=1e-10, atol=1e-12)
+    assert np.all(np.diff(Ed) >= -1e-12)
+    hop = Hop(H.device(ctx), dense[0]._basis, mode="matfree")
+    for p, e in zip(dense, Ed):
+        y = hop.apply(p._vec.contiguous())
+        assert torch.linalg.norm(y - e * p._vec).item() < 1e-11 * max(1.0, abs(e))
+
+
+@pytest.mark.parametrize("n", [100, 400])
+def test_dense_poles_degenerate_spectrum(n, ctx):
+    """Exactly degenerate eigenvalues (two copies of one block): every eigenvector
+    finite, H z = E z and Z orthonormal at rounding level."""
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n // 2, n // 2)) + 1j * rng.normal(size=(n // 2, n // 2))
+    M = np.kron(np.eye(2), A + A.conj().T)
+    T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+    E, _ = dense_poles(T.t(), None, ctx)
+    Z = T.t().cpu().numpy()
+    scale = np.max(np.abs(E))
+    assert np.all(np.isfinite(Z))
+    assert np.max(np.abs(E - np.linalg.eigvalsh(M))) < 1e-12 * scale
+    assert np.max(np.abs(M @ Z - Z * E[None, :])) < 1e-11 * scale
+    assert np.max(np.abs(Z.conj().T @ Z - np.eye(n))) < 1e-11
