@@ -209,10 +209,15 @@ def cpu_baseline(args, pr, basis, work):
         op.apply(psi, restrictions=pr["restrictions"], nthreads=threads)
         return time.perf_counter() - t0
 
+    # size the sample to about cpu_seconds: a short probe overestimates the time
+    # per row (thread start-up), so a sample that ran short is sized once more
     probe = 2000
     t = run(probe)
     rows = int(min(basis.n, max(probe, probe * args.cpu_seconds / max(t, 1e-3))))
     t = run(rows)
+    if t < 0.6 * args.cpu_seconds and rows < basis.n:
+        rows = int(min(basis.n, rows * args.cpu_seconds / max(t, 1e-3)))
+        t = run(rows)
     per_row = t / rows
     t_stream = sum((gs + depth) * dim * per_row for dim, gs, depth in work)
     return {"value": len(work) / t_stream, "unit": "spectra/s", "cores": threads, "kind": "port",

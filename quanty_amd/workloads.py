@@ -102,7 +102,12 @@ def dense_small(seed: int = 5, reps: int = 5):
     Hd, t_build = _sync_time(lambda: Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense())
     _, t_eig = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx))
     diff = float(np.max(np.abs(sd.G - sl.G)) / np.max(np.abs(sd.G)))
+    # broadening of the spectrum: Gaussian 0.5, Lorentzian from an energy table
+    table = [[-10.0, 0.2], [3.0, 0.2], [3.01, 0.6], [20.0, 0.6]]
+    sd.Broaden(0.5, table)
+    _, tb = _sync_time(lambda: [sd.Broaden(0.5, table) for _ in range(reps)])
     return {"config": "dense", "dim": basis.n, "seconds_dense": td / reps, "seconds_lanczos": tl / reps,
+            "seconds_broaden_2001_points": tb / reps,
             "seconds_dense_matrix": t_build, "seconds_zheevj_and_projection": t_eig,
             "lanczos200_vs_dense_max_rel_diff": diff}
 
