@@ -527,9 +527,47 @@ class Spectra:
     def energies(self):
         return self.Emin + np.arange(self.NE + 1) * (self.Emax - self.Emin) / self.NE
 
-    def Sum(self, weights):
-        return Spectra(self.Emin, self.Emax, self.NE, self.Gamma,
-                       [np.tensordot(np.asarray(weights), self.G.reshape(len(weights), -1), 1)])
+    def Sum(self, *weights):
+        """Spectra.Sum(s, w_1, w_2, ...) (LuaInterface/Spectra.cpp:527-613): one
+        spectrum per weight list, sum_k w_j[k] s[k], real or complex weights."""
+        if not weights:
+            raise QSError("Spectra.Sum needs at least one list of weights")
+        G = self.G.reshape(-1, self.NE + 1)
+        out = []
+        for j, w in enumerate(weights, 1):
+            w = np.asarray(w, dtype=np.complex128).ravel()
+            if w.size != G.shape[0]:
+                raise QSError(f"Spectra.Sum: weight list {j} has {w.size} entries for {G.shape[0]} spectra")
+            out.append(w @ G)
+        return Spectra(self.Emin, self.Emax, self.NE, self.Gamma, out, self.EminPole, self.EmaxPole)
+
+    # arithmetic (LuaInterface/Spectra.cpp:969-1752): a number adds to the real
+    # parts; spectra combine point by point
+    def _other(self, x):
+        if isinstance(x, Spectra):
+            if x.G.shape != self.G.shape or (x.Emin, x.Emax, x.NE) != (self.Emin, self.Emax, self.NE):
+                raise QSError("spectra on different grids or of different counts cannot be combined")
+            return x.G
+        return x
+
+    def _new(self, G):
+        return Spectra(self.Emin, self.Emax, self.NE, self.Gamma, G, self.EminPole, self.EmaxPole)
+
+    def __add__(self, x):
+        return self._new(self.G + self._other(x) if isinstance(x, Spectra) else self.G + float(np.real(x)))
+
+    __radd__ = __add__
+
+    def __sub__(self, x):
+        return self._new(self.G - self._other(x) if isinstance(x, Spectra) else self.G - float(np.real(x)))
+
+    def __mul__(self, x):
+        return self._new(self.G * self._other(x))
+
+    __rmul__ = __mul__
+
+    def __truediv__(self, x):
+        return self._new(self.G / self._other(x))
 
     def Copy(self):
         return Spectra(self.Emin, self.Emax, self.NE, self.Gamma, self.G.copy(), self.EminPole, self.EmaxPole)
@@ -575,18 +613,34 @@ class Spectra:
         return self
 
     def Print(self, file=None):
-        """Energy column then Re/Im of every spectrum, 17 significant digits."""
-        lines = []
-        for i, w in enumerate(self.energies):
-            row = [w]
-            for g in self.G:
-                row += [g[i].real, g[i].imag]
-            lines.append(" ".join(f"{x:23.16E}" for x in row))
-        text = "\n".join(lines) + "\n"
-        if file is None:
-            return text
-        with open(file, "w") as f:
-            f.write(text)
+        """The reference's spectra table (StreamPrintSpectra, IO/StreamPrint.cpp:4-60):
+        a header with the count, Emin/Emax, the pole range and dE/Gamma of every
+        spectrum, then one line per energy with Re and Im of every spectrum.
+        Returns the text; written to `file` when given ({{"file", name}})."""
+        G = self.G.reshape(-1, self.NE + 1)
+        ns = G.shape[0]
+        dE = (self.Emax - self.Emin) / self.NE
+        pmin = 0.0 if self.EminPole is None else self.EminPole
+        pmax = 0.0 if self.EmaxPole is None else self.EmaxPole
+        out = ["#Spectra: %i\n" % ns,
+               "Emin______Emax     " + "".join(" %22.15E %22.15E" % (self.Emin, self.Emax) for _ in range(ns)) + "\n",
+               "EminPole__EmaxPole " + "".join(" %22.15E %22.15E" % (pmin, pmax) for _ in range(ns)) + "\n",
+               "dE________Gamma    " + "".join(" %22.15E %22.15E" % (dE, self.Gamma) for _ in range(ns)) + "\n"]
+        head = "Energy             "
+        for i in range(min(ns, 10000)):  # labels as the reference writes them (up to 9999)
+            w = len(str(i))
+            head += "  Re[%i]" % i + " " * (19 - w) + "Im[%i]" % i + " " * (17 - w)
+        out.append(head + "\n")
+        for j in range(self.NE + 1):
+            row = "%19.12E" % (dE * j + self.Emin)
+            for g in G:
+                row += " %22.15E %22.15E" % (g[j].real, g[j].imag)
+            out.append(row + "\n")
+        text = "".join(out)
+        if file is not None:
+            with open(file, "w") as f:
+                f.write(text)
+        return text
 
 
 def _pole_range(a, b, E0):
