@@ -161,7 +161,8 @@ def _shell_args(args):
 def NewOperator(*args, **kw):
     """NewOperator(NF, NB, {{i, -j, ..., v}, ...}) or a named operator:
     "Cr"/"An" (NF, i), "Number" (NF, idxA, idxB, values), "U" (NF, up, dn, F) or
-    (NF, upA, dnA, upB, dnB, F, G), "ldots"/"Lx"/"Ly"/"Lz"/"Sx"/"Sy"/"Sz" (NF, up, dn),
+    (NF, upA, dnA, upB, dnB, F, G), "ldots", "Sx"/"Sy"/"Sz"/"Splus"/"Smin"/"Ssqr" and the
+    same for L and J (NF, up, dn),
     "CF" (NF, up, dn, Akm) or (NF, upA, dnA, upB, dnB, Akm)."""
     if not args:
         raise QSError("NewOperator called with 0 arguments, expect at least one")
@@ -201,9 +202,16 @@ def NewOperator(*args, **kw):
             return Operator(nf, O.coulomb(up, dn, list(rest[3])))
         upa, dna, upb, dnb = (list(x) for x in rest[1:5])
         return Operator(nf, O.coulomb_two_shells(upa, dna, upb, dnb, list(rest[5]), list(rest[6])))
-    if name in ("ldots", "Lx", "Ly", "Lz", "Sx", "Sy", "Sz"):
+    if name in ("ldots", "Lx", "Ly", "Lz", "Sx", "Sy", "Sz", "Jx", "Jy", "Jz", "Splus", "Smin", "Lplus", "Lmin",
+                "Jplus", "Jmin"):
         up, dn = _shell_args(rest[1:3])
         return Operator(nf, O.shell_operator(up, dn, name))
+    if name in ("Ssqr", "Lsqr", "Jsqr"):
+        # the square of the vector operator as products of its one-body components
+        up, dn = _shell_args(rest[1:3])
+        k = name[0]
+        comps = [Operator(nf, O.shell_operator(up, dn, k + c)) for c in "xyz"]
+        return comps[0] * comps[0] + comps[1] * comps[1] + comps[2] * comps[2]
     if name == "CF":
         if len(rest) == 4:
             up, dn = _shell_args(rest[1:3])

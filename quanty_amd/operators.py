@@ -188,6 +188,18 @@ def spin_orbital_matrix(l, which):
         return kron(Id, -0.5j * (sp - sm))
     if which == "ldots":
         return kron(Lz, sz) + 0.5 * (kron(Lp, sm) + kron(Lm, sp))
+    if which == "Splus":
+        return kron(Id, sp)
+    if which == "Smin":
+        return kron(Id, sm)
+    if which == "Lplus":
+        return kron(Lp, I2)
+    if which == "Lmin":
+        return kron(Lm, I2)
+    if which in ("Jx", "Jy", "Jz", "Jplus", "Jmin"):
+        s_, l_ = {"Jx": ("Sx", "Lx"), "Jy": ("Sy", "Ly"), "Jz": ("Sz", "Lz"), "Jplus": ("Splus", "Lplus"),
+                  "Jmin": ("Smin", "Lmin")}[which]
+        return spin_orbital_matrix(l, s_) + spin_orbital_matrix(l, l_)
     raise ValueError(which)
 
 

@@ -92,3 +92,29 @@ def test_spectra_width_tables_match_oracle_interpolation():
         S._widths([[1, 0.2], [0, 0.3]], "Gaussian")
     with pytest.raises(QSError):
         S._widths([[1, 0.2, 3]], "Lorentzian")
+
+
+def test_named_angular_momentum_operators_match_dense():
+    """Named shell operators (LuaInterface/Operator.cpp named forms) on a p shell:
+    S^2, L^2, J^2 have eigenvalues x(x+1); S+ = Sx + i Sy, J = L + S, and
+    [Jz, J+] = J+, checked on the Jordan-Wigner matrices of 6 modes."""
+    nf = 6
+    up, dn = [1, 3, 5], [0, 2, 4]
+    M = {n: dense_from_terms(nf, list(NewOperator(n, nf, up, dn).terms.items()))
+         for n in ("Sx", "Sy", "Sz", "Splus", "Smin", "Ssqr", "Lx", "Ly", "Lz", "Lplus", "Lmin", "Lsqr",
+                   "Jx", "Jy", "Jz", "Jplus", "Jmin", "Jsqr")}
+    assert np.allclose(M["Splus"], M["Sx"] + 1j * M["Sy"]) and np.allclose(M["Smin"], M["Sx"] - 1j * M["Sy"])
+    assert np.allclose(M["Lplus"], M["Lx"] + 1j * M["Ly"]) and np.allclose(M["Jmin"], M["Jx"] - 1j * M["Jy"])
+    for k in "xyz":
+        assert np.allclose(M["J" + k], M["L" + k] + M["S" + k])
+    assert np.allclose(M["Jz"] @ M["Jplus"] - M["Jplus"] @ M["Jz"], M["Jplus"])
+    assert np.allclose(M["Lx"] @ M["Ly"] - M["Ly"] @ M["Lx"], 1j * M["Lz"])
+    for sq, allowed in (("Ssqr", [s * (s + 1) for s in np.arange(0, 3.5, 0.5)]),
+                        ("Lsqr", [l * (l + 1) for l in range(0, 5)]),
+                        ("Jsqr", [j * (j + 1) for j in np.arange(0, 6.5, 0.5)])):
+        ev = np.linalg.eigvalsh(M[sq])
+        assert all(np.min(np.abs(np.array(allowed) - e)) < 1e-10 for e in ev)
+    # two electrons in a p shell: S = 0, 1 and L = 0, 1, 2 all occur
+    idx = [i for i in range(2 ** nf) if bin(i).count("1") == 2]
+    ev = np.linalg.eigvalsh(M["Lsqr"][np.ix_(idx, idx)])
+    assert set(np.round(ev).astype(int)) == {0, 2, 6}
