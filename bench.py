@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -218,10 +219,15 @@ def cpu_baseline(args, pr, basis, work):
     if t < 0.6 * args.cpu_seconds and rows < basis.n:
         rows = int(min(basis.n, rows * args.cpu_seconds / max(t, 1e-3)))
         t = run(rows)
-    per_row = t / rows
+    passes = 1
+    if rows == basis.n and t < 0.6 * args.cpu_seconds:  # the whole operator is quick: repeat it
+        passes = int(math.ceil(args.cpu_seconds / max(t, 1e-3)))
+        t = sum(run(rows) for _ in range(passes))
+    per_row = t / (rows * passes)
     t_stream = sum((gs + depth) * dim * per_row for dim, gs, depth in work)
     return {"value": len(work) / t_stream, "unit": "spectra/s", "cores": threads, "kind": "port",
-            "sample": f"oracle push-form H*psi over {rows} of {basis.n} rows of the 50-mode operator in {t:.2f} s "
+            "sample": f"oracle push-form H*psi over {rows} of {basis.n} rows of the 50-mode operator"
+                      f"{f' x {passes} passes' if passes > 1 else ''} in {t:.2f} s "
                       f"({threads} threads, {per_row * 1e6:.3f} us/row); the stream's {len(work)} requests need "
                       f"sum (gs steps + depth) * dim row-applications -> {t_stream:.0f} s"}
 

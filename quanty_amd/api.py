@@ -407,7 +407,16 @@ def _group_mask(nf, modes):
 
 
 def _sector_of_counts(nf, groups, counts, restrictions, ctx, max_dim=50_000_000) -> Basis:
-    restr = list(restrictions) + [(_group_mask(nf, g), c, c) for g, c in zip(groups, counts)]
+    # groups held empty (or full) form one restriction together: count 0 on the
+    # union (count = its size); operators leave many modes untouched
+    empty = [m for g, c in zip(groups, counts) if c == 0 for m in g]
+    full = [m for g, c in zip(groups, counts) if c == len(g) and c > 0 for m in g]
+    rest = [(g, c) for g, c in zip(groups, counts) if 0 < c < len(g)]
+    restr = list(restrictions) + [(_group_mask(nf, g), c, c) for g, c in rest]
+    if empty:
+        restr.append((_group_mask(nf, empty), 0, 0))
+    if full:
+        restr.append((_group_mask(nf, full), len(full), len(full)))
     n = Basis.count(nf, restr)
     if n > max_dim:
         raise QSError(f"the reachable space has {n} determinants; give Restrictions to reduce it")

@@ -291,3 +291,31 @@ def test_dense_poles_degenerate_spectrum(n, ctx):
     assert np.max(np.abs(E - np.linalg.eigvalsh(M))) < 1e-12 * scale
     assert np.max(np.abs(M @ Z - Z * E[None, :])) < 1e-11 * scale
     assert np.max(np.abs(Z.conj().T @ Z - np.eye(n))) < 1e-11
+
+
+def test_dense_two_word_determinants_match_matrix_free(ctx):
+    """76 modes (two-word determinants): every column of the dense matrix equals
+    the matrix-free H e_j, and CreateSpectra's dense method equals its Lanczos
+    method when the Krylov space is exhausted."""
+    nf = 76
+    rng = np.random.default_rng(8)
+    modes = np.r_[0:4, 62:66, 72:76]
+    terms = random_terms(nf, modes, 10, 6, rng)
+    op = Op(nf, terms, ctx)
+    cb = Basis.closure(op, Basis.from_words(nf, det_words(nf, [1, 63, 73]), ctx))
+    n = cb.n
+    M = Hop(op, cb, mode="matfree").dense().cpu().numpy()
+    mf = Hop(op, cb, mode="matfree")
+    for j in range(0, n, max(1, n // 12)):
+        e = torch.zeros(n, dtype=torch.complex128, device="cuda")
+        e[j] = 1
+        col = mf.apply(e).cpu().numpy()
+        assert np.allclose(M[:, j], col, atol=1e-14, rtol=0)
+    assert np.allclose(M, M.conj().T, atol=1e-13)
+    H = Operator(nf, dict(terms))
+    psi = NewWavefunction(nf, 0, [("".join("1" if i in (1, 63) else "0" for i in range(nf)), 1.0)])
+    T = NewOperator("Cr", nf, 73)
+    opts = [["Emin", -8.0], ["Emax", 8.0], ["NE", 300], ["Gamma", 0.25], ["NTri", 2 * n + 10], ["E0", 0.0]]
+    sd, _ = CreateSpectra(H, T, psi, opts)
+    sl, _ = CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]])
+    assert np.max(np.abs(sd.G - sl.G)) <= 1e-8 * np.max(np.abs(sl.G))
