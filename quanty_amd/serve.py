@@ -97,12 +97,27 @@ class Request:
     spectrum: np.ndarray | None = field(compare=False, default=None)
     a: np.ndarray | None = field(compare=False, default=None)
     b: np.ndarray | None = field(compare=False, default=None)
+    # the request's payload: NF, H and T terms, restrictions (made from seed and
+    # shape when absent)
+    problem: dict | None = field(compare=False, default=None, repr=False)
+
+
+def make_problem(seed: int, shape: dict) -> dict:
+    """The seeded impurity of a request: H, its sector restrictions and a random
+    one-body d<->bath transition T."""
+    pr = P.anderson(seed=seed, **shape)
+    rng = P.SplitMix64(seed ^ 0x5DEECE66D)
+    bath = [m for b in pr["baths"] for m in b]
+    return {"NF": pr["NF"], "H": pr["H"], "restrictions": pr["restrictions"],
+            "T": P.random_one_body_transition(rng, pr["dmodes"], bath)}
 
 
 def request_stream(n: int, rate: float, seed: int = 7, dim_lo=1e3, dim_hi=3e6, depth=(100, 400),
-                   gamma=(0.1, 0.5), nbath_max: int = 4) -> list[Request]:
+                   gamma=(0.1, 0.5), nbath_max: int = 4, payload: bool = True) -> list[Request]:
     """Poisson arrivals at `rate` req/s (all at t = 0 when rate is 0); dim
-    log-uniform; depth U{lo..hi}; Gamma U(lo, hi); at most nbath_max bath shells."""
+    log-uniform; depth U{lo..hi}; Gamma U(lo, hi); at most nbath_max bath shells.
+    payload: each request carries its operator terms, as a client's request
+    would (made here, before any clock starts)."""
     rng = P.SplitMix64(seed)
     t = 0.0
     out = []
@@ -113,6 +128,9 @@ def request_stream(n: int, rate: float, seed: int = 7, dim_lo=1e3, dim_hi=3e6, d
         d = depth[0] + int(rng.uniform() * (depth[1] - depth[0] + 1))
         g = rng.uniform(*gamma)
         out.append(Request(arrival=t, rid=i, seed=int(rng.next() & 0x7FFFFFFF), shape=shape, dim=dim, depth=d, gamma=g))
+    if payload:
+        for r in out:
+            r.problem = make_problem(r.seed, r.shape)
     return out
 
 
@@ -122,14 +140,12 @@ class _Job:
     def __init__(self, req: Request, ctx, gs_depth: int, start_temperature: float | None = 0.3,
                  start_floor: float = 1e-3):
         self.req = req
-        pr = P.anderson(seed=req.seed, **req.shape)
+        pr = req.problem if req.problem is not None else make_problem(req.seed, req.shape)
         self.nf = pr["NF"]
         self.basis = Basis.enumerate(self.nf, pr["restrictions"], ctx)
         self.H = Op(self.nf, pr["H"], ctx)
         self.hop = Hop(self.H, self.basis, mode="assembled")
-        rng = P.SplitMix64(req.seed ^ 0x5DEECE66D)
-        bath = [m for b in pr["baths"] for m in b]
-        self.T = Op(self.nf, P.random_one_body_transition(rng, pr["dmodes"], bath), ctx)
+        self.T = Op(self.nf, pr["T"], ctx)
         self.thop = Hop(self.T, self.basis, mode="matfree")
         g = torch.Generator(device="cuda").manual_seed(req.seed)
         dev = torch.device("cuda", ctx.device)
