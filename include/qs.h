@@ -302,15 +302,30 @@ qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const double *A, const
 qs_status qs_hop_dense(qs_ctx *ctx, const qs_hop *h, double *M);
 
 /* Eigen-decomposition of a dense Hermitian n x n matrix A (device, complex,
- * column-major; its lower triangle is read and it is overwritten by the
- * eigenvectors, column k = z_k) by rocSOLVER's Jacobi solver zheevj, and the projections of
- * nv vectors X (device, vector s at X + 2*s*n) on the eigenvectors:
+ * column-major; its lower triangle is read) and the projections of nv vectors
+ * X (device, vector s at X + 2*s*n) on the eigenvectors z_k:
  * E[k] (host, ascending), P[k*nv + s] = <z_k|x_s> (host, complex).
  * |P[k][s]|^2 are the pole weights of HermitianCompactMatrixToListOfPoles
  * (Core/BasicMath/BasicMath_CompactMatrixVector.cpp:3990-4026), and
  * conj(P[k][r]) P[k][c] the pole blocks of ...ToBlockListOfPoles (:4033-4113).
- * QS_ENOTCONVERGED when the Jacobi sweeps do not converge. */
-qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const double *X, double *E, double *P);
+ * vectors = 1: A is overwritten by the eigenvectors (column k = z_k;
+ * rocSOLVER's Jacobi solver zheevj). vectors = 0: eigenvalues and projections
+ * only, A is overwritten by the Householder reduction (zhetrd + zunmtr on the
+ * GPU, the tridiagonal QL on the host with its rotations applied to Q^+ X).
+ * Within a degenerate eigenvalue the eigenvector basis, hence the individual
+ * P, depends on the method; sums over the degenerate set do not.
+ * QS_ENOTCONVERGED when the solver does not converge. */
+qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const double *X, double *E, double *P,
+                         int vectors);
+
+/* All eigenvalues E (host, ascending) of a dense Hermitian n x n matrix A
+ * (device, complex, column-major, lower triangle read, overwritten) and the
+ * eigenvectors of the k lowest in V (device, n x k complex, column j = z_j):
+ * Householder reduction on the GPU (zhetrd), tridiagonal QL on the host with
+ * its rotations replayed on k unit vectors, back-transform on the GPU. The
+ * dense ground states of BlockLanczosGroundStateConserveBasisDense
+ * (Core/BlockLanczos.cpp:370-504). */
+qs_status qs_dense_lowest(qs_ctx *ctx, int n, double *A, int k, double *E, double *V);
 
 /* Spectra from poles E[0..npole) with projections P (npole*nv complex, as
  * written by qs_dense_poles), on the GPU; host arrays.

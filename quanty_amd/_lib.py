@@ -77,7 +77,8 @@ _SIGS = [
     ("qs_block_cf_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _D, _D, _D, C.c_int, _D, _P]),
     ("qs_broaden", C.c_int, [_P, C.c_int, C.c_int, _D, C.c_int, C.c_int, _P, _P, _P]),
     ("qs_hop_dense", C.c_int, [_P, _P, _P]),
-    ("qs_dense_poles", C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P, _P]),
+    ("qs_dense_poles", C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P, _P, C.c_int]),
+    ("qs_dense_lowest", C.c_int, [_P, C.c_int, _P, C.c_int, _P, _P]),
     ("qs_pole_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, C.c_int, _P, _D, _D, C.c_int, _D, _P]),
     ("qs_heev_host", C.c_int, [C.c_int, _P, _P, _P]),
     ("qs_tridiag_lowest_host", C.c_int, [C.c_int, _P, _P, C.POINTER(_D), _P]),

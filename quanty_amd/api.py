@@ -25,8 +25,8 @@ import torch
 from . import det as D
 from . import operators as O
 from ._lib import QSError, default_context
-from .engine import (Basis, Hop, Op, block_cf_spectrum, broaden, cf_spectrum, dense_poles, pole_spectrum,
-                     tridiag_extremes)
+from .engine import (Basis, Hop, Op, block_cf_spectrum, broaden, cf_spectrum, dense_lowest, dense_poles,
+                     pole_spectrum, tridiag_extremes)
 
 SQRT_EPS = math.sqrt(np.finfo(float).eps)
 
@@ -466,7 +466,8 @@ def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
     The basis is the particle sector of the start determinants, inside the
     Restrictions when given (the closure of H from StartRestrictions stays in it).
     Up to max(DenseBorder, Npsi) determinants the determinant matrix is
-    diagonalized (rocSOLVER zheevj); above, thick-restart block Lanczos."""
+    diagonalized (Householder reduction on the GPU, tridiagonal QL, the Npsi lowest
+    eigenvectors back-transformed on the GPU); above, thick-restart block Lanczos."""
     o = _options(options, {"Restrictions": None, "Zero": 100 * SQRT_EPS, "Epsilon": SQRT_EPS / 100,
                            "NKrylovMax": 500, "NKrylovStart": 100, "NKrylovStep": 10, "NKrylovMin": 50,
                            "DenseBorder": 1513, "NBitsKey": 16, "ShiftScheme": "Variance", "Seed": 1,
@@ -492,8 +493,8 @@ def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
         # dense: the determinant matrix and its eigenvectors
         # (BlockLanczosGroundStateConserveBasisDense, Core/BlockLanczos.cpp:370-504, 5713)
         Hd = Hop(H.device(ctx), basis, mode="matfree").dense()
-        dense_poles(Hd, None, ctx)
-        return [Wavefunction(nf, basis=basis, vec=Hd[:, k].contiguous()) for k in range(Npsi)]
+        _, V = dense_lowest(Hd, Npsi, ctx)
+        return [Wavefunction(nf, basis=basis, vec=V[k].contiguous()) for k in range(Npsi)]
     hop = Hop(H.device(ctx), basis, mode="auto")
     E, vecs, var, it = hop.ground_states(Npsi, tol=float(o["Zero"]), max_krylov=int(o["NKrylovMax"]),
                                          max_restarts=int(o["MaxRestarts"]), seed=int(o["Seed"]),
@@ -698,7 +699,7 @@ def _dense_spectra(H, cb, tpsi, psiEn, o, ctx):
     X = torch.from_numpy(Xh).to(dev)
     hop = Hop(H.device(ctx), cb, mode="matfree")
     Hd = hop.dense()
-    E, P = dense_poles(Hd, X, ctx)
+    E, P = dense_poles(Hd, X, ctx, vectors=bool(o["KrylovBasis"]))
     ne = int(o["NE"])
     gamma = float(o["Gamma"])
     if not o["Tensor"]:

@@ -24,6 +24,60 @@ namespace {
 // A: column-major n*n (A[i + j*n] = row i, col j), Hermitian; only its lower
 // triangle is read. Returns all eigenvalues ascending in w and the eigenvectors
 // of the nvec lowest in Z (column-major n x nvec).
+// Implicit QL with Wilkinson-type shifts on the symmetric tridiagonal matrix
+// (d[0..n-1], e[0..n-2]; e has n entries, e[n-1] ignored): d receives the
+// eigenvalues (unsorted), e is destroyed. Every plane rotation acting on
+// components (i, i+1) is handed to rot(i, c, s) in the order it is made: with
+// Z = G_1 G_2 ... G_K the eigenvector matrix, the row update
+// (r_i, r_{i+1}) <- (c r_i - s r_{i+1}, s r_i + c r_{i+1}) turns any block R
+// into Z^T R. False when an eigenvalue takes more than 200 iterations.
+template <class F>
+bool ql_rotations(int n, double *d, double *e, F &&rot) {
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    while (true) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[m]) <= eps * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 200) return false;
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = std::hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      bool early = false;
+      for (int i = m - 1; i >= l; --i) {
+        double f = s * e[i];
+        const double b = c * e[i];
+        r = std::hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= pp;
+          e[m] = 0.0;
+          early = true;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - pp;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        pp = s * r;
+        d[i + 1] = g + pp;
+        g = c * r - b;
+        rot(i, c, s);
+      }
+      if (early) continue;
+      d[l] -= pp;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  return true;
+}
+
 bool heev_impl(int n, const cd *Ain, double *w, cd *Z, int nvec) {
   if (n <= 0) return true;
   nvec = std::max(0, std::min(nvec, n));
@@ -117,59 +171,21 @@ bool heev_impl(int n, const cd *Ain, double *w, cd *Z, int nvec) {
   }
   std::vector<int> rot_i;
   std::vector<double> rot_c, rot_s;
-  const double eps = std::numeric_limits<double>::epsilon();
-  for (int l = 0; l < n; ++l) {
-    int iter = 0;
-    while (true) {
-      int m = l;
-      for (; m < n - 1; ++m) {
-        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-        if (std::fabs(e[m]) <= eps * dd) break;
+  const bool ok = ql_rotations(n, d.data(), e.data(), [&](int i, double c, double s) {
+    if (accumulate) {
+      double *z0 = &Zc[(size_t)i * n], *z1 = &Zc[(size_t)(i + 1) * n];
+      for (int k2 = 0; k2 < n; ++k2) {
+        const double f1 = z1[k2];
+        z1[k2] = s * z0[k2] + c * f1;
+        z0[k2] = c * z0[k2] - s * f1;
       }
-      if (m == l) break;
-      if (++iter > 200) return false;
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = std::hypot(g, 1.0);
-      g = d[m] - d[l] + e[l] / (g + std::copysign(r, g));
-      double s = 1.0, c = 1.0, pp = 0.0;
-      bool early = false;
-      for (int i = m - 1; i >= l; --i) {
-        double f = s * e[i];
-        const double b = c * e[i];
-        r = std::hypot(f, g);
-        e[i + 1] = r;
-        if (r == 0.0) {
-          d[i + 1] -= pp;
-          e[m] = 0.0;
-          early = true;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = d[i + 1] - pp;
-        r = (d[i] - g) * s + 2.0 * c * b;
-        pp = s * r;
-        d[i + 1] = g + pp;
-        g = c * r - b;
-        if (accumulate) {
-          double *z0 = &Zc[(size_t)i * n], *z1 = &Zc[(size_t)(i + 1) * n];
-          for (int k2 = 0; k2 < n; ++k2) {
-            const double f1 = z1[k2];
-            z1[k2] = s * z0[k2] + c * f1;
-            z0[k2] = c * z0[k2] - s * f1;
-          }
-        } else if (nvec > 0) {
-          rot_i.push_back(i);
-          rot_c.push_back(c);
-          rot_s.push_back(s);
-        }
-      }
-      if (early) continue;
-      d[l] -= pp;
-      e[l] = g;
-      e[m] = 0.0;
+    } else if (nvec > 0) {
+      rot_i.push_back(i);
+      rot_c.push_back(c);
+      rot_s.push_back(s);
     }
-  }
+  });
+  if (!ok) return false;
   std::vector<int> ord(n);
   std::iota(ord.begin(), ord.end(), 0);
   std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
@@ -219,6 +235,63 @@ bool heev_impl(int n, const cd *Ain, double *w, cd *Z, int nvec) {
 }
 
 }  // namespace
+
+bool qs_tridiag_project(int n, const double *d_in, const double *e_in, int nv, const cd *Y, double *w, cd *P) {
+  if (n <= 0) return true;
+  std::vector<double> d(d_in, d_in + n), e(n, 0.0);
+  for (int i = 0; i + 1 < n; ++i) e[i] = e_in[i];
+  std::vector<cd> R(Y, Y + (size_t)n * nv);  // row i: component i of every vector
+  const bool ok = ql_rotations(n, d.data(), e.data(), [&](int i, double c, double s) {
+    cd *r0 = &R[(size_t)i * nv], *r1 = &R[(size_t)(i + 1) * nv];
+    for (int k = 0; k < nv; ++k) {
+      const cd f1 = r1[k];
+      r1[k] = s * r0[k] + c * f1;
+      r0[k] = c * r0[k] - s * f1;
+    }
+  });
+  if (!ok) return false;
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
+  for (int k = 0; k < n; ++k) {
+    w[k] = d[ord[k]];
+    std::copy(&R[(size_t)ord[k] * nv], &R[(size_t)ord[k] * nv] + nv, P + (size_t)k * nv);
+  }
+  return true;
+}
+
+bool qs_tridiag_lowest_vectors(int n, const double *d_in, const double *e_in, int k, double *w, double *Z) {
+  if (n <= 0) return true;
+  k = std::max(0, std::min(k, n));
+  std::vector<double> d(d_in, d_in + n), e(n, 0.0);
+  for (int i = 0; i + 1 < n; ++i) e[i] = e_in[i];
+  std::vector<int> rot_i;
+  std::vector<double> rot_c, rot_s;
+  const bool ok = ql_rotations(n, d.data(), e.data(), [&](int i, double c, double s) {
+    rot_i.push_back(i);
+    rot_c.push_back(c);
+    rot_s.push_back(s);
+  });
+  if (!ok) return false;
+  std::vector<int> ord(n);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d[a] < d[b]; });
+  for (int j = 0; j < n; ++j) w[j] = d[ord[j]];
+  for (int j = 0; j < k; ++j) {
+    // Z = G_1 G_2 ... G_K, so Z e_j applies G_K first
+    double *x = Z + (size_t)j * n;
+    std::fill(x, x + n, 0.0);
+    x[ord[j]] = 1.0;
+    for (size_t t = rot_i.size(); t-- > 0;) {
+      const int i = rot_i[t];
+      const double c = rot_c[t], s = rot_s[t];
+      const double x0 = x[i], x1 = x[i + 1];
+      x[i] = c * x0 + s * x1;
+      x[i + 1] = -s * x0 + c * x1;
+    }
+  }
+  return true;
+}
 
 void qs_heev(int n, const double *A, double *w, double *Z) {
   heev_impl(n, reinterpret_cast<const cd *>(A), w, reinterpret_cast<cd *>(Z), n);

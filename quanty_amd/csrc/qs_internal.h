@@ -8,6 +8,7 @@
 #include <map>
 #include <string>
 #include <utility>
+#include <complex>
 #include <vector>
 
 #include "../../include/qs.h"
@@ -41,6 +42,18 @@ struct qs_ctx {
 
 // A basis holding n sorted, unique determinants copied from device memory.
 qs_status qs_basis_from_device_words(qs_ctx *ctx, int nf, const uint64_t *d_words, int64_t n, qs_basis **out);
+
+// Eigenvalues of the real symmetric tridiagonal matrix (d[0..n-1], e[0..n-2])
+// in ascending order (w) and the projections of nv vectors on its
+// eigenvectors: Y holds row i = component i of every vector (n x nv complex,
+// row-major); P row k = z_k^T Y for the k-th eigenvalue (qs_dense.cpp).
+bool qs_tridiag_project(int n, const double *d, const double *e, int nv, const std::complex<double> *Y, double *w,
+                        std::complex<double> *P);
+
+// Eigenvalues (ascending, w) of the real symmetric tridiagonal matrix (d, e)
+// and the eigenvectors of its k lowest, Z column j = z_j (n x k, column-major)
+// (qs_dense.cpp).
+bool qs_tridiag_lowest_vectors(int n, const double *d, const double *e, int k, double *w, double *Z);
 
 // Releases ctx->blas (qs_spectra.hip).
 void qs_blas_release(qs_ctx *ctx);

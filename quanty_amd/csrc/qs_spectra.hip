@@ -281,11 +281,126 @@ __global__ void k_poles_tensor(int npole, int nv, const double *__restrict__ E, 
 
 }  // namespace
 
-extern "C" qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const double *X, double *E, double *P) {
+// Y <- Q^+ Y for the Householder reduction of zhetrd (lower): Q = H_0 H_1 ...
+// H_{n-2}, H_k = I - tau_k v_k v_k^+, v_k = (0..0, 1, A[k+2..n-1, k]), so
+// Q^+ Y applies H_0^+ first. One workgroup per vector, reflectors in order,
+// the dot products summed in a fixed order; tau_k = 0 (a column already
+// reduced) is skipped. rocSOLVER's zunmtr returns NaN when some tau_k is 0,
+// as in block-diagonal (symmetric) Hamiltonians.
+__global__ __launch_bounds__(256) void k_apply_qh(int n, const double2 *__restrict__ A,
+                                                  const double2 *__restrict__ tau, double2 *__restrict__ Y) {
+  __shared__ double red[4];
+  double2 *y = Y + (size_t)blockIdx.x * n;
+  for (int k = 0; k + 1 < n; ++k) {
+    const double2 t = tau[k];
+    if (t.x == 0.0 && t.y == 0.0) continue;
+    const double2 *col = A + (size_t)k * n;
+    double sr = 0.0, si = 0.0;  // v^+ y
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) {
+      const double2 v = i == k + 1 ? make_double2(1.0, 0.0) : col[i];
+      const double2 w = y[i];
+      sr += v.x * w.x + v.y * w.y;
+      si += v.x * w.y - v.y * w.x;
+    }
+    sr = block_sum_values(sr, red);
+    si = block_sum_values(si, red);
+    // y -= conj(tau) v (v^+ y)
+    const double fr = t.x * sr + t.y * si, fi = t.x * si - t.y * sr;
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) {
+      const double2 v = i == k + 1 ? make_double2(1.0, 0.0) : col[i];
+      double2 w = y[i];
+      w.x -= v.x * fr - v.y * fi;
+      w.y -= v.x * fi + v.y * fr;
+      y[i] = w;
+    }
+    __syncthreads();
+  }
+}
+
+// Y <- Q Y (the back-transform of tridiagonal eigenvectors): H_{n-2} first,
+// y -= tau v (v^+ y). Layout and order of sums as in k_apply_qh.
+__global__ __launch_bounds__(256) void k_apply_q(int n, const double2 *__restrict__ A,
+                                                 const double2 *__restrict__ tau, double2 *__restrict__ Y) {
+  __shared__ double red[4];
+  double2 *y = Y + (size_t)blockIdx.x * n;
+  for (int k = n - 2; k >= 0; --k) {
+    const double2 t = tau[k];
+    if (t.x == 0.0 && t.y == 0.0) continue;
+    const double2 *col = A + (size_t)k * n;
+    double sr = 0.0, si = 0.0;  // v^+ y
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) {
+      const double2 v = i == k + 1 ? make_double2(1.0, 0.0) : col[i];
+      const double2 w = y[i];
+      sr += v.x * w.x + v.y * w.y;
+      si += v.x * w.y - v.y * w.x;
+    }
+    sr = block_sum_values(sr, red);
+    si = block_sum_values(si, red);
+    const double fr = t.x * sr - t.y * si, fi = t.x * si + t.y * sr;  // tau (v^+ y)
+    for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) {
+      const double2 v = i == k + 1 ? make_double2(1.0, 0.0) : col[i];
+      double2 w = y[i];
+      w.x -= v.x * fr - v.y * fi;
+      w.y -= v.x * fi + v.y * fr;
+      y[i] = w;
+    }
+    __syncthreads();
+  }
+}
+
+// Eigenvalues and projections only: Householder reduction A = Q T Q^+ on the
+// GPU (zhetrd; T real symmetric tridiagonal), Y = Q^+ X on the GPU
+// (k_apply_qh), then the tridiagonal QL on the host with its rotations applied
+// to the rows of Y, which turns Y into Z^T Y for the eigenvectors Z of T.
+static qs_status dense_poles_householder(qs_ctx *ctx, rocblas_handle hb, int n, double *A, int nv, const double *X,
+                                         double *E, double *P) {
+  char *mem = nullptr;
+  const size_t ybytes = (size_t)n * std::max(nv, 1) * 16;
+  QS_TRY(qs_malloc(ctx, (void **)&mem, ybytes + (size_t)n * 16 + (size_t)n * 16));
+  double2 *dY = (double2 *)mem;
+  double2 *dtau = (double2 *)(mem + ybytes);
+  double *dd = (double *)(mem + ybytes + (size_t)n * 16), *de = dd + n;
+  qs_status st = QS_OK;
+  hipError_t e = nv ? hipMemcpyAsync(dY, X, (size_t)n * nv * 16, hipMemcpyDeviceToDevice, ctx->stream) : hipSuccess;
+  if (e == hipSuccess) {
+    const rocblas_status rs = rocsolver_zhetrd(hb, rocblas_fill_lower, n, (rocblas_double_complex *)A, n, dd, de,
+                                               (rocblas_double_complex *)dtau);
+    if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zhetrd failed (status %d)", (int)rs);
+  }
+  if (e == hipSuccess && st == QS_OK && nv > 0) {
+    hipLaunchKernelGGL(k_apply_qh, dim3((unsigned)nv), dim3(256), 0, ctx->stream, n, (const double2 *)A,
+                       (const double2 *)dtau, dY);
+    e = hipGetLastError();
+  }
+  std::vector<double> hd(n), he(n);
+  std::vector<std::complex<double>> hy((size_t)n * nv);
+  if (e == hipSuccess && st == QS_OK) e = hipMemcpyAsync(hd.data(), dd, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && st == QS_OK && n > 1)
+    e = hipMemcpyAsync(he.data(), de, (size_t)(n - 1) * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && st == QS_OK && nv > 0)
+    e = hipMemcpyAsync(hy.data(), dY, (size_t)n * nv * 16, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && st == QS_OK) e = hipStreamSynchronize(ctx->stream);
+  else (void)hipStreamSynchronize(ctx->stream);
+  qs_free(mem);
+  if (st != QS_OK) return st;
+  if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_dense_poles", __FILE__, __LINE__);
+  // rows of Y: component i of every vector (hy holds vector s at s*n)
+  std::vector<std::complex<double>> rows((size_t)n * nv), proj((size_t)n * nv);
+  for (int sv = 0; sv < nv; ++sv)
+    for (int i = 0; i < n; ++i) rows[(size_t)i * nv + sv] = hy[(size_t)sv * n + i];
+  if (!qs_tridiag_project(n, hd.data(), he.data(), nv, rows.data(), E, proj.data()))
+    return qs_fail(ctx, QS_ENOTCONVERGED, "the tridiagonal QL did not converge");
+  std::copy(proj.begin(), proj.end(), reinterpret_cast<std::complex<double> *>(P));
+  return QS_OK;
+}
+
+extern "C" qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const double *X, double *E, double *P,
+                                    int vectors) {
   if (!ctx || n < 1 || !A || nv < 0 || !E || (nv > 0 && (!X || !P))) return QS_EINVAL;
   QS_HIP(ctx, hipSetDevice(ctx->device));
   rocblas_handle hb = nullptr;
   QS_TRY(blas_handle(ctx, &hb));
+  if (!vectors) return dense_poles_householder(ctx, hb, n, A, nv, X, E, P);
   char *mem = nullptr;
   const size_t pbytes = (size_t)n * nv * 16;
   QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)n * 8 + 256 + pbytes));
@@ -320,6 +435,47 @@ extern "C" qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const
   if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_dense_poles", __FILE__, __LINE__);
   if (hinfo != 0)
     return qs_fail(ctx, QS_ENOTCONVERGED, "zheevj did not converge in %d sweeps", QS_JACOBI_SWEEPS);
+  return QS_OK;
+}
+
+extern "C" qs_status qs_dense_lowest(qs_ctx *ctx, int n, double *A, int k, double *E, double *V) {
+  if (!ctx || n < 1 || !A || k < 0 || k > n || !E || (k > 0 && !V)) return QS_EINVAL;
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  rocblas_handle hb = nullptr;
+  QS_TRY(blas_handle(ctx, &hb));
+  char *mem = nullptr;
+  QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)n * 32));
+  double2 *dtau = (double2 *)mem;
+  double *dd = (double *)(mem + (size_t)n * 16), *de = dd + n;
+  qs_status st = QS_OK;
+  const rocblas_status rs = rocsolver_zhetrd(hb, rocblas_fill_lower, n, (rocblas_double_complex *)A, n, dd, de,
+                                             (rocblas_double_complex *)dtau);
+  if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zhetrd failed (status %d)", (int)rs);
+  std::vector<double> hd(n), he(n);
+  hipError_t e = hipSuccess;
+  if (st == QS_OK) e = hipMemcpyAsync(hd.data(), dd, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (st == QS_OK && e == hipSuccess && n > 1)
+    e = hipMemcpyAsync(he.data(), de, (size_t)(n - 1) * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (st == QS_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  else (void)hipStreamSynchronize(ctx->stream);
+  std::vector<double> z((size_t)n * k);
+  if (st == QS_OK && e == hipSuccess && !qs_tridiag_lowest_vectors(n, hd.data(), he.data(), k, E, z.data()))
+    st = qs_fail(ctx, QS_ENOTCONVERGED, "the tridiagonal QL did not converge");
+  if (st == QS_OK && e == hipSuccess && k > 0) {
+    std::vector<double2> zc((size_t)n * k);
+    for (size_t t = 0; t < zc.size(); ++t) zc[t] = make_double2(z[t], 0.0);
+    e = hipMemcpyAsync(V, zc.data(), zc.size() * 16, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_apply_q, dim3((unsigned)k), dim3(256), 0, ctx->stream, n, (const double2 *)A,
+                         (const double2 *)dtau, (double2 *)V);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    else (void)hipStreamSynchronize(ctx->stream);
+  }
+  qs_free(mem);
+  if (st != QS_OK) return st;
+  if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_dense_lowest", __FILE__, __LINE__);
   return QS_OK;
 }
 

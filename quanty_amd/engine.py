@@ -326,11 +326,12 @@ def broaden(G: np.ndarray, dE: float, kind: str, fwhm, ctx: Context | None = Non
     return re + 1j * im
 
 
-def dense_poles(Hd: torch.Tensor, X: torch.Tensor | None = None, ctx: Context | None = None):
-    """Eigen-decomposition of the dense Hermitian matrix Hd (from Hop.dense(); it is
-    overwritten by the eigenvectors: afterwards Hd[:, k] = z_k) and projections of
-    the vectors X (nv, n) on them. Returns E (n,) ascending and P (n, nv) with
-    P[k, s] = <z_k|x_s>."""
+def dense_poles(Hd: torch.Tensor, X: torch.Tensor | None = None, ctx: Context | None = None, vectors: bool = True):
+    """Eigen-decomposition of the dense Hermitian matrix Hd (from Hop.dense()) and
+    projections of the vectors X (nv, n) on the eigenvectors. Returns E (n,)
+    ascending and P (n, nv) with P[k, s] = <z_k|x_s>. vectors: Hd is overwritten
+    by the eigenvectors (Hd[:, k] = z_k); otherwise only E and P are made
+    (faster; Hd is overwritten by the Householder reduction)."""
     ctx = ctx or default_context()
     n = Hd.shape[0]
     st = Hd.t()
@@ -342,9 +343,24 @@ def dense_poles(Hd: torch.Tensor, X: torch.Tensor | None = None, ctx: Context | 
     E = np.zeros(n)
     P = np.zeros((n, max(nv, 1)), dtype=np.complex128)
     _torch_ready(Hd.device)
-    ctx.check(ctx.lib.qs_dense_poles(ctx.h, n, ptr(st), nv, ptr(X) if nv else None, ptr(E), ptr(P)),
+    ctx.check(ctx.lib.qs_dense_poles(ctx.h, n, ptr(st), nv, ptr(X) if nv else None, ptr(E), ptr(P), int(vectors)),
               "qs_dense_poles")
     return E, P[:, :nv]
+
+
+def dense_lowest(Hd: torch.Tensor, k: int, ctx: Context | None = None):
+    """All eigenvalues (ascending) of the dense Hermitian matrix Hd (from
+    Hop.dense(); overwritten) and the eigenvectors of the k lowest as a (k, n)
+    complex128 device tensor."""
+    ctx = ctx or default_context()
+    n = Hd.shape[0]
+    st = Hd.t()
+    assert Hd.shape == (n, n) and st.is_contiguous() and Hd.dtype == torch.complex128
+    E = np.zeros(n)
+    V = torch.empty((max(k, 1), n), dtype=torch.complex128, device=Hd.device)
+    _torch_ready(Hd.device)
+    ctx.check(ctx.lib.qs_dense_lowest(ctx.h, n, ptr(st), k, ptr(E), ptr(V)), "qs_dense_lowest")
+    return E, V[:k]
 
 
 def pole_spectrum(E: np.ndarray, P: np.ndarray, E0, emin: float, emax: float, ne: int, gamma: float,

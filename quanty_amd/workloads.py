@@ -77,7 +77,8 @@ def config0(reps: int = 5):
 def dense_small(seed: int = 5, reps: int = 5):
     """The dense small-problem path on a 1,410-determinant impurity (d + 1 bath
     shell, 16 electrons): ground state, one d<->bath transition, CreateSpectra by
-    the dense method (closure, determinant matrix, zheevj, poles) against
+    the dense method (closure, determinant matrix, Householder reduction and
+    tridiagonal QL, poles) against
     200-step Lanczos, 2001 points."""
     from .api import CreateSpectra, Operator, Wavefunction
 
@@ -97,10 +98,14 @@ def dense_small(seed: int = 5, reps: int = 5):
     sl, _ = CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]])
     _, td = _sync_time(lambda: [CreateSpectra(H, T, psi, opts) for _ in range(reps)])
     _, tl = _sync_time(lambda: [CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]]) for _ in range(reps)])
-    # stages of the dense method
+    # stages of the dense method: matrix, then eigenvalues + projections
+    # (Householder, as CreateSpectra uses), or every eigenvector (Jacobi, as
+    # KrylovBasis = true needs)
     from .engine import dense_poles
     Hd, t_build = _sync_time(lambda: Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense())
-    _, t_eig = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx))
+    _, t_poles = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=False))
+    Hd = Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense()
+    _, t_eig = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=True))
     diff = float(np.max(np.abs(sd.G - sl.G)) / np.max(np.abs(sd.G)))
     # broadening of the spectrum: Gaussian 0.5, Lorentzian from an energy table
     table = [[-10.0, 0.2], [3.0, 0.2], [3.01, 0.6], [20.0, 0.6]]
@@ -108,7 +113,8 @@ def dense_small(seed: int = 5, reps: int = 5):
     _, tb = _sync_time(lambda: [sd.Broaden(0.5, table) for _ in range(reps)])
     return {"config": "dense", "dim": basis.n, "seconds_dense": td / reps, "seconds_lanczos": tl / reps,
             "seconds_broaden_2001_points": tb / reps,
-            "seconds_dense_matrix": t_build, "seconds_zheevj_and_projection": t_eig,
+            "seconds_dense_matrix": t_build, "seconds_householder_poles": t_poles,
+            "seconds_jacobi_all_eigenvectors": t_eig,
             "lanczos200_vs_dense_max_rel_diff": diff}
 
 

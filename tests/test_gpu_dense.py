@@ -276,6 +276,40 @@ def test_eigensystem_dense_equals_krylov(ctx):
         assert torch.linalg.norm(y - e * p._vec).item() < 1e-11 * max(1.0, abs(e))
 
 
+@pytest.mark.parametrize("degenerate", [False, True])
+def test_dense_poles_householder_path_equals_jacobi(degenerate, ctx):
+    """qs_dense_poles without eigenvectors (Householder + tridiagonal QL with the
+    rotations applied to Q^+ X) against the Jacobi path and numpy: the same
+    eigenvalues, and the same spectra (degenerate poles may split their weight
+    differently; the spectra see only the sums)."""
+    n = 300
+    rng = np.random.default_rng(11 + degenerate)
+    if degenerate:
+        A = rng.normal(size=(n // 3, n // 3)) + 1j * rng.normal(size=(n // 3, n // 3))
+        M = np.kron(np.eye(3), A + A.conj().T)
+    else:
+        B = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        M = B + B.conj().T
+    Xh = rng.normal(size=(4, n)) + 1j * rng.normal(size=(4, n))
+    X = torch.from_numpy(Xh).cuda()
+    out = {}
+    for vec in (True, False):
+        T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+        out[vec] = dense_poles(T.t(), X, ctx, vectors=vec)
+    E1, P1 = out[True]
+    E0, P0 = out[False]
+    scale = np.max(np.abs(E1))
+    assert np.max(np.abs(E0 - E1)) < 1e-12 * scale
+    assert np.max(np.abs(E0 - np.linalg.eigvalsh(M))) < 1e-12 * scale
+    assert np.allclose((np.abs(P0) ** 2).sum(0), (np.abs(Xh) ** 2).sum(1), rtol=1e-12)
+    G1 = pole_spectrum(E1, P1, 0.0, -40.0, 40.0, 400, 0.5, ctx=ctx)
+    G0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 400, 0.5, ctx=ctx)
+    assert np.max(np.abs(G0 - G1)) < 1e-10 * np.max(np.abs(G1))
+    Gt1 = pole_spectrum(E1, P1, 0.0, -40.0, 40.0, 400, 0.5, tensor=True, ctx=ctx)
+    Gt0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 400, 0.5, tensor=True, ctx=ctx)
+    assert np.max(np.abs(Gt0 - Gt1)) < 1e-10 * np.max(np.abs(Gt1))
+
+
 @pytest.mark.parametrize("n", [100, 400])
 def test_dense_poles_degenerate_spectrum(n, ctx):
     """Exactly degenerate eigenvalues (two copies of one block): every eigenvector
@@ -319,3 +353,25 @@ def test_dense_two_word_determinants_match_matrix_free(ctx):
     sd, _ = CreateSpectra(H, T, psi, opts)
     sl, _ = CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]])
     assert np.max(np.abs(sd.G - sl.G)) <= 1e-8 * np.max(np.abs(sl.G))
+
+
+@pytest.mark.parametrize("degenerate", [False, True])
+def test_dense_lowest_eigenvectors(degenerate, ctx):
+    """qs_dense_lowest: every eigenvalue, and the k lowest eigenvectors
+    orthonormal with H z = E z at rounding level (degenerate included)."""
+    from quanty_amd.engine import dense_lowest
+    n, k = 240, 7
+    rng = np.random.default_rng(21 + degenerate)
+    if degenerate:
+        A = rng.normal(size=(n // 2, n // 2)) + 1j * rng.normal(size=(n // 2, n // 2))
+        M = np.kron(np.eye(2), A + A.conj().T)
+    else:
+        B = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        M = B + B.conj().T
+    T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+    E, V = dense_lowest(T.t(), k, ctx)
+    Z = V.cpu().numpy().T  # columns z_j
+    scale = np.max(np.abs(E))
+    assert np.max(np.abs(E - np.linalg.eigvalsh(M))) < 1e-12 * scale
+    assert np.max(np.abs(M @ Z - Z * E[None, :k])) < 1e-11 * scale
+    assert np.max(np.abs(Z.conj().T @ Z - np.eye(k))) < 1e-12
