@@ -1,8 +1,10 @@
 // Spectra on the GPU outside the Lanczos recurrence:
-//  - the dense small-problem path: eigen-decomposition of a dense Hermitian
-//    matrix (rocSOLVER zheevj, the counterpart of the reference's LAPACK
-//    zheev), projections of the transition vectors on the eigenvectors, and
-//    spectra from the resulting lists of poles (ListOfPolesToSpectrum,
+//  - the dense small-problem path (the counterpart of the reference's LAPACK
+//    zheev): eigenvalues of a dense Hermitian matrix and the projections of
+//    the transition vectors on its eigenvectors (Householder reduction on the
+//    GPU, tridiagonal QL on the host), the lowest eigenvectors, or all of them
+//    (rocSOLVER zheevj), and spectra from the resulting lists of poles
+//    (ListOfPolesToSpectrum,
 //    Core/Spectral.cpp:1516-1591; BlockListOfPolesToSpectraWithBroadening,
 //    Core/BasicMathBlock/BasicMath_BlockTypeConversion.cpp:588-660);
 //  - Gaussian and Lorentzian broadening of spectra on an equidistant energy
