@@ -726,6 +726,24 @@ def _dense_spectra(H, cb, tpsi, psiEn, o, ctx):
     return tuple(out)
 
 
+def _tri_spectra(A, Bv, psiEn, o, ctx):
+    """Spectra and shifted Tri tables of Lanczos coefficients (LuaInterface/ssp.cpp:2640-2757)."""
+    nsp = A.shape[0]
+    gamma, ne = float(o["Gamma"]), int(o["NE"])
+    lo, hi = zip(*[_pole_range(A[i], Bv[i], psiEn[i]) for i in range(nsp)])
+    eminpole, emaxpole = min(lo), max(hi)
+    emin, emax = _default_range(o, eminpole, emaxpole)
+    if gamma == 0:
+        gamma = 10 * (emax - emin) / ne
+    G = cf_spectrum(A, Bv, np.array(psiEn), float(emin), float(emax), ne, gamma, ctx)
+    tables = []
+    for i in range(nsp):
+        a = A[i].copy()
+        a[1:] -= psiEn[i]  # ShiftTriDiagonalMatrix shifts a[1..N] (BasicMath_TriDiagonalMatrixVector.cpp:133)
+        tables.append(TriResponse(a, Bv[i], 0.0, f"Spectrum {i}"))
+    return Spectra(float(emin), float(emax), ne, gamma, G, eminpole, emaxpole), tables
+
+
 def CreateSpectra(H: Operator, T, psi, options=None):
     """Spectra of transition operators T on wavefunctions psi (LuaCreateSpectra,
     LuaInterface/ssp.cpp:1906): E0 = <psi|H|psi> unless given, T psi (restricted
@@ -775,6 +793,21 @@ def CreateSpectra(H: Operator, T, psi, options=None):
                 return _dense_spectra(H, cb, tpsi, psiEn, o, ctx)
     # the space H explores from T psi: fix the occupation of every group of modes H conserves
     groups = conserved_groups(H)
+    nsp = len(tpsi)
+    # an operator that annihilates psi gives an empty T psi: the reference's
+    # Lanczos then has b[0] = 0 and the spectrum vanishes (Core/Lanczos.cpp:2189-2191)
+    zero = [len(tp.host()[0]) == 0 for tp in tpsi]
+    if all(zero):
+        A, Bv = np.zeros((nsp, ntri + 1)), np.zeros((nsp, ntri))
+        if o["Tensor"]:
+            emin = -10.0 if o["Emin"] is None else float(o["Emin"])
+            emax = 10.0 if o["Emax"] is None else float(o["Emax"])
+            ne = int(o["NE"])
+            gamma = float(o["Gamma"]) or 10 * (emax - emin) / ne
+            return (Spectra(emin, emax, ne, gamma, np.zeros((nsp * nsp, ne + 1), dtype=np.complex128)),
+                    {"A": np.zeros((ntri + 1, nsp, nsp), dtype=np.complex128),
+                     "B": np.zeros((ntri, nsp, nsp), dtype=np.complex128), "kept": np.zeros(ntri + 1, np.int32)})
+        return _tri_spectra(A, Bv, psiEn, o, ctx)
     cv = set()
     for tp in tpsi:
         w, _ = tp.host()
@@ -792,25 +825,13 @@ def CreateSpectra(H: Operator, T, psi, options=None):
             v[idx] = c
         tpsi[i] = Wavefunction(nf, basis=rows, vec=torch.from_numpy(v).to(torch.device("cuda", ctx.device)))
     hop = Hop(H.device(ctx), rows, mode="auto")
-    nsp = len(tpsi)
     if not o["Tensor"]:
         A = np.zeros((nsp, ntri + 1))
         Bv = np.zeros((nsp, ntri))
         for i, tp in enumerate(tpsi):
-            A[i], Bv[i] = hop.lanczos(tp._vec.contiguous(), ntri)
-        gamma, ne = float(o["Gamma"]), int(o["NE"])
-        lo, hi = zip(*[_pole_range(A[i], Bv[i], psiEn[i]) for i in range(nsp)])
-        eminpole, emaxpole = min(lo), max(hi)
-        emin, emax = _default_range(o, eminpole, emaxpole)
-        if gamma == 0:
-            gamma = 10 * (emax - emin) / ne
-        G = cf_spectrum(A, Bv, np.array(psiEn), float(emin), float(emax), ne, gamma, ctx)
-        tables = []
-        for i in range(nsp):
-            a = A[i].copy()
-            a[1:] -= psiEn[i]  # ShiftTriDiagonalMatrix shifts a[1..N] (BasicMath_TriDiagonalMatrixVector.cpp:133)
-            tables.append(TriResponse(a, Bv[i], 0.0, f"Spectrum {i}"))
-        return Spectra(float(emin), float(emax), ne, gamma, G, eminpole, emaxpole), tables
+            if not zero[i]:  # an empty T psi keeps its zero table
+                A[i], Bv[i] = hop.lanczos(tp._vec.contiguous(), ntri)
+        return _tri_spectra(A, Bv, psiEn, o, ctx)
     # tensor: one block Lanczos over all T psi, (nsp x nsp) spectra
     if len(set(psiEn)) != 1:
         raise QSError("Tensor spectra need one E0 for all wavefunctions (pass E0)")

@@ -375,3 +375,25 @@ def test_dense_lowest_eigenvectors(degenerate, ctx):
     assert np.max(np.abs(E - np.linalg.eigvalsh(M))) < 1e-12 * scale
     assert np.max(np.abs(M @ Z - Z * E[None, :k])) < 1e-11 * scale
     assert np.max(np.abs(Z.conj().T @ Z - np.eye(k))) < 1e-12
+
+
+def test_create_spectra_with_annihilated_psi(ctx):
+    """A transition operator that annihilates psi gives a zero spectrum and a
+    zero table, alone and beside a non-zero one, by both methods and in the
+    tensor form (the reference's Lanczos on an empty wavefunction has b[0] = 0)."""
+    nf, terms = chain(3, 0)
+    H = Operator(nf, dict(terms))
+    psi = NewWavefunction(nf, 0, [("10" + "00" * 3, 1.0)])
+    kill = NewOperator("Cr", nf, 0)   # mode 0 is occupied
+    live = NewOperator("Cr", nf, 1)
+    opts = [["Emin", -6.0], ["Emax", 8.0], ["NE", 100], ["Gamma", 0.3], ["NTri", 20], ["E0", 0.0]]
+    for method in ("Auto", "Lanczos"):
+        s0, t0 = CreateSpectra(H, kill, psi, opts + [["Method", method]])
+        assert np.all(s0.G == 0) and np.all(np.isfinite(s0.G))
+        assert np.all(t0[0]["b"] == 0)
+        s2, _ = CreateSpectra(H, [kill, live], psi, opts + [["Method", method]])
+        s1, _ = CreateSpectra(H, live, psi, opts + [["Method", method]])
+        assert np.all(s2.G[0] == 0)
+        assert np.max(np.abs(s2.G[1] - s1.G[0])) <= 1e-12 * np.max(np.abs(s1.G[0]))
+    st, _ = CreateSpectra(H, kill, psi, opts + [["Tensor", True]])
+    assert np.all(st.G == 0)
