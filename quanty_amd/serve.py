@@ -161,6 +161,8 @@ class _Job:
         self.gs_depth = min(gs_depth, self.basis.n)
         self.phase = "gs1"
         self.gs_prev = None  # (lowest Ritz value, residual) of the last ground-state check
+        self.restarts = 0  # ground-state restarts from the Ritz vector when the kept basis is full
+        self.steps_done = 0  # ground-state steps of the runs before the last restart
         self.slot = -1
         self.bytes = self.hop.stats()["stream_bytes"] + 6 * 16 * self.basis.n
         self.kept_bytes = 0  # Lanczos vectors kept during the ground-state phase
@@ -231,7 +233,8 @@ class SpectrumServer:
     admitted requests together."""
 
     def __init__(self, ctx, max_slots: int = 64, hbm_budget: float = 200e9, gs_depth: int = 60,
-                 gs_max: int = 500, gs_step: int = 16, gs_tol: float = 1e-12, chunk: int = 8, ne: int = 2000,
+                 gs_max: int = 500, gs_step: int = 16, gs_tol: float = 1e-12, gs_restarts: int = 8,
+                 chunk: int = 8, ne: int = 2000,
                  emin: float = -10.0, emax: float = 20.0, keep_basis: bool = True, prepare_ahead: bool = True,
                  prep_workers: int = 2, start_temperature: float | None = 0.3, start_floor: float = 1e-3):
         self.ctx = ctx
@@ -239,6 +242,7 @@ class SpectrumServer:
         self.budget = hbm_budget
         self.gs_depth = gs_depth
         self.gs_max = gs_max
+        self.gs_restarts = gs_restarts
         self.gs_step = gs_step
         self.gs_tol = gs_tol
         self.chunk = chunk
@@ -271,13 +275,26 @@ class SpectrumServer:
             E, z = tridiag_lowest(a[1:m + 1], b[1:m], lo, hi)
             res = float(abs(b[m] * z[m - 1]))
             job.gs_prev = (E, res)
-            if res ** 2 > self.gs_tol * max(1.0, E ** 2) and m < job.gs_max:
+            converged = res ** 2 <= self.gs_tol * max(1.0, E ** 2)
+            if not converged and m < job.gs_max:
                 job.gs_depth = min(job.gs_max, m + self.gs_step)
                 job.extend = True  # the caller sets the new depth (one launch for all such jobs)
                 return False
+            if not converged and self.keep_basis and job.restarts < self.gs_restarts:
+                # the kept basis is full: restart the Lanczos run from the Ritz vector
+                y = self.batch.ritz(job.slot, z, job.basis.zeros())
+                self.batch.remove(job.slot)
+                y /= torch.linalg.vector_norm(y)
+                job.restarts += 1
+                job.steps_done += m
+                job.gs_prev = None
+                job.gs_depth = min(self.gs_depth, job.gs_max)
+                job.slot = self.batch.add(job.hop, y.contiguous(), job.gs_depth, max_depth=job.gs_max,
+                                          keep_basis=True)
+                return False
             r.E0 = float(E)
             r.gs_residual = res
-            r.gs_steps = m
+            r.gs_steps = job.steps_done + m
             if not self.keep_basis:
                 self.batch.remove(job.slot)
                 job.slot = self.batch.add(job.hop, job.v0, m, ycoef=z)

@@ -268,3 +268,43 @@ def test_server_weighted_start_reaches_the_same_ground_state():
         for r in (a[rid], b[rid]):
             assert r.gs_residual ** 2 <= 1e-12 * max(1.0, r.E0 ** 2)
         assert abs(a[rid].E0 - b[rid].E0) <= 1e-8 * max(1.0, abs(a[rid].E0)), (rid, a[rid].E0, b[rid].E0)
+
+
+def test_server_restarts_ground_state_when_the_kept_basis_is_full():
+    """With room for only 40 kept Lanczos vectors, the ground state is restarted
+    from the Ritz vector until it meets the criterion; the spectrum then agrees
+    with the oracle's from the served ground state, and the plain run is
+    identical."""
+    from quanty_amd import default_context
+    reqs = serve.request_stream(1, rate=0.0, seed=11, dim_lo=800, dim_hi=1500)
+    srv = serve.SpectrumServer(default_context(), max_slots=4, chunk=8, gs_max=40)
+    keep = {}
+    orig = srv._advance
+
+    def spy(job, *args):
+        out = orig(job, *args)
+        if job.phase == "spec" and "psi" not in keep:
+            keep["basis"] = job.basis.words()
+            keep["psi"] = job.gs_vector.cpu().numpy()
+            keep["nf"] = job.nf
+            keep["restarts"] = job.restarts
+        return out
+
+    srv._advance = spy
+    (r,) = srv.serve(reqs, realtime=False)
+    assert keep["restarts"] >= 1 and r.gs_steps > 40
+    assert r.gs_residual ** 2 <= 1e-12 * max(1.0, r.E0 ** 2)
+    pr = P.anderson(seed=r.seed, **r.shape)
+    nf = keep["nf"]
+    psi = oracle.Vec.from_arrays(nf, D.words_to_bytes(keep["basis"], nf), keep["psi"])
+    rng = P.SplitMix64(r.seed ^ 0x5DEECE66D)
+    bath = [m for b in pr["baths"] for m in b]
+    T = P.random_one_body_transition(rng, pr["dmodes"], bath)
+    tpsi = oracle.Op(nf, T).apply(psi, restrictions=pr["restrictions"])
+    a, b = oracle.Op(nf, pr["H"]).lanczos(tpsi, r.depth, restrictions=pr["restrictions"])
+    want = oracle.cf(a, b, r.E0, srv.emin, srv.emax, srv.ne, r.gamma)
+    assert np.max(np.abs(r.spectrum - want)) <= 1e-8 * np.max(np.abs(want))
+    plain = serve.SpectrumServer(default_context(), max_slots=1, chunk=8, gs_max=40)
+    (p,) = plain.serve(serve.request_stream(1, rate=0.0, seed=11, dim_lo=800, dim_hi=1500), realtime=False,
+                       plain=True)
+    assert np.array_equal(p.spectrum, r.spectrum) and p.gs_steps == r.gs_steps
