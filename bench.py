@@ -188,7 +188,8 @@ def serve_section(args, ctx):
                        "dim_max": int(dims.max()), "dim_median": float(np.median(dims)), "depth": "U{100..400}",
                        "gamma": "U(0.1,0.5)", "ne_points": 2001, "slots": args.slots, "hbm_budget_GB": 200,
                        "gs": "Lanczos to |H psi - E psi|^2 <= 1e-12 max(1, E^2), Ritz vector from the kept "
-                             "Lanczos vectors",
+                             "Lanczos vectors (restarted from it when 500 are kept before convergence)",
+                       "gs_steps_max": int(max(w[1] for w in work)),
                        "gs_steps_mean": float(np.mean([w[1] for w in work])),
                        "row_steps": float(rs.sum()), "row_steps_share_dim_gt_3e5": float(rs[big].sum() / rs.sum())}}, \
         work
