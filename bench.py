@@ -167,9 +167,11 @@ def serve_section(args, ctx):
     serve.SpectrumServer(ctx, max_slots=args.slots).serve(warm, realtime=False)
     reqs = serve.request_stream(args.steps, args.rate, seed=args.seed + 100)
     t0 = time.perf_counter()
+    w0 = time.monotonic_ns()
     served = serve.SpectrumServer(ctx, max_slots=args.slots).serve(reqs, realtime=True)
     t_served = time.perf_counter() - t0
-    log(f"served stream: {t_served:.2f} s")
+    # the window on the monotonic clock, which kernel traces also use
+    log(f"served stream: {t_served:.2f} s (monotonic ns {w0} .. {time.monotonic_ns()})")
     reqs2 = serve.request_stream(args.steps, args.rate, seed=args.seed + 100)
     t0 = time.perf_counter()
     plain = serve.SpectrumServer(ctx, max_slots=1).serve(reqs2, realtime=True, plain=True)
