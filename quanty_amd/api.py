@@ -726,6 +726,33 @@ def _dense_spectra(H, cb, tpsi, psiEn, o, ctx):
     return tuple(out)
 
 
+def _block_tridiag_eigenvalues(A, B, kept, ctx):
+    """Eigenvalues (ascending) of the block tridiagonal matrix of a block Lanczos
+    run: diagonal blocks A[l] (kept[l] x kept[l]) and B[l] = V_l^+ H V_{l+1}
+    coupling levels l and l+1, for l = 1 .. the last level with kept > 0.
+    Assembled on the host, reduced on the GPU (Householder, tridiagonal QL)."""
+    nl = len(kept) - 1
+    for l in range(1, len(kept)):
+        if kept[l] == 0:
+            nl = l - 1
+            break
+    sizes = [int(kept[l]) for l in range(1, nl + 1)]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+    m = int(off[-1])
+    T = np.zeros((m, m), dtype=np.complex128)
+    for i, l in enumerate(range(1, nl + 1)):
+        p = sizes[i]
+        T[off[i]:off[i] + p, off[i]:off[i] + p] = A[l][:p, :p]
+        if i + 1 < nl:
+            q = sizes[i + 1]
+            T[off[i]:off[i] + p, off[i + 1]:off[i + 1] + q] = B[l][:p, :q]
+            T[off[i + 1]:off[i + 1] + q, off[i]:off[i] + p] = B[l][:p, :q].conj().T
+    # a Hermitian matrix read column-major is its conjugate: same eigenvalues
+    Hd = torch.from_numpy(T).to(torch.device("cuda", ctx.device)).t()
+    E, _ = dense_poles(Hd, None, ctx, vectors=False)
+    return E
+
+
 def _tri_spectra(A, Bv, psiEn, o, ctx):
     """Spectra and shifted Tri tables of Lanczos coefficients (LuaInterface/ssp.cpp:2640-2757)."""
     nsp = A.shape[0]
@@ -837,9 +864,17 @@ def CreateSpectra(H: Operator, T, psi, options=None):
         raise QSError("Tensor spectra need one E0 for all wavefunctions (pass E0)")
     V0 = torch.stack([tp._vec for tp in tpsi]).contiguous()
     A, B, kept = hop.block_lanczos(V0, ntri, float(o["SingularValue"]))
-    emin = -10.0 if o["Emin"] is None else float(o["Emin"])
-    emax = 10.0 if o["Emax"] is None else float(o["Emax"])
     ne = int(o["NE"])
+    eminpole = emaxpole = None
+    if o["Emin"] is None or o["Emax"] is None:
+        # default range from the eigenvalues of the block tridiagonal matrix
+        # (LuaInterface/ssp.cpp, tensor Lanczos branch: val[0], val[N*BlockSize-1])
+        ev = _block_tridiag_eigenvalues(A, B, kept, ctx)
+        eminpole, emaxpole = ev[0] - psiEn[0], ev[-1] - psiEn[0]
+        emin, emax = _default_range(o, eminpole, emaxpole)
+    else:
+        emin, emax = float(o["Emin"]), float(o["Emax"])
     gamma = float(o["Gamma"]) or 10 * (emax - emin) / ne
     G = block_cf_spectrum(A, B, kept, psiEn[0], emin, emax, ne, gamma, ctx)
-    return Spectra(emin, emax, ne, gamma, G.reshape(ne + 1, -1).T), {"A": A, "B": B, "kept": kept}
+    return (Spectra(emin, emax, ne, gamma, G.reshape(ne + 1, -1).T, eminpole, emaxpole),
+            {"A": A, "B": B, "kept": kept})

@@ -397,3 +397,30 @@ def test_create_spectra_with_annihilated_psi(ctx):
         assert np.max(np.abs(s2.G[1] - s1.G[0])) <= 1e-12 * np.max(np.abs(s1.G[0]))
     st, _ = CreateSpectra(H, kill, psi, opts + [["Tensor", True]])
     assert np.all(st.G == 0)
+
+
+def test_tensor_lanczos_default_range_from_block_eigenvalues(ctx):
+    """Tensor Lanczos without Emin/Emax takes the range from the eigenvalues of
+    its block tridiagonal matrix. With the block Krylov space exhausted those
+    are eigenvalues of H on the closure: the pole range lies inside the dense
+    method's (which sees every closure eigenvalue) and equals it when the
+    Krylov space is the whole closure."""
+    nf, H = _chain_api(5, 1)
+    T = [NewOperator("Cr", nf, 0), NewOperator("Cr", nf, 1)]
+    strings, coefs = chain_states(nf, 2)
+    psi = NewWavefunction(nf, 0, list(zip(strings, coefs)))
+    opts = [["NE", 400], ["Gamma", 0.3], ["NTri", 60], ["Tensor", True]]
+    sl, tab = CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]])
+    sd, _ = CreateSpectra(H, T, psi, opts)
+    kept = tab["kept"]
+    krylov_dim = int(sum(kept[1:]))
+    assert sl.EminPole >= sd.EminPole - 1e-9 and sl.EmaxPole <= sd.EmaxPole + 1e-9
+    tp = [operator_psi(t, psi) for t in T]
+    seed = np.unique(np.concatenate([t.host()[0] for t in tp]), axis=0)
+    cb = Basis.closure(Op(nf, list(H.terms.items()), ctx), Basis.from_words(nf, seed, ctx))
+    if krylov_dim == cb.n:
+        assert abs(sl.EminPole - sd.EminPole) < 1e-9 and abs(sl.EmaxPole - sd.EmaxPole) < 1e-9
+    # the range follows the reference's formula around the pole range
+    span = sl.EmaxPole - sl.EminPole
+    assert abs(sl.Emin - (sl.EminPole - 0.125 * span)) < 1e-12 * max(1, abs(sl.Emin))
+    assert abs(sl.Emax - (sl.EmaxPole + 0.125 * span)) < 1e-12 * max(1, abs(sl.Emax))
