@@ -590,6 +590,51 @@ class Spectra:
     def Copy(self):
         return Spectra(self.Emin, self.Emax, self.NE, self.Gamma, self.G.copy(), self.EminPole, self.EmaxPole)
 
+    Clone = Copy
+
+    def Shift(self, shift):
+        """The spectra on an energy axis moved by `shift` (a number, or one per
+        spectrum, all equal here since the spectra share one grid); the values
+        stay (LuaInterface/Spectra.cpp:217-274)."""
+        sh = np.atleast_1d(np.asarray(shift, dtype=float))
+        if sh.size > 1 and (sh.size != self.G.reshape(-1, self.NE + 1).shape[0] or np.any(sh != sh[0])):
+            raise QSError("Spectra.Shift: the spectra share one energy grid; give one shift")
+        d = float(sh[0])
+        pmin = None if self.EminPole is None else self.EminPole + d
+        pmax = None if self.EmaxPole is None else self.EmaxPole + d
+        return Spectra(self.Emin + d, self.Emax + d, self.NE, self.Gamma, self.G.copy(), pmin, pmax)
+
+    def MultiplyByEnergy(self):
+        """Every value times its energy E_j = Emin + j dE (Spectra.cpp:484-505)."""
+        return self._new(self.G * self.energies)
+
+    def Element(self, which):
+        """Spectrum `which` (1-based), or the listed ones (Spectra.cpp:695-811)."""
+        G = self.G.reshape(-1, self.NE + 1)
+        idx = [which] if isinstance(which, (int, np.integer)) else list(which)
+        if any(i < 1 or i > G.shape[0] for i in idx):
+            raise QSError(f"Spectra.Element: no element {idx} in spectra array with {G.shape[0]} elements")
+        return Spectra(self.Emin, self.Emax, self.NE, self.Gamma, G[[i - 1 for i in idx]].copy(), self.EminPole,
+                       self.EmaxPole)
+
+    def ToTable(self):
+        """One list per spectrum of [energy, complex value] pairs (Spectra.cpp:325-335)."""
+        E = self.energies
+        return [[[float(e), complex(v)] for e, v in zip(E, g)] for g in self.G.reshape(-1, self.NE + 1)]
+
+    @staticmethod
+    def FromTable(table):
+        """Spectra from ToTable's form (Spectra.cpp:337-480); the spectra must share one
+        equidistant grid."""
+        if not table or any(len(sp) < 2 for sp in table):
+            raise QSError("Spectra.FromTable needs spectra of at least two points")
+        E = np.array([[float(p[0]) for p in sp] for sp in table])
+        if np.any(E != E[0]):
+            raise QSError("Spectra.FromTable: the spectra must share one energy grid")
+        ne = E.shape[1] - 1
+        G = np.array([[complex(p[1]) for p in sp] for sp in table])
+        return Spectra(float(E[0, 0]), float(E[0, -1]), ne, 0.0, G)
+
     def _widths(self, w, which):
         """A FWHM as one number, or per grid point from a table {{E_1, FWHM_1}, ...}
         (ascending energies): linear between the two entries around a point,

@@ -87,3 +87,19 @@ def test_spectra_sum_and_arithmetic():
     assert np.allclose((S + 2).G, G + 2) and np.allclose((2 * S).G, 2 * G) and np.allclose((S - S).G, 0)
     with pytest.raises(QSError):
         S.Sum([1, 2])
+
+
+def test_spectra_shift_multiply_element_tables():
+    rng = np.random.default_rng(4)
+    G = rng.normal(size=(3, 21)) + 1j * rng.normal(size=(3, 21))
+    S = Spectra(-1.0, 1.0, 20, 0.1, G, -0.8, 0.9)
+    T = S.Shift(2.5)
+    assert (T.Emin, T.Emax, T.EminPole, T.EmaxPole) == (1.5, 3.5, 1.7, 3.4) and np.array_equal(T.G, G)
+    assert np.allclose(S.MultiplyByEnergy().G, G * np.linspace(-1, 1, 21)[None, :])
+    assert np.array_equal(S.Element(2).G, G[1:2]) and np.array_equal(S.Element([3, 1]).G, G[[2, 0]])
+    with pytest.raises(QSError):
+        S.Element(4)
+    with pytest.raises(QSError):
+        S.Shift([1.0, 2.0, 3.0])
+    back = Spectra.FromTable(S.ToTable())
+    assert np.allclose(back.G, G, rtol=0, atol=0) and (back.Emin, back.Emax, back.NE) == (-1.0, 1.0, 20)
