@@ -424,3 +424,33 @@ def test_tensor_lanczos_default_range_from_block_eigenvalues(ctx):
     span = sl.EmaxPole - sl.EminPole
     assert abs(sl.Emin - (sl.EminPole - 0.125 * span)) < 1e-12 * max(1, abs(sl.Emin))
     assert abs(sl.Emax - (sl.EmaxPole + 0.125 * span)) < 1e-12 * max(1, abs(sl.Emax))
+
+
+def test_dense_single_determinant_closure(ctx):
+    """H diagonal on T psi's only determinant: a closure of one determinant,
+    one pole at its energy with the whole weight."""
+    nf = 6
+    H = NewOperator(nf, 0, [[0, -0.0, 1.5], [1, -1, -0.25]])  # 1.5 n0 - 0.25 n1
+    psi = NewWavefunction(nf, 0, [("010000", 1.0)])
+    T = NewOperator("Cr", nf, 0)
+    spec, tables = CreateSpectra(H, T, psi, [["Emin", -2.0], ["Emax", 4.0], ["NE", 60], ["Gamma", 0.2],
+                                              ["E0", 0.0]])
+    assert tables[0]["type"] == "ListOfPoles"
+    assert np.allclose(tables[0]["a"], [0.0, 1.25]) and np.allclose(tables[0]["b"], [1.0])
+    w = -2.0 + np.arange(61) * 0.1
+    assert np.allclose(spec.G[0], 1.0 / (w - 1.25 + 0.1j), rtol=1e-12, atol=1e-14)
+
+
+def test_closure_two_word_with_restrictions(ctx):
+    """A restriction whose mask spans both words of 76-mode determinants cuts the
+    closure exactly as the oracle's restricted push form does."""
+    nf = 76
+    rng = np.random.default_rng(5)
+    modes = np.r_[0:4, 60:68, 72:76]
+    terms = random_terms(nf, modes, 12, 6, rng, hermitian=False)
+    mask = "".join("1" if i in (1, 2, 62, 63, 66) else "0" for i in range(nf))
+    restr = [(mask, 1, 2)]
+    seed = det_words(nf, [1, 63, 73])
+    b = Basis.closure(Op(nf, terms, ctx), Basis.from_words(nf, seed, ctx), restr)
+    want = oracle_closure(nf, terms, seed, restrictions=restr)
+    assert b.n == len(want) and np.array_equal(b.words(), want)
