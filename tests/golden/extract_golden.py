@@ -4,6 +4,7 @@ files are committed, so tests never read the reference).
 
   createspectra_tri.json  <- TestScripts/CreateSpectra.Quanty:188-260, 664-736
   eigensystem_f6.json     <- TestScripts/Eigensystem.Quanty:571, 669
+  eigensystem_hubbard.json <- TestScripts/Eigensystem.Quanty:1119-1136
 """
 import json
 import os
@@ -46,6 +47,14 @@ def main():
                "real": grab("expectedEn = {-12.11458547046453"),
                "complex": grab("expectedEn = {-12.11458548596329")},
               open(os.path.join(HERE, "eigensystem_f6.json"), "w"), indent=1)
+    json.dump({"source": "TestScripts/Eigensystem.Quanty:1119-1136 (energies from Quanty; the script "
+                         "checks them to 1e-4)",
+               "model": "1D Hubbard ring of 8 sites, modes 2i = site i down, 2i+1 = site i up; hopping "
+                        "+1 between nearest and +0.1 between next-nearest neighbours (spin-diagonal, "
+                        "periodic), U = 3.5 n_dn n_up per site; 4 down and 4 up electrons",
+               "nsites": 8, "t1": 1.0, "t2": 0.1, "U": 3.5,
+               "energies": grab("expectedEn = { -5.0089945483798")},
+              open(os.path.join(HERE, "eigensystem_hubbard.json"), "w"), indent=1)
 
 
 if __name__ == "__main__":

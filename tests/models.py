@@ -84,3 +84,20 @@ def dense_from_terms(nf, terms, cs=None):
             M = op @ M
         H += v * M
     return H
+
+
+def hubbard_ring(nsites=8, t1=1.0, t2=0.1, U=3.5):
+    """1D Hubbard ring (TestScripts/Eigensystem.Quanty:1121-1136): modes 2i = site i
+    down, 2i+1 = site i up; +t1 between nearest and +t2 between next-nearest
+    neighbours (periodic, spin-diagonal), U n_dn n_up on every site."""
+    nf = 2 * nsites
+    terms = []
+    for i in range(nsites):
+        for d, t in ((1, t1), (2, t2)):
+            j = (i + d) % nsites
+            for s in (0, 1):
+                a, b = 2 * i + s, 2 * j + s
+                terms.append(O.written([O.cr(b), O.an(a)], t))
+                terms.append(O.written([O.cr(a), O.an(b)], t))
+        terms.append(O.written([O.cr(2 * i), O.an(2 * i), O.cr(2 * i + 1), O.an(2 * i + 1)], U))
+    return nf, terms

@@ -327,3 +327,33 @@ def test_fwhm_table_is_linear_interpolation():
         got = oracle.fwhm_table(np.stack([E, F], 1), emin, dE, npts)
         x = emin + np.arange(npts) * dE
         assert np.allclose(got, np.interp(x, E, F), rtol=0, atol=1e-14)
+
+
+def test_hubbard_ring_energies_golden():
+    """The 10 lowest energies of the 8-site Hubbard ring held by
+    TestScripts/Eigensystem.Quanty:1119, from the oracle's push-form H*psi in a
+    matrix-free Lanczos (scipy eigsh) on the 4 down + 4 up sector (4900 dets).
+    The golden values carry Quanty's own convergence error (two degenerate pairs
+    differ by 2e-10 and 1e-8), so they are compared at 1e-7."""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+
+    from models import hubbard_ring
+    g = golden("eigensystem_hubbard.json")
+    nf, terms = hubbard_ring(g["nsites"], g["t1"], g["t2"], g["U"])
+    restr = [("1010101010101010", 4, 4), ("0101010101010101", 4, 4)]
+    dets = oracle.enumerate_sector(nf, restr)
+    assert len(dets) == 4900
+    index = {bytes(d): i for i, d in enumerate(dets)}
+    op = oracle.Op(nf, terms)
+
+    def mv(x):
+        x = np.asarray(x).ravel()
+        d, v = op.apply(oracle.Vec.from_arrays(nf, dets, x.astype(complex))).arrays()
+        y = np.zeros(len(dets), dtype=complex)
+        for det, val in zip(d, v):
+            y[index[bytes(det)]] += val
+        return y.real if np.isrealobj(x) else y
+
+    H = LinearOperator((4900, 4900), matvec=mv, dtype=float)
+    ev = np.sort(eigsh(H, k=10, which="SA", ncv=48, tol=1e-13, return_eigenvectors=False))
+    assert np.max(np.abs(ev - np.array(g["energies"]))) < 1e-7
