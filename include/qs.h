@@ -278,11 +278,19 @@ qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates, double tol
  * NKrylovStart / NKrylovStep / NKrylovMax settings (defaults 100 / 10 / 500,
  * Core/StructDef.cpp:298-300) as BlockLanczosGroundStateConserveBasisKrylov
  * uses them (Core/BlockLanczos.cpp:5130, 5311-5325). qs_ground_states is this
- * with krylov_start 100 and krylov_step 10. */
+ * with krylov_start 100, krylov_step 10 and flags 0. */
 qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int krylov_start,
-                              int krylov_step, int max_krylov, int max_restarts, uint64_t seed,
+                              int krylov_step, int max_krylov, int max_restarts, uint64_t seed, int flags,
                               const double *start, double *E_out, double *vecs, double *variance_out,
                               int *iterations_out);
+/* flags of qs_ground_states_ex. QS_GS_CYCLE_END: test convergence only when the
+ * basis reaches its restart size, as the reference's Krylov loop does (its
+ * deflation test sees the Ritz residual block once per restart cycle,
+ * Core/BlockLanczos.cpp:5176-5215). Without it the Ritz pairs are also tested
+ * at intermediate basis sizes and the solver may stop earlier; that stops
+ * within the same variance bound but resolves states that are much closer
+ * than the bound's residual (a field-split multiplet) less sharply. */
+#define QS_GS_CYCLE_END 1
 
 /* Tensor Green's function from qs_block_lanczos output, evaluated on the GPU:
  * G(w) = B0^* G_1 B0^T, G_j = (z - A_j - B_j G_{j+1} B_j^+)^-1 bottom-up,

@@ -71,8 +71,8 @@ _SIGS = [
     ("qs_lanczos", C.c_int, [_P, _P, _P, C.c_int, _P, _P]),
     ("qs_block_lanczos", C.c_int, [_P, _P, C.c_int, _P, C.c_int, _D, _P, _P, _P]),
     ("qs_ground_states", C.c_int, [_P, _P, C.c_int, _D, C.c_int, C.c_int, C.c_uint64, _P, _P, _P, _P, _P]),
-    ("qs_ground_states_ex", C.c_int, [_P, _P, C.c_int, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, _P, _P,
-                                      _P, _P, _P]),
+    ("qs_ground_states_ex", C.c_int, [_P, _P, C.c_int, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int,
+                                      _P, _P, _P, _P, _P]),
     ("qs_cf_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _D, _D, C.c_int, _D, _P, _P]),
     ("qs_block_cf_spectrum", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _D, _D, _D, C.c_int, _D, _P]),
     ("qs_broaden", C.c_int, [_P, C.c_int, C.c_int, _D, C.c_int, C.c_int, _P, _P, _P]),

@@ -498,7 +498,8 @@ def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
     hop = Hop(H.device(ctx), basis, mode="auto")
     E, vecs, var, it = hop.ground_states(Npsi, tol=float(o["Zero"]), max_krylov=int(o["NKrylovMax"]),
                                          max_restarts=int(o["MaxRestarts"]), seed=int(o["Seed"]),
-                                         krylov_start=int(o["NKrylovStart"]), krylov_step=int(o["NKrylovStep"]))
+                                         krylov_start=int(o["NKrylovStart"]), krylov_step=int(o["NKrylovStep"]),
+                                         cycle_end=True)
     return [Wavefunction(nf, basis=basis, vec=vecs[i].clone()) for i in range(Npsi)]
 
 

@@ -651,8 +651,8 @@ extern "C" qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const doubl
 // ---------------------------------------------------------------------------
 extern "C" qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int krylov_start,
                                          int krylov_step, int max_krylov, int max_restarts, uint64_t seed,
-                                         const double *start, double *E_out, double *vecs, double *variance_out,
-                                         int *iterations_out) {
+                                         int flags, const double *start, double *E_out, double *vecs,
+                                         double *variance_out, int *iterations_out) {
   if (!ctx || !h || nstates < 1 || nstates > QS_KMAX || !E_out || !vecs) return QS_EINVAL;
   if (h->rows != h->cols) return qs_fail(ctx, QS_EINVAL, "qs_ground_states needs rows == cols");
   QS_HIP(ctx, hipSetDevice(ctx->device));
@@ -665,6 +665,8 @@ extern "C" qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstat
   const int keep_max = std::min(QS_KMAX, std::max(2 * p, p + 6));
   if (krylov_start <= 0) krylov_start = 100;
   if (krylov_step < 0) krylov_step = 10;
+  // convergence tested only when the basis reaches its restart size
+  const bool cycle_end = (flags & QS_GS_CYCLE_END) != 0;
   const int mmax = (int)std::min<int64_t>(n, std::max(max_krylov, keep_max + 3 * p));
   // basis size before a thick restart: nstates + krylov_start at first, grown
   // by krylov_step + nstates per restart while below max_krylov
@@ -768,7 +770,7 @@ extern "C" qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstat
       if (wn > 0) QS_TRY(wk.vtw(wk.vs(Cb), wn, Wb, cur.w, R));
       const bool exhausted = wn == 0 || m + wn > n;
       const bool full = cap < n && m + wn + p > cap;
-      if (exhausted || full || m >= next_check) {
+      if (exhausted || full || (!cycle_end && m >= next_check)) {
         // Rayleigh-Ritz on the current basis (m vectors)
         std::vector<cd> Tm((size_t)m * m);
         for (int j = 0; j < m; ++j)
@@ -865,7 +867,7 @@ extern "C" qs_status qs_ground_states_ex(qs_ctx *ctx, const qs_hop *h, int nstat
 extern "C" qs_status qs_ground_states(qs_ctx *ctx, const qs_hop *h, int nstates, double tol, int max_krylov,
                                       int max_restarts, uint64_t seed, const double *start, double *E_out,
                                       double *vecs, double *variance_out, int *iterations_out) {
-  return qs_ground_states_ex(ctx, h, nstates, tol, 100, 10, max_krylov, max_restarts, seed, start, E_out, vecs,
+  return qs_ground_states_ex(ctx, h, nstates, tol, 100, 10, max_krylov, max_restarts, seed, 0, start, E_out, vecs,
                              variance_out, iterations_out);
 }
 

@@ -231,12 +231,14 @@ class Hop:
 
     def ground_states(self, nstates: int, tol: float = 1e-12, max_krylov: int = 500, max_restarts: int = 100,
                       seed: int = 1, start: torch.Tensor | None = None, krylov_start: int = 100,
-                      krylov_step: int = 10):
+                      krylov_step: int = 10, cycle_end: bool = False):
         """Lowest eigenpairs. Returns (E (nstates,), vecs (nstates, n) tensor, variances, iterations).
 
         The Krylov basis holds nstates + krylov_start vectors before the first
         thick restart and grows by krylov_step + nstates per restart up to
-        max_krylov (NKrylovStart / NKrylovStep / NKrylovMax)."""
+        max_krylov (NKrylovStart / NKrylovStep / NKrylovMax). cycle_end tests
+        convergence only when the basis reaches its restart size, as the
+        reference's Krylov loop does (Core/BlockLanczos.cpp:5176-5215)."""
         E = np.zeros(nstates)
         var = np.zeros(nstates)
         it = C.c_int()
@@ -245,7 +247,7 @@ class Hop:
             start = start.contiguous()
         _torch_ready(vecs.device)
         self.ctx.check(self.lib.qs_ground_states_ex(self.ctx.h, self.h, nstates, tol, krylov_start, krylov_step,
-                                                    max_krylov, max_restarts, seed, ptr(start), ptr(E), ptr(vecs),
+                                                    max_krylov, max_restarts, seed, 1 if cycle_end else 0, ptr(start), ptr(E), ptr(vecs),
                                                     ptr(var), C.byref(it)), "qs_ground_states_ex")
         return E, vecs, var, it.value
 

@@ -5,6 +5,7 @@ files are committed, so tests never read the reference).
   createspectra_tri.json  <- TestScripts/CreateSpectra.Quanty:188-260, 664-736
   eigensystem_f6.json     <- TestScripts/Eigensystem.Quanty:571, 669
   eigensystem_hubbard.json <- TestScripts/Eigensystem.Quanty:1119-1136
+  eigensystem_mlft.json   <- TestScripts/Eigensystem.Quanty:763-770 (real d5 ligand-field cluster)
 """
 import json
 import os
@@ -55,6 +56,18 @@ def main():
                "nsites": 8, "t1": 1.0, "t2": 0.1, "U": 3.5,
                "energies": grab("expectedEn = { -5.0089945483798")},
               open(os.path.join(HERE, "eigensystem_hubbard.json"), "w"), indent=1)
+    i = src.index("::mlft::")
+    i = src.index("valuesExpected = {", i)
+    blk = src[i:src.index("NF=20", i)]
+    rows = [[float(x) for x in r.split(",")] for r in re.findall(r"\{([^{}]*)\}", blk)]
+    json.dump({"source": "TestScripts/Eigensystem.Quanty:763-770 (Quanty output; the script checks to 2e-6)",
+               "model": "d5 ligand-field cluster, NF 20: 3d modes 0-9 and ligand d modes 10-19 (dn even, up "
+                        "odd); U 5.5, Delta 8, F2 9.35, F4 5.78, 10Dq 0.67, 10DqL 1.06, Veg 1.92, Vt2g 1.15, "
+                        "zeta 0.04, B 1e-6 (2 Sz + Lz on 3d); start: 5 in 3d, 10 in L; 3d occupation 5..7",
+               "observables": ["H", "S^2", "L^2", "J^2", "Sz_3d", "Lz_3d", "ldots_3d", "F2_3d", "F4_3d",
+                               "Neg_3d", "Nt2g_3d", "Neg_Ld", "Nt2g_Ld", "N_3d"],
+               "values": rows},
+              open(os.path.join(HERE, "eigensystem_mlft.json"), "w"), indent=1)
 
 
 if __name__ == "__main__":

@@ -101,3 +101,49 @@ def hubbard_ring(nsites=8, t1=1.0, t2=0.1, U=3.5):
                 terms.append(O.written([O.cr(a), O.an(b)], t))
         terms.append(O.written([O.cr(2 * i), O.an(2 * i), O.cr(2 * i + 1), O.an(2 * i + 1)], U))
     return nf, terms
+
+
+def mlft_d5():
+    """d5 ligand-field cluster (TestScripts/Eigensystem.Quanty:794-990): a 3d shell
+    (modes 0-9) and a ligand d shell (modes 10-19), dn even and up odd. Returns
+    (nf, H terms, {name: terms} of the linear observables, {"S"|"L"|"J": [x, y, z]
+    component terms summed over both shells}). Every cubic crystal-field operator
+    of the script is an eg/t2g projector sum: CF with Oh potential {e_eg, e_t2g}
+    is e_eg P_eg + e_t2g P_t2g on the shell (or between the shells)."""
+    from quanty_amd.problems import shell
+
+    nd, U, Delta = 5, 5.5, 8.0
+    F2, F4, tenDq, tenDqL, Veg, Vt2g, zeta, B = 9.35, 5.78, 0.67, 1.06, 1.92, 1.15, 0.04, 1e-6
+    ed = (10 * Delta - nd * (19 + nd) * U / 2) / (10 + nd)
+    eL = nd * ((1 + nd) * U / 2 - Delta) / (10 + nd)
+    F0 = U + (F2 + F4) * 2 / 63
+    up_d, dn_d = shell(0, 2)
+    up_l, dn_l = shell(10, 2)
+    md, ml = O.shell_modes(up_d, dn_d), O.shell_modes(up_l, dn_l)
+    eg, t2g = O.oh_d_projectors()
+    Peg, Pt2g = O.d_shell_matrix_to_spin(eg), O.d_shell_matrix_to_spin(t2g)
+
+    def hop(P):
+        return O.add(O.one_body(md, ml, P), O.one_body(ml, md, P))
+
+    obs = {
+        "Sz_3d": O.shell_operator(up_d, dn_d, "Sz"),
+        "Lz_3d": O.shell_operator(up_d, dn_d, "Lz"),
+        "ldots_3d": O.shell_operator(up_d, dn_d, "ldots"),
+        "F2_3d": O.coulomb(up_d, dn_d, [0, 1, 0]),
+        "F4_3d": O.coulomb(up_d, dn_d, [0, 0, 1]),
+        "Neg_3d": O.one_body(md, md, Peg),
+        "Nt2g_3d": O.one_body(md, md, Pt2g),
+        "Neg_Ld": O.one_body(ml, ml, Peg),
+        "Nt2g_Ld": O.one_body(ml, ml, Pt2g),
+        "N_3d": O.number(md),
+    }
+    H = O.add(O.coulomb(up_d, dn_d, [F0, F2, F4]), O.shell_operator(up_d, dn_d, "ldots", zeta),
+              O.scale(O.add(O.scale(obs["Sz_3d"], 2), obs["Lz_3d"]), B),
+              O.one_body(md, md, tenDq * (0.6 * Peg - 0.4 * Pt2g)),
+              O.one_body(ml, ml, tenDqL * (0.6 * Peg - 0.4 * Pt2g)),
+              O.scale(hop(Peg), Veg), O.scale(hop(Pt2g), Vt2g),
+              O.scale(O.number(md), ed), O.scale(O.number(ml), eL))
+    comps = {k: [O.add(O.shell_operator(up_d, dn_d, k + c), O.shell_operator(up_l, dn_l, k + c)) for c in "xyz"]
+             for k in "SLJ"}
+    return 20, H, obs, comps

@@ -357,3 +357,70 @@ def test_hubbard_ring_energies_golden():
     H = LinearOperator((4900, 4900), matvec=mv, dtype=float)
     ev = np.sort(eigsh(H, k=10, which="SA", ncv=48, tol=1e-13, return_eigenvectors=False))
     assert np.max(np.abs(ev - np.array(g["energies"]))) < 1e-7
+
+
+def _sector_ops(nf, dets, restr):
+    """(apply, to_vec) over a sorted-by-key determinant list: to_vec places the
+    oracle's output determinants on the list's positions."""
+    def key(d):
+        d = np.asarray(d, dtype=np.uint64)
+        k = np.zeros(len(d), dtype=np.uint64)
+        for j in range(d.shape[1]):
+            k = (k << np.uint64(8)) | d[:, j]
+        return k
+
+    kd = key(dets)
+    order = np.argsort(kd)
+    ks = kd[order]
+    n = len(dets)
+
+    def to_vec(d, v):
+        k = key(d)
+        pos = np.minimum(np.searchsorted(ks, k), n - 1)
+        assert np.all(ks[pos] == k)
+        y = np.zeros(n, dtype=complex)
+        np.add.at(y, order[pos], v)
+        return y
+
+    def apply(op, x, restricted=False):
+        psi = oracle.Vec.from_arrays(nf, dets, np.asarray(x, dtype=complex))
+        return op.apply(psi, restrictions=restr if restricted else None).arrays()
+
+    return apply, to_vec
+
+
+def test_mlft_d5_expectation_values_golden():
+    """The d5 ligand-field cluster of TestScripts/Eigensystem.Quanty:750-1012: the
+    6 lowest states (a Zeeman-split 6A1 sextet, 2e-6 apart) from the oracle's
+    restricted H*psi in scipy eigsh, and their 14 expectation values (energy, the
+    cluster's S^2, L^2, J^2 as sums of squared component norms, 3d spin, orbital
+    and spin-orbit moments, F2 and F4 energies, eg/t2g occupations of both shells,
+    3d count) against the table of :763-770, which the script checks to 2e-6.
+    Pins the shell operators, Coulomb, cubic crystal field and hybridization
+    construction as well as the restricted push H*psi."""
+    from scipy.sparse.linalg import LinearOperator, eigsh
+
+    from models import mlft_d5
+    g = golden("eigensystem_mlft.json")
+    nf, H, obs, comps = mlft_d5()
+    restr = [("1" * 20, 15, 15), ("1" * 10 + "0" * 10, 5, 7)]
+    dets = oracle.enumerate_sector(nf, restr)
+    assert len(dets) == 252 + 2100 + 5400
+    apply, to_vec = _sector_ops(nf, dets, restr)
+    h = oracle.Op(nf, H)
+
+    def mv(x):
+        y = to_vec(*apply(h, np.asarray(x).ravel(), restricted=True))
+        return y.real
+
+    w, V = eigsh(LinearOperator((len(dets),) * 2, matvec=mv, dtype=float), k=6, which="SA", ncv=40, tol=1e-14)
+    ops = {k: oracle.Op(nf, t) for k, t in obs.items()}
+    cops = {k: [oracle.Op(nf, c) for c in v] for k, v in comps.items()}
+    for r, i in enumerate(np.argsort(w)):
+        x = V[:, i].astype(complex)
+        row = [w[i]]
+        for k in "SLJ":
+            row.append(sum(np.vdot(v, v).real for _, v in (apply(c, x) for c in cops[k])))
+        for k in g["observables"][4:]:
+            row.append(np.vdot(x, to_vec(*apply(ops[k], x))).real)
+        assert np.max(np.abs(np.array(row) - np.array(g["values"][r]))) < 2e-8, r
