@@ -6,6 +6,7 @@ files are committed, so tests never read the reference).
   eigensystem_f6.json     <- TestScripts/Eigensystem.Quanty:571, 669
   eigensystem_hubbard.json <- TestScripts/Eigensystem.Quanty:1119-1136
   eigensystem_mlft.json   <- TestScripts/Eigensystem.Quanty:763-770 (real d5 ligand-field cluster)
+                             and :1032-1039 (the same with the field along y)
 """
 import json
 import os
@@ -60,13 +61,20 @@ def main():
     i = src.index("valuesExpected = {", i)
     blk = src[i:src.index("NF=20", i)]
     rows = [[float(x) for x in r.split(",")] for r in re.findall(r"\{([^{}]*)\}", blk)]
+    i = src.index("Complex ligand field theory")
+    i = src.index("valuesExpected = {", i)
+    blk = src[i:src.index("Hamiltonian", i)]
+    rows_y = [[float(x) for x in r.split(",")] for r in re.findall(r"\{([^{}]*)\}", blk)]
     json.dump({"source": "TestScripts/Eigensystem.Quanty:763-770 (Quanty output; the script checks to 2e-6)",
                "model": "d5 ligand-field cluster, NF 20: 3d modes 0-9 and ligand d modes 10-19 (dn even, up "
                         "odd); U 5.5, Delta 8, F2 9.35, F4 5.78, 10Dq 0.67, 10DqL 1.06, Veg 1.92, Vt2g 1.15, "
                         "zeta 0.04, B 1e-6 (2 Sz + Lz on 3d); start: 5 in 3d, 10 in L; 3d occupation 5..7",
                "observables": ["H", "S^2", "L^2", "J^2", "Sz_3d", "Lz_3d", "ldots_3d", "F2_3d", "F4_3d",
                                "Neg_3d", "Nt2g_3d", "Neg_Ld", "Nt2g_Ld", "N_3d"],
-               "values": rows},
+               "values": rows,
+               "source_y": "TestScripts/Eigensystem.Quanty:1032-1039: B (2 Sy + Ly) instead of B (2 Sz + Lz); "
+                           "columns 5 and 6 are Sy_3d and Ly_3d",
+               "values_y": rows_y},
               open(os.path.join(HERE, "eigensystem_mlft.json"), "w"), indent=1)
 
 

@@ -103,10 +103,12 @@ def hubbard_ring(nsites=8, t1=1.0, t2=0.1, U=3.5):
     return nf, terms
 
 
-def mlft_d5():
-    """d5 ligand-field cluster (TestScripts/Eigensystem.Quanty:794-990): a 3d shell
+def mlft_d5(axis="z"):
+    """d5 ligand-field cluster (TestScripts/Eigensystem.Quanty:794-990, and with
+    axis "y" the field along y of :1063-1066): a 3d shell
     (modes 0-9) and a ligand d shell (modes 10-19), dn even and up odd. Returns
-    (nf, H terms, {name: terms} of the linear observables, {"S"|"L"|"J": [x, y, z]
+    (nf, H terms, {name: terms} of the linear observables (S and L along the
+    field axis as "S<axis>_3d", "L<axis>_3d"), {"S"|"L"|"J": [x, y, z]
     component terms summed over both shells}). Every cubic crystal-field operator
     of the script is an eg/t2g projector sum: CF with Oh potential {e_eg, e_t2g}
     is e_eg P_eg + e_t2g P_t2g on the shell (or between the shells)."""
@@ -127,8 +129,8 @@ def mlft_d5():
         return O.add(O.one_body(md, ml, P), O.one_body(ml, md, P))
 
     obs = {
-        "Sz_3d": O.shell_operator(up_d, dn_d, "Sz"),
-        "Lz_3d": O.shell_operator(up_d, dn_d, "Lz"),
+        "S_3d": O.shell_operator(up_d, dn_d, "S" + axis),
+        "L_3d": O.shell_operator(up_d, dn_d, "L" + axis),
         "ldots_3d": O.shell_operator(up_d, dn_d, "ldots"),
         "F2_3d": O.coulomb(up_d, dn_d, [0, 1, 0]),
         "F4_3d": O.coulomb(up_d, dn_d, [0, 0, 1]),
@@ -139,11 +141,13 @@ def mlft_d5():
         "N_3d": O.number(md),
     }
     H = O.add(O.coulomb(up_d, dn_d, [F0, F2, F4]), O.shell_operator(up_d, dn_d, "ldots", zeta),
-              O.scale(O.add(O.scale(obs["Sz_3d"], 2), obs["Lz_3d"]), B),
+              O.scale(O.add(O.scale(obs["S_3d"], 2), obs["L_3d"]), B),
               O.one_body(md, md, tenDq * (0.6 * Peg - 0.4 * Pt2g)),
               O.one_body(ml, ml, tenDqL * (0.6 * Peg - 0.4 * Pt2g)),
               O.scale(hop(Peg), Veg), O.scale(hop(Pt2g), Vt2g),
               O.scale(O.number(md), ed), O.scale(O.number(ml), eL))
     comps = {k: [O.add(O.shell_operator(up_d, dn_d, k + c), O.shell_operator(up_l, dn_l, k + c)) for c in "xyz"]
              for k in "SLJ"}
+    obs["S%s_3d" % axis] = obs.pop("S_3d")
+    obs["L%s_3d" % axis] = obs.pop("L_3d")
     return 20, H, obs, comps

@@ -131,17 +131,20 @@ def test_eigensystem_hubbard_ring_golden():
         assert torch.linalg.norm(y - e * p._vec).item() < 1e-6
 
 
-def test_eigensystem_mlft_d5_golden():
+@pytest.mark.parametrize("axis", ["z", "y"])
+def test_eigensystem_mlft_d5_golden(axis):
     """TestScripts/Eigensystem.Quanty:750-1012 through the API: Eigensystem of
     the d5 ligand-field cluster from StartRestrictions (5 in 3d, 10 in the
     ligand shell) inside Restrictions (3d holds 5..7; 7752 determinants, the
     block Lanczos path), default options as in the script; the 6 lowest states'
     14 expectation values against the table of :763-770 at the script's 2e-6.
     The 6 states are a sextet split by a 1e-6 field, so the check also requires
-    each state to be resolved from its neighbours 2e-6 away."""
+    each state to be resolved from its neighbours 2e-6 away. Axis "y" is the
+    script's complex case (:1032-1066): the field and the moments along y."""
     from models import mlft_d5
     g = golden("eigensystem_mlft.json")
-    nf, H, obs, comps = mlft_d5()
+    nf, H, obs, comps = mlft_d5(axis)
+    names = ["S%s_3d" % axis, "L%s_3d" % axis] + g["observables"][6:]
     Hop = Operator(nf, H)
     start = [nf, 0, ["1111111111 0000000000", 5, 5], ["0000000000 1111111111", 10, 10]]
     psis = Eigensystem(Hop, start, 6, [["Restrictions", [nf, 0, ["1111111111 0000000000", 5, 7]]]])
@@ -149,7 +152,7 @@ def test_eigensystem_mlft_d5_golden():
     for k in "SLJ":
         c = [Operator(nf, t) for t in comps[k]]
         sq[k] = c[0] * c[0] + c[1] * c[1] + c[2] * c[2]
-    ops = [Hop, sq["S"], sq["L"], sq["J"]] + [Operator(nf, obs[k]) for k in g["observables"][4:]]
+    ops = [Hop, sq["S"], sq["L"], sq["J"]] + [Operator(nf, obs[k]) for k in names]
     got = np.array([[dot(p, o * p).real for o in ops] for p in psis])
-    want = np.array(g["values"][:6])
+    want = np.array((g["values"] if axis == "z" else g["values_y"])[:6])
     assert np.max(np.abs(got - want)) < 2e-6, np.max(np.abs(got - want), axis=1)

@@ -389,31 +389,37 @@ def _sector_ops(nf, dets, restr):
     return apply, to_vec
 
 
-def test_mlft_d5_expectation_values_golden():
+@pytest.mark.parametrize("axis", ["z", "y"])
+def test_mlft_d5_expectation_values_golden(axis):
     """The d5 ligand-field cluster of TestScripts/Eigensystem.Quanty:750-1012: the
     6 lowest states (a Zeeman-split 6A1 sextet, 2e-6 apart) from the oracle's
     restricted H*psi in scipy eigsh, and their 14 expectation values (energy, the
     cluster's S^2, L^2, J^2 as sums of squared component norms, 3d spin, orbital
     and spin-orbit moments, F2 and F4 energies, eg/t2g occupations of both shells,
-    3d count) against the table of :763-770, which the script checks to 2e-6.
+    3d count) against the table of :763-770, which the script checks to 2e-6;
+    with the field along y (a complex Hamiltonian) against :1032-1039.
     Pins the shell operators, Coulomb, cubic crystal field and hybridization
     construction as well as the restricted push H*psi."""
     from scipy.sparse.linalg import LinearOperator, eigsh
 
     from models import mlft_d5
     g = golden("eigensystem_mlft.json")
-    nf, H, obs, comps = mlft_d5()
+    nf, H, obs, comps = mlft_d5(axis)
+    want = g["values"] if axis == "z" else g["values_y"]
+    names = ["S%s_3d" % axis, "L%s_3d" % axis] + g["observables"][6:]
     restr = [("1" * 20, 15, 15), ("1" * 10 + "0" * 10, 5, 7)]
     dets = oracle.enumerate_sector(nf, restr)
     assert len(dets) == 252 + 2100 + 5400
     apply, to_vec = _sector_ops(nf, dets, restr)
     h = oracle.Op(nf, H)
 
+    dtype = float if axis == "z" else complex
+
     def mv(x):
         y = to_vec(*apply(h, np.asarray(x).ravel(), restricted=True))
-        return y.real
+        return y.real if dtype is float else y
 
-    w, V = eigsh(LinearOperator((len(dets),) * 2, matvec=mv, dtype=float), k=6, which="SA", ncv=40, tol=1e-14)
+    w, V = eigsh(LinearOperator((len(dets),) * 2, matvec=mv, dtype=dtype), k=6, which="SA", ncv=40, tol=1e-14)
     ops = {k: oracle.Op(nf, t) for k, t in obs.items()}
     cops = {k: [oracle.Op(nf, c) for c in v] for k, v in comps.items()}
     for r, i in enumerate(np.argsort(w)):
@@ -421,6 +427,6 @@ def test_mlft_d5_expectation_values_golden():
         row = [w[i]]
         for k in "SLJ":
             row.append(sum(np.vdot(v, v).real for _, v in (apply(c, x) for c in cops[k])))
-        for k in g["observables"][4:]:
+        for k in names:
             row.append(np.vdot(x, to_vec(*apply(ops[k], x))).real)
-        assert np.max(np.abs(np.array(row) - np.array(g["values"][r]))) < 2e-8, r
+        assert np.max(np.abs(np.array(row) - np.array(want[r]))) < 2e-8, r
