@@ -158,6 +158,28 @@ def _shell_args(args):
     return list(args[0]), list(args[1])
 
 
+def PotentialExpandedOnClm(group: str, l: int, energies):
+    """A crystal-field potential on renormalized spherical harmonics,
+    [[k, m, A_km], ...], whose eigenvalues on an l shell are the given energies of
+    the irreducible representations (LuaPotentialExpandedOnClm,
+    LuaInterface/ssp.cpp:908; the input to NewOperator("CF", ...)). Cubic
+    symmetry: "Oh" with l = 2 and energies {e_eg, e_t2g}, or "C1"/"Oh" with
+    l < 2 and one energy (a spherical shift). In the d shell <m|C^4_0|m> is
+    2/7 for m = 0 (eg), -4/21 for |m| = 1 (t2g) and traceless, so
+    A00 = (2 e_eg + 3 e_t2g)/5, A40 = 21/10 (e_eg - e_t2g) and the cubic
+    invariant fixes A4+-4 = sqrt(5/14) A40."""
+    e = [float(x) for x in np.atleast_1d(energies)]
+    if group == "C1" or (group == "Oh" and l < 2):
+        return [[0, 0, e[0]]]
+    if group == "Oh" and l == 2:
+        if len(e) != 2:
+            raise QSError("PotentialExpandedOnClm(\"Oh\", 2, {e_eg, e_t2g}) needs two energies")
+        a40 = 2.1 * (e[0] - e[1])
+        a44 = math.sqrt(5.0 / 14.0) * a40
+        return [[0, 0, (2 * e[0] + 3 * e[1]) / 5], [4, 0, a40], [4, -4, a44], [4, 4, a44]]
+    raise QSError(f"Potential of \"{group}\" symmetry for angular momentum {l} not found")
+
+
 def NewOperator(*args, **kw):
     """NewOperator(NF, NB, {{i, -j, ..., v}, ...}) or a named operator:
     "Cr"/"An" (NF, i), "Number" (NF, idxA, idxB, values), "U" (NF, up, dn, F) or

@@ -118,3 +118,60 @@ def test_named_angular_momentum_operators_match_dense():
     idx = [i for i in range(2 ** nf) if bin(i).count("1") == 2]
     ev = np.linalg.eigvalsh(M["Lsqr"][np.ix_(idx, idx)])
     assert set(np.round(ev).astype(int)) == {0, 2, 6}
+
+
+@pytest.mark.parametrize("energies", [(0.6, -0.4), (1.0, 0.0), (0.0, 1.0), (-0.3, 2.2)])
+def test_potential_oh_d_equals_eg_t2g_projectors(energies):
+    """NewOperator("CF", ..., PotentialExpandedOnClm("Oh", 2, {e_eg, e_t2g})) is
+    e_eg P_eg + e_t2g P_t2g on the d shell (the cubic harmonics eg = {z^2,
+    x^2-y^2}, t2g = {xy, yz, zx}), and between two shells the same projector
+    coupling (TestScripts/Eigensystem.Quanty:894-930 builds 10Dq, the eg/t2g
+    counts and the hybridization this way)."""
+    from quanty_amd import operators as O
+    from quanty_amd.api import PotentialExpandedOnClm
+    akm = PotentialExpandedOnClm("Oh", 2, list(energies))
+    up, dn = [1, 3, 5, 7, 9], [0, 2, 4, 6, 8]
+    upl, dnl = [11, 13, 15, 17, 19], [10, 12, 14, 16, 18]
+    eg, t2g = O.oh_d_projectors()
+    M = O.d_shell_matrix_to_spin(energies[0] * eg + energies[1] * t2g)
+    md, ml = O.shell_modes(up, dn), O.shell_modes(upl, dnl)
+    for got, want in ((NewOperator("CF", 20, up, dn, akm), O.one_body(md, md, M)),
+                      (NewOperator("CF", 20, up, dn, upl, dnl, akm), O.one_body(md, ml, M))):
+        diff = got - Operator(20, want)
+        assert max([abs(v) for v in diff.terms.values()] or [0.0]) < 1e-14
+    assert PotentialExpandedOnClm("Oh", 1, [0.5]) == [[0, 0, 0.5]]
+    with pytest.raises(QSError):
+        PotentialExpandedOnClm("D4h", 2, [1, 2, 3, 4])
+
+
+def test_mlft_hamiltonian_as_the_script_writes_it():
+    """The d5 ligand-field Hamiltonian written with the named operators the way
+    TestScripts/Eigensystem.Quanty:794-990 writes it (NewOperator "U", "ldots",
+    "Sz", "Lz", "CF" with PotentialExpandedOnClm, "Number") equals the model
+    whose eigenstates reproduce the script's expectation-value table."""
+    from models import mlft_d5
+    from quanty_amd.api import PotentialExpandedOnClm
+    NF = 20
+    up_d, dn_d = [1, 3, 5, 7, 9], [0, 2, 4, 6, 8]
+    up_l, dn_l = [11, 13, 15, 17, 19], [10, 12, 14, 16, 18]
+    nd, U, Delta = 5, 5.5, 8.0
+    F2, F4, tenDq, tenDqL, Veg, Vt2g, zeta, B = 9.35, 5.78, 0.67, 1.06, 1.92, 1.15, 0.04, 1e-6
+    ed = (10 * Delta - nd * (19 + nd) * U / 2) / (10 + nd)
+    eL = nd * ((1 + nd) * U / 2 - Delta) / (10 + nd)
+    F0 = U + (F2 + F4) * 2 / 63
+    akm = PotentialExpandedOnClm("Oh", 2, [0.6, -0.4])
+    eg = PotentialExpandedOnClm("Oh", 2, [1, 0])
+    t2g = PotentialExpandedOnClm("Oh", 2, [0, 1])
+    N_3d = NewOperator("Number", NF, up_d, up_d, [1] * 5) + NewOperator("Number", NF, dn_d, dn_d, [1] * 5)
+    N_L = NewOperator("Number", NF, up_l, up_l, [1] * 5) + NewOperator("Number", NF, dn_l, dn_l, [1] * 5)
+    H = (F0 * NewOperator("U", NF, up_d, dn_d, [1, 0, 0]) + F2 * NewOperator("U", NF, up_d, dn_d, [0, 1, 0])
+         + F4 * NewOperator("U", NF, up_d, dn_d, [0, 0, 1]) + zeta * NewOperator("ldots", NF, up_d, dn_d)
+         + B * (2 * NewOperator("Sz", NF, up_d, dn_d) + NewOperator("Lz", NF, up_d, dn_d))
+         + tenDq * NewOperator("CF", NF, up_d, dn_d, akm) + tenDqL * NewOperator("CF", NF, up_l, dn_l, akm)
+         + Veg * (NewOperator("CF", NF, up_d, dn_d, up_l, dn_l, eg) + NewOperator("CF", NF, up_l, dn_l, up_d, dn_d, eg))
+         + Vt2g * (NewOperator("CF", NF, up_d, dn_d, up_l, dn_l, t2g)
+                   + NewOperator("CF", NF, up_l, dn_l, up_d, dn_d, t2g))
+         + ed * N_3d + eL * N_L)
+    _, terms, _, _ = mlft_d5("z")
+    diff = H - Operator(NF, terms)
+    assert max([abs(v) for v in diff.terms.values()] or [0.0]) < 1e-13
