@@ -907,8 +907,19 @@ def CreateSpectra(H: Operator, T, psi, options=None):
     for tp in tpsi:
         w, _ = tp.host()
         cv |= _counts_of_words(nf, groups, w)
-    if len(cv) != 1:
-        raise QSError(f"T psi spans {len(cv)} sectors of the conserved mode groups; one is supported")
+    if len(cv) > 1:
+        # the T psi fall in different sectors (e.g. T = c+ on modes of two
+        # decoupled groups): merge the groups whose occupation differs into one
+        # group with a fixed total, a space H leaves invariant that holds every
+        # T psi (the extra sectors it may add are not coupled to them)
+        vary = [k for k in range(len(groups)) if len({c[k] for c in cv}) > 1]
+        totals = {sum(c[k] for k in vary) for c in cv}
+        if len(totals) != 1:
+            raise QSError("T psi holds different particle numbers on the modes H couples; one sector is supported")
+        keep = [k for k in range(len(groups)) if k not in vary]
+        c0 = next(iter(cv))
+        groups = [groups[k] for k in keep] + [sorted(m for k in vary for m in groups[k])]
+        cv = {tuple(c0[k] for k in keep) + (totals.pop(),)}
     rows = _sector_of_counts(nf, groups, cv.pop(), R, ctx)
     for i, tp in enumerate(tpsi):
         w, c = tp.host()
@@ -933,15 +944,12 @@ def CreateSpectra(H: Operator, T, psi, options=None):
     V0 = torch.stack([tp._vec for tp in tpsi]).contiguous()
     A, B, kept = hop.block_lanczos(V0, ntri, float(o["SingularValue"]))
     ne = int(o["NE"])
-    eminpole = emaxpole = None
-    if o["Emin"] is None or o["Emax"] is None:
-        # default range from the eigenvalues of the block tridiagonal matrix
-        # (LuaInterface/ssp.cpp, tensor Lanczos branch: val[0], val[N*BlockSize-1])
-        ev = _block_tridiag_eigenvalues(A, B, kept, ctx)
-        eminpole, emaxpole = ev[0] - psiEn[0], ev[-1] - psiEn[0]
-        emin, emax = _default_range(o, eminpole, emaxpole)
-    else:
-        emin, emax = float(o["Emin"]), float(o["Emax"])
+    # the pole range is the extreme eigenvalues of the block tridiagonal matrix,
+    # reported whether or not Emin/Emax are given, and the default energy range
+    # follows from it (LuaInterface/ssp.cpp:2824-2846, tensor Lanczos branch)
+    ev = _block_tridiag_eigenvalues(A, B, kept, ctx)
+    eminpole, emaxpole = ev[0] - psiEn[0], ev[-1] - psiEn[0]
+    emin, emax = _default_range(o, eminpole, emaxpole)
     gamma = float(o["Gamma"]) or 10 * (emax - emin) / ne
     G = block_cf_spectrum(A, B, kept, psiEn[0], emin, emax, ne, gamma, ctx)
     return (Spectra(emin, emax, ne, gamma, G.reshape(ne + 1, -1).T, eminpole, emaxpole),

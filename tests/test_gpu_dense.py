@@ -9,6 +9,8 @@
   space) and against the exact resolvent when NTri is below the dimension.
 Tolerances: eigenvalues 1e-12 of the spectral radius; spectra 1e-10 relative
 L-infinity (dense vs resolvent) and the parity bar 1e-8 (dense vs Lanczos)."""
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -16,6 +18,7 @@ import torch
 import oracle
 from models import chain, chain_states, dense_from_terms, jw_matrices
 from quanty_amd import det as D
+from quanty_amd import operators as O
 from quanty_amd import problems as P
 from quanty_amd.api import CreateSpectra, NewOperator, NewWavefunction, Operator, operator_psi
 from quanty_amd.engine import Basis, Hop, Op, dense_poles, pole_spectrum
@@ -454,3 +457,84 @@ def test_closure_two_word_with_restrictions(ctx):
     b = Basis.closure(Op(nf, terms, ctx), Basis.from_words(nf, seed, ctx), restr)
     want = oracle_closure(nf, terms, seed, restrictions=restr)
     assert b.n == len(want) and np.array_equal(b.words(), want)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_create_spectra_pole_range_tensor(cplx):
+    """TestScripts/CreateSpectraPoleRange.Quanty: a one-body H from the 20 x 20
+    matrix H_ij = cos(i+2j) + cos(2i+j) (+ i sin(i-j)) + i delta_ij (1-based),
+    T = c+_0, c+_1, c+_2 on the vacuum, tensor block Lanczos (DenseBorder 0),
+    explicit Emin/Emax, NTri 5. The chain must not deflate (15 x 15 block
+    tridiagonal matrix) and EminPole/EmaxPole are its extreme eigenvalues (1e-10),
+    which by interlacing lie inside H's one-particle spectrum."""
+    N = 20
+    i = np.arange(1, N + 1)[:, None]
+    j = np.arange(1, N + 1)[None, :]
+    M = np.cos(i + 2 * j) + np.cos(2 * i + j) + np.diag(np.arange(1, N + 1)).astype(float)
+    if cplx:
+        M = M + 1j * np.sin(i - j)
+    H = Operator(N, O.one_body(list(range(N)), list(range(N)), M))
+    T = [NewOperator(N, 0, [[k, 1]]) for k in range(3)]
+    psi0 = NewWavefunction(N, 0, [["0" * N, 1]])
+    S, tab = CreateSpectra(H, T, psi0, [["Tensor", True], ["DenseBorder", 0], ["Emin", -30], ["Emax", 30],
+                                         ["NE", 100], ["Gamma", 0.1], ["NTri", 5]])
+    A, B, kept = tab["A"], tab["B"], tab["kept"]
+    sizes = [int(k) for k in kept[1:] if k > 0]
+    assert sum(sizes) == 15 and S.Emin == -30 and S.Emax == 30
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    Tm = np.zeros((15, 15), dtype=complex)
+    for l, p in enumerate(sizes, 1):
+        Tm[off[l - 1]:off[l], off[l - 1]:off[l]] = A[l][:p, :p]
+        if l < len(sizes):
+            q = sizes[l]
+            Tm[off[l - 1]:off[l], off[l]:off[l] + q] = B[l][:p, :q]
+            Tm[off[l]:off[l] + q, off[l - 1]:off[l]] = B[l][:p, :q].conj().T
+    val = np.linalg.eigvalsh(Tm)
+    assert abs(S.EminPole - val[0]) < 1e-10 and abs(S.EmaxPole - val[-1]) < 1e-10
+    lam = np.linalg.eigvalsh(M)
+    assert lam[0] - 1e-10 <= S.EminPole and S.EmaxPole <= lam[-1] + 1e-10
+
+
+def _deflation_cases(cplx):
+    """The three one-body models of TestScripts/CreateSpectraDeflation.Quanty
+    (1-based matrix indices as there): (N, H matrix, T mode list)."""
+    def ph(k):
+        return complex(math.cos(k), math.sin(k)) if cplx else 1.0
+
+    def hop(M, i, j, t):
+        M[i - 1, j - 1] += t
+        M[j - 1, i - 1] += np.conj(t)
+
+    M = np.diag([-1, 1, 0.5, -0.5, 2, -2]).astype(complex)
+    for i, j, t in ((1, 3, 0.7 * ph(1)), (2, 4, 0.9 * ph(2)), (3, 5, 0.4), (4, 6, 0.6), (1, 2, 0.3 * ph(3))):
+        hop(M, i, j, t)
+    yield 6, M, [0, 0, 1, 1, 1]
+    M = np.diag([0, 0, 0, 0, 0, 1.5, -1.5, 3, -3]).astype(complex)
+    for i, j, t in ((1, 6, ph(1)), (2, 6, ph(2)), (3, 7, ph(3)), (4, 7, ph(4)), (5, 7, ph(5)), (6, 8, 0.8),
+                    (7, 9, 0.5)):
+        hop(M, i, j, t)
+    yield 9, M, [0, 1, 2, 3, 4]
+    M = np.diag([0.6, 0.6, -0.4, -0.4, -0.4, -3, -3, -3]).astype(complex)
+    for k in range(3):
+        hop(M, 3 + k, 6 + k, 1.2 * ph(k + 1))
+    yield 8, M, [0, 1, 2, 3, 4]
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_create_spectra_block_deflation(cplx):
+    """TestScripts/CreateSpectraDeflation.Quanty: tensor spectra from block
+    Lanczos (DenseBorder 0, NTri 10) equal the dense method's to 1e-10 of their
+    maximum when the start block is linearly dependent ({a, a, b, b, b}), when
+    the chain deflates in its second block, and when the eg orbitals decouple
+    after the first block (then the Krylov blocks hold 5 and 3 vectors)."""
+    for case, (N, M, modes) in enumerate(_deflation_cases(cplx)):
+        H = Operator(N, O.one_body(list(range(N)), list(range(N)), M))
+        T = [NewOperator(N, 0, [[k, 1]]) for k in modes]
+        psi0 = NewWavefunction(N, 0, [["0" * N, 1]])
+        opts = [["Tensor", True], ["Emin", -10], ["Emax", 10], ["NE", 200], ["Gamma", 0.2]]
+        D, _ = CreateSpectra(H, T, psi0, opts + [["DenseBorder", 100000]])
+        S, tab = CreateSpectra(H, T, psi0, opts + [["DenseBorder", 0], ["NTri", 10]])
+        d = np.max(np.abs(S.G - D.G)) / np.max(np.abs(D.G))
+        assert d < 1e-10, (case, d)
+        if case == 2:
+            assert [int(k) for k in tab["kept"][1:] if k > 0] == [5, 3]
