@@ -17,6 +17,7 @@ Conventions kept from the reference:
 from __future__ import annotations
 
 import math
+import warnings
 from typing import Iterable
 
 import numpy as np
@@ -156,6 +157,25 @@ def dagger_terms(terms: dict) -> dict:
 
 def _shell_args(args):
     return list(args[0]), list(args[1])
+
+
+def DeterminantString(nf: int, *index_lists) -> str:
+    """A determinant string of nf modes with a 1 at every mode named in the
+    index lists (LuaDeterminantString, LuaInterface/ssp.cpp:5409): the form
+    restriction masks and NewWavefunction take. Indices outside [0, nf) are
+    skipped with a warning, repeated ones count once."""
+    if not index_lists:
+        raise QSError("DeterminantString called with 1 arguments. Number of Fermions and one or several Lists "
+                      "of Indices expected.")
+    on = set()
+    for lst in index_lists:
+        for i in np.atleast_1d(lst):
+            i = int(i)
+            if 0 <= i < nf:
+                on.add(i)
+            else:
+                warnings.warn(f"Index {i} is not within [0,NF-1]=[0,{nf - 1}].")
+    return "".join("1" if i in on else "0" for i in range(int(nf)))
 
 
 def PotentialExpandedOnClm(group: str, l: int, energies):

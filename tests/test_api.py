@@ -175,3 +175,21 @@ def test_mlft_hamiltonian_as_the_script_writes_it():
     _, terms, _, _ = mlft_d5("z")
     diff = H - Operator(NF, terms)
     assert max([abs(v) for v in diff.terms.values()] or [0.0]) < 1e-13
+
+
+def test_determinant_string():
+    """TestScripts/DeterminantString.Quanty with the script's index lists spelled
+    out (22 modes: H 1s = 0-1, Fe 2s = 2-3, Fe 2p = 4-9, Fe 3s = 10-11,
+    Fe 3d = 12-21)."""
+    from quanty_amd.api import DeterminantString
+    NF = 22
+    H_1s, Fe_2p, Fe_3d = [0, 1], list(range(4, 10)), list(range(12, 22))
+    assert DeterminantString(NF, [0, 1, 2, 3]) == "1111000000000000000000"
+    assert DeterminantString(NF, Fe_2p) == "0000111111000000000000"
+    assert DeterminantString(NF, H_1s) == "1100000000000000000000"
+    assert DeterminantString(NF, list(range(2, 22))) == "0011111111111111111111"
+    assert DeterminantString(NF, H_1s, Fe_2p, Fe_3d) == "1100111111001111111111"
+    psi = NewWavefunction(NF, 0, [[DeterminantString(NF, Fe_3d), 1]])
+    assert psi == NewWavefunction(22, 0, [["0000000000001111111111", 1]])
+    with pytest.warns(UserWarning):
+        assert DeterminantString(4, [1, 1, 7]) == "0100"
