@@ -374,6 +374,30 @@ def NewWavefunction(nf, nb, table, options=None):
     return Wavefunction(nf, words, np.array([keys[r] for r in rows], dtype=np.complex128))
 
 
+def Braket(*args):
+    """Braket(a, b) = <a|b> or Braket(a, O, b) = <a|O|b> (the "Braket" global,
+    LuaInterface/ssp.cpp:6131). a and b may each be a wavefunction or a list of
+    them: a list on one side gives a list, lists on both sides a table
+    [i][j] = <a_i|(O)|b_j>. O|b_j> is formed once per b_j."""
+    if len(args) == 2:
+        a, b = args
+        op = None
+    elif len(args) == 3:
+        a, op, b = args
+    else:
+        raise QSError(f"Braket called with {len(args)} arguments, expected 2 or 3")
+    a_list, b_list = isinstance(a, (list, tuple)), isinstance(b, (list, tuple))
+    bs = list(b) if b_list else [b]
+    if op is not None:
+        bs = [operator_psi(op, x) for x in bs]
+    rows = [[dot(x, y) for y in bs] for x in (list(a) if a_list else [a])]
+    if a_list and b_list:
+        return rows
+    if a_list:
+        return [r[0] for r in rows]
+    return rows[0] if b_list else rows[0][0]
+
+
 def dot(a: Wavefunction, b: Wavefunction):
     """<a|b>, a conjugated; real when |Im| < 10(eps|Re| + DBL_MIN) (LuaWavefunctionMul)."""
     if a._basis is not None and a._basis is b._basis:

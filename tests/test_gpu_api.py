@@ -156,3 +156,46 @@ def test_eigensystem_mlft_d5_golden(axis):
     got = np.array([[dot(p, o * p).real for o in ops] for p in psis])
     want = np.array((g["values"] if axis == "z" else g["values_y"])[:6])
     assert np.max(np.abs(got - want)) < 2e-6, np.max(np.abs(got - want), axis=1)
+
+
+def test_braket_tables():
+    """TestScripts/Braket.Quanty: <psi_i|psi_j>, <psi_i|O1|psi_j> and
+    <psi_i|O2|psi_j> of three normalized 5-mode wavefunctions, with a raw-table
+    operator O1 and a one-body operator O2 from a 5 x 5 matrix, against the
+    script's values at its 1e-13; function, list and list-list forms."""
+    from quanty_amd import operators as O
+    from quanty_amd.api import Braket
+    I = 1j
+    psi1 = NewWavefunction(5, 0, [["11100", 1], ["11010", 2], ["10110", 4], ["01110", 8], ["11001", 16]]).Normalize()
+    psi2 = NewWavefunction(5, 0, [["10101", 1], ["01101", 2], ["10011", I * 4], ["01011", 8],
+                                  ["00111", I * 16]]).Normalize()
+    psi3 = NewWavefunction(5, 0, [["11100", 1], ["10110", 2], ["11001", 4], ["01101", 2 - 8 * I],
+                                  ["01011", 16]]).Normalize()
+    psis = [psi1, psi2, psi3]
+    O1 = NewOperator(5, 0, [[0, -0.0, 1, -1, 1], [0, -0.0, 0.5], [1, -1, 0.5], [3, -2, I], [2, -3, -I]])
+    M = np.array([[0.643774, 0.601109, -0.139955, -0.185993, 0.681491],
+                  [0.601109, 0.53571, 0.0323266, -0.217606, -0.641618],
+                  [-0.139955, 0.0323266, 0.333757, 0.459147, -0.572746],
+                  [-0.185993, -0.217606, 0.459147, -0.0612072, 0.0570009],
+                  [0.681491, -0.641618, -0.572746, 0.0570009, -0.206424]])
+    O2 = Operator(5, O.one_body(list(range(5)), list(range(5)), M))
+
+    def herm(d11, d12, d13, d22, d23, d33):
+        return np.array([[d11, d12, d13], [np.conj(d12), d22, d23], [np.conj(d13), np.conj(d23), d33]])
+
+    want = {None: herm(1, 0, 0.21283160644656, 1, 0.38484619247871 - 0.046648023330752 * I, 1),
+            "O1": herm(1.6480938416422, 0, 0.39067719539505 + 0.005831002916344 * I, 0.14809384164223,
+                       0.37901518956236 - 0.069972034996128 * I, 1.3159420289855),
+            "O2": herm(1.0289384498534, 0.23315867331378 - 0.11185215249267 * I,
+                       0.64167492996089 - 0.078760014128188 * I, 0.12524842170088,
+                       0.12149853368012 - 0.3540028973014 * I, 0.54756175362319)}
+    for key, op in ((None, None), ("O1", O1), ("O2", O2)):
+        mid = () if op is None else (op,)
+        W = want[key]
+        for i in range(3):
+            for j in range(3):
+                assert abs(Braket(psis[i], *mid, psis[j]) - W[i, j]) < 1e-13, (key, i, j)
+        assert np.max(np.abs(np.array(Braket(psis, *mid, psis)) - W)) < 1e-13
+        assert np.max(np.abs(np.array(Braket(psis, *mid, psi2)) - W[:, 1])) < 1e-13
+        assert np.max(np.abs(np.array(Braket(psi3, *mid, psis)) - W[2])) < 1e-13
+        assert np.max(np.abs(np.array(Braket(psis, *mid, [psi1, psi3])) - W[:, [0, 2]])) < 1e-13
