@@ -398,6 +398,18 @@ def Braket(*args):
     return rows[0] if b_list else rows[0][0]
 
 
+def BraketDiagonal(*args):
+    """Braket with lists on both sides paired element by element: the list
+    [<a_i|(O)|b_i>] ("BraketDiagonal", LuaInterface/ssp.cpp:6132); one
+    wavefunction on either side gives the same as Braket."""
+    a, b = args[0], args[-1]
+    if isinstance(a, (list, tuple)) and isinstance(b, (list, tuple)):
+        if len(a) != len(b):
+            raise QSError(f"BraketDiagonal needs lists of equal length, got {len(a)} and {len(b)}")
+        return [Braket(x, *args[1:-1], y) for x, y in zip(a, b)]
+    return Braket(*args)
+
+
 def dot(a: Wavefunction, b: Wavefunction):
     """<a|b>, a conjugated; real when |Im| < 10(eps|Re| + DBL_MIN) (LuaWavefunctionMul)."""
     if a._basis is not None and a._basis is b._basis:

@@ -159,12 +159,12 @@ def test_eigensystem_mlft_d5_golden(axis):
 
 
 def test_braket_tables():
-    """TestScripts/Braket.Quanty: <psi_i|psi_j>, <psi_i|O1|psi_j> and
+    """TestScripts/Braket.Quanty and BraketDiagonal.Quanty: <psi_i|psi_j>, <psi_i|O1|psi_j> and
     <psi_i|O2|psi_j> of three normalized 5-mode wavefunctions, with a raw-table
     operator O1 and a one-body operator O2 from a 5 x 5 matrix, against the
     script's values at its 1e-13; function, list and list-list forms."""
     from quanty_amd import operators as O
-    from quanty_amd.api import Braket
+    from quanty_amd.api import Braket, BraketDiagonal
     I = 1j
     psi1 = NewWavefunction(5, 0, [["11100", 1], ["11010", 2], ["10110", 4], ["01110", 8], ["11001", 16]]).Normalize()
     psi2 = NewWavefunction(5, 0, [["10101", 1], ["01101", 2], ["10011", I * 4], ["01011", 8],
@@ -199,3 +199,10 @@ def test_braket_tables():
         assert np.max(np.abs(np.array(Braket(psis, *mid, psi2)) - W[:, 1])) < 1e-13
         assert np.max(np.abs(np.array(Braket(psi3, *mid, psis)) - W[2])) < 1e-13
         assert np.max(np.abs(np.array(Braket(psis, *mid, [psi1, psi3])) - W[:, [0, 2]])) < 1e-13
+        # TestScripts/BraketDiagonal.Quanty: the same values, lists paired element by element
+        for i in range(3):
+            assert abs(BraketDiagonal(psis[i], *mid, psis[2 - i]) - W[i, 2 - i]) < 1e-13
+        assert np.max(np.abs(np.array(BraketDiagonal(psis, *mid, psi3)) - W[:, 2])) < 1e-13
+        assert np.max(np.abs(np.array(BraketDiagonal(psi1, *mid, psis)) - W[0])) < 1e-13
+        got = BraketDiagonal(psis, *mid, [psi3, psi1, psi2])
+        assert np.max(np.abs(np.array(got) - np.array([W[0, 2], W[1, 0], W[2, 1]]))) < 1e-13
