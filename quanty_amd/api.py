@@ -178,6 +178,21 @@ def DeterminantString(nf: int, *index_lists) -> str:
     return "".join("1" if i in on else "0" for i in range(int(nf)))
 
 
+def IndexToBitMask(nf: int, indices) -> str:
+    """The mask string of nf modes with the given modes set ("IndexToBitMask",
+    LuaInterface/ssp.cpp:6223)."""
+    return DeterminantString(nf, indices)
+
+
+def BitMaskToIndex(mask: str):
+    """(NF, [set modes]) of a mask string, spaces ignored ("BitMaskToIndex",
+    LuaInterface/ssp.cpp:6224)."""
+    bits = [c for c in str(mask) if c != " "]
+    if any(c not in "01" for c in bits):
+        raise QSError(f"BitMaskToIndex: the mask may hold only 0, 1 and spaces: {mask!r}")
+    return len(bits), [i for i, c in enumerate(bits) if c == "1"]
+
+
 def PotentialExpandedOnClm(group: str, l: int, energies):
     """A crystal-field potential on renormalized spherical harmonics,
     [[k, m, A_km], ...], whose eigenvalues on an l shell are the given energies of

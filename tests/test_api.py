@@ -193,3 +193,12 @@ def test_determinant_string():
     assert psi == NewWavefunction(22, 0, [["0000000000001111111111", 1]])
     with pytest.warns(UserWarning):
         assert DeterminantString(4, [1, 1, 7]) == "0100"
+
+
+def test_bitmask_index_codec():
+    """TestScripts/BitMaskToIndex.Quanty and IndexToBitMask.Quanty."""
+    from quanty_amd.api import BitMaskToIndex, IndexToBitMask
+    for m in ("00000 10101 000", "0000010101000", "  0000010101000", "0000010101000   "):
+        assert BitMaskToIndex(m) == (13, [5, 7, 9])
+    assert IndexToBitMask(15, [0, 1, 2]) == "111000000000000"
+    assert IndexToBitMask(15, [0, 1, 2, 14]) == "111000000000001"
