@@ -52,20 +52,26 @@ qs_status qs_ctx_destroy(qs_ctx *ctx);
 const char *qs_last_error(const qs_ctx *ctx);
 /* Device memory helpers so C callers need no HIP headers. All device memory
  * of the library (these, and what bases, operators, batches and solvers hold)
- * comes from a process-wide cache: freed blocks stay allocated and are reused
- * for later requests of a similar size, since memory handed back to the driver
- * is cleared when it is allocated again (up to a second for the multi-GB
- * blocks of large problems). Cached blocks go back to the driver when a new
- * block would leave less than 8 GiB of the device free for other allocators.
- * Free or destroy only after the work that uses the memory has completed
- * (qs_sync for an async context). */
+ * is carved from chunks of device memory that stay with the process (at least
+ * 4 GiB each; a larger block gets a chunk of its own size): the driver clears
+ * memory when it hands it out, tens of milliseconds per GB, so a stream of
+ * requests that returned and re-took its multi-GB blocks would spend seconds
+ * in the driver. A freed block is merged with its free neighbours and reused
+ * for later blocks of any size; allocating and freeing one large block
+ * reserves that much for the library. Wholly free chunks go back to the
+ * driver when a new chunk would leave less than 8 GiB of the device free for
+ * other allocators, and by qs_trim_memory. Free or destroy only after the work
+ * that uses the memory has completed (qs_sync for an async context).
+ * QS_ALLOC_TRACE=1 in the environment reports every chunk taken from or
+ * returned to the driver on stderr. */
 qs_status qs_dev_alloc(qs_ctx *ctx, size_t bytes, void **ptr);
 qs_status qs_dev_free(qs_ctx *ctx, void *ptr);
-/* Return cached blocks of ctx's device to the driver, largest first, until at
- * most keep_bytes stay cached (0: all). The cache also gives blocks back by
- * itself when an allocation would otherwise fail. */
+/* Return wholly free chunks of ctx's device to the driver, largest first,
+ * until at most keep_bytes stay free (0: every wholly free chunk). The
+ * library also gives chunks back by itself when an allocation would otherwise
+ * fail. */
 qs_status qs_trim_memory(qs_ctx *ctx, size_t keep_bytes);
-/* Bytes in use and bytes cached on ctx's device. */
+/* Bytes in live blocks and free bytes held in chunks, on ctx's device. */
 qs_status qs_memory_stats(qs_ctx *ctx, size_t *live_bytes, size_t *cached_bytes);
 qs_status qs_memcpy_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes);
 qs_status qs_memcpy_d2h(qs_ctx *ctx, void *dst, const void *src, size_t bytes);
@@ -108,7 +114,9 @@ qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
  * (Core/Hash.cpp, Core/ComplexWaveFunctionArithmetic.cpp:64-353). */
 qs_status qs_basis_enumerate(qs_ctx *ctx, int nf, int nres, const uint8_t *masks,
                              const int *lo, const int *hi, qs_basis **out);
-/* Count only (no device allocation of the determinant list). */
+/* Count only (no device allocation of the determinant list, no size limit);
+ * counts above INT64_MAX saturate at INT64_MAX. qs_basis_enumerate refuses
+ * sectors of 2^32 - 1 or more determinants. */
 qs_status qs_basis_count(qs_ctx *ctx, int nf, int nres, const uint8_t *masks,
                          const int *lo, const int *hi, int64_t *n);
 /* Build a basis from an explicit list of n determinants (host, NW words each,
@@ -126,6 +134,17 @@ qs_status qs_basis_from_words(qs_ctx *ctx, int nf, const uint64_t *words, int64_
 qs_status qs_basis_closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,
                            const uint8_t *masks, const int *lo, const int *hi,
                            int64_t max_dim, qs_basis **out);
+/* Image of a determinant set under an operator: every target t|d> != 0 of
+ * op's terms over the determinants d of `seed` (a term counts when it acts on
+ * d, whatever its amplitude), inside at most 16 restrictions; sorted, lookups
+ * by binary search. This is the output basis of the push-form OperatorPsi /
+ * OperatorPsiRestricted (Core/OperatorWaveFunction.cpp:1750-2258, 2288-2794):
+ * the determinants O|psi> reaches from psi's support, with no enumeration of
+ * the whole particle sector. QS_ENOTCONVERGED and *out = NULL when the image
+ * holds more than max_dim determinants. */
+qs_status qs_basis_image(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,
+                         const uint8_t *masks, const int *lo, const int *hi, int64_t max_dim,
+                         qs_basis **out);
 int64_t qs_basis_size(const qs_basis *b);
 int qs_basis_nf(const qs_basis *b);
 int qs_basis_words(const qs_basis *b); /* NW: 1 or 2 */
@@ -205,6 +224,15 @@ qs_status qs_norm(qs_ctx *ctx, int64_t n, const double *a, double *nrm);
  * a_out: nsteps+1 doubles, b_out: nsteps doubles (host). */
 qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, int nsteps, double *a_out,
                      double *b_out);
+/* Lanczos that keeps its Krylov basis (CreateSpectra with KrylovBasis = true):
+ * LanczosTriDiagonalizeKrylovCC (Core/Lanczos.cpp:3705-3854). Each new vector
+ * is orthogonalized twice against every kept vector, normalized, and set to
+ * zero when its squared overlaps with the kept vectors still exceed eps (the
+ * Krylov space is exhausted); a[nsteps] = <v|H|v> of the last vector.
+ * a_out: nsteps+1, b_out: nsteps doubles (host); V_out: device, nsteps vectors
+ * of n complex back to back (vector i at V_out + 2*i*n), v_0 = v0/|v0|. */
+qs_status qs_lanczos_krylov(qs_ctx *ctx, const qs_hop *h, const double *v0, int nsteps, double eps,
+                            double *a_out, double *b_out, double *V_out);
 /* Block Lanczos for k start vectors V0 (k vectors back to back, device):
  * block recurrence of LanczosBlockTriDiagonalizeCC (Core/BlockLanczos.cpp:2131-2371):
  * Loewdin-orthonormalised start block, B0 = <v|V0>, per step A_k = V_k^+ H V_k,

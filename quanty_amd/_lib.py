@@ -45,6 +45,7 @@ _SIGS = [
     ("qs_basis_count", C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, C.POINTER(_I64)]),
     ("qs_basis_from_words", C.c_int, [_P, C.c_int, _P, _I64, C.POINTER(_P)]),
     ("qs_basis_closure", C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P, _I64, C.POINTER(_P)]),
+    ("qs_basis_image", C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P, _I64, C.POINTER(_P)]),
     ("qs_basis_size", _I64, [_P]),
     ("qs_basis_nf", C.c_int, [_P]),
     ("qs_basis_words", C.c_int, [_P]),
@@ -69,6 +70,7 @@ _SIGS = [
     ("qs_dot", C.c_int, [_P, _I64, _P, _P, C.POINTER(_D), C.POINTER(_D)]),
     ("qs_norm", C.c_int, [_P, _I64, _P, C.POINTER(_D)]),
     ("qs_lanczos", C.c_int, [_P, _P, _P, C.c_int, _P, _P]),
+    ("qs_lanczos_krylov", C.c_int, [_P, _P, _P, C.c_int, C.c_double, _P, _P, _P]),
     ("qs_block_lanczos", C.c_int, [_P, _P, C.c_int, _P, C.c_int, _D, _P, _P, _P]),
     ("qs_ground_states", C.c_int, [_P, _P, C.c_int, _D, C.c_int, C.c_int, C.c_uint64, _P, _P, _P, _P, _P]),
     ("qs_ground_states_ex", C.c_int, [_P, _P, C.c_int, _D, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int,

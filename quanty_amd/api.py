@@ -26,7 +26,7 @@ import torch
 from . import det as D
 from . import operators as O
 from ._lib import QSError, default_context
-from .engine import (Basis, Hop, Op, block_cf_spectrum, broaden, cf_spectrum, dense_lowest, dense_poles,
+from .engine import (Basis, Batch, Hop, Op, block_cf_spectrum, broaden, cf_spectrum, dense_lowest, dense_poles,
                      pole_spectrum, tridiag_extremes)
 
 SQRT_EPS = math.sqrt(np.finfo(float).eps)
@@ -35,18 +35,71 @@ SQRT_EPS = math.sqrt(np.finfo(float).eps)
 # ---------------------------------------------------------------------------
 # options
 # ---------------------------------------------------------------------------
-def _options(opts, allowed: dict) -> dict:
-    """Parse a list of {name, value} pairs against `allowed` (name -> default)."""
+def _options(opts, allowed: dict, aliases: dict | None = None, unsupported: dict | None = None,
+             given: set | None = None) -> dict:
+    """Parse a list of {name, value} pairs against `allowed` (name -> default),
+    as LuaCheckOptions does: an unknown name is an error; aliases map the
+    reference's second spelling of a name onto the first (the later pair wins,
+    both write one setting); names in `unsupported` (name -> reason) are
+    options of the reference outside this engine and raise with the reason.
+    `given` receives the names the caller set."""
     out = dict(allowed)
     if opts is None:
         return out
+    aliases = aliases or {}
     items = opts.items() if isinstance(opts, dict) else opts
     for item in items:
         name, value = item[0], item[1]
+        if unsupported and name in unsupported:
+            raise QSError(f"option {name!r} is not supported: {unsupported[name]}")
+        name = aliases.get(name, name)
         if name not in allowed:
-            raise QSError(f"unknown option {name!r}; possible: {', '.join(allowed)}")
+            raise QSError(f"unknown option {name!r}; possible: {', '.join(list(allowed) + list(aliases))}")
         out[name] = value
+        if given is not None:
+            given.add(name)
     return out
+
+
+def _flag_untruncated(o, given, where):
+    """The reference drops determinants whose amplitude falls below Epsilon
+    (ComplexWaveFunctionTruncateBasis, Core/ComplexWaveFunctionArithmetic.cpp:1073);
+    this engine keeps every determinant of the space, i.e. runs at Epsilon = 0,
+    the limit the reference's results approach. A caller's Epsilon is accepted
+    and flagged."""
+    if "Epsilon" in given and float(o["Epsilon"]) > 0:
+        warnings.warn(f"{where}: Epsilon = {o['Epsilon']} is not applied; no determinant is truncated "
+                      "(the engine works at Epsilon = 0)", stacklevel=3)
+
+
+DBL_EPSILON = float(np.finfo(float).eps)
+DBL_MIN = float(np.finfo(float).tiny)
+
+
+def hermitian_class(H: "Operator", zero: float) -> int:
+    """QOperatorHermitian (Core/OperatorArithmetic.cpp:6517-6568): the normal-ordered
+    H^+ - H chopped at 100 DBL_MIN is empty -> 1 (Hermitian); empty after a chop
+    at `zero` -> 2 (Hermitian up to zero); otherwise 0."""
+    d = O.normal_order(list(dagger_terms(H.terms).items()) + [(c, -v) for c, v in H.terms.items()],
+                       100 * DBL_MIN)
+    if not d:
+        return 1
+    if all(abs(v) <= zero for v in d.values()):
+        return 2
+    return 0
+
+
+def hermitian_part(H: "Operator", zero: float) -> "Operator":
+    """The repair LuaEigensystem applies to an almost-Hermitian operator
+    (LuaInterface/ssp.cpp:4164-4205): H <- normal order of (H + H^+)/2 chopped at
+    DBL_EPSILON zero, twice."""
+    eps = DBL_EPSILON * zero
+
+    def sym(terms):
+        t = O.normal_order(list(terms.items()) + list(dagger_terms(terms).items()))
+        return {c: 0.5 * v for c, v in t.items() if abs(0.5 * v) > eps}
+
+    return Operator(H.NF, sym(sym(H.terms)), restrictions=H.Restrictions, name=H.Name)
 
 
 def _restrictions(R, nf):
@@ -180,16 +233,26 @@ def DeterminantString(nf: int, *index_lists) -> str:
 
 def IndexToBitMask(nf: int, indices) -> str:
     """The mask string of nf modes with the given modes set ("IndexToBitMask",
-    LuaInterface/ssp.cpp:6223)."""
-    return DeterminantString(nf, indices)
+    LuaIndexToBitMask, LuaInterface/ssp.cpp:6038-6062): a negative nf, or an
+    index below zero or at or above nf, is an error."""
+    nf = int(nf)
+    if nf < 0:
+        raise QSError("IndexToBitmask called with a negative value for NF")
+    idx = [int(i) for i in np.atleast_1d(indices)] if np.size(indices) else []
+    for pos, i in enumerate(idx, 1):
+        if i < 0:
+            raise QSError(f"Index at position {pos} smaller than zero ({i}).")
+        if i >= nf:
+            raise QSError(f"Index at position {pos} larger or equal than NF ({nf},{i}).")
+    on = set(idx)
+    return "".join("1" if i in on else "0" for i in range(nf))
 
 
 def BitMaskToIndex(mask: str):
-    """(NF, [set modes]) of a mask string, spaces ignored ("BitMaskToIndex",
-    LuaInterface/ssp.cpp:6224)."""
-    bits = [c for c in str(mask) if c != " "]
-    if any(c not in "01" for c in bits):
-        raise QSError(f"BitMaskToIndex: the mask may hold only 0, 1 and spaces: {mask!r}")
+    """(NF, [set modes]) of a mask string: NF counts the '0' and '1' characters,
+    every other character is skipped (LuaBitMaskToIndex, LuaInterface/ssp.cpp:6017;
+    Core/Utilities.cpp:5-31)."""
+    bits = [c for c in str(mask) if c in "01"]
     return len(bits), [i for i, c in enumerate(bits) if c == "1"]
 
 
@@ -322,7 +385,9 @@ class Wavefunction:
 
     def particle_numbers(self):
         w, _ = self.host()
-        return {sum(bin(int(x)).count("1") for x in row) for row in w}
+        if len(w) == 0:
+            return set()
+        return {int(x) for x in np.unique(np.bitwise_count(w).sum(axis=1))}
 
     # -- algebra ------------------------------------------------------------
     def _combine(self, other, fa, fb):
@@ -524,22 +589,42 @@ def _counts_of_words(nf, groups, words):
 
 
 def operator_psi(op: Operator, psi: Wavefunction, restrictions=None) -> Wavefunction:
-    """O|psi> (OperatorPsi / OperatorPsiRestricted): output over the sector the
-    operator maps psi to, filtered by the operator's or the given restrictions."""
+    """O|psi> (OperatorPsi / OperatorPsiRestricted, Core/OperatorWaveFunction.cpp:5573-5625):
+    the output holds the determinants the operator's terms reach from psi's
+    determinants (the reference's push form grows exactly that set), inside the
+    operator's or the given restrictions. When psi is large and fills much of
+    its particle sector, the sector itself (cheaper to enumerate than the
+    image) is the output basis; the coefficients are the same, the extra
+    determinants hold zeros."""
     ctx = default_context()
     if op.NF != psi.NF:
         raise QSError("operator and wavefunction have different numbers of fermions")
     nf = op.NF
     R = _restrictions(restrictions if restrictions is not None else op.Restrictions, nf)
     cols, x = psi.device(ctx)
-    npsi = psi.particle_numbers()
+    rows = None
     dn = op.particle_change()
-    if len(npsi) != 1 or len(dn) != 1:
-        raise QSError("H*psi needs a wavefunction and an operator with one fixed particle number change")
-    rows = _sector(nf, R, npsi.pop() + dn.pop(), ctx)
+    if cols.n >= _SECTOR_ROUTE_MIN and len(dn) == 1:
+        npsi = psi.particle_numbers()
+        if len(npsi) == 1:
+            n_out = npsi.pop() + dn.pop()
+            restr = list(R) + [("1" * nf, n_out, n_out)]
+            nsec = Basis.count(nf, restr)
+            if 0 <= nsec <= min(50_000_000, _SECTOR_ROUTE_RATIO * cols.n):
+                rows = Basis.enumerate(nf, restr, ctx)
+    if rows is None:
+        rows = Basis.image(op.device(ctx), cols, R or None)
+    if rows.n == 0:
+        return Wavefunction(nf, np.zeros((0, D.nwords(nf)), dtype=np.uint64), np.zeros(0, dtype=np.complex128))
     hop = Hop(op.device(ctx), rows, cols, mode="matfree")
     y = hop.apply(x)
     return Wavefunction(nf, basis=rows, vec=y)
+
+
+# psi of at least this many determinants whose particle sector (after the
+# operator) holds at most this many times as many use the sector as output basis
+_SECTOR_ROUTE_MIN = 1 << 16
+_SECTOR_ROUTE_RATIO = 64
 
 
 def _expect(op: Operator, psi: Wavefunction) -> complex:
@@ -554,46 +639,109 @@ def _expect(op: Operator, psi: Wavefunction) -> complex:
 # ---------------------------------------------------------------------------
 # Eigensystem
 # ---------------------------------------------------------------------------
-def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
-    """Lowest Npsi eigenstates (LuaEigensystem, LuaInterface/ssp.cpp:4039).
-    The basis is the particle sector of the start determinants, inside the
-    Restrictions when given (the closure of H from StartRestrictions stays in it).
-    Up to max(DenseBorder, Npsi) determinants the determinant matrix is
-    diagonalized (Householder reduction on the GPU, tridiagonal QL, the Npsi lowest
-    eigenvectors back-transformed on the GPU); above, thick-restart block Lanczos."""
-    o = _options(options, {"Restrictions": None, "Zero": 100 * SQRT_EPS, "Epsilon": SQRT_EPS / 100,
-                           "NKrylovMax": 500, "NKrylovStart": 100, "NKrylovStep": 10, "NKrylovMin": 50,
-                           "DenseBorder": 1513, "NBitsKey": 16, "ShiftScheme": "Variance", "Seed": 1,
-                           "MaxRestarts": 200})
-    ctx = default_context()
-    nf = H.NF
-    if H.particle_change() - {0}:
-        raise QSError("Eigensystem needs a particle-number conserving Hamiltonian")
-    start = _restrictions(StartRestrictions, nf)
-    groups = conserved_groups(H)
-    counts = []
+_EIG_OPTIONS = {"Restrictions": None, "Zero": 100 * SQRT_EPS, "Epsilon": SQRT_EPS / 100, "SingularValue": SQRT_EPS,
+                "ExpandBasis": True, "NKrylovMax": 500, "NKrylovStart": 100, "NKrylovStep": 10, "NKrylovMin": 50,
+                "DenseBorder": 1513, "NBitsKey": 16, "ShiftScheme": "Variance", "Seed": 1, "MaxRestarts": 200}
+_EIG_ALIASES = {"restrictions": "Restrictions", "zero": "Zero", "epsilon": "Epsilon"}
+
+
+def _start_sectors(nf, groups, start, limit=256):
+    """Occupation vectors of the conserved mode groups that determinants inside
+    the start restrictions can have: per group the counts the start allows,
+    combined and kept when some determinant has all of them at once."""
+    per = []
     for g in groups:
         m = _group_mask(nf, g)
-        cs = [c for c in range(len(g) + 1) if Basis.count(nf, start + [(m, c, c)]) > 0]
-        if len(cs) != 1:
-            raise QSError(f"StartRestrictions allow occupations {cs} of the conserved mode group {g}; "
-                          "one start sector is supported")
-        counts.append(cs[0])
-    basis = _sector_of_counts(nf, groups, counts, _restrictions(o["Restrictions"], nf), ctx)
-    if basis.n < Npsi:
-        raise QSError(f"the space has {basis.n} determinants, fewer than Npsi = {Npsi}")
-    if basis.n <= max(int(o["DenseBorder"]), Npsi):
-        # dense: the determinant matrix and its eigenvectors
-        # (BlockLanczosGroundStateConserveBasisDense, Core/BlockLanczos.cpp:370-504, 5713)
+        per.append([c for c in range(len(g) + 1) if Basis.count(nf, start + [(m, c, c)]) > 0])
+    combos = [()]
+    for g, cs in zip(groups, per):
+        combos = [t + (c,) for t in combos for c in cs]
+        if len(combos) > 16 * limit:
+            raise QSError(f"StartRestrictions span more than {limit} sectors of the modes H conserves")
+    masks = [_group_mask(nf, g) for g in groups]
+    out = [c for c in combos
+           if Basis.count(nf, start + [(m, k, k) for m, k in zip(masks, c)]) > 0] if len(combos) > 1 else combos
+    if not out or not per or any(not cs for cs in per):
+        raise QSError("no determinant satisfies the StartRestrictions")
+    if len(out) > limit:
+        raise QSError(f"StartRestrictions span {len(out)} sectors of the modes H conserves; at most {limit}")
+    return out
+
+
+def _lowest(H: Operator, basis: Basis, k: int, o: dict, ctx):
+    """The k lowest eigenpairs of H on one basis: the determinant matrix diagonalized
+    up to DenseBorder determinants (BlockLanczosGroundStateConserveBasisDense,
+    Core/BlockLanczos.cpp:370-504, 5713), thick-restart block Lanczos above."""
+    if basis.n <= max(int(o["DenseBorder"]), k):
         Hd = Hop(H.device(ctx), basis, mode="matfree").dense()
-        _, V = dense_lowest(Hd, Npsi, ctx)
-        return [Wavefunction(nf, basis=basis, vec=V[k].contiguous()) for k in range(Npsi)]
+        E, V = dense_lowest(Hd, k, ctx)
+        return E[:k], [V[i].contiguous() for i in range(k)]
     hop = Hop(H.device(ctx), basis, mode="auto")
-    E, vecs, var, it = hop.ground_states(Npsi, tol=float(o["Zero"]), max_krylov=int(o["NKrylovMax"]),
-                                         max_restarts=int(o["MaxRestarts"]), seed=int(o["Seed"]),
-                                         krylov_start=int(o["NKrylovStart"]), krylov_step=int(o["NKrylovStep"]),
-                                         cycle_end=True)
-    return [Wavefunction(nf, basis=basis, vec=vecs[i].clone()) for i in range(Npsi)]
+    E, vecs, _, _ = hop.ground_states(k, tol=float(o["Zero"]), max_krylov=int(o["NKrylovMax"]),
+                                      max_restarts=int(o["MaxRestarts"]), seed=int(o["Seed"]),
+                                      krylov_start=int(o["NKrylovStart"]), krylov_step=int(o["NKrylovStep"]),
+                                      cycle_end=True)
+    return E, [vecs[i].clone() for i in range(k)]
+
+
+def Eigensystem(H: Operator, StartRestrictions, Npsi: int, options=None):
+    """Lowest Npsi eigenstates (LuaEigensystem, LuaInterface/ssp.cpp:4039-4272).
+
+    * Options as the reference spells them (both spellings of Restrictions,
+      Zero and Epsilon). NKrylovMin is accepted and, as in the reference, not
+      used; NBitsKey sizes the reference's hash tables, which this engine does
+      not have; ShiftScheme ("Automatic", "Variance", "None") shifts the
+      reference's Krylov iteration and changes its convergence path only, the
+      thick-restart solver here needs no shift; Epsilon is flagged when set
+      (no truncation here); SingularValue is accepted (the block solver keeps
+      its start block full rank by itself).
+    * H is checked for Hermiticity as the reference does: not Hermitian up to
+      Zero raises; Hermitian up to Zero is repaired into its Hermitian part.
+    * The space: with ExpandBasis (default) every sector of the mode groups H
+      conserves that the StartRestrictions reach (the closure of H*psi the
+      reference grows, Core/BlockLanczos.cpp:5849-5858), inside the
+      Restrictions; with ExpandBasis = false the determinants of the
+      StartRestrictions themselves (BlockLanczosGroundStateConserveBasis).
+      The Npsi lowest states of every sector are found and the Npsi lowest of
+      all are returned, ascending in energy."""
+    given: set = set()
+    o = _options(options, _EIG_OPTIONS, _EIG_ALIASES, given=given)
+    _flag_untruncated(o, given, "Eigensystem")
+    if str(o["ShiftScheme"]) not in ("Automatic", "Variance", "None"):
+        raise QSError(f'Eigensystem: unknown ShiftScheme "{o["ShiftScheme"]}", possible are "Automatic", '
+                      '"Variance" and "None"')
+    nf = H.NF
+    Npsi = int(Npsi)
+    if Npsi < 1:
+        raise QSError("Eigensystem needs Npsi >= 1")
+    if H.particle_change() - {0}:
+        raise QSError("Eigensystem needs a particle-number conserving Hamiltonian")
+    herm = hermitian_class(H, float(o["Zero"]))
+    if herm == 0:
+        raise QSError("Eigensystem called with an operator that is not Hermitian")
+    if herm == 2:
+        H = hermitian_part(H, float(o["Zero"]))
+    start = _restrictions(StartRestrictions, nf)
+    R = _restrictions(o["Restrictions"], nf)
+    ctx = default_context()
+    if not o["ExpandBasis"]:
+        bases = [Basis.enumerate(nf, start + R, ctx)]
+    else:
+        groups = conserved_groups(H)
+        bases = [_sector_of_counts(nf, groups, c, R, ctx) for c in _start_sectors(nf, groups, start)]
+    found = []
+    total = 0
+    for basis in bases:
+        total += basis.n
+        k = min(Npsi, basis.n)
+        if k == 0:
+            continue
+        E, vecs = _lowest(H, basis, k, o, ctx)
+        found += [(float(E[i]), len(found), Wavefunction(nf, basis=basis, vec=vecs[i])) for i in range(k)]
+    if len(found) < Npsi:
+        raise QSError(f"the space has {total} determinants, fewer than Npsi = {Npsi}")
+    found.sort(key=lambda t: (t[0], t[1]))
+    return [w for _, _, w in found[:Npsi]]
 
 
 # ---------------------------------------------------------------------------
@@ -910,6 +1058,39 @@ def _tri_spectra(A, Bv, psiEn, o, ctx):
     return Spectra(float(emin), float(emax), ne, gamma, G, eminpole, emaxpole), tables
 
 
+_CS_OPTIONS = {"Tensor": False, "KrylovBasis": False, "NTri": 200, "Epsilon": SQRT_EPS / 100,
+               "SingularValue": SQRT_EPS, "Restrictions": None, "Emin": None, "Emax": None, "NE": 1000,
+               "Gamma": 0.0, "E0": None, "DenseBorder": 1513, "ForceDenseMethod": False, "Method": None}
+_CS_UNSUPPORTED = {n: "bath-excitation, grid and file options of the reference are outside this engine"
+                   for n in ("GridPoints", "NTriFull", "NBathExcitations", "NonCrossing", "file")}
+QS_KMAX = 16  # vectors in one block of the block Lanczos (qs_block_lanczos)
+
+
+def _lanczos_many(hop: Hop, vs: list, ntri: int, ctx):
+    """Lanczos tables of several start vectors on one operator: all of them as
+    slots of one batch, advanced together (one set of launches per step), when
+    the binding is assembled; one after another otherwise. Each slot's
+    recurrence is that of qs_lanczos (LanczosTriDiagonalizeCC on a fixed basis)."""
+    if not vs:
+        return []
+    if not hop.stats()["assembled"] or len(vs) == 1:
+        return [hop.lanczos(v, ntri) for v in vs]
+    out = []
+    for g in range(0, len(vs), _BATCH_SLOTS):
+        part = vs[g:g + _BATCH_SLOTS]
+        batch = Batch(len(part), ctx)
+        slots = [batch.add(hop, v, ntri) for v in part]
+        batch.run(ntri)
+        steps, a, b = batch.coeffs(slots, ntri + 1)
+        if np.any(steps < ntri):
+            raise QSError("batched Lanczos stopped early")
+        out += [(a[i, :ntri + 1].copy(), b[i, :ntri].copy()) for i in range(len(part))]
+    return out
+
+
+_BATCH_SLOTS = 64
+
+
 def CreateSpectra(H: Operator, T, psi, options=None):
     """Spectra of transition operators T on wavefunctions psi (LuaCreateSpectra,
     LuaInterface/ssp.cpp:1906): E0 = <psi|H|psi> unless given, T psi (restricted
@@ -923,17 +1104,25 @@ def CreateSpectra(H: Operator, T, psi, options=None):
         (T, psi) — or one block Lanczos over all of them with Tensor = true —
         and the continued fraction; the tables are tridiagonal.
     Spectra on NE+1 points with Lorentzian FWHM Gamma."""
-    o = _options(options, {"Tensor": False, "KrylovBasis": False, "NTri": 200, "epsilon": SQRT_EPS / 100,
-                           "Epsilon": SQRT_EPS / 100, "SingularValue": SQRT_EPS, "restrictions": None,
-                           "Restrictions": None, "Emin": None, "Emax": None, "NE": 1000, "Gamma": 0.0,
-                           "E0": None, "DenseBorder": 1513, "ForceDenseMethod": False, "Method": None})
-    ctx = default_context()
+    given: set = set()
+    o = _options(options, _CS_OPTIONS, {"epsilon": "Epsilon", "restrictions": "Restrictions"}, _CS_UNSUPPORTED,
+                 given=given)
+    _flag_untruncated(o, given, "CreateSpectra")
     Ts = list(T) if isinstance(T, (list, tuple)) else [T]
     psis = list(psi) if isinstance(psi, (list, tuple)) else [psi]
     if not Ts or not psis:
         raise QSError("CreateSpectra called with an empty table of operators or wavefunctions")
     nf = H.NF
-    R = _restrictions(o["Restrictions"] if o["Restrictions"] is not None else o["restrictions"], nf)
+    # Hermiticity as LuaCreateSpectra tests it (LuaInterface/ssp.cpp:2180-2203), at
+    # the default convergence zero (Core/StructDef.cpp:302): an operator that is
+    # not Hermitian is an error; one Hermitian up to that zero is used as given
+    if hermitian_class(H, 100 * SQRT_EPS) == 0:
+        raise QSError("CreateSpectra called with an operator that is not Hermitian")
+    if o["Tensor"] and len(Ts) * len(psis) > QS_KMAX:
+        raise QSError(f"Tensor CreateSpectra holds {len(Ts) * len(psis)} (T, psi) pairs; the block Lanczos "
+                      f"takes at most {QS_KMAX}")
+    R = _restrictions(o["Restrictions"], nf)
+    ctx = default_context()
     ntri = int(o["NTri"])
     method = str(o["Method"] or "Auto").lower()
     if method not in ("auto", "dense", "lanczos"):
@@ -1005,10 +1194,25 @@ def CreateSpectra(H: Operator, T, psi, options=None):
     if not o["Tensor"]:
         A = np.zeros((nsp, ntri + 1))
         Bv = np.zeros((nsp, ntri))
-        for i, tp in enumerate(tpsi):
-            if not zero[i]:  # an empty T psi keeps its zero table
-                A[i], Bv[i] = hop.lanczos(tp._vec.contiguous(), ntri)
+        live = [i for i in range(nsp) if not zero[i]]  # an empty T psi keeps its zero table
+        if o["KrylovBasis"]:
+            # LanczosTriDiagonalizeKrylov (Core/Lanczos.cpp:3705): fully
+            # re-orthogonalized, the Krylov vectors returned as wavefunctions, the
+            # empty ones left out (LuaPushTable_WaveFunctionType_RemoveEmpty,
+            # LuaInterface/ssp.cpp:2730-2752)
+            kb = [[] for _ in range(nsp)]
+            for i in live:
+                A[i], Bv[i], V = hop.lanczos_krylov(tpsi[i]._vec.contiguous(), ntri, float(o["Epsilon"]))
+                kb[i] = [Wavefunction(nf, basis=rows, vec=V[j].clone()) for j in range(ntri)
+                         if bool(torch.any(V[j] != 0))]
+            sp, tables = _tri_spectra(A, Bv, psiEn, o, ctx)
+            return sp, tables, (kb[0] if nsp == 1 else kb)
+        coef = _lanczos_many(hop, [tpsi[i]._vec.contiguous() for i in live], ntri, ctx)
+        for i, (a, b) in zip(live, coef):
+            A[i], Bv[i] = a, b
         return _tri_spectra(A, Bv, psiEn, o, ctx)
+    if o["KrylovBasis"]:
+        raise QSError("KrylovBasis with Tensor is supported by the dense method only (Method \"Dense\")")
     # tensor: one block Lanczos over all T psi, (nsp x nsp) spectra
     if len(set(psiEn)) != 1:
         raise QSError("Tensor spectra need one E0 for all wavefunctions (pass E0)")

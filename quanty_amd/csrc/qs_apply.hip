@@ -158,15 +158,12 @@ extern "C" qs_status qs_op_create(qs_ctx *ctx, int nf, int nterms, const int *le
     qs_op_destroy(op);
     return st;
   }
-  hipError_t e = hipMemcpyAsync(op->d_terms, buf.data(), buf.size(), hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(op->d_val, val.data(), val.size() * 8, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(op->d_par, par.data(), par.size() * 8, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) {
+  st = qs_h2d(ctx, op->d_terms, buf.data(), buf.size());
+  if (st == QS_OK) st = qs_h2d(ctx, op->d_val, val.data(), val.size() * 8);
+  if (st == QS_OK) st = qs_h2d(ctx, op->d_par, par.data(), par.size() * 8);
+  if (st != QS_OK) {
     qs_op_destroy(op);
-    return qs_hip_fail(ctx, e, "qs_op_create", __FILE__, __LINE__);
+    return st;
   }
   *out = op;
   return QS_OK;
@@ -703,11 +700,10 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   std::vector<uint32_t> slen((size_t)nslices), bcnt((size_t)nslices);
   unsigned long long c[3] = {0, 0, 0};
   if (nslices) {
-    QS_HIP(ctx, hipMemcpyAsync(slen.data(), h->d_slice_len, (size_t)nslices * 4, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    QS_HIP(ctx, hipMemcpyAsync(bcnt.data(), blk_cnt, (size_t)nslices * 4, hipMemcpyDeviceToHost, ctx->stream));
+    QS_TRY(qs_d2h(ctx, slen.data(), h->d_slice_len, (size_t)nslices * 4));
+    QS_TRY(qs_d2h(ctx, bcnt.data(), blk_cnt, (size_t)nslices * 4));
   }
-  QS_HIP(ctx, hipMemcpyAsync(c, ctr, 24, hipMemcpyDeviceToHost, ctx->stream));
+  QS_TRY(qs_d2h(ctx, c, ctr, 24));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<uint64_t> sptr((size_t)nslices + 1, 0), bptr((size_t)nslices + 1, 0);
   for (int64_t s = 0; s < nslices; ++s) {
@@ -730,8 +726,8 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   QS_TRY(qs_malloc(ctx, (void **)&h->d_col, (size_t)std::max<uint64_t>(ent, 1) * 4));
   if (!h->view.packed) QS_TRY(qs_malloc(ctx, (void **)&h->d_tid, (size_t)std::max<uint64_t>(ent, 1) * 2));
   QS_TRY(qs_malloc(ctx, (void **)&h->d_blk, (size_t)std::max<uint64_t>(nblk, 1) * 16));
-  QS_HIP(ctx, hipMemcpyAsync(h->d_slice_ptr, sptr.data(), sptr.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-  QS_HIP(ctx, hipMemcpyAsync(h->d_blk_ptr, bptr.data(), bptr.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  QS_TRY(qs_h2d(ctx, h->d_slice_ptr, sptr.data(), sptr.size() * 8));
+  QS_TRY(qs_h2d(ctx, h->d_blk_ptr, bptr.data(), bptr.size() * 8));
   if (n > 0) {
     if (op->nw == 1) launch_build<1, true>(ctx, h, blocks, L, ctr, blk_cnt);
     else launch_build<2, true>(ctx, h, blocks, L, ctr, blk_cnt);

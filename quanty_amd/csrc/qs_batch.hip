@@ -103,6 +103,14 @@ k_b_init(SlotDev *slots, int slot, const double2 *__restrict__ v0, double *__res
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// A new slot's descriptor (passed by value, so no host copy is staged) and
+// its zeroed coefficient arrays.
+__global__ void k_b_slot_set(SlotDev *slots, int slot, SlotDev S, double *__restrict__ scal, int nscal) {
+  for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x); i < nscal; i += (int)(gridDim.x * blockDim.x))
+    scal[i] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) slots[slot] = S;
+}
+
 __global__ void __launch_bounds__(QS_BLOCK)
 k_b_init_finish(SlotDev *slots, int slot, const double *__restrict__ part, int np) {
   __shared__ double red[QS_BLOCK];
@@ -374,13 +382,13 @@ static qs_status rebuild(qs_batch *B) {
     QS_TRY(qs_malloc(ctx, (void **)&B->d_p2, (size_t)B->cap2 * 8));
   }
   if (!w1.empty())
-    QS_HIP(ctx, hipMemcpyAsync(B->d_w1, w1.data(), w1.size() * sizeof(WorkItem), hipMemcpyHostToDevice, ctx->stream));
+    QS_TRY(qs_h2d(ctx, B->d_w1, w1.data(), w1.size() * sizeof(WorkItem)));
   if (!w2.empty())
-    QS_HIP(ctx, hipMemcpyAsync(B->d_w2, w2.data(), w2.size() * sizeof(WorkItem), hipMemcpyHostToDevice, ctx->stream));
-  QS_HIP(ctx, hipMemcpyAsync(B->d_r1, r1.data(), r1.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-  QS_HIP(ctx, hipMemcpyAsync(B->d_r2, r2.data(), r2.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    QS_TRY(qs_h2d(ctx, B->d_w2, w2.data(), w2.size() * sizeof(WorkItem)));
+  QS_TRY(qs_h2d(ctx, B->d_r1, r1.data(), r1.size() * 4));
+  QS_TRY(qs_h2d(ctx, B->d_r2, r2.data(), r2.size() * 4));
   const int64_t counts[2] = {(int64_t)w1.size(), (int64_t)w2.size()};
-  QS_HIP(ctx, hipMemcpyAsync(B->d_n, counts, sizeof counts, hipMemcpyHostToDevice, ctx->stream));
+  QS_TRY(qs_h2d(ctx, B->d_n, counts, sizeof counts));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (old_ptrs[0] != B->d_w1 || old_ptrs[1] != B->d_w2 || old_ptrs[2] != B->d_p1 || old_ptrs[3] != B->d_p2)
     drop_graphs(B);
@@ -477,13 +485,18 @@ extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double 
   S.active = 1;
   S.pad = max_depth;  // capacity of the coefficient arrays
   hipError_t e = hipSuccess;
-  if (want_y) e = hipMemcpyAsync(yc, ycoef, (size_t)depth * 8, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(scal, 0, 3 * sc, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(B->d_slots + slot, &S, sizeof(SlotDev), hipMemcpyHostToDevice, ctx->stream);
+  qs_status st = QS_OK;
+  if (want_y) st = qs_h2d(ctx, yc, ycoef, (size_t)depth * 8);
+  if (st == QS_OK) {
+    const int nscal = 3 * (max_depth + 1);
+    hipLaunchKernelGGL(k_b_slot_set, dim3((unsigned)std::min(64, (nscal + QS_BLOCK - 1) / QS_BLOCK)), dim3(QS_BLOCK),
+                       0, ctx->stream, B->d_slots, slot, S, scal, nscal);
+    e = hipGetLastError();
+  }
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + QS_BLOCK - 1) / QS_BLOCK));
   double *part = nullptr;
-  void *scr;
-  qs_status st = qs_scratch(ctx, (size_t)nb * 8, &scr);
+  void *scr = nullptr;
+  if (st == QS_OK) st = qs_scratch(ctx, (size_t)nb * 8, &scr);
   part = (double *)scr;
   if (st == QS_OK && e == hipSuccess) {
     hipLaunchKernelGGL(k_b_init, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, (const double2 *)v0, part);
@@ -629,12 +642,12 @@ extern "C" qs_status qs_batch_poll(qs_batch *B, int slot, int *step, double *a, 
   qs_ctx *ctx = B->ctx;
   QS_HIP(ctx, hipSetDevice(ctx->device));
   SlotDev S;
-  QS_HIP(ctx, hipMemcpyAsync(&S, B->d_slots + slot, sizeof(SlotDev), hipMemcpyDeviceToHost, ctx->stream));
+  QS_TRY(qs_d2h(ctx, &S, B->d_slots + slot, sizeof(SlotDev)));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (step) *step = S.step;
   const SlotDev &H = B->h_slots[slot];
-  if (a) QS_HIP(ctx, hipMemcpyAsync(a, H.a, (size_t)(H.depth + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
-  if (b) QS_HIP(ctx, hipMemcpyAsync(b, H.b, (size_t)(H.depth + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  if (a) QS_TRY(qs_d2h(ctx, a, H.a, (size_t)(H.depth + 1) * 8));
+  if (b) QS_TRY(qs_d2h(ctx, b, H.b, (size_t)(H.depth + 1) * 8));
   if (y) {
     if (!H.y) return qs_fail(ctx, QS_EINVAL, "slot %d accumulates no vector", slot);
     QS_HIP(ctx, hipMemcpyAsync(y, H.y, (size_t)H.n * 16, hipMemcpyDeviceToDevice, ctx->stream));
@@ -650,12 +663,12 @@ extern "C" qs_status qs_batch_ritz(qs_batch *B, int slot, int m, const double *z
   if (!H.V) return qs_fail(ctx, QS_EINVAL, "slot %d keeps no Lanczos vectors", slot);
   QS_HIP(ctx, hipSetDevice(ctx->device));
   SlotDev S;
-  QS_HIP(ctx, hipMemcpyAsync(&S, B->d_slots + slot, sizeof(SlotDev), hipMemcpyDeviceToHost, ctx->stream));
+  QS_TRY(qs_d2h(ctx, &S, B->d_slots + slot, sizeof(SlotDev)));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (m > S.step) return qs_fail(ctx, QS_EINVAL, "Ritz vector over %d vectors, slot %d has %d", m, slot, S.step);
   void *scr;
   QS_TRY(qs_scratch(ctx, (size_t)std::max(m, 1) * 8, &scr));
-  if (m > 0) QS_HIP(ctx, hipMemcpyAsync(scr, z, (size_t)m * 8, hipMemcpyHostToDevice, ctx->stream));
+  if (m > 0) QS_TRY(qs_h2d(ctx, scr, z, (size_t)m * 8));
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((H.n + QS_BLOCK - 1) / QS_BLOCK, (int64_t)ctx->num_cu * 8));
   hipLaunchKernelGGL(k_b_ritz, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, m, (const double *)scr,
                      (double2 *)y);
@@ -669,7 +682,7 @@ extern "C" qs_status qs_batch_remove(qs_batch *B, int slot) {
   qs_ctx *ctx = B->ctx;
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   SlotDev z{};
-  QS_HIP(ctx, hipMemcpyAsync(B->d_slots + slot, &z, sizeof(SlotDev), hipMemcpyHostToDevice, ctx->stream));
+  QS_TRY(qs_h2d(ctx, B->d_slots + slot, &z, sizeof(SlotDev)));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   pool_put(B, B->mem[slot], B->mem_bytes[slot]);
   B->mem[slot] = nullptr;
@@ -718,11 +731,11 @@ extern "C" qs_status qs_batch_coeffs(qs_batch *B, int n, const int *slots, int l
   QS_TRY(qs_scratch(ctx, (size_t)n * 4 + 64 + body, &scr));
   int *d_list = (int *)scr;
   char *d_out = (char *)scr + (((size_t)n * 4 + 63) / 64) * 64;
-  QS_HIP(ctx, hipMemcpyAsync(d_list, slots, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+  QS_TRY(qs_h2d(ctx, d_list, slots, (size_t)n * 4));
   hipLaunchKernelGGL(k_b_gather, dim3(n), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, n, d_list, ld, d_out);
   QS_HIP(ctx, hipGetLastError());
   std::vector<char> h(body);
-  QS_HIP(ctx, hipMemcpyAsync(h.data(), d_out, body, hipMemcpyDeviceToHost, ctx->stream));
+  QS_TRY(qs_d2h(ctx, h.data(), d_out, body));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   memcpy(steps, h.data(), (size_t)n * 4);
   memcpy(a, h.data() + head, (size_t)n * ld * 8);
@@ -755,7 +768,7 @@ extern "C" qs_status qs_batch_set_depths(qs_batch *B, int n, const int *slots, c
   }
   void *scr;
   QS_TRY(qs_scratch(ctx, pairs.size() * 4, &scr));
-  QS_HIP(ctx, hipMemcpyAsync(scr, pairs.data(), pairs.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+  QS_TRY(qs_h2d(ctx, scr, pairs.data(), pairs.size() * 4));
   hipLaunchKernelGGL(k_b_set_depths, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, B->d_slots, n,
                      (const int *)scr);
   QS_HIP(ctx, hipGetLastError());
@@ -778,7 +791,7 @@ extern "C" qs_status qs_batch_steps(qs_batch *B, int *steps) {
   QS_TRY(qs_scratch(ctx, (size_t)B->cap * 4, &scr));
   hipLaunchKernelGGL(k_b_steps, dim3((B->cap + 255) / 256), dim3(256), 0, ctx->stream, B->d_slots, B->cap, (int *)scr);
   QS_HIP(ctx, hipGetLastError());
-  QS_HIP(ctx, hipMemcpyAsync(steps, scr, (size_t)B->cap * 4, hipMemcpyDeviceToHost, ctx->stream));
+  QS_TRY(qs_d2h(ctx, steps, scr, (size_t)B->cap * 4));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QS_OK;
 }

@@ -426,7 +426,7 @@ struct Work {
     hipLaunchKernelGGL(k_bdot_reduce, dim3(m * w), dim3(QS_BLOCK), 0, ctx->stream, (int)nch, m * w,
                        (const double2 *)vpart, vC);
     QS_HIP(ctx, hipGetLastError());
-    QS_HIP(ctx, hipMemcpyAsync(C.data(), vC, (size_t)m * w * 16, hipMemcpyDeviceToHost, ctx->stream));
+    QS_TRY(qs_d2h(ctx, C.data(), vC, (size_t)m * w * 16));
     QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return QS_OK;
   }
@@ -435,7 +435,7 @@ struct Work {
     if (w == 0) return QS_OK;
     if (w > QS_KMAX) return qs_fail(ctx, QS_EINVAL, "block wider than %d", QS_KMAX);
     QS_TRY(grow(&vC, &vC_bytes, (size_t)std::max(m, 1) * w * 16));
-    if (m > 0) QS_HIP(ctx, hipMemcpyAsync(vC, C.data(), (size_t)m * w * 16, hipMemcpyHostToDevice, ctx->stream));
+    if (m > 0) QS_TRY(qs_h2d(ctx, vC, C.data(), (size_t)m * w * 16));
 #define QS_VUPD(WC)                                                                                         \
   hipLaunchKernelGGL(k_vupd<WC>, dim3(grid_for(ctx, n)), dim3(QS_BLOCK), 0, ctx->stream, n, m, V,             \
                      (const double2 *)vC, W, (int64_t)1, n, beta)
@@ -528,7 +528,10 @@ qs_status block_apply_v(qs_ctx *ctx, const qs_hop *h, const double2 *x, int k, d
 // ---------------------------------------------------------------------------
 extern "C" qs_status qs_block_lanczos(qs_ctx *ctx, const qs_hop *h, int k, const double *V0, int nsteps,
                                       double sv_cut, double *A_out, double *B_out, int *kept_out) {
-  if (!ctx || !h || !V0 || !A_out || !B_out || k < 1 || k > QS_KMAX || nsteps < 1) return QS_EINVAL;
+  if (!ctx) return QS_EINVAL;
+  if (!h || !V0 || !A_out || !B_out || nsteps < 1) return qs_fail(ctx, QS_EINVAL, "qs_block_lanczos: bad arguments");
+  if (k < 1 || k > QS_KMAX)
+    return qs_fail(ctx, QS_EINVAL, "qs_block_lanczos: %d start vectors; a block holds 1 to %d", k, QS_KMAX);
   if (h->rows != h->cols) return qs_fail(ctx, QS_EINVAL, "qs_block_lanczos needs rows == cols");
   QS_HIP(ctx, hipSetDevice(ctx->device));
   const int64_t n = h->rows->n;
@@ -643,6 +646,79 @@ extern "C" qs_status qs_block_cf_spectrum(qs_ctx *ctx, int k, int n, const doubl
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   qs_free(mem);
   if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_block_cf_spectrum", __FILE__, __LINE__);
+  return QS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Lanczos with the Krylov basis kept (KrylovBasis = true)
+// ---------------------------------------------------------------------------
+// LanczosTriDiagonalizeKrylovCC (Core/Lanczos.cpp:3705-3854): every new vector
+// H v_{i-1} is orthogonalized against all kept vectors twice (a[i] is its
+// overlap with v_{i-1} before any subtraction), normalized (b[i] its norm;
+// below 100 DBL_MIN it becomes zero), and zeroed when its squared overlaps
+// with the kept vectors still exceed eps (the Krylov space is exhausted);
+// a[nsteps] = <v_{nsteps-1}|H|v_{nsteps-1}>. Gram-Schmidt runs in classical
+// form on the FP64 matrix cores (k_vtw / k_vupd), where the reference subtracts
+// vector by vector; both leave the basis orthonormal to working precision.
+extern "C" qs_status qs_lanczos_krylov(qs_ctx *ctx, const qs_hop *h, const double *v0, int nsteps, double eps,
+                                       double *a_out, double *b_out, double *V_out) {
+  if (!ctx) return QS_EINVAL;
+  if (!h || !v0 || !a_out || !b_out || !V_out || nsteps < 1)
+    return qs_fail(ctx, QS_EINVAL, "qs_lanczos_krylov: bad arguments");
+  if (h->rows != h->cols) return qs_fail(ctx, QS_EINVAL, "qs_lanczos_krylov needs rows == cols");
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  const int64_t n = h->rows->n;
+  double2 *V = (double2 *)V_out;
+  Work wk;
+  QS_TRY(wk.init(ctx, n));
+  double2 *Wb = nullptr;
+  QS_TRY(qs_malloc(ctx, (void **)&Wb, (size_t)std::max<int64_t>(n, 1) * 16));
+  std::vector<double> a(nsteps + 1, 0.0), b(nsteps, 0.0);
+  const double tiny = 100 * 2.2250738585072014e-308;
+  auto vec = [&](int i) { return V + (size_t)i * n; };
+  auto run = [&]() -> qs_status {
+    QS_HIP(ctx, hipMemcpyAsync(vec(0), v0, (size_t)n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    std::vector<cd> C;
+    QS_TRY(wk.vtw(wk.vs(vec(0)), 1, vec(0), 1, C));
+    b[0] = std::sqrt(std::max(C[0].real(), 0.0));
+    QS_TRY(wk.vupd(wk.vs(vec(0)), 0, C, vec(0), 1, b[0] < tiny ? 0.0 : 1.0 / b[0]));
+    for (int i = 1; i < nsteps; ++i) {
+      double2 *w = vec(i);
+      QS_TRY(block_apply_v(ctx, h, vec(i - 1), 1, w));
+      const VSet K = wk.vs(vec(0));  // the kept vectors 0 .. i-1
+      for (int pass = 0; pass < 2; ++pass) {
+        QS_TRY(wk.vtw(K, i, w, 1, C));
+        if (pass == 0) a[i] = C[(size_t)i - 1].real();
+        for (auto &c : C) c = -c;
+        QS_TRY(wk.vupd(K, i, C, w, 1, 1.0));
+      }
+      QS_TRY(wk.vtw(wk.vs(w), 1, w, 1, C));
+      b[i] = std::sqrt(std::max(C[0].real(), 0.0));
+      QS_TRY(wk.vupd(K, 0, C, w, 1, b[i] < tiny ? 0.0 : 1.0 / b[i]));
+      if (b[i] >= tiny) {  // a vector that still overlaps the kept ones closes the space
+        QS_TRY(wk.vtw(K, i, w, 1, C));
+        double ov = 0.0;
+        for (int j = i - 1; j >= 0; --j) {
+          ov += std::norm(C[(size_t)j]);
+          if (ov > eps) {
+            QS_TRY(wk.vupd(K, 0, C, w, 1, 0.0));
+            break;
+          }
+        }
+      }
+    }
+    // a[nsteps] = <v|H|v> for the last kept vector
+    QS_TRY(block_apply_v(ctx, h, vec(nsteps - 1), 1, Wb));
+    QS_TRY(wk.vtw(wk.vs(vec(nsteps - 1)), 1, Wb, 1, C));
+    a[nsteps] = C[0].real();
+    return QS_OK;
+  };
+  const qs_status st = run();
+  (void)hipStreamSynchronize(ctx->stream);
+  qs_free(Wb);
+  if (st != QS_OK) return st;
+  memcpy(a_out, a.data(), a.size() * sizeof(double));
+  memcpy(b_out, b.data(), b.size() * sizeof(double));
   return QS_OK;
 }
 

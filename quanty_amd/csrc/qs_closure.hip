@@ -10,6 +10,8 @@
 // passes for two-word determinants), made unique, and targets already in the
 // set are dropped. What is left is the next frontier; it is merged into the
 // set by sorting the union. The closure is reached when a round adds nothing.
+// The image of a set (one round from an empty set) is the output basis of the
+// reference's push-form OperatorPsi (Core/OperatorWaveFunction.cpp:1750-2258).
 #include "qs_internal.h"
 #include "qs_kernels.h"
 
@@ -192,18 +194,21 @@ qs_status sort_new(qs_ctx *ctx, Bufs &bufs, uint64_t *in, int64_t m, const uint6
   void *tmp;
   QS_TRY(bufs.get(ctx, tb, &tmp));
   QS_HIP(ctx, hipcub::DeviceSelect::Flagged(tmp, tb, kin, flag, kout, dn, m, ctx->stream));
-  QS_HIP(ctx, hipMemcpyAsync(count, dn, 8, hipMemcpyDeviceToHost, ctx->stream));
+  QS_TRY(qs_d2h(ctx, count, dn, 8));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QS_OK;
 }
 
+// image = false: the closure of seed. image = true: one round only, from an
+// empty set, so the result is the set of targets of the seed's determinants
+// (the seed itself only where a term maps a determinant onto one of them).
 template <int NW>
 qs_status closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, const ResD &R, int64_t max_dim,
-                  qs_basis **out) {
+                  bool image, qs_basis **out) {
   Bufs bufs;
   const int64_t nt = op->nterms;
   // S: the set so far (sorted); F: the last round's new determinants (sorted)
-  int64_t ns = seed->n;
+  int64_t ns = image ? 0 : seed->n;
   uint64_t *S;
   QS_TRY(qs_malloc(ctx, (void **)&S, (size_t)(ns ? ns : 1) * 8 * NW));
   std::vector<void *> owned{S};
@@ -217,8 +222,8 @@ qs_status closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, const ResD
     release();
     return qs_hip_fail(ctx, e, "qs_basis_closure", __FILE__, __LINE__);
   }
-  const uint64_t *F = S;
-  int64_t nf = ns;
+  const uint64_t *F = image ? seed->d_words : S;
+  int64_t nf = seed->n;
   unsigned long long *dcnt;
   qs_status st = bufs.get(ctx, 8, (void **)&dcnt);
   while (st == QS_OK && nf > 0 && nt > 0) {
@@ -273,7 +278,13 @@ qs_status closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, const ResD
       nacc = nu;
       if (st == QS_OK && ns + nacc > max_dim) st = QS_ENOTCONVERGED;
     }
-    if (st != QS_OK || nacc == 0) break;
+    if (st != QS_OK) break;
+    if (image) {  // the targets of one round are the result
+      F = acc;
+      ns = nacc;
+      break;
+    }
+    if (nacc == 0) break;
     // S <- sort(S u acc); the next frontier is acc
     uint64_t *cat, *S2;
     st = bufs.get(ctx, (size_t)(ns + nacc) * 8 * NW, (void **)&cat);
@@ -306,7 +317,7 @@ qs_status closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, const ResD
     return st;
   }
   qs_basis *b = nullptr;
-  st = qs_basis_from_device_words(ctx, seed->nf, S, ns, &b);
+  st = qs_basis_from_device_words(ctx, seed->nf, image ? F : S, ns, &b);
   release();
   if (st != QS_OK) return st;
   *out = b;
@@ -315,14 +326,14 @@ qs_status closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, const ResD
 
 }  // namespace
 
-extern "C" qs_status qs_basis_closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,
-                                      const uint8_t *masks, const int *lo, const int *hi, int64_t max_dim,
-                                      qs_basis **out) {
+namespace {
+qs_status closure_entry(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres, const uint8_t *masks,
+                        const int *lo, const int *hi, int64_t max_dim, bool image, qs_basis **out) {
   if (!ctx || !op || !seed || !out || nres < 0 || (nres > 0 && (!masks || !lo || !hi))) return QS_EINVAL;
   *out = nullptr;
   if (op->nf != seed->nf) return qs_fail(ctx, QS_EINVAL, "operator has %d modes, the seed %d", op->nf, seed->nf);
   if (nres > CL_MAXRES) return qs_fail(ctx, QS_EINVAL, "qs_basis_closure: at most %d restrictions", CL_MAXRES);
-  if (seed->n > max_dim)
+  if (!image && seed->n > max_dim)
     return qs_fail(ctx, QS_ENOTCONVERGED, "qs_basis_closure: the seed already has more than max_dim = %lld",
                    (long long)max_dim);
   ResD R{};
@@ -334,5 +345,19 @@ extern "C" qs_status qs_basis_closure(qs_ctx *ctx, const qs_op *op, const qs_bas
     R.hi[r] = hi[r];
   }
   QS_HIP(ctx, hipSetDevice(ctx->device));
-  return seed->nw == 1 ? closure<1>(ctx, op, seed, R, max_dim, out) : closure<2>(ctx, op, seed, R, max_dim, out);
+  return seed->nw == 1 ? closure<1>(ctx, op, seed, R, max_dim, image, out)
+                       : closure<2>(ctx, op, seed, R, max_dim, image, out);
+}
+}  // namespace
+
+extern "C" qs_status qs_basis_closure(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,
+                                      const uint8_t *masks, const int *lo, const int *hi, int64_t max_dim,
+                                      qs_basis **out) {
+  return closure_entry(ctx, op, seed, nres, masks, lo, hi, max_dim, false, out);
+}
+
+extern "C" qs_status qs_basis_image(qs_ctx *ctx, const qs_op *op, const qs_basis *seed, int nres,
+                                    const uint8_t *masks, const int *lo, const int *hi, int64_t max_dim,
+                                    qs_basis **out) {
+  return closure_entry(ctx, op, seed, nres, masks, lo, hi, max_dim, true, out);
 }

@@ -1,6 +1,7 @@
 // Context, device memory, and restricted-occupation bases (enumeration by
 // exact ranking, lookups by rank table or binary search).
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstring>
 #include <iterator>
@@ -8,6 +9,7 @@
 
 #include <mutex>
 #include <unordered_map>
+#include "qs_arena.h"
 #include "qs_internal.h"
 
 // ---------------------------------------------------------------------------
@@ -29,124 +31,146 @@ qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *f
   return qs_fail(ctx, QS_EHIP, "%s: %s (%s:%d)", what, hipGetErrorString(e), file, line);
 }
 
-// Device memory cache. Blocks freed by the library stay allocated and are
-// handed to later requests of the same size class: device memory the driver takes
-// back is cleared when it is allocated again, which costs up to a second for
-// the multi-GB Krylov bases and assembled operators of a serving stream and
-// stalls every other allocating thread meanwhile. Cached blocks are returned to
-// the driver (largest first, as many as it takes) when a new block would leave
-// less than QS_FREE_RESERVE of the device free or would not fit, and by
-// qs_trim_memory.
-// device memory left free for other allocators when the cache takes a new block
+// Device memory. The library's blocks are carved from chunks of device
+// memory that stay with the process (QsArena, qs_arena.h): the driver clears
+// memory when it hands it out, tens of milliseconds per GB, and other threads'
+// allocations wait meanwhile, so a serving stream that returned and re-took its
+// multi-GB Krylov bases and assembled operators per request would spend
+// seconds in the driver. A chunk holds at least QS_CHUNK bytes (a larger
+// request gets a chunk of its own size) and is only handed back to the driver
+// when it is wholly free and a new chunk would otherwise leave less than
+// QS_FREE_RESERVE of the device to other allocators, or by qs_trim_memory.
+// Allocating and freeing one large block once reserves that much memory for
+// later blocks of any size.
 #define QS_FREE_RESERVE ((size_t)8 << 30)
+#define QS_CHUNK ((size_t)4 << 30)
 
 namespace {
-struct DevBlock {
-  int dev;
-  size_t bytes;
-};
-std::mutex g_alloc_mu;
-std::unordered_map<void *, DevBlock> g_live;
-std::map<int, std::multimap<size_t, void *>> g_cached;  // device -> (bytes, block)
+std::mutex g_alloc_mu;                      // guards the maps only; driver calls run outside it
+std::map<int, QsArena> g_arena;             // device -> its chunks and blocks
+std::unordered_map<void *, int> g_owner;    // live block -> device
+}  // namespace
 
-// Block sizes: multiples of 256 B below 1 MiB; above, eight classes per
-// octave (m * 2^(e-3), m = 8..15), at most 12.5% over the request, so that
-// requests of similar size share blocks.
-size_t alloc_size(size_t bytes) {
-  if (bytes == 0) bytes = 256;
-  if (bytes < ((size_t)1 << 20)) return (bytes + 255) / 256 * 256;
-  int e = 63 - __builtin_clzll((unsigned long long)bytes);
-  const size_t g = (size_t)1 << (e - 3);
-  return (bytes + g - 1) / g * g;
+// QS_ALLOC_TRACE=1 in the environment: every chunk taken from or returned to
+// the driver is reported on stderr (monotonic ns, bytes, duration).
+static bool alloc_trace() {
+  static const bool on = [] {
+    const char *v = getenv("QS_ALLOC_TRACE");
+    return v && v[0] == '1';
+  }();
+  return on;
 }
 
-// largest waste accepted when reusing a cached block for `bytes` (a class-
-// rounded size): the next class up above 1 MiB, twice the size below
-size_t reuse_slack(size_t bytes) { return bytes < ((size_t)1 << 20) ? bytes : bytes / 7; }
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
-}  // namespace
+static void driver_free(void *p, size_t bytes) {
+  const double t0 = alloc_trace() ? now_s() : 0.0;
+  (void)hipFree(p);
+  if (alloc_trace()) fprintf(stderr, "[qs alloc] %.0f hipFree %zu B %.3f ms\n", 1e9 * t0, bytes, 1e3 * (now_s() - t0));
+}
+
+static hipError_t driver_malloc(void **p, size_t bytes) {
+  const double t0 = alloc_trace() ? now_s() : 0.0;
+  const hipError_t e = hipMalloc(p, bytes);
+  if (alloc_trace()) fprintf(stderr, "[qs alloc] %.0f hipMalloc %zu B %.3f ms\n", 1e9 * t0, bytes, 1e3 * (now_s() - t0));
+  return e;
+}
+
+// Wholly free chunks of `dev` back to the driver until at most keep bytes stay free.
+static void release_chunks(int dev, size_t keep) {
+  std::vector<std::pair<uintptr_t, size_t>> out;
+  {
+    std::lock_guard<std::mutex> lock(g_alloc_mu);
+    out = g_arena[dev].release_free_chunks(keep);
+  }
+  for (auto &c : out) driver_free((void *)c.first, c.second);
+}
 
 qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes) {
   *p = nullptr;
-  const size_t want = alloc_size(bytes);
-  std::lock_guard<std::mutex> lock(g_alloc_mu);
-  auto &cache = g_cached[ctx->device];
-  auto it = cache.lower_bound(want);
-  if (it != cache.end() && it->first <= want + reuse_slack(want)) {
-    *p = it->second;
-    g_live[*p] = DevBlock{ctx->device, it->first};
-    cache.erase(it);
-    return QS_OK;
-  }
-  // a new block: keep QS_FREE_RESERVE of device memory free for other
-  // allocators in the process (cached blocks go back to the driver, largest
-  // first, to make room)
-  size_t dev_free = 0, dev_total = 0;
-  if (!cache.empty() && hipMemGetInfo(&dev_free, &dev_total) == hipSuccess) {
-    while (!cache.empty() && dev_free < want + QS_FREE_RESERVE) {
-      auto big = std::prev(cache.end());
-      (void)hipFree(big->second);
-      dev_free += big->first;
-      cache.erase(big);
+  const size_t want = QsArena::round(bytes);
+  const size_t gran = (size_t)2 << 20;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    {
+      std::lock_guard<std::mutex> lock(g_alloc_mu);
+      const uintptr_t a = g_arena[ctx->device].take(want);
+      if (a) {
+        *p = (void *)a;
+        g_owner[*p] = ctx->device;
+        return QS_OK;
+      }
     }
-  }
-  (void)hipGetLastError();
-  hipError_t e = hipMalloc(p, want);
-  // out of memory: give cached blocks back to the driver, largest first, until
-  // the allocation fits
-  while (e != hipSuccess && !cache.empty()) {
+    // a new chunk, keeping QS_FREE_RESERVE of the device for other allocators
+    const size_t exact = (want + gran - 1) / gran * gran;
+    size_t chunk = std::max(exact, QS_CHUNK);
+    size_t dev_free = 0, dev_total = 0;
+    if (hipMemGetInfo(&dev_free, &dev_total) == hipSuccess) {
+      if (dev_free < chunk + QS_FREE_RESERVE) chunk = exact;
+      if (dev_free < chunk + QS_FREE_RESERVE) {
+        release_chunks(ctx->device, 0);
+        (void)hipMemGetInfo(&dev_free, &dev_total);
+      }
+    }
     (void)hipGetLastError();
-    auto big = std::prev(cache.end());
-    (void)hipFree(big->second);
-    cache.erase(big);
-    e = hipMalloc(p, want);
+    void *base = nullptr;
+    hipError_t e = driver_malloc(&base, chunk);
+    if (e != hipSuccess && chunk > exact) {
+      (void)hipGetLastError();
+      chunk = exact;
+      e = driver_malloc(&base, chunk);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      release_chunks(ctx->device, 0);
+      e = driver_malloc(&base, chunk);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return qs_fail(ctx, QS_ENOMEM_DEVICE, "hipMalloc of %zu bytes failed: %s", chunk, hipGetErrorString(e));
+    }
+    std::lock_guard<std::mutex> lock(g_alloc_mu);
+    QsArena &ar = g_arena[ctx->device];
+    ar.add_chunk((uintptr_t)base, chunk);
+    const uintptr_t a = ar.take(want);
+    if (a) {
+      *p = (void *)a;
+      g_owner[*p] = ctx->device;
+      return QS_OK;
+    }
+    // another thread took the new chunk first: try again
   }
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    *p = nullptr;
-    return qs_fail(ctx, QS_ENOMEM_DEVICE, "hipMalloc of %zu bytes failed: %s", want, hipGetErrorString(e));
-  }
-  g_live[*p] = DevBlock{ctx->device, want};
-  return QS_OK;
+  return qs_fail(ctx, QS_ENOMEM_DEVICE, "could not place a block of %zu bytes", want);
 }
 
 void qs_free(void *p) {
   if (!p) return;
-  std::lock_guard<std::mutex> lock(g_alloc_mu);
-  auto it = g_live.find(p);
-  if (it == g_live.end()) {
-    (void)hipFree(p);  // not from qs_malloc
-    return;
+  {
+    std::lock_guard<std::mutex> lock(g_alloc_mu);
+    auto it = g_owner.find(p);
+    if (it != g_owner.end()) {
+      g_arena[it->second].give((uintptr_t)p);
+      g_owner.erase(it);
+      return;
+    }
   }
-  g_cached[it->second.dev].insert({it->second.bytes, p});
-  g_live.erase(it);
+  (void)hipFree(p);  // not from qs_malloc
 }
 
 extern "C" qs_status qs_trim_memory(qs_ctx *ctx, size_t keep_bytes) {
   if (!ctx) return QS_EINVAL;
   QS_HIP(ctx, hipSetDevice(ctx->device));
-  std::lock_guard<std::mutex> lock(g_alloc_mu);
-  auto &cache = g_cached[ctx->device];
-  size_t total = 0;
-  for (auto &b : cache) total += b.first;
-  while (total > keep_bytes && !cache.empty()) {  // largest blocks first
-    auto big = std::prev(cache.end());
-    total -= big->first;
-    (void)hipFree(big->second);
-    cache.erase(big);
-  }
+  release_chunks(ctx->device, keep_bytes);
   return QS_OK;
 }
 
 extern "C" qs_status qs_memory_stats(qs_ctx *ctx, size_t *live_bytes, size_t *cached_bytes) {
   if (!ctx) return QS_EINVAL;
   std::lock_guard<std::mutex> lock(g_alloc_mu);
-  size_t l = 0, c = 0;
-  for (auto &b : g_live)
-    if (b.second.dev == ctx->device) l += b.second.bytes;
-  for (auto &b : g_cached[ctx->device]) c += b.first;
-  if (live_bytes) *live_bytes = l;
-  if (cached_bytes) *cached_bytes = c;
+  const QsArena &ar = g_arena[ctx->device];
+  if (live_bytes) *live_bytes = ar.live_bytes();
+  if (cached_bytes) *cached_bytes = ar.free_bytes();
   return QS_OK;
 }
 
@@ -207,10 +231,54 @@ extern "C" qs_status qs_ctx_create(int device, qs_ctx **out) {
   return QS_OK;
 }
 
+// pinned staging: at most this many bytes per chunk
+#define QS_PINNED_MAX ((size_t)32 << 20)
+
+static qs_status pinned_get(qs_ctx *ctx, size_t bytes) {
+  if (ctx->pinned_bytes >= bytes) return QS_OK;
+  if (ctx->pinned) {
+    (void)hipHostFree(ctx->pinned);
+    ctx->pinned = nullptr;
+    ctx->pinned_bytes = 0;
+  }
+  size_t want = (size_t)1 << 20;
+  while (want < bytes) want <<= 1;
+  QS_HIP(ctx, hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+  ctx->pinned_bytes = want;
+  return QS_OK;
+}
+
+qs_status qs_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  for (size_t off = 0; off < bytes;) {
+    const size_t c = std::min(bytes - off, QS_PINNED_MAX);
+    QS_TRY(pinned_get(ctx, c));
+    QS_HIP(ctx, hipStreamSynchronize(ctx->stream));  // the buffer's previous copy is done
+    memcpy(ctx->pinned, (const char *)src + off, c);
+    QS_HIP(ctx, hipMemcpyAsync((char *)dst + off, ctx->pinned, c, hipMemcpyHostToDevice, ctx->stream));
+    off += c;
+  }
+  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+qs_status qs_d2h(qs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  for (size_t off = 0; off < bytes;) {
+    const size_t c = std::min(bytes - off, QS_PINNED_MAX);
+    QS_TRY(pinned_get(ctx, c));
+    QS_HIP(ctx, hipMemcpyAsync(ctx->pinned, (const char *)src + off, c, hipMemcpyDeviceToHost, ctx->stream));
+    QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy((char *)dst + off, ctx->pinned, c);
+    off += c;
+  }
+  if (bytes == 0) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
 extern "C" qs_status qs_ctx_destroy(qs_ctx *ctx) {
   if (!ctx) return QS_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->scratch) qs_free(ctx->scratch);
   qs_blas_release(ctx);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -231,15 +299,11 @@ extern "C" qs_status qs_dev_free(qs_ctx *ctx, void *ptr) {
 }
 extern "C" qs_status qs_memcpy_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes) {
   if (!ctx) return QS_EINVAL;
-  QS_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
-  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return QS_OK;
+  return qs_h2d(ctx, dst, src, bytes);
 }
 extern "C" qs_status qs_memcpy_d2h(qs_ctx *ctx, void *dst, const void *src, size_t bytes) {
   if (!ctx) return QS_EINVAL;
-  QS_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return QS_OK;
+  return qs_d2h(ctx, dst, src, bytes);
 }
 extern "C" void *qs_ctx_stream(qs_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
@@ -305,6 +369,18 @@ extern "C" qs_status qs_sync(qs_ctx *ctx) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// Determinant counts saturate at INT64_MAX: a sector can hold more than 2^63
+// determinants (68 modes, 34 particles), and a wrapped count would pass every
+// size check as a negative number.
+inline int64_t sat_add(int64_t a, int64_t b) {
+  int64_t r;
+  return __builtin_add_overflow(a, b, &r) ? INT64_MAX : r;
+}
+inline int64_t sat_mul(int64_t a, int64_t b) {
+  int64_t r;
+  return __builtin_mul_overflow(a, b, &r) ? INT64_MAX : r;
+}
+
 struct StateKey {
   std::vector<int16_t> c;
   bool operator<(const StateKey &o) const { return c < o.c; }
@@ -316,7 +392,7 @@ struct StateKey {
 // add, so each state meets a handful of distinct successors instead of 256.
 // State ids follow the packed keys in ascending order.
 bool rank_table_packed(int nf, int nres, const uint8_t *masks, const int *lo, const int *hi, RankTable &rt,
-                       std::string &err) {
+                       std::string &err, bool count_only) {
   const int nbytes = (nf + 7) / 8;
   const int nch = nbytes;
   struct Group {
@@ -394,12 +470,13 @@ bool rank_table_packed(int nf, int nres, const uint8_t *masks, const int *lo, co
         uint64_t t;
         if (!step(keys[c][i], g.inc, t)) continue;
         const int64_t j = find(c + 1, t);
-        if (j >= 0) tot += cnt[c + 1][j] * (int64_t)g.vs.size();
+        if (j >= 0) tot = sat_add(tot, sat_mul(cnt[c + 1][j], (int64_t)g.vs.size()));
       }
       cnt[c][i] = tot;
     }
   }
   rt.total = cnt[0][0];
+  if (count_only) return true;
   if (rt.total >= ((int64_t)1 << 32) - 1) {
     err = "basis larger than 2^32-2 determinants";
     return false;
@@ -451,7 +528,7 @@ bool rank_table_packed(int nf, int nres, const uint8_t *masks, const int *lo, co
 }  // namespace
 
 qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int *lo,
-                              const int *hi, RankTable &rt, std::string &err) {
+                              const int *hi, RankTable &rt, std::string &err, bool count_only) {
   if (nres >= 0 && nres <= 8 && nf > 0 && nf <= 128) {
     // no count exceeds 128 (nf <= 128), so bounds are clamped into a byte
     int lo8[8], hi8[8];
@@ -459,7 +536,7 @@ qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int 
       lo8[r] = std::max(0, std::min(lo[r], 129));
       hi8[r] = std::min(hi[r], 128);
     }
-    if (!rank_table_packed(nf, nres, masks, lo8, hi8, rt, err)) return QS_EINVAL;
+    if (!rank_table_packed(nf, nres, masks, lo8, hi8, rt, err, count_only)) return QS_EINVAL;
     return QS_OK;
   }
   if (nf <= 0 || nf > 128) {
@@ -524,12 +601,13 @@ qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int 
         }
         if (!ok) continue;
         auto it = bnd[c + 1].find(t);
-        if (it != bnd[c + 1].end()) tot += it->second;
+        if (it != bnd[c + 1].end()) tot = sat_add(tot, it->second);
       }
       kv.second = tot;
     }
   }
   rt.total = bnd[0].begin()->second;
+  if (count_only) return QS_OK;
   if (rt.total >= ((int64_t)1 << 32) - 1) {
     err = "basis larger than 2^32-2 determinants";
     return QS_EINVAL;
@@ -665,7 +743,7 @@ extern "C" qs_status qs_basis_count(qs_ctx *ctx, int nf, int nres, const uint8_t
                                     const int *lo, const int *hi, int64_t *n) {
   RankTable rt;
   std::string err;
-  qs_status st = qs_build_rank_table(nf, nres, masks, lo, hi, rt, err);
+  qs_status st = qs_build_rank_table(nf, nres, masks, lo, hi, rt, err, true);
   if (st != QS_OK) return qs_fail(ctx, st, "%s", err.c_str());
   *n = rt.total;
   return QS_OK;
@@ -677,7 +755,7 @@ extern "C" qs_status qs_basis_enumerate(qs_ctx *ctx, int nf, int nres, const uin
   *out = nullptr;
   RankTable rt;
   std::string err;
-  qs_status st = qs_build_rank_table(nf, nres, masks, lo, hi, rt, err);
+  qs_status st = qs_build_rank_table(nf, nres, masks, lo, hi, rt, err, false);
   if (st != QS_OK) return qs_fail(ctx, st, "qs_basis_enumerate: %s", err.c_str());
   QS_HIP(ctx, hipSetDevice(ctx->device));
   qs_basis *b = new qs_basis();
@@ -700,9 +778,13 @@ extern "C" qs_status qs_basis_enumerate(qs_ctx *ctx, int nf, int nres, const uin
     qs_basis_destroy(b);
     return st;
   }
-  hipError_t e = hipMemcpyAsync(b->d_tbl, tbl.data(), tbl.size() * sizeof(uint2),
-                                hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess && b->n > 0) {
+  st = qs_h2d(ctx, b->d_tbl, tbl.data(), tbl.size() * sizeof(uint2));
+  if (st != QS_OK) {
+    qs_basis_destroy(b);
+    return st;
+  }
+  hipError_t e = hipSuccess;
+  if (b->n > 0) {
     int64_t blocks = (b->n + 255) / 256;
     if (b->nw == 1)
       hipLaunchKernelGGL(k_unrank<1>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, b->d_tbl,

@@ -38,6 +38,8 @@ struct qs_ctx {
   int batch_graph = 1;             // QS_PARAM_BATCH_GRAPH
   int term_lds = QS_TERM_LDS_DEFAULT;  // QS_PARAM_TERM_LDS
   void *blas = nullptr;            // rocblas handle for the dense eigensolver, made on first use
+  void *pinned = nullptr;          // the ctx's own pinned staging buffer (qs_h2d / qs_d2h)
+  size_t pinned_bytes = 0;
 };
 
 // A basis holding n sorted, unique determinants copied from device memory.
@@ -193,6 +195,13 @@ qs_status qs_hip_fail(qs_ctx *ctx, hipError_t e, const char *what, const char *f
   } while (0)
 
 qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes);
+// Host <-> device copies staged through the ctx's own pinned buffer, ordered
+// on ctx->stream; both return after the copy (and the stream's earlier work)
+// has completed. Copies from pageable memory go through the runtime's shared
+// staging path, where one thread's small copy waits behind another thread's
+// large one; with a pinned buffer per ctx, threads do not wait on each other.
+qs_status qs_h2d(qs_ctx *ctx, void *dst, const void *src, size_t bytes);
+qs_status qs_d2h(qs_ctx *ctx, void *dst, const void *src, size_t bytes);
 // Return a qs_malloc block to the library's device memory cache (null is a
 // no-op). The block must no longer be in use by queued work.
 void qs_free(void *p);
@@ -205,8 +214,10 @@ qs_status qs_scratch(qs_ctx *ctx, size_t bytes, void **p);
 int64_t qs_resident_blocks(qs_ctx *ctx, const void *kernel, size_t shm);
 
 // ---- internal entry points across translation units ----------------------
+// count_only: fill rt.total only (saturated at INT64_MAX), with no size limit
+// and no tables; otherwise sectors of 2^32-1 or more determinants are refused.
 qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int *lo,
-                              const int *hi, RankTable &rt, std::string &err);
+                              const int *hi, RankTable &rt, std::string &err, bool count_only);
 qs_status qs_dot_internal(qs_ctx *ctx, int64_t n, const double *a, const double *b,
                           double *re, double *im);
 

@@ -55,7 +55,7 @@ qs_status qs_dot_internal(qs_ctx *ctx, int64_t n, const double *a, const double 
                      (const double2 *)b, p, p + nb);
   QS_HIP(ctx, hipGetLastError());
   std::vector<double> h((size_t)nb * 2);
-  QS_HIP(ctx, hipMemcpyAsync(h.data(), p, (size_t)nb * 16, hipMemcpyDeviceToHost, ctx->stream));
+  QS_TRY(qs_d2h(ctx, h.data(), p, (size_t)nb * 16));
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   double sr = 0.0, si = 0.0;
   for (int i = 0; i < nb; ++i) {
@@ -276,8 +276,8 @@ extern "C" qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, 
     e = hipGetLastError();
     std::swap(uc, up);
   }
-  if (e == hipSuccess && st == QS_OK) e = hipMemcpyAsync(a_out, S.a, (size_t)(nsteps + 1) * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess && st == QS_OK) e = hipMemcpyAsync(b_out, S.b, (size_t)nsteps * 8, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess && st == QS_OK) st = qs_d2h(ctx, a_out, S.a, (size_t)(nsteps + 1) * 8);
+  if (e == hipSuccess && st == QS_OK) st = qs_d2h(ctx, b_out, S.b, (size_t)nsteps * 8);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   qs_free(mem);
   if (st != QS_OK) return st;
@@ -325,19 +325,21 @@ extern "C" qs_status qs_cf_spectrum(qs_ctx *ctx, int nspec, int n, const double 
   char *mem = nullptr;
   QS_TRY(qs_malloc(ctx, (void **)&mem, (na + nbv + nspec + 2 * np) * 8));
   double *da = (double *)mem, *db = da + na, *de0 = db + nbv, *dre = de0 + nspec, *dim = dre + np;
-  hipError_t e = hipMemcpyAsync(da, a, na * 8, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(db, b, nbv * 8, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(de0, E0, (size_t)nspec * 8, hipMemcpyHostToDevice, ctx->stream);
-  if (e == hipSuccess) {
+  qs_status st = qs_h2d(ctx, da, a, na * 8);
+  if (st == QS_OK) st = qs_h2d(ctx, db, b, nbv * 8);
+  if (st == QS_OK) st = qs_h2d(ctx, de0, E0, (size_t)nspec * 8);
+  hipError_t e = hipSuccess;
+  if (st == QS_OK) {
     const double de = (emax - emin) / (double)ne;
     hipLaunchKernelGGL(k_cf, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, nspec, n, da, db, de0,
                        emin, de, ne, gamma, dre, dim);
     e = hipGetLastError();
   }
-  if (e == hipSuccess) e = hipMemcpyAsync(re, dre, np * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(im, dim, np * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (st == QS_OK && e == hipSuccess) st = qs_d2h(ctx, re, dre, np * 8);
+  if (st == QS_OK && e == hipSuccess) st = qs_d2h(ctx, im, dim, np * 8);
+  (void)hipStreamSynchronize(ctx->stream);
   qs_free(mem);
+  if (st != QS_OK) return st;
   if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_cf_spectrum", __FILE__, __LINE__);
   return QS_OK;
 }

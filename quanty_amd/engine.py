@@ -6,6 +6,7 @@ for device memory); every computation goes through libqs.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 
 import numpy as np
 import torch
@@ -93,6 +94,21 @@ class Basis:
         ctx.check(st, "qs_basis_closure")
         return cls(h, ctx, list(restrictions) if restrictions else None)
 
+    @classmethod
+    def image(cls, op: "Op", seed: "Basis", restrictions=None, max_dim: int = 1 << 62) -> "Basis | None":
+        """Every target of op's terms on seed's determinants (inside the
+        restrictions), built on the GPU: the output basis of a push-form O|psi>;
+        None when it holds more than max_dim determinants."""
+        ctx = seed.ctx
+        nres, masks, lo, hi = restriction_arrays(seed.nf, restrictions or [])
+        h = C.c_void_p()
+        st = ctx.lib.qs_basis_image(ctx.h, op.h, seed.h, nres, ptr(masks), ptr(lo), ptr(hi), int(max_dim),
+                                    C.byref(h))
+        if st == QS_ENOTCONVERGED:
+            return None
+        ctx.check(st, "qs_basis_image")
+        return cls(h, ctx, list(restrictions) if restrictions else None)
+
     def words(self) -> np.ndarray:
         out = np.zeros((self.n, self.nw), dtype=np.uint64)
         if self.n:
@@ -122,12 +138,15 @@ class Basis:
 
 def pack_terms(terms):
     """[(codes (o[0] first), complex value), ...] -> (len int32, codes uint16, values float64)."""
-    lens = np.array([len(c) for c, _ in terms], dtype=np.int32)
-    codes = np.array([x for c, _ in terms for x in c] or [0], dtype=np.uint16)
-    vals = np.zeros(2 * max(len(terms), 1), dtype=np.float64)
-    for i, (_, v) in enumerate(terms):
-        vals[2 * i] = complex(v).real
-        vals[2 * i + 1] = complex(v).imag
+    terms = list(terms)
+    if not terms:
+        return np.zeros(0, dtype=np.int32), np.zeros(1, dtype=np.uint16), np.zeros(2, dtype=np.float64)
+    cs, vs = zip(*terms)
+    lens = np.fromiter(map(len, cs), dtype=np.int32, count=len(cs))
+    codes = np.fromiter(itertools.chain.from_iterable(cs), dtype=np.uint16, count=int(lens.sum()))
+    if codes.size == 0:
+        codes = np.zeros(1, dtype=np.uint16)
+    vals = np.ascontiguousarray(np.array(vs, dtype=np.complex128).view(np.float64))
     return lens, codes, vals
 
 
@@ -216,6 +235,17 @@ class Hop:
         _torch_ready(v0.device)
         self.ctx.check(self.lib.qs_lanczos(self.ctx.h, self.h, ptr(v0), nsteps, ptr(a), ptr(b)), "qs_lanczos")
         return a, b
+
+    def lanczos_krylov(self, v0: torch.Tensor, nsteps: int, eps: float = 1.4901161193847656e-10):
+        """Lanczos with full re-orthogonalization that keeps its basis. Returns a
+        (nsteps+1,), b (nsteps,) and the normalized Krylov vectors (nsteps, n)."""
+        a = np.zeros(nsteps + 1)
+        b = np.zeros(nsteps)
+        V = torch.zeros((nsteps, self.rows.n), dtype=torch.complex128, device=_device(self.ctx))
+        _torch_ready(v0.device)
+        self.ctx.check(self.lib.qs_lanczos_krylov(self.ctx.h, self.h, ptr(v0), nsteps, float(eps), ptr(a), ptr(b),
+                                                  ptr(V)), "qs_lanczos_krylov")
+        return a, b, V
 
     def block_lanczos(self, V0: torch.Tensor, nsteps: int, sv_cut: float = 1.4901161193847656e-08):
         """V0: (k, n) complex128 start vectors. Returns A (nsteps+1, k, k), B (nsteps, k, k), kept (nsteps+1)."""

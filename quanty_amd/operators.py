@@ -136,15 +136,19 @@ def coulomb_two_shells(up_a, dn_a, up_b, dn_b, F, G):
 # ---------------------------------------------------------------------------
 # one-body
 # ---------------------------------------------------------------------------
+@lru_cache(maxsize=1024)
+def _one_body_codes(rows: tuple, cols: tuple) -> tuple:
+    """Ladder codes (o[0] first) of c+_rows[i] c_cols[j], row-major over (i, j)."""
+    return tuple((an(b), cr(a)) for a in rows for b in cols)
+
+
 def one_body(rows, cols, M):
-    """sum_ij M[i][j] c+_rows[i] c_cols[j]."""
-    M = np.asarray(M)
-    out = []
-    for i, a in enumerate(rows):
-        for j, b in enumerate(cols):
-            if M[i, j] != 0:
-                out.append(written([cr(a), an(b)], M[i, j]))
-    return out
+    """sum_ij M[i][j] c+_rows[i] c_cols[j] (zero entries left out)."""
+    M = np.asarray(M, dtype=np.complex128).reshape(len(rows), len(cols))
+    codes = _one_body_codes(tuple(int(a) for a in rows), tuple(int(b) for b in cols))
+    flat = M.ravel()
+    nz = np.flatnonzero(flat)
+    return [(codes[k], complex(flat[k])) for k in nz.tolist()]
 
 
 def number(indices, values=None):
@@ -242,6 +246,44 @@ def add(*lists):
         for c, v in lst:
             out[c] = out.get(c, 0.0) + complex(v)
     return [(c, v) for c, v in out.items() if v != 0]
+
+
+@lru_cache(maxsize=1 << 16)
+def _ordered(codes: tuple) -> tuple:
+    """A ladder string as a sum of strings in canonical order, codes strictly
+    ascending in application order (annihilators before creators, each by
+    ascending mode; written: creators left of annihilators). Adjacent factors
+    anticommute, {c_i, c+_i} = 1 adds the string without the pair, and a
+    repeated code makes the string vanish. Returns ((codes, coefficient), ...)."""
+    for k in range(len(codes) - 1):
+        a, b = codes[k], codes[k + 1]
+        if a == b:
+            return ()
+        if a > b:
+            out: dict = {}
+            for c, v in _ordered(codes[:k] + (b, a) + codes[k + 2:]):
+                out[c] = out.get(c, 0) - v
+            if a ^ b == CR:
+                for c, v in _ordered(codes[:k] + codes[k + 2:]):
+                    out[c] = out.get(c, 0) + v
+            return tuple((c, v) for c, v in out.items() if v != 0)
+    return ((codes, 1),)
+
+
+def normal_order(terms, eps: float = 0.0) -> dict:
+    """Terms rewritten as a sum of canonically ordered strings with their
+    coefficients added, every coefficient with |v| <= eps dropped — the
+    normal-ordered form the reference keeps operators in
+    (OperatorAddLaderNormalOrder, Core/OperatorArithmetic.cpp:1212) and the
+    chop of OperatorRemoveSmallValues (:1309-1420; |v| > eps is kept). The
+    canonical order here is this module's own; sums that cancel cancel in any
+    canonical order."""
+    out: dict = {}
+    items = terms.items() if isinstance(terms, dict) else terms
+    for c, v in items:
+        for cc, f in _ordered(tuple(int(x) for x in c)):
+            out[cc] = out.get(cc, 0) + f * complex(v)
+    return {c: v for c, v in out.items() if abs(v) > eps}
 
 
 def dagger(terms):

@@ -20,6 +20,7 @@ import json
 import math
 import os
 import queue
+import sys
 import threading
 import time
 from dataclasses import dataclass, field
@@ -38,8 +39,10 @@ _FAMILY_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "impurit
 
 
 def impurity_family():
-    """Anderson-impurity shapes with their sector sizes, from about 10^2 to 10^8
-    (the table is cached in impurity_family.json next to this module)."""
+    """Anderson-impurity shapes with their sector sizes, from about 10^2 to 10^8.
+    The table ships as impurity_family.json next to this module (written by
+    `python -m quanty_amd.serve --write-family`); without it the sizes are
+    counted here, in memory."""
     global _FAMILY
     if _FAMILY is not None:
         return _FAMILY
@@ -47,6 +50,11 @@ def impurity_family():
         data = json.load(open(_FAMILY_FILE))
         _FAMILY = [(int(n), dict(kw, d_range=tuple(kw["d_range"]))) for n, kw in data]
         return _FAMILY
+    _FAMILY = _count_family()
+    return _FAMILY
+
+
+def _count_family():
     out = []
     for nbath in (1, 2, 3, 4):
         for full in range(0, nbath + 1):
@@ -65,18 +73,20 @@ def impurity_family():
                             if n >= 50:
                                 out.append((n, kw))
     out.sort(key=lambda t: t[0])
-    _FAMILY = out
-    with open(_FAMILY_FILE, "w") as f:
-        json.dump(out, f)
     return out
+
+
+_SHAPES: dict = {}
 
 
 def pick_shape(target_dim: float, nbath_max: int = 4):
     """The impurity shape (at most nbath_max bath shells) whose sector size is
-    closest to target_dim on a log scale."""
-    fam = [t for t in impurity_family() if t[1]["nbath"] <= nbath_max]
-    best = min(fam, key=lambda t: abs(math.log(t[0]) - math.log(target_dim)))
-    return best
+    closest to target_dim on a log scale (the first such in family order on a tie)."""
+    if nbath_max not in _SHAPES:
+        fam = [t for t in impurity_family() if t[1]["nbath"] <= nbath_max]
+        _SHAPES[nbath_max] = (fam, np.log(np.array([t[0] for t in fam], dtype=float)))
+    fam, logs = _SHAPES[nbath_max]
+    return fam[int(np.argmin(np.abs(logs - math.log(target_dim))))]
 
 
 @dataclass(order=True)
@@ -95,6 +105,7 @@ class Request:
     gs_residual: float = field(compare=False, default=0.0)
     gs_steps: int = field(compare=False, default=0)
     spectrum: np.ndarray | None = field(compare=False, default=None)
+    gs_state: np.ndarray | None = field(compare=False, default=None, repr=False)
     a: np.ndarray | None = field(compare=False, default=None)
     b: np.ndarray | None = field(compare=False, default=None)
     # the request's payload: NF, H and T terms, restrictions (made from seed and
@@ -257,6 +268,21 @@ class SpectrumServer:
         self.batch = Batch(max_slots, ctx)
         self.jobs: dict[int, _Job] = {}
         self.used = 0
+        # requests whose normalized ground state is copied to the host
+        # (Request.gs_state) for an independent check of the result
+        self.keep_states: set = set()
+
+    def reserve(self, extra: float = 16e9):
+        """Take the HBM budget (plus `extra` for the preparation threads'
+        temporaries) from the driver once, as one chunk of the library's
+        device memory, so that requests are placed in memory the process
+        already holds: the driver clears memory it hands out, tens of
+        milliseconds per GB, and serializes other threads' allocations
+        meanwhile."""
+        import ctypes as C
+        p = C.c_void_p()
+        self.ctx.check(self.ctx.lib.qs_dev_alloc(self.ctx.h, int(self.budget + extra), C.byref(p)), "qs_dev_alloc")
+        self.ctx.check(self.ctx.lib.qs_dev_free(self.ctx.h, p), "qs_dev_free")
 
     def _advance(self, job: _Job, coeffs=None):
         """Move a job whose slot reached its depth to its next phase. coeffs:
@@ -312,6 +338,8 @@ class SpectrumServer:
                 self.batch.remove(job.slot)
             y /= torch.linalg.vector_norm(y)
             job.gs_vector = y  # the normalized ground state (device)
+            if r.rid in self.keep_states:
+                r.gs_state = y.cpu().numpy()
             tpsi = job.thop.apply(y.contiguous(), ctx=self.ctx)
             job.slot = self.batch.add(job.hop, tpsi, r.depth)
             job.phase = "spec"
@@ -375,6 +403,13 @@ class SpectrumServer:
         inflight: dict[int, int] = {}  # rid -> reserved bytes of requests being prepared
         kept = self.gs_max + 1 if self.keep_basis else 0
         slots = 1 if plain else self.max_slots
+        # The preparation threads run Python between their library calls; the
+        # serving loop, which releases the interpreter lock in every library
+        # call, would otherwise wait up to the default 5 ms switch interval to
+        # get it back after each call while the batch on the GPU runs dry.
+        switch = sys.getswitchinterval()
+        if prep is not None:
+            sys.setswitchinterval(2e-4)
         t0 = time.perf_counter()
         i = 0
         try:
@@ -416,6 +451,7 @@ class SpectrumServer:
         finally:
             if prep is not None:
                 prep.close()
+            sys.setswitchinterval(switch)
         return done
 
 
@@ -424,3 +460,10 @@ def latency_stats(done: list[Request]) -> dict:
     span = max(r.t_done for r in done) - min(r.arrival for r in done)
     return {"n": len(done), "spectra_per_s": len(done) / span, "p50_s": float(np.percentile(lat, 50)),
             "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()), "span_s": span}
+
+
+if __name__ == "__main__":
+    import sys
+    if "--write-family" in sys.argv:
+        with open(_FAMILY_FILE, "w") as f:
+            json.dump(_count_family(), f)

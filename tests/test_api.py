@@ -202,3 +202,111 @@ def test_bitmask_index_codec():
         assert BitMaskToIndex(m) == (13, [5, 7, 9])
     assert IndexToBitMask(15, [0, 1, 2]) == "111000000000000"
     assert IndexToBitMask(15, [0, 1, 2, 14]) == "111000000000001"
+    # the reference's error behaviour (LuaIndexToBitMask, LuaInterface/ssp.cpp:6050-6053)
+    # and BitMaskToIndex skipping every character but 0 and 1 (Core/Utilities.cpp:5-31)
+    with pytest.raises(QSError, match="smaller than zero"):
+        IndexToBitMask(15, [0, -1])
+    with pytest.raises(QSError, match="larger or equal than NF"):
+        IndexToBitMask(15, [3, 15])
+    with pytest.raises(QSError, match="negative value for NF"):
+        IndexToBitMask(-1, [])
+    assert IndexToBitMask(4, []) == "0000"
+    assert BitMaskToIndex("00a1-1|0x") == (5, [2, 3])
+    assert BitMaskToIndex("") == (0, [])
+
+
+def test_count_saturates_instead_of_wrapping():
+    """Sector counts past 2^63 (68 modes holding 34, 76 modes holding 38) saturate
+    at INT64_MAX instead of wrapping to negative numbers; enumeration refuses
+    sectors of 2^32 - 1 or more determinants with a clear error."""
+    from quanty_amd.engine import Basis
+    for nf in (68, 76):
+        n = Basis.count(nf, [("1" * nf, nf // 2, nf // 2)])
+        assert n == 2 ** 63 - 1
+    # exact below the limit: C(64, 32) fits in int64
+    from math import comb
+    assert Basis.count(64, [("1" * 64, 32, 32)]) == comb(64, 32)
+    assert Basis.count(40, [("1" * 40, 20, 20)]) == comb(40, 20)
+
+
+def test_normal_order_against_jordan_wigner():
+    """The canonical normal order of random ladder strings (lengths 1-4, 5
+    modes) equals the string as a Jordan-Wigner matrix; strings that only
+    reorder one product give identical canonical terms."""
+    from models import dense_from_terms, jw_matrices
+    from quanty_amd import operators as Ops
+    rng = np.random.default_rng(3)
+    nf = 5
+    cs = jw_matrices(nf)
+    for _ in range(40):
+        terms = []
+        for _ in range(4):
+            ln = int(rng.integers(1, 5))
+            codes = tuple(int(rng.integers(0, nf)) | (Ops.CR if rng.random() < 0.5 else 0) for _ in range(ln))
+            terms.append((codes, complex(rng.normal(), rng.normal())))
+        no = Ops.normal_order(terms)
+        assert np.allclose(dense_from_terms(nf, list(no.items()), cs), dense_from_terms(nf, terms, cs), atol=1e-12)
+        for c in no:  # canonical: strictly ascending codes
+            assert all(a < b for a, b in zip(c, c[1:]))
+    # c+_0 c_1 = - c_1 c+_0
+    a = Ops.normal_order([((Ops.an(1), Ops.cr(0)), 1.0)])
+    b = Ops.normal_order([((Ops.cr(0), Ops.an(1)), -1.0)])
+    assert a == b
+
+
+def test_hermiticity_classes_and_repair():
+    """QOperatorHermitian's three answers (Core/OperatorArithmetic.cpp:6517-6568)
+    and the almost-Hermitian repair of LuaEigensystem (ssp.cpp:4164-4205);
+    CreateSpectra and Eigensystem refuse a non-Hermitian operator before any
+    device work."""
+    from quanty_amd.api import CreateSpectra, Eigensystem, hermitian_class, hermitian_part
+    nf = 4
+    H = NewOperator(nf, 0, [[0, -1, 0.5 + 0.2j], [1, -0.0, 0.5 - 0.2j], [0, -0.0, 1.0], [2, 3, -3, -2, 2.0]])
+    assert hermitian_class(H, 1e-6) == 1
+    # the same operator written in another order is still Hermitian
+    H2 = NewOperator(nf, 0, [[0, -1, 0.5 + 0.2j], [-0.0, 1, -0.5 + 0.2j], [0, -0.0, 1.0], [3, 2, -2, -3, 2.0]])
+    assert hermitian_class(H2, 1e-6) == 1
+    almost = H + NewOperator(nf, 0, [[1, -2, 1e-9]])
+    assert hermitian_class(almost, 1e-6) == 2
+    fixed = hermitian_part(almost, 1e-6)
+    assert hermitian_class(fixed, 1e-6) == 1
+    assert abs(fixed.terms[(2, 0x8001)] - 0.5e-9) < 1e-20
+    bad = H + NewOperator(nf, 0, [[1, -2, 1e-3]])
+    assert hermitian_class(bad, 1e-6) == 0
+    psi = NewWavefunction(nf, 0, [["1100", 1.0]])
+    with pytest.raises(QSError, match="not Hermitian"):
+        Eigensystem(bad, [nf, 0, ["1111", 2, 2]], 1)
+    with pytest.raises(QSError, match="not Hermitian"):
+        CreateSpectra(bad, NewOperator("An", nf, 0), psi)
+
+
+def test_option_names_of_the_reference():
+    """Eigensystem and CreateSpectra take every option name of LuaEigensystem /
+    LuaCreateSpectra (ssp.cpp:4062-4078, 1958-1981) that this engine covers,
+    reject unknown ones and name the unsupported ones, before any device work."""
+    from quanty_amd.api import (_CS_OPTIONS, _CS_UNSUPPORTED, _EIG_ALIASES, _EIG_OPTIONS, CreateSpectra,
+                                Eigensystem, _options)
+    for name in ["DenseBorder", "NKrylovStart", "NKrylovMax", "NKrylovStep", "NKrylovMin", "NBitsKey", "epsilon",
+                 "Epsilon", "SingularValue", "zero", "Zero", "restrictions", "Restrictions", "ExpandBasis",
+                 "ShiftScheme"]:
+        assert name in _EIG_OPTIONS or name in _EIG_ALIASES
+    o = _options([["zero", 1e-9], ["Zero", 1e-10]], _EIG_OPTIONS, _EIG_ALIASES)
+    assert o["Zero"] == 1e-10  # both spellings write one setting; the later pair wins
+    o = _options([["restrictions", [4, 0, ["1100", 1, 2]]]], _EIG_OPTIONS, _EIG_ALIASES)
+    assert o["Restrictions"] == [4, 0, ["1100", 1, 2]]
+    nf = 4
+    H = NewOperator(nf, 0, [[0, -0.0, 1.0]])
+    with pytest.raises(QSError, match="unknown option"):
+        Eigensystem(H, [nf, 0, ["1111", 2, 2]], 1, [["NoSuchOption", 1]])
+    with pytest.raises(QSError, match="ShiftScheme"):
+        Eigensystem(H, [nf, 0, ["1111", 2, 2]], 1, [["ShiftScheme", "Sideways"]])
+    psi = NewWavefunction(nf, 0, [["1100", 1.0]])
+    for name in _CS_UNSUPPORTED:
+        with pytest.raises(QSError, match="not supported"):
+            CreateSpectra(H, NewOperator("An", nf, 0), psi, [[name, 1]])
+    for name in ["Tensor", "KrylovBasis", "NTri", "epsilon", "Epsilon", "SingularValue", "restrictions",
+                 "Restrictions", "Emin", "Emax", "NE", "Gamma", "E0", "DenseBorder", "ForceDenseMethod", "Method"]:
+        assert name in _CS_OPTIONS or name in ("epsilon", "restrictions")
+    many = [NewOperator("An", nf, i % nf) for i in range(17)]
+    with pytest.raises(QSError, match="at most 16"):
+        CreateSpectra(H, many, psi, [["Tensor", True]])

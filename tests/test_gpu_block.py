@@ -162,9 +162,13 @@ def test_ground_states_krylov_schedules_agree():
             assert dot(r, r).real < 1e-10 * max(1.0, float(np.sum(E ** 2)))
 
 
-def test_device_memory_cache_reuses_and_trims():
-    """Freed library memory is cached and reused (no growth over repeated
-    create/destroy cycles), and qs_trim_memory empties the cache."""
+def test_device_memory_is_reused_and_trimmed():
+    """Freed library memory stays in the process's chunks and is reused (no
+    growth over repeated create/destroy cycles); a block larger than a chunk
+    gets a chunk of its own, which qs_trim_memory hands back to the driver
+    once it is wholly free; the freed space of a large block serves smaller
+    blocks."""
+    import ctypes as C
     from quanty_amd import default_context
     ctx = default_context()
     pr = P.anderson(**SMALL_GS)
@@ -173,21 +177,35 @@ def test_device_memory_cache_reuses_and_trims():
     op = Op(nf, pr["H"])
     warm = Hop(op, b, mode="assembled")  # first use may grow the context's scratch space
     del warm
-    live0, _ = ctx.memory_stats()
+    live0, free0 = ctx.memory_stats()
     hop = Hop(op, b, mode="assembled")
     live1, _ = ctx.memory_stats()
     assert live1 > live0
     del hop
-    live2, cached2 = ctx.memory_stats()
-    assert live2 == live0 and cached2 >= live1 - live0
-    for _ in range(3):  # the same sizes come back from the cache
+    live2, free2 = ctx.memory_stats()
+    assert live2 == live0 and free2 == free0
+    for _ in range(3):  # the same space comes back
         hop = Hop(op, b, mode="assembled")
         del hop
-        live, cached = ctx.memory_stats()
-        assert live == live0 and cached == cached2
-    keep = cached2 // 2
-    ctx.trim_memory(keep)
-    live, cached = ctx.memory_stats()
-    assert live == live0 and cached <= keep
-    ctx.trim_memory()
-    assert ctx.memory_stats() == (live0, 0)
+        assert ctx.memory_stats() == (live0, free0)
+    big = free0 + (6 << 30)  # more than the free space held: a chunk of its own
+    p = C.c_void_p()
+    ctx.check(ctx.lib.qs_dev_alloc(ctx.h, big, C.byref(p)), "qs_dev_alloc")
+    assert ctx.memory_stats()[0] >= live0 + big
+    ctx.check(ctx.lib.qs_dev_free(ctx.h, p), "qs_dev_free")
+    live, free = ctx.memory_stats()
+    assert live == live0 and free >= free0 + big
+    # two blocks of 2 GiB fit into the freed chunk without a new one
+    q1, q2 = C.c_void_p(), C.c_void_p()
+    ctx.check(ctx.lib.qs_dev_alloc(ctx.h, 2 << 30, C.byref(q1)), "qs_dev_alloc")
+    ctx.check(ctx.lib.qs_dev_alloc(ctx.h, 2 << 30, C.byref(q2)), "qs_dev_alloc")
+    assert ctx.memory_stats()[1] == free - (4 << 30)
+    ctx.check(ctx.lib.qs_dev_free(ctx.h, q1), "qs_dev_free")
+    ctx.check(ctx.lib.qs_dev_free(ctx.h, q2), "qs_dev_free")
+    assert ctx.memory_stats() == (live, free)
+    ctx.trim_memory(0)
+    live3, free3 = ctx.memory_stats()
+    assert live3 == live0 and free3 <= free - big
+    hop = Hop(op, b, mode="assembled")  # still works after the trim
+    assert ctx.memory_stats()[0] == live1
+    del hop

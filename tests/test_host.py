@@ -137,3 +137,17 @@ def test_tridiagonal_lowest_with_bracket_hints():
             E, z = tridiag_lowest(d, e, lo, hi)
             assert abs(E - w0) < 1e-13 * s, (trial, lo, hi, E, w0)
             assert np.linalg.norm(T @ z - E * z) < 1e-11 * s
+
+
+def test_device_memory_arena_bookkeeping(tmp_path):
+    """The sub-allocator behind the library's device memory (qs_arena.h),
+    compiled for the host: randomized take/give over adjacent chunks keeps live
+    blocks disjoint and inside their chunks, merges freed neighbours (never
+    across chunk edges) and accounts every byte."""
+    import subprocess
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "arena_check.cpp")
+    exe = str(tmp_path / "arena_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-o", exe, src], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    assert "arena ok" in out.stdout
