@@ -1,16 +1,20 @@
 """Benchmark: spectra/s and p50/p99 latency of the continuously batched
-spectrum server against plain one-at-a-time GPU Lanczos on the same Poisson
-stream of impurity-spectrum requests, with the H*psi kernel of the 50-mode
-Anderson impurity (2,179,235 determinants) against the HBM roofline and the
-oracle timed on the host cores.
+spectrum server on the 10,000-request Poisson stream of impurity-spectrum
+requests, against plain one-at-a-time GPU Lanczos, with the H*psi kernel of
+the 50-mode Anderson impurity (2,179,235 determinants) against the HBM
+roofline, a sample of served results re-computed by the oracle, and complete
+requests timed on the host cores.
 
-A step is one served request: a seeded Anderson impurity of the 50-mode
-family (dim log-uniform in 10^3..3*10^6), its ground state (Lanczos to
-|H psi - E psi|^2 <= 1e-12 max(1, E^2)), T|psi0> for a random one-body
-d<->bath transition, a Lanczos tridiagonalization of depth U{100..400} and the
-continued fraction on 2001 points with Gamma U(0.1, 0.5). `--steps` requests
-arrive as a Poisson stream; `value` is the served stream's spectra/s (requests
-over the span from the first arrival to the last completion).
+A request: a seeded Anderson impurity of the 50-mode family (dim log-uniform
+in 10^3..3*10^6), its ground state (Lanczos to |H psi - E psi|^2 <=
+1e-12 max(1, E^2)), T|psi0> for a random one-body d<->bath transition, a
+Lanczos tridiagonalization of depth U{100..400} and the continued fraction on
+2001 points with Gamma U(0.1, 0.5). One step = one served request; the timed
+stream holds --requests requests (10,000, the serving workload) whatever
+--steps says; `value` is its spectra/s (requests over the span from the first
+arrival to the last completion). The plain baseline serves the stream's first
+--plain-requests requests one at a time, and the server serves the same
+prefix again so that the two are compared on identical work.
 """
 from __future__ import annotations
 
@@ -37,9 +41,15 @@ def log(msg):
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=48, help="requests in the timed stream")
-    p.add_argument("--warmup", type=int, default=48,
-                   help="requests of the timed stream served untimed before it (memory warm-up)")
+    p.add_argument("--steps", type=int, default=None,
+                   help="accepted for the harness; the timed stream length is --requests")
+    p.add_argument("--warmup", type=int, default=None,
+                   help="accepted for the harness; the untimed warm-up is --warm-requests")
+    p.add_argument("--requests", type=int, default=10000, help="requests in the timed served stream")
+    p.add_argument("--plain-requests", type=int, default=1000,
+                   help="prefix of the stream served one at a time (and again by the server) for the ratio")
+    p.add_argument("--warm-requests", type=int, default=64,
+                   help="requests of the stream served untimed first (library caches, worker contexts)")
     p.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate of the stream (req/s)")
     p.add_argument("--slots", type=int, default=64)
     p.add_argument("--seed", type=int, default=1)
@@ -47,12 +57,22 @@ def parse():
     p.add_argument("--ne", type=int, default=2000)
     p.add_argument("--hpsi-reps", type=int, default=20)
     p.add_argument("--spectra", type=int, default=5, help="plain spectra timed on the 2.18M basis")
-    p.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
+    p.add_argument("--cpu-seconds", type=float, default=20.0, help="target length of the CPU baseline sample")
+    p.add_argument("--oracle-max-dim", type=int, default=12000,
+                   help="largest served request re-computed by the oracle")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-oracle", action="store_true")
     p.add_argument("--no-serve", action="store_true", help="H*psi and the 2.18M spectrum only")
     p.add_argument("--shift-min", type=int, default=None,
                    help="shift-block threshold of the assembled operator (library default when omitted)")
     return p.parse_args()
+
+
+def threads():
+    """Host threads of the CPU legs: the box's CPU share (OMP_NUM_THREADS when
+    set), else every core."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    return int(env) if env and env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
 
 
 def hpsi_section(args, ctx, torch):
@@ -99,6 +119,7 @@ def hpsi_section(args, ctx, torch):
     assert np.all(np.isfinite(gl))
     log(f"plain spectrum on the 2.18M basis: {spec_ms:.1f} ms")
 
+    # H*psi with HIP events recorded on the library's own stream
     stream = torch.cuda.ExternalStream(ctx.lib.qs_ctx_stream(ctx.h), device=dev)
     x = torch.from_numpy(g.normal(size=n) + 1j * g.normal(size=n)).to(dev)
     y = basis.zeros()
@@ -129,9 +150,14 @@ def hpsi_section(args, ctx, torch):
     copy_gbps = 2 * src.numel() * 8 / (copy_ms * 1e-3) / 1e9
     del src, dst
 
-    # algorithmic bytes per H*psi: determinant + x_i + y_i per row, 16 B per (row, term) hit
-    bytes_alg = n * (8 + 16 + 16) + 16 * st["nnz_eff"]
-    achieved = bytes_alg / (hpsi_ms * 1e-3) / 1e9
+    t = hpsi_ms * 1e-3
+    # bytes this layout must move from HBM per H*psi: the assembled index stream
+    # (row determinants, ELL counts, diagonal, shift-block headers, entries),
+    # x read once and y written once
+    needed = st["stream_bytes"] + 2 * 16 * n
+    # the per-nonzero gather model (16 B per (row, term) hit): most of these
+    # gathers hit L2 / the Infinity Cache, so it is a gather rate, not HBM traffic
+    gather_bytes = n * (8 + 16 + 16) + 16 * st["nnz_eff"]
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         tr = json.load(open(TRAFFIC_FILE))
@@ -141,105 +167,196 @@ def hpsi_section(args, ctx, torch):
     hp = {"dim": n, "ms": hpsi_ms, "nnz_eff": st["nnz_eff"], "nnz_offdiag": st["nnz_offdiag"],
           "nnz_stored": st["nnz_stored"], "nblocks": st["nblocks"], "block_hits": st["block_hits"],
           "ell_slots": st["ell_slots"], "packed": st["packed"], "shift_min": st["shift_min"],
-          "stream_bytes": st["stream_bytes"], "bytes_algorithmic": bytes_alg, "GBps_algorithmic": achieved,
-          "compulsory_bytes": n * (8 + 32), "setup_s": {"enumerate": t_enum, "assemble": t_asm},
-          "plain_spectrum_ms": spec_ms, "plain_spectrum": f"T|psi0> + {args.ntri} Lanczos steps + CF on "
-                                                        f"{args.ne + 1} points, random psi0"}
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+          "stream_bytes": st["stream_bytes"], "bytes_needed": needed, "gather_bytes": gather_bytes,
+          "gather_rate_GBps": gather_bytes / t / 1e9, "compulsory_bytes": n * (8 + 32),
+          "setup_s": {"enumerate": t_enum, "assemble": t_asm}, "plain_spectrum_ms": spec_ms,
+          "plain_spectrum": f"T|psi0> + {args.ntri} Lanczos steps + CF on {args.ne + 1} points, random psi0"}
+    roof = {"bound": "hbm", "achieved": needed / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": needed / t / 1e9 / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_GBps": None if traffic is None else traffic / t / 1e9,
+            "traffic_frac": None if traffic is None else traffic / t / 1e9 / HBM_PEAK_GBS,
             "kernel": "k_spmv<1,1,true,true> (assembled H*psi, 50-mode Anderson impurity, 2,179,235 dets)",
             "copy_1GiB_GBps": copy_gbps}
     return hp, roof, (pr, basis)
 
 
-def serve_section(args, ctx):
-    """The same Poisson stream served with continuous batching and one request
-    at a time."""
-    from quanty_amd import serve
-
-    # warm-up: the first `warmup` requests of the timed stream, served untimed by
-    # the same server, so the library's device memory cache holds blocks of the
-    # sizes the stream needs (as in a server that has been running). Memory new
-    # to the process is cleared by the driver when it is allocated: up to ~200 GB
-    # of it, seconds, on a device another process has just used. The timed run
-    # recomputes everything; only memory blocks and HIP graphs are reused.
-    warm = serve.request_stream(args.steps, 0.0, seed=args.seed + 100)[:args.warmup]
-    serve.SpectrumServer(ctx, max_slots=args.slots).serve(warm, realtime=False)
-    reqs = serve.request_stream(args.steps, args.rate, seed=args.seed + 100)
-    t0 = time.perf_counter()
-    w0 = time.monotonic_ns()
-    served = serve.SpectrumServer(ctx, max_slots=args.slots).serve(reqs, realtime=True)
-    t_served = time.perf_counter() - t0
-    # the window on the monotonic clock, which kernel traces also use
-    log(f"served stream: {t_served:.2f} s (monotonic ns {w0} .. {time.monotonic_ns()})")
-    reqs2 = serve.request_stream(args.steps, args.rate, seed=args.seed + 100)
-    t0 = time.perf_counter()
-    plain = serve.SpectrumServer(ctx, max_slots=1).serve(reqs2, realtime=True, plain=True)
-    t_plain = time.perf_counter() - t0
-    log(f"plain stream: {t_plain:.2f} s")
-    by_rid = {r.rid: r for r in plain}
-    same = all(np.array_equal(r.spectrum, by_rid[r.rid].spectrum) for r in served)
-    s1, s2 = serve.latency_stats(served), serve.latency_stats(plain)
+def _stream_summary(reqs, done, args):
     dims = np.array([r.dim for r in reqs])
-    work = [(r.dim, r.gs_steps, r.depth) for r in served]
+    work = [(r.dim, r.gs_steps, r.depth) for r in done]
     rs = np.array([dim * (gs + depth) for dim, gs, depth in work], dtype=float)
     big = np.array([dim > 3e5 for dim, _, _ in work])
-    return {"served": s1, "plain": s2, "speedup": s1["spectra_per_s"] / s2["spectra_per_s"],
-            "wall_s": {"served": t_served, "plain": t_plain}, "identical_spectra": bool(same),
-            "stream": {"requests": args.steps, "rate_per_s": args.rate, "dim_min": int(dims.min()),
-                       "dim_max": int(dims.max()), "dim_median": float(np.median(dims)), "depth": "U{100..400}",
-                       "gamma": "U(0.1,0.5)", "ne_points": 2001, "slots": args.slots, "hbm_budget_GB": 200,
-                       "gs": "Lanczos to |H psi - E psi|^2 <= 1e-12 max(1, E^2), Ritz vector from the kept "
-                             "Lanczos vectors (restarted from it when 500 are kept before convergence)",
-                       "gs_steps_max": int(max(w[1] for w in work)),
-                       "gs_steps_mean": float(np.mean([w[1] for w in work])),
-                       "row_steps": float(rs.sum()), "row_steps_share_dim_gt_3e5": float(rs[big].sum() / rs.sum())}}, \
-        work
+    return {"requests": len(reqs), "rate_per_s": args.rate, "dim_min": int(dims.min()), "dim_max": int(dims.max()),
+            "dim_median": float(np.median(dims)), "depth": "U{100..400}", "gamma": "U(0.1,0.5)",
+            "ne_points": args.ne + 1, "slots": args.slots, "hbm_budget_GB": 200,
+            "gs": "Lanczos to |H psi - E psi|^2 <= 1e-12 max(1, E^2), Ritz vector from the kept Lanczos vectors "
+                  "(restarted from it when 500 are kept before convergence)",
+            "gs_steps_max": int(max(w[1] for w in work)), "gs_steps_mean": float(np.mean([w[1] for w in work])),
+            "row_steps": float(rs.sum()), "row_steps_share_dim_gt_3e5": float(rs[big].sum() / rs.sum())}
 
 
-def cpu_baseline(args, pr, basis, work):
-    """The oracle's push-form H*psi (the reference algorithm: per (det, term)
-    byte-wise ladder and hash insert) timed on a bounded sample of source rows
-    of the 50-mode operator with all host threads (at most 16). The stream's
-    CPU time is estimated as sum over requests of (ground-state steps + depth)
-    * dim * (time per row): H*psi only, no vector work, so it favours the CPU."""
+def _oracle_sample(reqs, limit):
+    """Served requests the oracle re-computes: the smallest dim, and the ones
+    nearest 3e3 and 1e4, all at most `limit` dims."""
+    small = [r for r in reqs if r.dim <= limit]
+    if not small:
+        return []
+    pick = {min(small, key=lambda r: r.dim).rid}
+    for target in (3e3, 1e4):
+        pick.add(min(small, key=lambda r: abs(math.log(r.dim / target))).rid)
+    return sorted(pick)
+
+
+def serve_section(args, ctx):
+    """The stream served with continuous batching; a prefix of it served again
+    and one request at a time."""
+    from quanty_amd import serve
+
+    t0 = time.perf_counter()
+    reqs = serve.request_stream(args.requests, args.rate, seed=args.seed + 100)
+    log(f"stream of {len(reqs)} requests made in {time.perf_counter() - t0:.1f} s")
+    # warm-up: the stream's first requests served untimed (library caches, HIP
+    # graphs, preparation contexts); the timed runs recompute everything
+    t0 = time.perf_counter()
+    serve.SpectrumServer(ctx, max_slots=args.slots).reserve()
+    log(f"HBM budget reserved in {time.perf_counter() - t0:.1f} s")
+    warm = serve.request_stream(min(args.warm_requests, args.requests), 0.0, seed=args.seed + 100)
+    serve.SpectrumServer(ctx, max_slots=args.slots).serve(warm, realtime=False)
+    # the timed stream; the ground states of a few small requests are kept for the oracle
+    srv = serve.SpectrumServer(ctx, max_slots=args.slots)
+    srv.keep_states = set(_oracle_sample(reqs, args.oracle_max_dim))
+    w0 = time.monotonic_ns()
+    t0 = time.perf_counter()
+    served = srv.serve(reqs, realtime=True)
+    t_served = time.perf_counter() - t0
+    log(f"served stream of {len(reqs)}: {t_served:.1f} s (monotonic ns {w0} .. {time.monotonic_ns()})")
+    # the same prefix served again and one request at a time
+    npre = min(args.plain_requests, args.requests)
+    pre = serve.request_stream(npre, args.rate, seed=args.seed + 100)
+    t0 = time.perf_counter()
+    pre_served = serve.SpectrumServer(ctx, max_slots=args.slots).serve(pre, realtime=True)
+    t_pre = time.perf_counter() - t0
+    pre2 = serve.request_stream(npre, args.rate, seed=args.seed + 100)
+    t0 = time.perf_counter()
+    plain = serve.SpectrumServer(ctx, max_slots=1).serve(pre2, realtime=True, plain=True)
+    t_plain = time.perf_counter() - t0
+    log(f"prefix of {npre}: served {t_pre:.1f} s, plain {t_plain:.1f} s")
+    by_rid = {r.rid: r for r in plain}
+    same = all(np.array_equal(r.spectrum, by_rid[r.rid].spectrum) for r in pre_served)
+    full_by_rid = {r.rid: r for r in served}
+    same_full = all(np.array_equal(full_by_rid[r.rid].spectrum, r.spectrum) for r in pre_served)
+    s_full = serve.latency_stats(served)
+    s_pre, s_plain = serve.latency_stats(pre_served), serve.latency_stats(plain)
+    return {"served": s_full,
+            "prefix": {"requests": npre, "served": s_pre, "plain": s_plain,
+                       "speedup": s_pre["spectra_per_s"] / s_plain["spectra_per_s"],
+                       "identical_spectra_served_vs_plain": bool(same),
+                       "identical_spectra_prefix_vs_full_stream": bool(same_full)},
+            "wall_s": {"served": t_served, "prefix_served": t_pre, "prefix_plain": t_plain},
+            "stream": _stream_summary(reqs, served, args)}, served
+
+
+def oracle_check(served, nthreads):
+    """Re-compute kept served requests with the oracle from the served ground
+    state: the state meets the convergence criterion under the oracle's own
+    H*psi, and T psi0, the Lanczos table and the continued fraction give the
+    served spectrum to the stated relative L-infinity error."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from quanty_amd import det as D
+    from quanty_amd import serve
 
-    nf = pr["NF"]
-    threads = min(os.cpu_count() or 1, 16)
-    op = oracle.Op(nf, pr["H"])
-    words = basis.words()
-    g = np.random.default_rng(0)
+    out = []
+    for r in served:
+        if r.gs_state is None:
+            continue
+        pr = serve.make_problem(r.seed, r.shape)
+        nf = pr["NF"]
+        dets = oracle.enumerate_sector(nf, pr["restrictions"])
+        assert dets.shape[0] == r.dim == r.gs_state.shape[0]
+        psi = oracle.Vec.from_arrays(nf, dets, r.gs_state)
+        H = oracle.Op(nf, pr["H"])
+        hp = H.apply(psi, restrictions=pr["restrictions"], nthreads=nthreads)
+        E = psi.dot(hp).real
+        d, v = hp.arrays()
+        idx = {bytes(row): i for i, row in enumerate(dets)}
+        res = np.array(r.gs_state) * (-E)
+        for row, val in zip(d, v):
+            res[idx[bytes(row)]] += val
+        res2 = float(np.vdot(res, res).real)
+        tp = oracle.Op(nf, pr["T"]).apply(psi, restrictions=pr["restrictions"], nthreads=nthreads)
+        a, b = H.lanczos(tp, r.depth, restrictions=pr["restrictions"], nthreads=nthreads)
+        want = oracle.cf(a, b, E, -10.0, 20.0, 2000, r.gamma)
+        err = float(np.max(np.abs(r.spectrum - want)) / np.max(np.abs(want)))
+        out.append({"rid": r.rid, "dim": r.dim, "depth": r.depth, "E0_served": r.E0, "E0_oracle": E,
+                    "gs_residual2_oracle": res2, "gs_criterion": 1e-12 * max(1.0, E * E),
+                    "spectrum_rel_linf": err})
+    return {"requests": out, "max_spectrum_rel_linf": max((o["spectrum_rel_linf"] for o in out), default=None),
+            "all_ground_states_within_criterion": all(o["gs_residual2_oracle"] <= 1.01 * o["gs_criterion"]
+                                                      for o in out)}
 
-    def run(rows):
-        sel = np.sort(g.choice(basis.n, rows, replace=False))
-        psi = oracle.Vec.from_arrays(nf, D.words_to_bytes(words[sel], nf), np.ones(rows, dtype=complex))
+
+def cpu_baseline(args, pr, basis, served, nthreads):
+    """Complete requests on the host cores with the oracle (the reference's
+    push-form, hash-map algorithm): ground state by Lanczos with kept vectors
+    (as many steps as the served run needed) and its Ritz vector, T psi0, the
+    Lanczos table of the request's depth, the continued fraction. The smallest
+    requests of the stream are timed end to end until about cpu_seconds have
+    passed; one H*psi pass on the 2,179,235-dim operator gives the cost per
+    row-step at the large end. The stream's CPU time is estimated from both
+    (requests at most 2e4 dims at the small-request rate, larger at the
+    large-operator rate) and stated as an estimate."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from quanty_amd import serve
+
+    small = sorted(served, key=lambda r: r.dim)
+    t_run = 0.0
+    row_steps = 0.0
+    done = []
+    for r in small:
+        if t_run > 0.7 * args.cpu_seconds or r.dim > 20000:
+            break
+        prb = serve.make_problem(r.seed, r.shape)
+        nf = prb["NF"]
         t0 = time.perf_counter()
-        op.apply(psi, restrictions=pr["restrictions"], nthreads=threads)
-        return time.perf_counter() - t0
-
-    # size the sample to about cpu_seconds: a short probe overestimates the time
-    # per row (thread start-up), so a sample that ran short is sized once more
-    probe = 2000
-    t = run(probe)
-    rows = int(min(basis.n, max(probe, probe * args.cpu_seconds / max(t, 1e-3))))
-    t = run(rows)
-    if t < 0.6 * args.cpu_seconds and rows < basis.n:
-        rows = int(min(basis.n, rows * args.cpu_seconds / max(t, 1e-3)))
-        t = run(rows)
-    passes = 1
-    if rows == basis.n and t < 0.6 * args.cpu_seconds:  # the whole operator is quick: repeat it
-        passes = int(math.ceil(args.cpu_seconds / max(t, 1e-3)))
-        t = sum(run(rows) for _ in range(passes))
-    per_row = t / (rows * passes)
-    t_stream = sum((gs + depth) * dim * per_row for dim, gs, depth in work)
-    return {"value": len(work) / t_stream, "unit": "spectra/s", "cores": threads, "kind": "port",
-            "sample": f"oracle push-form H*psi over {rows} of {basis.n} rows of the 50-mode operator"
-                      f"{f' x {passes} passes' if passes > 1 else ''} in {t:.2f} s "
-                      f"({threads} threads, {per_row * 1e6:.3f} us/row); the stream's {len(work)} requests need "
-                      f"sum (gs steps + depth) * dim row-applications -> {t_stream:.0f} s"}
+        dets = oracle.enumerate_sector(nf, prb["restrictions"])
+        g = np.random.default_rng(r.seed)
+        v0 = oracle.Vec.from_arrays(nf, dets, g.normal(size=len(dets)) + 1j * g.normal(size=len(dets)))
+        H = oracle.Op(nf, prb["H"])
+        m = max(2, min(r.gs_steps, 500))
+        a, b, keep = H.lanczos(v0, m, restrictions=prb["restrictions"], nthreads=nthreads, keep=True)
+        w, z = np.linalg.eigh(np.diag(a[1:m + 1]) + np.diag(b[1:m], 1) + np.diag(b[1:m], -1))
+        psi = oracle.combine(keep, z[:, 0])
+        del keep
+        d, c = psi.arrays()
+        psi = oracle.Vec.from_arrays(nf, d, c / np.linalg.norm(c))
+        E0 = psi.dot(H.apply(psi, restrictions=prb["restrictions"], nthreads=nthreads)).real
+        tp = oracle.Op(nf, prb["T"]).apply(psi, restrictions=prb["restrictions"], nthreads=nthreads)
+        a2, b2 = H.lanczos(tp, r.depth, restrictions=prb["restrictions"], nthreads=nthreads)
+        G = oracle.cf(a2, b2, E0, -10.0, 20.0, 2000, r.gamma)
+        dt = time.perf_counter() - t0
+        assert np.all(np.isfinite(G))
+        t_run += dt
+        row_steps += r.dim * (m + 1 + r.depth)
+        done.append((r.dim, dt))
+    small_rate = t_run / max(row_steps, 1.0)  # s per row-step, small requests
+    # one H*psi pass over the large operator
+    from quanty_amd import det as D
+    words = basis.words()
+    psi = oracle.Vec.from_arrays(pr["NF"], D.words_to_bytes(words, pr["NF"]), np.ones(basis.n, dtype=complex))
+    op = oracle.Op(pr["NF"], pr["H"])
+    t0 = time.perf_counter()
+    op.apply(psi, restrictions=pr["restrictions"], nthreads=nthreads)
+    big_rate = (time.perf_counter() - t0) / basis.n
+    est = sum(r.dim * (r.gs_steps + 1 + r.depth) * (small_rate if r.dim <= 20000 else big_rate) for r in served)
+    return {"value": len(served) / est, "unit": "spectra/s", "cores": nthreads, "kind": "port",
+            "sample": f"{len(done)} complete requests of the stream (dims {done[0][0]}..{done[-1][0]}) timed end to "
+                      f"end on {nthreads} host threads in {t_run:.1f} s ({small_rate * 1e9:.0f} ns per row-step); "
+                      f"one H*psi pass on the 2,179,235-dim operator: {big_rate * 1e9:.0f} ns per row. "
+                      f"Estimate for the {len(served)}-request stream: {est:.0f} s (requests above 2e4 dims at the "
+                      f"large-operator rate per row-step, H*psi only, which favours the CPU)"
+                      if done else "no request small enough",
+            "nproc": os.cpu_count()}
 
 
 def main():
@@ -257,11 +374,18 @@ def main():
         from quanty_amd._lib import QS_PARAM_SHIFT_MIN
         ctx.set_param(QS_PARAM_SHIFT_MIN, args.shift_min)
     hp, roof, (pr, basis) = hpsi_section(args, ctx, torch)
-    srv, work = (None, None) if args.no_serve else serve_section(args, ctx)
+    srv, served = (None, None) if args.no_serve else serve_section(args, ctx)
+    nth = threads()
+    check = None
+    if srv and not args.no_oracle:
+        t0 = time.perf_counter()
+        check = oracle_check(served, nth)
+        log(f"oracle check done in {time.perf_counter() - t0:.1f} s")
     cpu = None
-    if not args.no_cpu and work:
-        cpu = cpu_baseline(args, pr, basis, work)
-        log("cpu baseline done")
+    if not args.no_cpu and served:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(args, pr, basis, served, nth)
+        log(f"cpu baseline done in {time.perf_counter() - t0:.1f} s")
     if srv:
         value = srv["served"]["spectra_per_s"]
         steps = srv["served"]["n"]
@@ -276,7 +400,7 @@ def main():
         "unit": "spectra/s",
         "n_gpus": 1,
         "steps": steps,
-        "warmup": args.warmup,
+        "warmup": min(args.warm_requests, args.requests) if srv else 1,
         "ms_per_step": 1e3 * span / max(steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
@@ -286,8 +410,11 @@ def main():
         "config": {"workload": "impurity-spectrum stream: Poisson requests, dim log-uniform 1e3..3e6 (50-mode "
                                "Anderson family), depth U{100..400}, Gamma U(0.1,0.5), 200 GB HBM budget; "
                                "H*psi on the 50-mode N=28 impurity (2,179,235 dets)",
-                   "requests": args.steps, "rate_per_s": args.rate, "parallelism": "single GPU"},
+                   "requests": steps if srv else 0, "rate_per_s": args.rate, "parallelism": "single GPU",
+                   "cli": {"steps": args.steps, "warmup": args.warmup},
+                   "step": "one served request; the stream length is --requests, not --steps"},
         "serving": srv,
+        "oracle_check": check,
         "hpsi": hp,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -52,6 +52,9 @@ def load():
     L.or_opsi_range.restype = P
     L.or_opsi_range.argtypes = [P, P, I64, I64, C.c_int, P, P, P, C.c_int]
     L.or_lanczos.argtypes = [P, P, C.c_int, D, D, C.c_int, P, P, P, P, P]
+    L.or_lanczos_mt.argtypes = [P, P, C.c_int, D, D, C.c_int, P, P, P, P, P, C.c_int, P]
+    L.or_combine.restype = P
+    L.or_combine.argtypes = [P, C.c_int, P]
     L.or_cf.argtypes = [P, P, C.c_int, D, D, D, C.c_int, D, P, P]
     L.or_enumerate.restype = I64
     L.or_enumerate.argtypes = [C.c_int, C.c_int, P, P, P, P, I64]
@@ -146,11 +149,16 @@ class Op:
         h = self.L.or_opsi_range(self.h, psi.h, i0, i1, nres, _p(masks), _p(lo), _p(hi), nthreads)
         return Vec(self.nf, h)
 
-    def lanczos(self, psi: Vec, n: int, epsilon=0.0, gamma=0.0, restrictions=None):
+    def lanczos(self, psi: Vec, n: int, epsilon=0.0, gamma=0.0, restrictions=None, nthreads=1, keep=False):
+        """(a, b) of n Lanczos steps; keep: also the n normalized Lanczos vectors."""
         nres, masks, lo, hi = _restr(self.nf, restrictions)
         a = np.zeros(n + 1)
         b = np.zeros(n)
-        self.L.or_lanczos(self.h, psi.h, n, epsilon, gamma, nres, _p(masks), _p(lo), _p(hi), _p(a), _p(b))
+        hs = (C.c_void_p * n)() if keep else None
+        self.L.or_lanczos_mt(self.h, psi.h, n, epsilon, gamma, nres, _p(masks), _p(lo), _p(hi), _p(a), _p(b),
+                             int(nthreads), hs)
+        if keep:
+            return a, b, [Vec(self.nf, C.c_void_p(h)) for h in hs]
         return a, b
 
     def __del__(self):
@@ -216,3 +224,11 @@ def fwhm_table(table, emin, dE, npts) -> np.ndarray:
 
 def max_threads() -> int:
     return int(load().or_max_threads())
+
+
+def combine(vs, z) -> Vec:
+    """sum_i z[i] vs[i] over oracle vectors (real z), in index order."""
+    L = load()
+    arr = (C.c_void_p * len(vs))(*[v.h for v in vs])
+    zz = np.ascontiguousarray(z, dtype=np.float64)
+    return Vec(vs[0].nf, C.c_void_p(L.or_combine(arr, len(vs), _p(zz))))

@@ -354,8 +354,12 @@ orvec *or_opsi(const orop *O, const orvec *psi, int nres, const uint8_t *masks, 
 /* ------------------------------------------------------------------------ */
 /* Lanczos tridiagonalization (LanczosTriDiagonalizeCC, Core/Lanczos.cpp:2169-2264) */
 /* ------------------------------------------------------------------------ */
-void or_lanczos(const orop *H, const orvec *psi, int N, double epsilon, double gamma, int nres,
-                const uint8_t *masks, const int *lo, const int *hi, double *a, double *b) {
+/* nthreads: threads of each O*psi (or_opsi_range). keep (N handles, or NULL):
+ * receives a copy of every normalized Lanczos vector v_0 .. v_{N-1} (after
+ * truncation), for a Ritz vector by or_combine. */
+void or_lanczos_mt(const orop *H, const orvec *psi, int N, double epsilon, double gamma, int nres,
+                   const uint8_t *masks, const int *lo, const int *hi, double *a, double *b, int nthreads,
+                   orvec **keep) {
   orvec *v[3] = {NULL, NULL, NULL};
   double err = 1.0, junk;
   v[0] = vec_copy(psi);
@@ -363,7 +367,8 @@ void or_lanczos(const orop *H, const orvec *psi, int N, double epsilon, double g
   b[0] = vec_norm(v[0]);
   vec_scale(v[0], 1.0 / b[0]);
   v[0] = or_truncate(v[0], epsilon);
-  v[1] = or_opsi(H, v[0], nres, masks, lo, hi);
+  if (keep) keep[0] = vec_copy(v[0]);
+  v[1] = or_opsi_range(H, v[0], 0, v[0]->n, nres, masks, lo, hi, nthreads);
   or_dot(v[1], v[0], &a[1], &junk);
   int i, vi = 0, vip = 0, vim = 0;
   for (i = 1; i < N; ++i) {
@@ -374,9 +379,10 @@ void or_lanczos(const orop *H, const orvec *psi, int N, double epsilon, double g
     b[i] = vec_norm(v[vi]);
     vec_scale(v[vi], 1.0 / b[i]);
     v[vi] = or_truncate(v[vi], epsilon / err);
+    if (keep) keep[i] = vec_copy(v[vi]);
     err *= b[i] * b[i] / (b[i] * b[i] + 4.0 * gamma * gamma);
     if (v[vip]) or_vec_free(v[vip]);
-    v[vip] = or_opsi(H, v[vi], nres, masks, lo, hi);
+    v[vip] = or_opsi_range(H, v[vi], 0, v[vi]->n, nres, masks, lo, hi, nthreads);
     or_dot(v[vip], v[vi], &a[i + 1], &junk);
     vec_axpy_union(v[vip], v[vim], -b[i]);
     or_vec_free(v[vim]);
@@ -384,6 +390,19 @@ void or_lanczos(const orop *H, const orvec *psi, int N, double epsilon, double g
   }
   for (int k = 0; k < 3; ++k)
     if (v[k]) or_vec_free(v[k]);
+}
+
+void or_lanczos(const orop *H, const orvec *psi, int N, double epsilon, double gamma, int nres,
+                const uint8_t *masks, const int *lo, const int *hi, double *a, double *b) {
+  or_lanczos_mt(H, psi, N, epsilon, gamma, nres, masks, lo, hi, a, b, 1, NULL);
+}
+
+/* y = sum_i z[i] vs[i] (real coefficients), accumulated in index order into
+ * the union of the supports: the Ritz vector of kept Lanczos vectors. */
+orvec *or_combine(orvec *const *vs, int n, const double *z) {
+  orvec *y = or_vec_new(vs[0]->nbytes);
+  for (int i = 0; i < n; ++i) vec_axpy_union(y, vs[i], z[i]);
+  return y;
 }
 
 /* ------------------------------------------------------------------------ */

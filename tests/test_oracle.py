@@ -430,3 +430,31 @@ def test_mlft_d5_expectation_values_golden(axis):
         for k in names:
             row.append(np.vdot(x, to_vec(*apply(ops[k], x))).real)
         assert np.max(np.abs(np.array(row) - np.array(want[r]))) < 2e-8, r
+
+
+def test_threaded_lanczos_keeps_orthonormal_vectors():
+    """or_lanczos_mt: with several threads the coefficients equal the
+    single-thread recurrence to rounding, the kept vectors are the normalized
+    Lanczos vectors (orthonormal while the recurrence keeps orthogonality,
+    v_0 = psi/|psi|), and or_combine forms sum z_i v_i: the Ritz vector of the
+    lowest eigenvalue of the table has the Rayleigh quotient that eigenvalue."""
+    import oracle
+    from models import chain
+    nf, terms = chain(5, 1)
+    op = oracle.Op(nf, terms)
+    rng = np.random.default_rng(2)
+    dets = oracle.enumerate_sector(nf, [("1" * nf, 6, 6)])
+    psi = oracle.Vec.from_arrays(nf, dets, rng.normal(size=len(dets)) + 1j * rng.normal(size=len(dets)))
+    a1, b1 = op.lanczos(psi, 12)
+    a2, b2, keep = op.lanczos(psi, 12, nthreads=4, keep=True)
+    assert np.allclose(a1, a2, rtol=1e-12, atol=1e-12) and np.allclose(b1, b2, rtol=1e-12)
+    G = np.array([[x.dot(y) for y in keep] for x in keep])
+    assert np.max(np.abs(G - np.eye(len(keep)))) < 1e-10
+    d0, c0 = psi.arrays()
+    _, k0 = keep[0].arrays()
+    assert np.allclose(k0, c0 / np.linalg.norm(c0), atol=1e-14)
+    T = np.diag(a2[1:]) + np.diag(b2[1:], 1) + np.diag(b2[1:], -1)
+    w, z = np.linalg.eigh(T)
+    y = oracle.combine(keep, z[:, 0])
+    hy = op.apply(y)
+    assert abs(y.dot(hy).real / y.dot(y).real - w[0]) < 1e-9
