@@ -209,6 +209,18 @@ qs_status qs_apply(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_
 qs_status qs_expect(qs_ctx *ctx, const qs_op *op, const qs_basis *b, const double *x,
                     double *re, double *im);
 
+/* <x|O|x> with no allocation at all: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same value as
+ * qs_expect, written to out (device, 2 doubles: re, im) instead of the host.
+ * O x and the partial sums live in a static device array of 64 MiB that every
+ * GPU the library runs on holds until the process ends, so the basis may hold
+ * at most 4,193,280 determinants (larger bases are refused with QS_EINVAL;
+ * use qs_expect). Calls that overlap on one GPU share that array and race:
+ * serialize them (one stream, or one call at a time). With an async ctx the
+ * call returns after enqueueing its kernels. */
+qs_status qs_expect_noalloc(qs_ctx *ctx, const qs_op *op, const qs_basis *b, const double *x, double *out);
+
 /* ---- vector helpers (device pointers, n complex entries) ---------------- */
 /* <a|b> with a conjugated (ComplexWaveFunctionDotProductSameBasis,
  * Core/ComplexWaveFunctionArithmetic.cpp:961). Fixed-order, deterministic. */

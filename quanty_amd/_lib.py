@@ -67,6 +67,7 @@ _SIGS = [
     ("qs_hop_destroy", C.c_int, [_P]),
     ("qs_apply", C.c_int, [_P, _P, _P, _P, _P, _P]),
     ("qs_expect", C.c_int, [_P, _P, _P, _P, C.POINTER(_D), C.POINTER(_D)]),
+    ("qs_expect_noalloc", C.c_int, [_P, _P, _P, _P, _P]),
     ("qs_dot", C.c_int, [_P, _I64, _P, _P, C.POINTER(_D), C.POINTER(_D)]),
     ("qs_norm", C.c_int, [_P, _I64, _P, C.POINTER(_D)]),
     ("qs_lanczos", C.c_int, [_P, _P, _P, C.c_int, _P, _P]),

@@ -11,6 +11,7 @@
 #include <map>
 #include <tuple>
 
+#include <mutex>
 #include "qs_internal.h"
 #include "qs_kernels.h"
 
@@ -932,6 +933,98 @@ extern "C" qs_status qs_apply(qs_ctx *ctx, const qs_op *op, const qs_basis *rows
   QS_TRY(check_pair(ctx, op, rows, cols));
   QS_HIP(ctx, hipSetDevice(ctx->device));
   QS_TRY(launch_mf(ctx, op, rows, cols, rows == cols, x, y));
+  if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// <x|O|x> without any allocation: the product O x and the per-block partial
+// sums live in `scratch`, a file-scope device array that every GPU the
+// library runs on holds until the process ends. Each call overwrites all of
+// it that it reads (O x over every row, one partial per block), so nothing has
+// to start at zero; calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+#define QS_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define QS_NOALLOC_BLOCKS 1024
+static __device__ __align__(256) unsigned char scratch[QS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch_addr;  // device -> address of `scratch` on that device
+
+// The device address of `scratch` on ctx's device, looked up once per device
+// with that device current (each GPU holds its own copy at its own address).
+qs_status scratch_address(qs_ctx *ctx, void **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(ctx->device);
+  if (it == g_scratch_addr.end()) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return qs_fail(ctx, QS_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", ctx->device,
+                     hipGetErrorString(e));
+    it = g_scratch_addr.emplace(ctx->device, p).first;
+  }
+  *out = it->second;
+  return QS_OK;
+}
+}  // namespace
+
+// partial[b] = sum over block b's rows of conj(x_i) y_i (re at [b], im at [nb + b])
+__global__ void __launch_bounds__(QS_BLOCK)
+k_expect_partial(int64_t n, const double2 *__restrict__ x, const double2 *__restrict__ y, double *__restrict__ part) {
+  __shared__ double red[QS_BLOCK / 64];
+  double sr = 0.0, si = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = x[i], c = y[i];
+    sr += a.x * c.x + a.y * c.y;
+    si += a.x * c.y - a.y * c.x;
+  }
+  const double r = block_sum_values(sr, red);
+  const double m = block_sum_values(si, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = r;
+    part[gridDim.x + blockIdx.x] = m;
+  }
+}
+
+// out[0] + i out[1] = the fixed-order sum of the nb partials
+__global__ void __launch_bounds__(QS_BLOCK)
+k_expect_finish(int nb, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double red[QS_BLOCK / 64];
+  const double r = block_sum_fixed(part, nb, red);
+  const double m = block_sum_fixed(part + nb, nb, red);
+  if (threadIdx.x == 0) {
+    out[0] = r;
+    out[1] = m;
+  }
+}
+
+extern "C" qs_status qs_expect_noalloc(qs_ctx *ctx, const qs_op *op, const qs_basis *b, const double *x,
+                                       double *out) {
+  if (!ctx) return QS_EINVAL;
+  if (!x || !out) return qs_fail(ctx, QS_EINVAL, "qs_expect_noalloc: null vector or output");
+  QS_TRY(check_pair(ctx, op, b, b));
+  const int64_t n = b->n;
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(QS_NOALLOC_BLOCKS, (n + QS_BLOCK - 1) / QS_BLOCK));
+  const size_t vec_bytes = ((size_t)n * 16 + 255) / 256 * 256;
+  const size_t need = vec_bytes + (size_t)nb * 16;
+  if (need > QS_NOALLOC_SCRATCH_BYTES)
+    return qs_fail(ctx, QS_EINVAL,
+                   "qs_expect_noalloc: a basis of %lld determinants needs %zu bytes of scratch; the static "
+                   "scratch holds %zu bytes (at most %lld determinants); use qs_expect",
+                   (long long)n, need, QS_NOALLOC_SCRATCH_BYTES,
+                   (long long)((QS_NOALLOC_SCRATCH_BYTES - QS_NOALLOC_BLOCKS * 16) / 16));
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  void *base = nullptr;
+  QS_TRY(scratch_address(ctx, &base));
+  double *y = (double *)base;
+  double *part = (double *)((char *)base + vec_bytes);
+  QS_TRY(launch_mf(ctx, op, b, b, true, x, y));
+  hipLaunchKernelGGL(k_expect_partial, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, n, (const double2 *)x,
+                     (const double2 *)y, part);
+  hipLaunchKernelGGL(k_expect_finish, dim3(1), dim3(QS_BLOCK), 0, ctx->stream, nb, (const double *)part, out);
+  QS_HIP(ctx, hipGetLastError());
   if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QS_OK;
 }
