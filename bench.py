@@ -206,6 +206,12 @@ def _oracle_sample(reqs, limit):
     return sorted(pick)
 
 
+def _progress(label):
+    def report(done, total, seconds):
+        log(f"{label}: {done}/{total} requests done after {seconds:.1f} s")
+    return report
+
+
 def serve_section(args, ctx):
     """The stream served with continuous batching; a prefix of it served again
     and one request at a time."""
@@ -226,18 +232,20 @@ def serve_section(args, ctx):
     srv.keep_states = set(_oracle_sample(reqs, args.oracle_max_dim))
     w0 = time.monotonic_ns()
     t0 = time.perf_counter()
-    served = srv.serve(reqs, realtime=True)
+    served = srv.serve(reqs, realtime=True, progress=_progress("served"))
     t_served = time.perf_counter() - t0
     log(f"served stream of {len(reqs)}: {t_served:.1f} s (monotonic ns {w0} .. {time.monotonic_ns()})")
     # the same prefix served again and one request at a time
     npre = min(args.plain_requests, args.requests)
     pre = serve.request_stream(npre, args.rate, seed=args.seed + 100)
     t0 = time.perf_counter()
-    pre_served = serve.SpectrumServer(ctx, max_slots=args.slots).serve(pre, realtime=True)
+    pre_served = serve.SpectrumServer(ctx, max_slots=args.slots).serve(pre, realtime=True,
+                                                                       progress=_progress("prefix served"))
     t_pre = time.perf_counter() - t0
     pre2 = serve.request_stream(npre, args.rate, seed=args.seed + 100)
     t0 = time.perf_counter()
-    plain = serve.SpectrumServer(ctx, max_slots=1).serve(pre2, realtime=True, plain=True)
+    plain = serve.SpectrumServer(ctx, max_slots=1).serve(pre2, realtime=True, plain=True,
+                                                         progress=_progress("prefix plain"))
     t_plain = time.perf_counter() - t0
     log(f"prefix of {npre}: served {t_pre:.1f} s, plain {t_plain:.1f} s")
     by_rid = {r.rid: r for r in plain}

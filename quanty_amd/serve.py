@@ -390,11 +390,13 @@ class SpectrumServer:
         self.jobs[job.req.rid] = job
         self.used += job.bytes + job.kept_bytes
 
-    def serve(self, requests: list[Request], realtime: bool = True, plain: bool = False) -> list[Request]:
+    def serve(self, requests: list[Request], realtime: bool = True, plain: bool = False,
+              progress=None) -> list[Request]:
         """Run the stream. realtime: arrivals are honoured on the wall clock;
         plain: one request at a time, prepared when its turn comes (the
         baseline). Otherwise admitted requests are prepared on a worker thread
-        while the batch steps."""
+        while the batch steps. progress: called with (completed, total,
+        seconds) each time another 5% of the stream has completed."""
         pending = sorted(requests)
         waiting: list[Request] = []
         done: list[Request] = []
@@ -447,7 +449,12 @@ class SpectrumServer:
                         time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
                     continue
                 self.batch.run(self.chunk)
+                before = len(done)
                 self._step_jobs(t0, done)
+                if progress is not None:
+                    step = max(1, len(pending) // 20)
+                    if len(done) // step > before // step:
+                        progress(len(done), len(pending), time.perf_counter() - t0)
         finally:
             if prep is not None:
                 prep.close()
