@@ -263,14 +263,28 @@ def serve_section(args, ctx):
             "stream": _stream_summary(reqs, served, args)}, served
 
 
-def oracle_check(served, nthreads):
+def cpu_leg(args, pr, basis, served, nthreads):
+    """The host-side leg, the only one that runs the oracle: first as the
+    checker of a sample of served requests (oracle_check), then timed as the
+    CPU baseline (cpu_baseline). Returns (cpu_baseline, oracle_check)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    check = None
+    if not args.no_oracle:
+        t0 = time.perf_counter()
+        check = oracle_check(oracle, served, nthreads)
+        log(f"oracle check done in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    cpu = cpu_baseline(oracle, args, pr, basis, served, nthreads)
+    log(f"cpu baseline done in {time.perf_counter() - t0:.1f} s")
+    return cpu, check
+
+
+def oracle_check(oracle, served, nthreads):
     """Re-compute kept served requests with the oracle from the served ground
     state: the state meets the convergence criterion under the oracle's own
     H*psi, and T psi0, the Lanczos table and the continued fraction give the
     served spectrum to the stated relative L-infinity error."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    from quanty_amd import det as D
     from quanty_amd import serve
 
     out = []
@@ -303,52 +317,59 @@ def oracle_check(served, nthreads):
                                                       for o in out)}
 
 
-def cpu_baseline(args, pr, basis, served, nthreads):
-    """Complete requests on the host cores with the oracle (the reference's
-    push-form, hash-map algorithm): ground state by Lanczos with kept vectors
-    (as many steps as the served run needed) and its Ritz vector, T psi0, the
-    Lanczos table of the request's depth, the continued fraction. The smallest
-    requests of the stream are timed end to end until about cpu_seconds have
-    passed; one H*psi pass on the 2,179,235-dim operator gives the cost per
-    row-step at the large end. The stream's CPU time is estimated from both
-    (requests at most 2e4 dims at the small-request rate, larger at the
-    large-operator rate) and stated as an estimate."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
+def _cpu_request(oracle, r, nthreads):
+    """One complete request on the host: ground state by Lanczos with kept
+    vectors (as many steps as the served run needed) and its Ritz vector,
+    T psi0, the request's Lanczos table and the continued fraction. Returns the
+    seconds and the row-steps (dim x Lanczos steps)."""
     from quanty_amd import serve
+    prb = serve.make_problem(r.seed, r.shape)
+    nf = prb["NF"]
+    t0 = time.perf_counter()
+    dets = oracle.enumerate_sector(nf, prb["restrictions"])
+    g = np.random.default_rng(r.seed)
+    v0 = oracle.Vec.from_arrays(nf, dets, g.normal(size=len(dets)) + 1j * g.normal(size=len(dets)))
+    H = oracle.Op(nf, prb["H"])
+    m = max(2, min(r.gs_steps, 500))
+    a, b, keep = H.lanczos(v0, m, restrictions=prb["restrictions"], nthreads=nthreads, keep=True)
+    _, z = np.linalg.eigh(np.diag(a[1:m + 1]) + np.diag(b[1:m], 1) + np.diag(b[1:m], -1))
+    psi = oracle.combine(keep, z[:, 0])
+    del keep
+    d, c = psi.arrays()
+    psi = oracle.Vec.from_arrays(nf, d, c / np.linalg.norm(c))
+    E0 = psi.dot(H.apply(psi, restrictions=prb["restrictions"], nthreads=nthreads)).real
+    tp = oracle.Op(nf, prb["T"]).apply(psi, restrictions=prb["restrictions"], nthreads=nthreads)
+    a2, b2 = H.lanczos(tp, r.depth, restrictions=prb["restrictions"], nthreads=nthreads)
+    G = oracle.cf(a2, b2, E0, -10.0, 20.0, 2000, r.gamma)
+    assert np.all(np.isfinite(G))
+    return time.perf_counter() - t0, float(r.dim) * (m + 1 + r.depth)
 
-    small = sorted(served, key=lambda r: r.dim)
-    t_run = 0.0
-    row_steps = 0.0
-    done = []
-    for r in small:
-        if t_run > 0.7 * args.cpu_seconds or r.dim > 20000:
+
+def cpu_baseline(oracle, args, pr, basis, served, nthreads):
+    """Complete requests on the host cores with the oracle (the reference's
+    push-form, hash-map algorithm, see _cpu_request): requests of the stream
+    near 1e3, 2e3, 5e3, 1e4 and 2e4 dims are timed end to end (in that order,
+    while the time target allows); one H*psi pass on the 2,179,235-dim
+    operator gives the cost per row-step at the large end. The stream's CPU
+    time is estimated from both (requests up to 2e4 dims at the rate of the
+    timed requests, larger ones at the large-operator H*psi rate, which leaves
+    out their vector work and so favours the CPU) and stated as an estimate."""
+    pool = [r for r in served if r.dim <= 20000]
+    picks = []
+    for target in (1e3, 2e3, 5e3, 1e4, 2e4):
+        if pool:
+            r = min(pool, key=lambda q: abs(math.log(q.dim / target)))
+            if r not in picks:
+                picks.append(r)
+    t_run, row_steps, done = 0.0, 0.0, []
+    for r in picks:
+        if t_run > 0.6 * args.cpu_seconds:
             break
-        prb = serve.make_problem(r.seed, r.shape)
-        nf = prb["NF"]
-        t0 = time.perf_counter()
-        dets = oracle.enumerate_sector(nf, prb["restrictions"])
-        g = np.random.default_rng(r.seed)
-        v0 = oracle.Vec.from_arrays(nf, dets, g.normal(size=len(dets)) + 1j * g.normal(size=len(dets)))
-        H = oracle.Op(nf, prb["H"])
-        m = max(2, min(r.gs_steps, 500))
-        a, b, keep = H.lanczos(v0, m, restrictions=prb["restrictions"], nthreads=nthreads, keep=True)
-        w, z = np.linalg.eigh(np.diag(a[1:m + 1]) + np.diag(b[1:m], 1) + np.diag(b[1:m], -1))
-        psi = oracle.combine(keep, z[:, 0])
-        del keep
-        d, c = psi.arrays()
-        psi = oracle.Vec.from_arrays(nf, d, c / np.linalg.norm(c))
-        E0 = psi.dot(H.apply(psi, restrictions=prb["restrictions"], nthreads=nthreads)).real
-        tp = oracle.Op(nf, prb["T"]).apply(psi, restrictions=prb["restrictions"], nthreads=nthreads)
-        a2, b2 = H.lanczos(tp, r.depth, restrictions=prb["restrictions"], nthreads=nthreads)
-        G = oracle.cf(a2, b2, E0, -10.0, 20.0, 2000, r.gamma)
-        dt = time.perf_counter() - t0
-        assert np.all(np.isfinite(G))
+        dt, rs = _cpu_request(oracle, r, nthreads)
         t_run += dt
-        row_steps += r.dim * (m + 1 + r.depth)
+        row_steps += rs
         done.append((r.dim, dt))
-    small_rate = t_run / max(row_steps, 1.0)  # s per row-step, small requests
-    # one H*psi pass over the large operator
+    small_rate = t_run / max(row_steps, 1.0)  # s per row-step
     from quanty_amd import det as D
     words = basis.words()
     psi = oracle.Vec.from_arrays(pr["NF"], D.words_to_bytes(words, pr["NF"]), np.ones(basis.n, dtype=complex))
@@ -357,14 +378,15 @@ def cpu_baseline(args, pr, basis, served, nthreads):
     op.apply(psi, restrictions=pr["restrictions"], nthreads=nthreads)
     big_rate = (time.perf_counter() - t0) / basis.n
     est = sum(r.dim * (r.gs_steps + 1 + r.depth) * (small_rate if r.dim <= 20000 else big_rate) for r in served)
+    timed = ", ".join(f"{dm} dims {dt:.2f} s" for dm, dt in done)
     return {"value": len(served) / est, "unit": "spectra/s", "cores": nthreads, "kind": "port",
-            "sample": f"{len(done)} complete requests of the stream (dims {done[0][0]}..{done[-1][0]}) timed end to "
-                      f"end on {nthreads} host threads in {t_run:.1f} s ({small_rate * 1e9:.0f} ns per row-step); "
-                      f"one H*psi pass on the 2,179,235-dim operator: {big_rate * 1e9:.0f} ns per row. "
-                      f"Estimate for the {len(served)}-request stream: {est:.0f} s (requests above 2e4 dims at the "
-                      f"large-operator rate per row-step, H*psi only, which favours the CPU)"
-                      if done else "no request small enough",
-            "nproc": os.cpu_count()}
+            "sample": f"{len(done)} complete requests of the stream timed end to end on {nthreads} host threads "
+                      f"({timed}; {small_rate * 1e9:.0f} ns per row-step); one H*psi pass on the 2,179,235-dim "
+                      f"operator: {big_rate * 1e9:.0f} ns per row. Estimate for the {len(served)}-request stream: "
+                      f"{est:.0f} s (requests above 2e4 dims at the large-operator H*psi rate per row-step, "
+                      f"which leaves out their vector work and favours the CPU)",
+            "nproc": os.cpu_count(),
+            "threads_note": "the box's CPU share for one GPU (OMP_NUM_THREADS)"}
 
 
 def main():
@@ -383,17 +405,9 @@ def main():
         ctx.set_param(QS_PARAM_SHIFT_MIN, args.shift_min)
     hp, roof, (pr, basis) = hpsi_section(args, ctx, torch)
     srv, served = (None, None) if args.no_serve else serve_section(args, ctx)
-    nth = threads()
-    check = None
-    if srv and not args.no_oracle:
-        t0 = time.perf_counter()
-        check = oracle_check(served, nth)
-        log(f"oracle check done in {time.perf_counter() - t0:.1f} s")
-    cpu = None
+    cpu, check = None, None
     if not args.no_cpu and served:
-        t0 = time.perf_counter()
-        cpu = cpu_baseline(args, pr, basis, served, nth)
-        log(f"cpu baseline done in {time.perf_counter() - t0:.1f} s")
+        cpu, check = cpu_leg(args, pr, basis, served, threads())
     if srv:
         value = srv["served"]["spectra_per_s"]
         steps = srv["served"]["n"]

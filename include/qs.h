@@ -98,8 +98,14 @@ qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
  * QS_PARAM_TERM_LDS (default 24576, range 0..65536): assembled H*psi kernels
  * stage an operator's term tables (value and parity mask per term) in LDS when
  * they take at most this many bytes, and read them from global memory
- * otherwise. Larger tables in LDS leave fewer workgroups per CU. */
-enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3, QS_PARAM_TERM_LDS = 4 };
+ * otherwise. Larger tables in LDS leave fewer workgroups per CU.
+ * QS_PARAM_ELL_ORDER (default 1): the order of a row's individually stored
+ * entries in operators assembled after the call; 0 keeps the term order, 1
+ * sorts them by column, so that the lanes of a slice gather from nearby
+ * columns at each entry position (2% faster H*psi on the 2,179,235-row
+ * impurity). Results agree to rounding (the sum runs in another order). */
+enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3, QS_PARAM_TERM_LDS = 4,
+       QS_PARAM_ELL_ORDER = 5 };
 qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value);
 qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
 

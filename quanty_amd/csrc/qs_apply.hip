@@ -479,6 +479,44 @@ k_sell_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *
   if (lane == 0) atomicAdd(ctr + 1, ne + bhits);
 }
 
+// Reorder every row's ELL entries by column (one thread per row; rows with
+// more than QS_SORT_MAX entries keep their order). The padding slots after a
+// row's count stay where they are.
+#define QS_SORT_MAX 96
+template <bool PACKED>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_sell_sort_rows(int64_t nrows, const uint16_t *__restrict__ rowcnt, const uint64_t *__restrict__ slice_ptr,
+                 uint32_t colbits, uint32_t *__restrict__ colv, uint16_t *__restrict__ tidv) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nrows) return;
+  const int cnt = rowcnt[row];
+  if (cnt < 2 || cnt > QS_SORT_MAX) return;
+  const uint64_t base = slice_ptr[row >> 6] + (uint64_t)(row & 63);
+  const uint32_t cmask = PACKED ? ((1u << colbits) - 1u) : 0xFFFFFFFFu;
+  uint32_t ev[QS_SORT_MAX];
+  uint16_t tv[QS_SORT_MAX];
+  for (int j = 0; j < cnt; ++j) {
+    ev[j] = colv[base + (uint64_t)j * 64];
+    tv[j] = PACKED ? 0 : tidv[base + (uint64_t)j * 64];
+  }
+  for (int j = 1; j < cnt; ++j) {  // insertion sort by column
+    const uint32_t e = ev[j];
+    const uint16_t t = tv[j];
+    int k = j - 1;
+    while (k >= 0 && (ev[k] & cmask) > (e & cmask)) {
+      ev[k + 1] = ev[k];
+      tv[k + 1] = tv[k];
+      --k;
+    }
+    ev[k + 1] = e;
+    tv[k + 1] = t;
+  }
+  for (int j = 0; j < cnt; ++j) {
+    colv[base + (uint64_t)j * 64] = ev[j];
+    if (!PACKED) tidv[base + (uint64_t)j * 64] = tv[j];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // assembled apply: one wavefront per slice, lane per row
 // ---------------------------------------------------------------------------
@@ -733,6 +771,15 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
     if (op->nw == 1) launch_build<1, true>(ctx, h, blocks, L, ctr, blk_cnt);
     else launch_build<2, true>(ctx, h, blocks, L, ctr, blk_cnt);
     QS_HIP(ctx, hipGetLastError());
+    if (ctx->ell_order == 1) {
+      if (h->view.packed)
+        hipLaunchKernelGGL(k_sell_sort_rows<true>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, n, h->d_rowcnt,
+                           h->d_slice_ptr, h->view.colbits, h->d_col, h->d_tid);
+      else
+        hipLaunchKernelGGL(k_sell_sort_rows<false>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, n, h->d_rowcnt,
+                           h->d_slice_ptr, h->view.colbits, h->d_col, h->d_tid);
+      QS_HIP(ctx, hipGetLastError());
+    }
   }
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   SellD &A = h->view;

@@ -333,6 +333,10 @@ extern "C" qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value) {
         return qs_fail(ctx, QS_EINVAL, "QS_PARAM_TERM_LDS must be in 0..%d", QS_TERM_LDS_MAX);
       ctx->term_lds = (int)value;
       return QS_OK;
+    case QS_PARAM_ELL_ORDER:
+      if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_ELL_ORDER must be 0 or 1");
+      ctx->ell_order = (int)value;
+      return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
   }
@@ -352,6 +356,9 @@ extern "C" qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value) {
       return QS_OK;
     case QS_PARAM_TERM_LDS:
       *value = ctx->term_lds;
+      return QS_OK;
+    case QS_PARAM_ELL_ORDER:
+      *value = ctx->ell_order;
       return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);

@@ -37,6 +37,7 @@ struct qs_ctx {
   int ell_pack = 1;                // QS_PARAM_ELL_PACK
   int batch_graph = 1;             // QS_PARAM_BATCH_GRAPH
   int term_lds = QS_TERM_LDS_DEFAULT;  // QS_PARAM_TERM_LDS
+  int ell_order = 1;               // QS_PARAM_ELL_ORDER
   void *blas = nullptr;            // rocblas handle for the dense eigensolver, made on first use
   void *pinned = nullptr;          // the ctx's own pinned staging buffer (qs_h2d / qs_d2h)
   size_t pinned_bytes = 0;

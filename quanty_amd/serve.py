@@ -29,7 +29,9 @@ import numpy as np
 import torch
 
 from . import problems as P
+from ._lib import QS_PARAM_SHIFT_MIN
 from .engine import Basis, Batch, Hop, Op, cf_spectrum, tridiag_lowest
+from .tune import layout_for
 
 # (nbath, full_baths, nelec, d_range, max_holes, max_elec) -> sector size, cached
 _FAMILY = None
@@ -106,6 +108,7 @@ class Request:
     gs_steps: int = field(compare=False, default=0)
     spectrum: np.ndarray | None = field(compare=False, default=None)
     gs_state: np.ndarray | None = field(compare=False, default=None, repr=False)
+    rejected: str | None = field(compare=False, default=None)  # why the server refused it
     a: np.ndarray | None = field(compare=False, default=None)
     b: np.ndarray | None = field(compare=False, default=None)
     # the request's payload: NF, H and T terms, restrictions (made from seed and
@@ -155,6 +158,10 @@ class _Job:
         self.nf = pr["NF"]
         self.basis = Basis.enumerate(self.nf, pr["restrictions"], ctx)
         self.H = Op(self.nf, pr["H"], ctx)
+        # the layout the autotuner fitted for this basis size (tune.fit_family)
+        sm = layout_for(self.basis.n)
+        if sm is not None:
+            ctx.set_param(QS_PARAM_SHIFT_MIN, sm)
         self.hop = Hop(self.H, self.basis, mode="assembled")
         self.T = Op(self.nf, pr["T"], ctx)
         self.thop = Hop(self.T, self.basis, mode="matfree")
@@ -271,6 +278,7 @@ class SpectrumServer:
         # requests whose normalized ground state is copied to the host
         # (Request.gs_state) for an independent check of the result
         self.keep_states: set = set()
+        self.rejected: list[Request] = []  # requests refused by the budget
 
     def reserve(self, extra: float = 16e9):
         """Take the HBM budget (plus `extra` for the preparation threads'
@@ -392,11 +400,14 @@ class SpectrumServer:
 
     def serve(self, requests: list[Request], realtime: bool = True, plain: bool = False,
               progress=None) -> list[Request]:
-        """Run the stream. realtime: arrivals are honoured on the wall clock;
-        plain: one request at a time, prepared when its turn comes (the
-        baseline). Otherwise admitted requests are prepared on a worker thread
-        while the batch steps. progress: called with (completed, total,
-        seconds) each time another 5% of the stream has completed."""
+        """Run the stream; returns the completed requests. realtime: arrivals
+        are honoured on the wall clock; plain: one request at a time, prepared
+        when its turn comes (the baseline). Otherwise admitted requests are
+        prepared on a worker thread while the batch steps. A request whose
+        device memory alone exceeds the HBM budget is refused on arrival
+        (Request.rejected says why; self.rejected lists them). progress:
+        called with (completed, total, seconds) each time another 5% of the
+        stream has completed."""
         pending = sorted(requests)
         waiting: list[Request] = []
         done: list[Request] = []
@@ -420,10 +431,18 @@ class SpectrumServer:
                 while i < len(pending) and (not realtime or pending[i].arrival <= now):
                     waiting.append(pending[i])
                     i += 1
-                # admission: FIFO, by free slots and HBM budget
+                # admission: FIFO, by free slots and HBM budget; a request that
+                # would not fit the budget even alone is refused at once
                 while waiting and len(self.jobs) + len(inflight) < slots:
                     r = waiting[0]
                     est = estimate_bytes(r.dim, kept)
+                    if est > self.budget:
+                        waiting.pop(0)
+                        r.t_admit = r.t_done = time.perf_counter() - t0
+                        r.rejected = f"needs about {est / 1e9:.1f} GB of device memory; the budget is " \
+                                     f"{self.budget / 1e9:.1f} GB"
+                        self.rejected.append(r)
+                        continue
                     if (self.jobs or inflight) and self.used + est > self.budget:
                         break
                     waiting.pop(0)

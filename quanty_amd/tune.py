@@ -10,10 +10,14 @@ same for the largest term table (values and parity masks) staged in LDS:
 staging saves global reads per entry but takes LDS from other workgroups.
 
     python3 -m quanty_amd.tune            # the 50-mode Anderson impurity, 2,179,235 dets
+    python3 -m quanty_amd.tune --family   # the table the server assembles requests with
 """
 from __future__ import annotations
 
 import argparse
+import json
+import math
+import os
 import time
 
 import numpy as np
@@ -100,6 +104,52 @@ def tune_term_lds(op: Op, basis: Basis, candidates=LDS_CANDIDATES, reps: int = 1
     return best, rows
 
 
+LAYOUT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "layout_table.json")
+_TABLE = None
+
+
+def fit_family(dims=(1e3, 1e4, 1e5, 5e5, 1.5e6, 3e6), candidates=(65, 56, 48, 32), reps: int = 10,
+               seed: int = 1, out: str = LAYOUT_TABLE):
+    """The shift-block threshold the server assembles each request with: for
+    impurity shapes of the serving family near each of `dims`, the fastest
+    threshold of `candidates` by timed H*psi; written to `out` as rows
+    {dim, threshold, timings}. layout_for() picks the row nearest a
+    request's dim on a log scale."""
+    from . import default_context
+    from . import problems as P
+    from . import serve
+
+    ctx = default_context()
+    rows = []
+    for target in dims:
+        dim, shape = serve.pick_shape(target)
+        pr = P.anderson(seed=seed, **shape)
+        basis = Basis.enumerate(pr["NF"], pr["restrictions"], ctx)
+        op = Op(pr["NF"], pr["H"], ctx)
+        best, rs = tune_shift_min(op, basis, candidates=candidates, reps=reps, apply=False, seed=seed)
+        rows.append({"dim": int(basis.n), "shape": {k: list(v) if isinstance(v, tuple) else v for k, v in shape.items()},
+                     "shift_min": int(best), "us": {str(r["shift_min"]): round(r["us"], 2) for r in rs}})
+        print(f"dim {basis.n}: threshold {best}  " + "  ".join(f"{r['shift_min']}: {r['us']:.1f} us" for r in rs),
+              flush=True)
+    with open(out, "w") as f:
+        json.dump({"what": "shift-block threshold by basis size, fastest timed H*psi per impurity shape",
+                   "candidates": list(candidates), "rows": rows}, f, indent=1)
+    return rows
+
+
+def layout_for(dim: int) -> int | None:
+    """The shift-block threshold for a basis of `dim` determinants from the
+    fitted table (the measured shape nearest in log dim), or None without a
+    table."""
+    global _TABLE
+    if _TABLE is None:
+        _TABLE = json.load(open(LAYOUT_TABLE))["rows"] if os.path.exists(LAYOUT_TABLE) else []
+    if not _TABLE or dim < 1:
+        return None
+    row = min(_TABLE, key=lambda r: abs(math.log(r["dim"]) - math.log(dim)))
+    return int(row["shift_min"])
+
+
 def main():
     from . import default_context
     from . import problems as P
@@ -107,7 +157,12 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--family", action="store_true",
+                    help="fit the per-size threshold table of the serving family (layout_table.json)")
     args = ap.parse_args()
+    if args.family:
+        fit_family(reps=args.reps, seed=args.seed)
+        return
     ctx = default_context()
     pr = P.anderson(seed=args.seed)
     t0 = time.perf_counter()
