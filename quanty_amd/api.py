@@ -259,23 +259,20 @@ def BitMaskToIndex(mask: str):
 def PotentialExpandedOnClm(group: str, l: int, energies):
     """A crystal-field potential on renormalized spherical harmonics,
     [[k, m, A_km], ...], whose eigenvalues on an l shell are the given energies of
-    the irreducible representations (LuaPotentialExpandedOnClm,
-    LuaInterface/ssp.cpp:908; the input to NewOperator("CF", ...)). Cubic
-    symmetry: "Oh" with l = 2 and energies {e_eg, e_t2g}, or "C1"/"Oh" with
-    l < 2 and one energy (a spherical shift). In the d shell <m|C^4_0|m> is
-    2/7 for m = 0 (eg), -4/21 for |m| = 1 (t2g) and traceless, so
-    A00 = (2 e_eg + 3 e_t2g)/5, A40 = 21/10 (e_eg - e_t2g) and the cubic
-    invariant fixes A4+-4 = sqrt(5/14) A40."""
-    e = [float(x) for x in np.atleast_1d(energies)]
-    if group == "C1" or (group == "Oh" and l < 2):
-        return [[0, 0, e[0]]]
-    if group == "Oh" and l == 2:
-        if len(e) != 2:
-            raise QSError("PotentialExpandedOnClm(\"Oh\", 2, {e_eg, e_t2g}) needs two energies")
-        a40 = 2.1 * (e[0] - e[1])
-        a44 = math.sqrt(5.0 / 14.0) * a40
-        return [[0, 0, (2 * e[0] + 3 * e[1]) / 5], [4, 0, a40], [4, -4, a44], [4, 4, a44]]
-    raise QSError(f"Potential of \"{group}\" symmetry for angular momentum {l} not found")
+    the irreducible representations, in the reference's order and orientation
+    (LuaPotentialExpandedOnClm, LuaInterface/ssp.cpp:908; the input to
+    NewOperator("CF", ...)). Groups: C1 and Ci (one energy, any l), Oh, O and Td
+    (l <= 1 one energy; l = 2 {e_g, t_2g}), D4h, C4v, D4 and D2d (l = 1 {a, e};
+    l = 2 {a1 (z^2), b1 (x^2-y^2), b2 (xy), e (xz, yz)}), D2h (l = 2 {a_g x^2-y^2,
+    a_g z^2, their mixing, b1g xy, b2g xz, b3g yz}) and D3d (l = 2 {a1g, eg(1),
+    eg(2), their mixing}); see crystal_field.py for the construction."""
+    from .crystal_field import potential_expanded_on_clm
+    try:
+        return potential_expanded_on_clm(str(group), int(l), energies)
+    except KeyError:
+        raise QSError(f"Potential of \"{group}\" symmetry for angular momentum {l} not found") from None
+    except ValueError as e:
+        raise QSError(f"PotentialExpandedOnClm(\"{group}\", {l}, ...): {e}") from None
 
 
 def NewOperator(*args, **kw):
