@@ -141,7 +141,7 @@ def test_potential_oh_d_equals_eg_t2g_projectors(energies):
         assert max([abs(v) for v in diff.terms.values()] or [0.0]) < 1e-14
     assert PotentialExpandedOnClm("Oh", 1, [0.5]) == [[0, 0, 0.5]]
     with pytest.raises(QSError):
-        PotentialExpandedOnClm("D4h", 2, [1, 2, 3, 4])
+        PotentialExpandedOnClm("D6h", 2, [1, 2, 3, 4])
 
 
 def test_mlft_hamiltonian_as_the_script_writes_it():
