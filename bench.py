@@ -230,10 +230,15 @@ def serve_section(args, ctx):
     # the timed stream; the ground states of a few small requests are kept for the oracle
     srv = serve.SpectrumServer(ctx, max_slots=args.slots)
     srv.keep_states = set(_oracle_sample(reqs, args.oracle_max_dim))
+    h0, m0 = serve.PATTERNS.hits, serve.PATTERNS.misses
     w0 = time.monotonic_ns()
     t0 = time.perf_counter()
     served = srv.serve(reqs, realtime=True, progress=_progress("served"))
     t_served = time.perf_counter() - t0
+    cache = {"hits": serve.PATTERNS.hits - h0, "misses": serve.PATTERNS.misses - m0,
+             "bytes_end": serve.PATTERNS.bytes, "cap_bytes": serve.PATTERNS.cap,
+             "what": "requests that bound their H to a cached assembled stream of the same operator structure "
+                     "(qs_hop_create_like) instead of enumerating and assembling"}
     log(f"served stream of {len(reqs)}: {t_served:.1f} s (monotonic ns {w0} .. {time.monotonic_ns()})")
     # the same prefix served again and one request at a time
     npre = min(args.plain_requests, args.requests)
@@ -260,6 +265,7 @@ def serve_section(args, ctx):
                        "identical_spectra_served_vs_plain": bool(same),
                        "identical_spectra_prefix_vs_full_stream": bool(same_full)},
             "wall_s": {"served": t_served, "prefix_served": t_pre, "prefix_plain": t_plain},
+            "pattern_cache": cache,
             "stream": _stream_summary(reqs, served, args)}, served
 
 

@@ -103,9 +103,13 @@ qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
  * entries in operators assembled after the call; 0 keeps the term order, 1
  * sorts them by column, so that the lanes of a slice gather from nearby
  * columns at each entry position (2% faster H*psi on the 2,179,235-row
- * impurity). Results agree to rounding (the sum runs in another order). */
+ * impurity). Results agree to rounding (the sum runs in another order).
+ * QS_PARAM_ALLOC_TOP (default 0): 1 places this context's later device
+ * allocations at the top of the library's free device memory, for data that
+ * stays long (a cache of assembled operators): the free space below then
+ * stays in large pieces for the short-lived blocks placed best-fit. */
 enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3, QS_PARAM_TERM_LDS = 4,
-       QS_PARAM_ELL_ORDER = 5 };
+       QS_PARAM_ELL_ORDER = 5, QS_PARAM_ALLOC_TOP = 6 };
 qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value);
 qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
 
@@ -184,6 +188,16 @@ enum { QS_HOP_MATFREE = 0, QS_HOP_ASSEMBLED = 1, QS_HOP_AUTO = 2 };
  * `mem_budget_bytes` (0 = 50% of free HBM). */
 qs_status qs_hop_create(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, const qs_basis *cols,
                         int mode, size_t mem_budget_bytes, qs_hop **out);
+/* Bind op to the bases of `like`, an assembled binding whose operator compiles
+ * to the same terms (masks and order; values may differ), sharing like's
+ * connection stream: only the term values and the pre-summed diagonal are
+ * op's own, so nothing is enumerated or assembled. The shared stream lives
+ * until the last binding that uses it is destroyed; the bases must outlive
+ * every such binding. QS_EINVAL with a message when the terms differ. No
+ * counterpart in the reference, which has no assembled operator: for callers
+ * that apply one operator family with many parameter sets (a server's
+ * requests of one impurity shape). */
+qs_status qs_hop_create_like(qs_ctx *ctx, const qs_op *op, const qs_hop *like, qs_hop **out);
 /* Number of (row, term) pairs with nonzero phase whose source lies in cols,
  * the stored off-diagonal stream length (incl. slice padding; diagonal terms
  * are pre-summed), its bytes, and the off-diagonal entries without padding.

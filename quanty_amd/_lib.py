@@ -18,6 +18,7 @@ QS_OK = 0
 QS_HOP_MATFREE, QS_HOP_ASSEMBLED, QS_HOP_AUTO = 0, 1, 2
 QS_ENOTCONVERGED = 5
 QS_PARAM_SHIFT_MIN, QS_PARAM_ELL_PACK, QS_PARAM_BATCH_GRAPH, QS_PARAM_TERM_LDS, QS_PARAM_ELL_ORDER = 1, 2, 3, 4, 5
+QS_PARAM_ALLOC_TOP = 6
 QS_BATCH_KEEP_BASIS = 1
 STATUS = {0: "QS_OK", 1: "QS_ENOMEM_DEVICE", 2: "QS_ENOMEM_HOST", 3: "QS_EINVAL",
           4: "QS_EHIP", 5: "QS_ENOTCONVERGED", 6: "QS_EUNSUPPORTED"}
@@ -57,6 +58,7 @@ _SIGS = [
     ("qs_op_nterms", C.c_int, [_P]),
     ("qs_op_destroy", C.c_int, [_P]),
     ("qs_hop_create", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_size_t, C.POINTER(_P)]),
+    ("qs_hop_create_like", C.c_int, [_P, _P, _P, C.POINTER(_P)]),
     ("qs_hop_stats", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
                                C.POINTER(_I64)]),
     ("qs_hop_layout", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),

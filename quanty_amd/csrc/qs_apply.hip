@@ -666,10 +666,20 @@ qs_status qs_launch_mf_ext(qs_ctx *ctx, const qs_hop *h, const double *x, double
   return launch_mf(ctx, h->op, h->rows, h->cols, h->same, x, y);
 }
 
-static void free_stream(qs_hop *h) {
-  void *bufs[] = {h->d_blk_ptr, h->d_blk, h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid, h->d_diag};
-  for (void *b : bufs)
+HopIndex::~HopIndex() {
+  for (void *b : buf)
     qs_free(b);
+}
+
+static void free_stream(qs_hop *h) {
+  if (h->index) {
+    h->index.reset();  // the arrays go with the last binding that shares them
+  } else {
+    void *bufs[] = {h->d_blk_ptr, h->d_blk, h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid};
+    for (void *b : bufs)
+      qs_free(b);
+  }
+  qs_free(h->d_diag);
   h->d_blk_ptr = nullptr;
   h->d_blk = nullptr;
   h->d_slice_ptr = nullptr;
@@ -798,7 +808,146 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   A.val = (const double2 *)op->d_val;
   A.par = op->d_par;
   A.diag = h->d_diag;
+  h->index = std::make_shared<HopIndex>();
+  void *own[7] = {h->d_blk_ptr, h->d_blk, h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid};
+  std::copy(own, own + 7, h->index->buf);
   h->assembled = 1;
+  return QS_OK;
+}
+
+// Pre-summed diagonal of the rows: for every row, the sum over the diagonal
+// terms (listed in `which`, in term order) that hit it of (-1)^popcount(d & P_t) V_t.
+// The same terms in the same order as k_sell_build's count pass, so the two
+// give bit-identical diagonals.
+template <int NW>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_diag_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *__restrict__ terms,
+             const int *__restrict__ which, int nwhich, void *__restrict__ diag, int real_diag) {
+  __shared__ TermD<NW> sh[QS_TILE];
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = row < nrows;
+  uint64_t d[NW];
+#pragma unroll
+  for (int w = 0; w < NW; ++w) d[w] = active ? rows[row * NW + w] : 0;
+  double dr = 0.0, di = 0.0;
+  for (int base = 0; base < nwhich; base += QS_TILE) {
+    const int cnt = min(QS_TILE, nwhich - base);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) sh[i] = terms[which[base + i]];
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const TermD<NW> &t = sh[j];
+      bool hit = active;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) hit = hit && ((d[w] & t.C[w]) == t.V[w]);
+      if (hit) {
+        uint32_t p = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) p += __popcll(d[w] & t.P[w]);
+        dr += (p & 1u) ? -t.re : t.re;
+        di += (p & 1u) ? -t.im : t.im;
+      }
+    }
+  }
+  if (active) {
+    if (real_diag) static_cast<double *>(diag)[row] = dr;
+    else static_cast<double2 *>(diag)[row] = make_double2(dr, di);
+  }
+}
+
+extern "C" qs_status qs_hop_create_like(qs_ctx *ctx, const qs_op *op, const qs_hop *like, qs_hop **out) {
+  if (!ctx || !op || !like || !out) return QS_EINVAL;
+  *out = nullptr;
+  if (!like->assembled || !like->index)
+    return qs_fail(ctx, QS_EINVAL, "qs_hop_create_like needs an assembled binding to share");
+  const qs_op *lo = like->op;
+  if (op->nf != lo->nf || op->nw != lo->nw)
+    return qs_fail(ctx, QS_EINVAL, "qs_hop_create_like: %d modes against the shared binding's %d", op->nf, lo->nf);
+  if (op->nterms != lo->nterms)
+    return qs_fail(ctx, QS_EINVAL, "qs_hop_create_like: %d terms against the shared binding's %d", op->nterms,
+                   lo->nterms);
+  for (int t = 0; t < op->nterms; ++t) {
+    const TermHost &a = op->h_terms[t], &b = lo->h_terms[t];
+    for (int w = 0; w < op->nw; ++w)
+      if (a.C[w] != b.C[w] || a.V[w] != b.V[w] || a.F[w] != b.F[w] || a.P[w] != b.P[w])
+        return qs_fail(ctx, QS_EINVAL, "qs_hop_create_like: term %d acts differently from the shared binding's", t);
+  }
+  QS_HIP(ctx, hipSetDevice(ctx->device));
+  qs_hop *h = new qs_hop();
+  h->op = op;
+  h->rows = like->rows;
+  h->cols = like->cols;
+  h->same = like->same;
+  h->assembled = 1;
+  h->nnz_eff = like->nnz_eff;
+  h->nnz_stored = like->nnz_stored;
+  h->nnz_offdiag = like->nnz_offdiag;
+  h->nblocks = like->nblocks;
+  h->block_hits = like->block_hits;
+  h->ell_slots = like->ell_slots;
+  h->shift_min = like->shift_min;
+  h->nslices = like->nslices;
+  h->index = like->index;
+  h->d_blk_ptr = like->d_blk_ptr;
+  h->d_blk = like->d_blk;
+  h->d_slice_ptr = like->d_slice_ptr;
+  h->d_slice_len = like->d_slice_len;
+  h->d_rowcnt = like->d_rowcnt;
+  h->d_col = like->d_col;
+  h->d_tid = like->d_tid;
+  h->view = like->view;
+  h->view.val = (const double2 *)op->d_val;
+  h->view.par = op->d_par;
+  h->view.diag = nullptr;
+  const int64_t n = h->rows->n;
+  size_t diag_elem = 0;
+  qs_status st = QS_OK;
+  if (h->same) {
+    bool real_diag = true;
+    std::vector<int> which;
+    for (int t = 0; t < op->nterms; ++t) {
+      const TermHost &x = op->h_terms[t];
+      if ((x.F[0] | x.F[1]) == 0) {
+        which.push_back(t);
+        if (x.im != 0.0) real_diag = false;
+      }
+    }
+    h->view.real_diag = real_diag ? 1 : 0;
+    diag_elem = real_diag ? 8 : 16;
+    st = qs_malloc(ctx, (void **)&h->d_diag, (size_t)std::max<int64_t>(n, 1) * diag_elem);
+    void *scr = nullptr;
+    if (st == QS_OK && !which.empty()) st = qs_scratch(ctx, which.size() * sizeof(int), &scr);
+    if (st == QS_OK && !which.empty()) st = qs_h2d(ctx, scr, which.data(), which.size() * sizeof(int));
+    if (st == QS_OK && n > 0) {
+      const unsigned blocks = (unsigned)((n + QS_BLOCK - 1) / QS_BLOCK);
+      if (op->nw == 1)
+        hipLaunchKernelGGL(k_diag_build<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, h->rows->d_words, n,
+                           (const TermD<1> *)op->d_terms, (const int *)scr, (int)which.size(), (void *)h->d_diag,
+                           h->view.real_diag);
+      else
+        hipLaunchKernelGGL(k_diag_build<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, h->rows->d_words, n,
+                           (const TermD<2> *)op->d_terms, (const int *)scr, (int)which.size(), (void *)h->d_diag,
+                           h->view.real_diag);
+      const hipError_t e = hipGetLastError();
+      if (e == hipSuccess) {
+        const hipError_t e2 = hipStreamSynchronize(ctx->stream);
+        if (e2 != hipSuccess) st = qs_hip_fail(ctx, e2, "qs_hop_create_like", __FILE__, __LINE__);
+      } else {
+        st = qs_hip_fail(ctx, e, "qs_hop_create_like", __FILE__, __LINE__);
+      }
+    }
+    h->view.diag = h->d_diag;
+  }
+  if (st != QS_OK) {
+    free_stream(h);
+    delete h;
+    return st;
+  }
+  const uint64_t per_entry = h->view.packed ? 4 : 6;
+  h->stream_bytes = (int64_t)((uint64_t)h->ell_slots * per_entry + (uint64_t)h->nblocks * 16 +
+                              (uint64_t)n * (2 + 8 * op->nw) + (uint64_t)(h->nslices + 1) * 20 +
+                              (uint64_t)n * diag_elem);
+  *out = h;
   return QS_OK;
 }
 

@@ -47,6 +47,27 @@ class QsArena {
     return p;
   }
 
+  // A block of round(bytes) bytes at the top of the highest free block that
+  // holds it, or 0. For long-lived data: kept at the top of the address
+  // range, it leaves the free space below in few large pieces for the
+  // short-lived blocks best-fit places there.
+  uintptr_t take_top(size_t bytes) {
+    const size_t want = round(bytes);
+    for (auto it = by_addr_.rbegin(); it != by_addr_.rend(); ++it) {
+      if (it->second < want) continue;
+      const uintptr_t start = it->first;
+      const size_t have = it->second;
+      erase_free(std::next(it).base());
+      if (have > want) insert_free(start, have - want);
+      const uintptr_t p = start + (have - want);
+      live_[p] = want;
+      live_bytes_ += want;
+      free_bytes_ -= want;
+      return p;
+    }
+    return 0;
+  }
+
   // Return a block from take(); false when p is not a live block of this arena.
   bool give(uintptr_t p) {
     auto it = live_.find(p);

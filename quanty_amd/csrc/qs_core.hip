@@ -95,7 +95,8 @@ qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes) {
   for (int attempt = 0; attempt < 8; ++attempt) {
     {
       std::lock_guard<std::mutex> lock(g_alloc_mu);
-      const uintptr_t a = g_arena[ctx->device].take(want);
+      QsArena &ar = g_arena[ctx->device];
+      const uintptr_t a = ctx->alloc_top ? ar.take_top(want) : ar.take(want);
       if (a) {
         *p = (void *)a;
         g_owner[*p] = ctx->device;
@@ -133,7 +134,7 @@ qs_status qs_malloc(qs_ctx *ctx, void **p, size_t bytes) {
     std::lock_guard<std::mutex> lock(g_alloc_mu);
     QsArena &ar = g_arena[ctx->device];
     ar.add_chunk((uintptr_t)base, chunk);
-    const uintptr_t a = ar.take(want);
+    const uintptr_t a = ctx->alloc_top ? ar.take_top(want) : ar.take(want);
     if (a) {
       *p = (void *)a;
       g_owner[*p] = ctx->device;
@@ -337,6 +338,10 @@ extern "C" qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value) {
       if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_ELL_ORDER must be 0 or 1");
       ctx->ell_order = (int)value;
       return QS_OK;
+    case QS_PARAM_ALLOC_TOP:
+      if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_ALLOC_TOP must be 0 or 1");
+      ctx->alloc_top = (int)value;
+      return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
   }
@@ -359,6 +364,9 @@ extern "C" qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value) {
       return QS_OK;
     case QS_PARAM_ELL_ORDER:
       *value = ctx->ell_order;
+      return QS_OK;
+    case QS_PARAM_ALLOC_TOP:
+      *value = ctx->alloc_top;
       return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);

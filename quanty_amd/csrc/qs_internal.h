@@ -9,6 +9,7 @@
 #include <string>
 #include <utility>
 #include <complex>
+#include <memory>
 #include <vector>
 
 #include "../../include/qs.h"
@@ -38,6 +39,7 @@ struct qs_ctx {
   int batch_graph = 1;             // QS_PARAM_BATCH_GRAPH
   int term_lds = QS_TERM_LDS_DEFAULT;  // QS_PARAM_TERM_LDS
   int ell_order = 1;               // QS_PARAM_ELL_ORDER
+  int alloc_top = 0;               // QS_PARAM_ALLOC_TOP: device blocks from the top of free memory
   void *blas = nullptr;            // rocblas handle for the dense eigensolver, made on first use
   void *pinned = nullptr;          // the ctx's own pinned staging buffer (qs_h2d / qs_d2h)
   size_t pinned_bytes = 0;
@@ -153,6 +155,14 @@ struct SellD {
   const void *diag = nullptr;          // pre-summed diagonal (rows == cols) or null: double2, or double if real_diag
 };
 
+// Device arrays of an assembled index stream (everything but the term tables
+// and the diagonal), shared by the bindings made from it with
+// qs_hop_create_like and freed when the last of them goes.
+struct HopIndex {
+  void *buf[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  ~HopIndex();
+};
+
 struct qs_hop {
   const qs_op *op = nullptr;
   const qs_basis *rows = nullptr;
@@ -177,6 +187,7 @@ struct qs_hop {
   uint32_t *d_col = nullptr;
   uint16_t *d_tid = nullptr;
   double *d_diag = nullptr;
+  std::shared_ptr<HopIndex> index;  // owner of the d_* arrays above except d_diag, once assembled
 };
 
 // ---- error helpers --------------------------------------------------------

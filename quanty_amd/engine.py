@@ -188,6 +188,20 @@ class Hop:
                                               C.byref(h)), "qs_hop_create")
         self.h = h
 
+    @classmethod
+    def like(cls, op: Op, template: "Hop") -> "Hop":
+        """op bound to template's bases, sharing template's assembled stream
+        (qs_hop_create_like): op must compile to the same terms as
+        template's operator, values aside."""
+        self = cls.__new__(cls)
+        self.op, self.rows, self.cols = op, template.rows, template.cols
+        self.ctx = op.ctx
+        self.lib = self.ctx.lib
+        h = C.c_void_p()
+        self.ctx.check(self.lib.qs_hop_create_like(self.ctx.h, op.h, template.h, C.byref(h)), "qs_hop_create_like")
+        self.h = h
+        return self
+
     def stats(self) -> dict:
         a, b, c, d, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int(), C.c_int64()
         self.ctx.check(self.lib.qs_hop_stats(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d), C.byref(e)),

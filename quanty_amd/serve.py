@@ -29,7 +29,7 @@ import numpy as np
 import torch
 
 from . import problems as P
-from ._lib import QS_PARAM_SHIFT_MIN
+from ._lib import QS_PARAM_ALLOC_TOP, QS_PARAM_SHIFT_MIN, QSError
 from .engine import Basis, Batch, Hop, Op, cf_spectrum, tridiag_lowest
 from .tune import layout_for
 
@@ -148,21 +148,101 @@ def request_stream(n: int, rate: float, seed: int = 7, dim_lo=1e3, dim_hi=3e6, d
     return out
 
 
+class PatternCache:
+    """Assembled connection streams by operator structure, shared by requests
+    whose Hamiltonians differ only in their values (the same impurity shape
+    with another seed): such a request binds its operator to a cached stream
+    with qs_hop_create_like instead of enumerating its basis and assembling.
+    Keyed by (modes, restrictions, the ladder codes of H's terms in order), so
+    a hit is exact; the library checks the compiled terms again and refuses a
+    mismatch (a coefficient that vanishes drops a term), in which case the
+    request is assembled on its own. Least recently used entries go once the
+    streams exceed `cap_bytes`; the server counts the cached bytes against its
+    HBM budget. A stream stays on the device while any request uses it.
+    Process-wide: preparation workers and successive servers share it."""
+
+    def __init__(self, cap_bytes: float = 24e9):
+        self.cap = cap_bytes
+        self.lock = threading.Lock()
+        self.entries: "dict[tuple, tuple]" = {}  # key -> (basis, template hop, bytes), in use order
+        self.bytes = 0
+        self.hits = 0
+        self.misses = 0
+
+    @staticmethod
+    def key(pr) -> tuple:
+        return (pr["NF"], tuple((str(m), int(a), int(b)) for m, a, b in pr["restrictions"]),
+                tuple(tuple(c) for c, _ in pr["H"]))
+
+    def get(self, key):
+        with self.lock:
+            e = self.entries.pop(key, None)
+            if e is None:
+                self.misses += 1
+                return None
+            self.entries[key] = e  # most recently used last
+            self.hits += 1
+            return e[0], e[1]
+
+    def put(self, key, basis, hop, nbytes: int):
+        if nbytes > self.cap:
+            return
+        with self.lock:
+            if key in self.entries:
+                return
+            self.entries[key] = (basis, hop, nbytes)
+            self.bytes += nbytes
+            while self.bytes > self.cap and self.entries:
+                k = next(iter(self.entries))
+                self.bytes -= self.entries.pop(k)[2]
+
+    def clear(self):
+        with self.lock:
+            self.entries.clear()
+            self.bytes = 0
+            self.hits = self.misses = 0
+
+
+PATTERNS = PatternCache()
+
+
 class _Job:
     """Device-resident state of an admitted request."""
 
     def __init__(self, req: Request, ctx, gs_depth: int, start_temperature: float | None = 0.3,
-                 start_floor: float = 1e-3):
+                 start_floor: float = 1e-3, patterns: PatternCache | None = PATTERNS):
         self.req = req
         pr = req.problem if req.problem is not None else make_problem(req.seed, req.shape)
         self.nf = pr["NF"]
-        self.basis = Basis.enumerate(self.nf, pr["restrictions"], ctx)
         self.H = Op(self.nf, pr["H"], ctx)
-        # the layout the autotuner fitted for this basis size (tune.fit_family)
-        sm = layout_for(self.basis.n)
-        if sm is not None:
-            ctx.set_param(QS_PARAM_SHIFT_MIN, sm)
-        self.hop = Hop(self.H, self.basis, mode="assembled")
+        key = PatternCache.key(pr) if patterns is not None else None
+        hit = patterns.get(key) if patterns is not None else None
+        self.hop = None
+        self.shared = False
+        if hit is not None:
+            try:
+                self.hop = Hop.like(self.H, hit[1])
+                self.basis = hit[0]
+                self.shared = True
+            except QSError:
+                self.hop = None
+        if self.hop is None:
+            # a basis and stream the cache will keep go to the top of the
+            # library's device memory, away from the requests' short-lived blocks
+            if patterns is not None:
+                ctx.set_param(QS_PARAM_ALLOC_TOP, 1)
+            try:
+                self.basis = Basis.enumerate(self.nf, pr["restrictions"], ctx)
+                # the layout the autotuner fitted for this basis size (tune.fit_family)
+                sm = layout_for(self.basis.n)
+                if sm is not None:
+                    ctx.set_param(QS_PARAM_SHIFT_MIN, sm)
+                self.hop = Hop(self.H, self.basis, mode="assembled")
+            finally:
+                if patterns is not None:
+                    ctx.set_param(QS_PARAM_ALLOC_TOP, 0)
+            if patterns is not None:
+                patterns.put(key, self.basis, self.hop, self.hop.stats()["stream_bytes"])
         self.T = Op(self.nf, pr["T"], ctx)
         self.thop = Hop(self.T, self.basis, mode="matfree")
         g = torch.Generator(device="cuda").manual_seed(req.seed)
@@ -182,7 +262,10 @@ class _Job:
         self.restarts = 0  # ground-state restarts from the Ritz vector when the kept basis is full
         self.steps_done = 0  # ground-state steps of the runs before the last restart
         self.slot = -1
-        self.bytes = self.hop.stats()["stream_bytes"] + 6 * 16 * self.basis.n
+        # device bytes the request holds: its vectors, and the assembled stream
+        # unless it shares a cached one (then only its own diagonal)
+        own = 8 * self.basis.n if self.shared else self.hop.stats()["stream_bytes"]
+        self.bytes = own + 6 * 16 * self.basis.n
         self.kept_bytes = 0  # Lanczos vectors kept during the ground-state phase
 
     def release(self):
@@ -212,9 +295,10 @@ class _Prep:
     stream, start vector) on worker threads, each with its own library context,
     so that the preparation overlaps the batch's GPU steps."""
 
-    def __init__(self, device: int, gs_depth: int, workers: int = 1, start=(0.3, 1e-3)):
+    def __init__(self, device: int, gs_depth: int, workers: int = 1, start=(0.3, 1e-3), patterns=None):
         self.gs_depth = gs_depth
         self.start = start
+        self.patterns = patterns
         self.todo: queue.Queue = queue.Queue()
         self.ready: queue.Queue = queue.Queue()
         self.ctxs = _worker_contexts(device, max(1, workers))
@@ -228,7 +312,7 @@ class _Prep:
             if r is None:
                 return
             try:
-                job = _Job(r, ctx, self.gs_depth, *self.start)
+                job = _Job(r, ctx, self.gs_depth, *self.start, patterns=self.patterns)
                 torch.cuda.current_stream().synchronize()  # the start vector; library calls are synchronous
                 self.ready.put(job)
             except BaseException as e:  # handed to the serving loop, which raises it
@@ -254,8 +338,11 @@ class SpectrumServer:
                  gs_max: int = 500, gs_step: int = 16, gs_tol: float = 1e-12, gs_restarts: int = 8,
                  chunk: int = 8, ne: int = 2000,
                  emin: float = -10.0, emax: float = 20.0, keep_basis: bool = True, prepare_ahead: bool = True,
-                 prep_workers: int = 2, start_temperature: float | None = 0.3, start_floor: float = 1e-3):
+                 prep_workers: int = 2, start_temperature: float | None = 0.3, start_floor: float = 1e-3,
+                 share_patterns: bool = True):
         self.ctx = ctx
+        # requests of one operator structure share one assembled stream (PatternCache)
+        self.patterns = PATTERNS if share_patterns else None
         self.max_slots = max_slots
         self.budget = hbm_budget
         self.gs_depth = gs_depth
@@ -392,7 +479,20 @@ class SpectrumServer:
     def _start(self, job: _Job):
         """Put a prepared job into the batch."""
         job.gs_max = min(self.gs_max, job.basis.n)
-        job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max, keep_basis=self.keep_basis)
+        try:
+            job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max,
+                                      keep_basis=self.keep_basis)
+        except QSError as e:
+            if "QS_ENOMEM_DEVICE" not in str(e) or self.patterns is None or not self.patterns.entries:
+                raise
+            # the cached streams may split the free device memory into pieces
+            # too small for this slot's Krylov basis: drop the cache (streams
+            # still in use stay with their requests) and try once more
+            import gc
+            self.patterns.clear()
+            gc.collect()
+            job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max,
+                                      keep_basis=self.keep_basis)
         if self.keep_basis:
             job.kept_bytes = 16 * (job.gs_max + 1) * job.basis.n
         self.jobs[job.req.rid] = job
@@ -412,7 +512,8 @@ class SpectrumServer:
         waiting: list[Request] = []
         done: list[Request] = []
         prep = _Prep(self.ctx.device, self.gs_depth, self.prep_workers,
-                     (self.start_temperature, self.start_floor)) if (self.prepare_ahead and not plain) else None
+                     (self.start_temperature, self.start_floor), self.patterns) \
+            if (self.prepare_ahead and not plain) else None
         inflight: dict[int, int] = {}  # rid -> reserved bytes of requests being prepared
         kept = self.gs_max + 1 if self.keep_basis else 0
         slots = 1 if plain else self.max_slots
@@ -443,12 +544,14 @@ class SpectrumServer:
                                      f"{self.budget / 1e9:.1f} GB"
                         self.rejected.append(r)
                         continue
-                    if (self.jobs or inflight) and self.used + est > self.budget:
+                    cached = self.patterns.bytes if self.patterns is not None else 0
+                    if (self.jobs or inflight) and self.used + cached + est > self.budget:
                         break
                     waiting.pop(0)
                     r.t_admit = time.perf_counter() - t0
                     if prep is None:
-                        self._start(_Job(r, self.ctx, self.gs_depth, self.start_temperature, self.start_floor))
+                        self._start(_Job(r, self.ctx, self.gs_depth, self.start_temperature, self.start_floor,
+                                         patterns=self.patterns))
                     else:
                         inflight[r.rid] = est
                         self.used += est

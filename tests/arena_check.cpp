@@ -36,7 +36,8 @@ int main() {
     for (int step = 0; step < 4000; ++step) {
       if (live.empty() || rng() % 3) {
         const size_t want = 1 + rng() % (1 << 18);
-        const uintptr_t p = a.take(want);
+        const bool top = rng() % 4 == 0;  // long-lived blocks from the top
+        const uintptr_t p = top ? a.take_top(want) : a.take(want);
         if (p == 0) continue;
         const size_t sz = QsArena::round(want);
         CHECK(p % QsArena::kAlign == 0);
@@ -74,6 +75,21 @@ int main() {
     for (auto &c : chunks) biggest = std::max(biggest, c.second);
     CHECK(a.take(biggest) != 0);
     CHECK(a.take(total) == 0);
+  }
+  {
+    // take_top: the block ends where the highest free block that holds it ends
+    QsArena a;
+    a.add_chunk(1 << 20, 1 << 20);
+    a.add_chunk(4 << 20, 1 << 20);
+    const uintptr_t t = a.take_top(1000);
+    CHECK(t == (5 << 20) - QsArena::round(1000));
+    const uintptr_t b = a.take(1000);  // best-fit: the smaller (first chunk's) block... any block
+    CHECK(b != 0 && b != t);
+    const uintptr_t big = a.take_top((1 << 20) - 512);  // only the first chunk still holds this much
+    CHECK(big == (1 << 20) + 512 || big == 0);
+    CHECK(a.give(t) && a.give(b));
+    if (big) CHECK(a.give(big));
+    CHECK(a.live_bytes() == 0);
   }
   std::printf("arena ok\n");
   return 0;
