@@ -69,6 +69,7 @@ struct qs_batch {
   WorkItem *d_w1 = nullptr, *d_w2 = nullptr;
   int64_t n1 = 0, n2 = 0, cap1 = 0, cap2 = 0;
   int *d_r1 = nullptr, *d_r2 = nullptr;  // per slot: [begin, end) of its entries, 2 ints each
+  int *d_off = nullptr;                    // per slot: first K1 item, first K2 item (worklist build)
   double *d_p1 = nullptr, *d_p2 = nullptr;  // partials per slice (K1) and per work item (K2)
   size_t lds1 = 0;                          // K1 dynamic LDS: the largest term table that is staged
   bool fast1 = true;                        // every slot: one-word determinants, packed entries
@@ -313,6 +314,69 @@ k_b_ritz(const SlotDev *__restrict__ slots, int slot, int m, const double *__res
   }
 }
 
+// Worklists from the slot table, on the device, so a change in the set of
+// slots needs no host upload and no wait for the steps in flight (the stream
+// orders the build after them). A slot's K1 items are its groups of
+// QS_K1_GROUP slices, its K2 items its row chunks, slots in index order.
+// Scan: one block; thread t of each tile of blockDim.x slots counts slot
+// base + t, and an inclusive scan in LDS gives the offsets.
+#define QS_WL_BLOCK 1024
+__global__ void __launch_bounds__(QS_WL_BLOCK)
+k_b_worklist_scan(const SlotDev *__restrict__ slots, int cap, int *__restrict__ off, int *__restrict__ r1,
+                  int *__restrict__ r2, int64_t *__restrict__ counts) {
+  __shared__ int s1[QS_WL_BLOCK], s2[QS_WL_BLOCK];
+  int base1 = 0, base2 = 0;
+  for (int t0 = 0; t0 < cap; t0 += blockDim.x) {
+    const int s = t0 + (int)threadIdx.x;
+    int c1 = 0, c2 = 0;
+    if (s < cap && slots[s].active) {
+      c1 = (int)((slots[s].nslices + QS_K1_GROUP - 1) / QS_K1_GROUP);
+      c2 = (int)((slots[s].n + QS_K2_CHUNK - 1) / QS_K2_CHUNK);
+    }
+    s1[threadIdx.x] = c1;
+    s2[threadIdx.x] = c2;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+      const int a1 = threadIdx.x >= (unsigned)o ? s1[threadIdx.x - o] : 0;
+      const int a2 = threadIdx.x >= (unsigned)o ? s2[threadIdx.x - o] : 0;
+      __syncthreads();
+      s1[threadIdx.x] += a1;
+      s2[threadIdx.x] += a2;
+      __syncthreads();
+    }
+    if (s < cap) {
+      const int e1 = base1 + s1[threadIdx.x], e2 = base2 + s2[threadIdx.x];
+      off[2 * s] = e1 - c1;
+      off[2 * s + 1] = e2 - c2;
+      // K1 partials are kept per slice position of an item
+      r1[2 * s] = (e1 - c1) * QS_K1_GROUP;
+      r1[2 * s + 1] = e1 * QS_K1_GROUP;
+      r2[2 * s] = e2 - c2;
+      r2[2 * s + 1] = e2;
+    }
+    base1 += s1[blockDim.x - 1];
+    base2 += s2[blockDim.x - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    counts[0] = base1;
+    counts[1] = base2;
+  }
+}
+
+// One block per slot writes its items at the offsets of k_b_worklist_scan.
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b_worklist_fill(const SlotDev *__restrict__ slots, const int *__restrict__ off, WorkItem *__restrict__ w1,
+                  WorkItem *__restrict__ w2) {
+  const int s = blockIdx.x;
+  if (!slots[s].active) return;
+  const int c1 = (int)((slots[s].nslices + QS_K1_GROUP - 1) / QS_K1_GROUP);
+  const int c2 = (int)((slots[s].n + QS_K2_CHUNK - 1) / QS_K2_CHUNK);
+  const int o1 = off[2 * s], o2 = off[2 * s + 1];
+  for (int k = threadIdx.x; k < c1; k += blockDim.x) w1[o1 + k] = WorkItem{s, k};
+  for (int k = threadIdx.x; k < c2; k += blockDim.x) w2[o2 + k] = WorkItem{s, k};
+}
+
 // ---------------------------------------------------------------------------
 // host
 // ---------------------------------------------------------------------------
@@ -332,31 +396,23 @@ static void pool_put(qs_batch *B, void *p, size_t bytes) {
 
 static qs_status rebuild(qs_batch *B) {
   qs_ctx *ctx = B->ctx;
-  std::vector<WorkItem> w1, w2;
-  std::vector<int> r1(2 * B->cap, 0), r2(2 * B->cap, 0);
-  for (int s = 0; s < B->cap; ++s) {
-    r1[2 * s] = (int)w1.size();
-    r2[2 * s] = (int)w2.size();
-    if (B->used[s]) {
-      const SlotDev &S = B->h_slots[s];
-      for (int64_t k = 0; k * QS_K1_GROUP < S.nslices; ++k) w1.push_back({s, (int)k});
-      for (int64_t k = 0; k * QS_K2_CHUNK < S.n; ++k) w2.push_back({s, (int)k});
-    }
-    r1[2 * s + 1] = (int)w1.size();
-    r2[2 * s + 1] = (int)w2.size();
-    // K1 partials are kept per slice position of an item
-    r1[2 * s] *= QS_K1_GROUP;
-    r1[2 * s + 1] *= QS_K1_GROUP;
-  }
+  // item counts from the host's copy of the slot table (the device builds
+  // the same lists from its own copy)
+  int64_t n1 = 0, n2 = 0;
   size_t lds = 0;
   bool fast = true;
   for (int s = 0; s < B->cap; ++s)
     if (B->used[s]) {
-      const SellD &A = B->h_slots[s].A;
+      const SlotDev &S = B->h_slots[s];
+      n1 += (S.nslices + QS_K1_GROUP - 1) / QS_K1_GROUP;
+      n2 += (S.n + QS_K2_CHUNK - 1) / QS_K2_CHUNK;
+      const SellD &A = S.A;
       const size_t tb = term_table_bytes(A.nterms, A.nw);
       if (tb <= (size_t)ctx->term_lds) lds = std::max(lds, tb);
       fast = fast && A.nw == 1 && A.packed;
     }
+  if (n1 >= ((int64_t)1 << 31) / QS_K1_GROUP || n2 >= ((int64_t)1 << 31))
+    return qs_fail(ctx, QS_EINVAL, "the batch's work lists would exceed 2^31 items");
   B->lds1 = lds;
   B->fast1 = fast;
   auto grow = [&](WorkItem **p, int64_t *cap, int64_t need) -> qs_status {
@@ -371,8 +427,8 @@ static qs_status rebuild(qs_batch *B) {
   };
   const int64_t c1 = B->cap1, c2 = B->cap2;
   const void *old_ptrs[4] = {B->d_w1, B->d_w2, B->d_p1, B->d_p2};
-  QS_TRY(grow(&B->d_w1, &B->cap1, std::max<size_t>(1, w1.size())));
-  QS_TRY(grow(&B->d_w2, &B->cap2, std::max<size_t>(1, w2.size())));
+  QS_TRY(grow(&B->d_w1, &B->cap1, std::max<int64_t>(1, n1)));
+  QS_TRY(grow(&B->d_w2, &B->cap2, std::max<int64_t>(1, n2)));
   if (B->cap1 != c1 || !B->d_p1) {
     qs_free(B->d_p1);
     QS_TRY(qs_malloc(ctx, (void **)&B->d_p1, (size_t)B->cap1 * QS_K1_GROUP * 8));
@@ -381,19 +437,15 @@ static qs_status rebuild(qs_batch *B) {
     qs_free(B->d_p2);
     QS_TRY(qs_malloc(ctx, (void **)&B->d_p2, (size_t)B->cap2 * 8));
   }
-  if (!w1.empty())
-    QS_TRY(qs_h2d(ctx, B->d_w1, w1.data(), w1.size() * sizeof(WorkItem)));
-  if (!w2.empty())
-    QS_TRY(qs_h2d(ctx, B->d_w2, w2.data(), w2.size() * sizeof(WorkItem)));
-  QS_TRY(qs_h2d(ctx, B->d_r1, r1.data(), r1.size() * 4));
-  QS_TRY(qs_h2d(ctx, B->d_r2, r2.data(), r2.size() * 4));
-  const int64_t counts[2] = {(int64_t)w1.size(), (int64_t)w2.size()};
-  QS_TRY(qs_h2d(ctx, B->d_n, counts, sizeof counts));
-  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  hipLaunchKernelGGL(k_b_worklist_scan, dim3(1), dim3(QS_WL_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_off,
+                     B->d_r1, B->d_r2, B->d_n);
+  hipLaunchKernelGGL(k_b_worklist_fill, dim3(B->cap), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_off,
+                     B->d_w1, B->d_w2);
+  QS_HIP(ctx, hipGetLastError());
   if (old_ptrs[0] != B->d_w1 || old_ptrs[1] != B->d_w2 || old_ptrs[2] != B->d_p1 || old_ptrs[3] != B->d_p2)
     drop_graphs(B);
-  B->n1 = counts[0];
-  B->n2 = counts[1];
+  B->n1 = n1;
+  B->n2 = n2;
   B->dirty = false;
   return QS_OK;
 }
@@ -416,6 +468,7 @@ extern "C" qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out)
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r1, (size_t)max_slots * 8);
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r2, (size_t)max_slots * 8);
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_n, 2 * sizeof(int64_t));
+  if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_off, (size_t)max_slots * 8);
   if (st == QS_OK) {
     hipError_t e = hipMemsetAsync(B->d_slots, 0, (size_t)max_slots * sizeof(SlotDev), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -680,9 +733,10 @@ extern "C" qs_status qs_batch_ritz(qs_batch *B, int slot, int m, const double *z
 extern "C" qs_status qs_batch_remove(qs_batch *B, int slot) {
   if (!B || slot < 0 || slot >= B->cap || !B->used[slot]) return QS_EINVAL;
   qs_ctx *ctx = B->ctx;
-  QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  SlotDev z{};
-  QS_TRY(qs_h2d(ctx, B->d_slots + slot, &z, sizeof(SlotDev)));
+  // an empty descriptor (active = 0) after the steps in flight; the slot's
+  // memory goes back once the stream has passed it
+  hipLaunchKernelGGL(k_b_slot_set, dim3(1), dim3(1), 0, ctx->stream, B->d_slots, slot, SlotDev{}, (double *)nullptr, 0);
+  QS_HIP(ctx, hipGetLastError());
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   pool_put(B, B->mem[slot], B->mem_bytes[slot]);
   B->mem[slot] = nullptr;
@@ -817,6 +871,7 @@ extern "C" qs_status qs_batch_destroy(qs_batch *B) {
   qs_free(B->d_n);
   qs_free(B->d_r1);
   qs_free(B->d_r2);
+  qs_free(B->d_off);
   qs_free(B->d_p1);
   qs_free(B->d_p2);
   delete B;
