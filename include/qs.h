@@ -198,6 +198,32 @@ qs_status qs_hop_create(qs_ctx *ctx, const qs_op *op, const qs_basis *rows, cons
  * that apply one operator family with many parameter sets (a server's
  * requests of one impurity shape). */
 qs_status qs_hop_create_like(qs_ctx *ctx, const qs_op *op, const qs_hop *like, qs_hop **out);
+/* Bind op to a restricted sector `basis` (rows == cols) in the product
+ * layout: the modes are split into group A (amask: NF bits, mode i at bit
+ * 7-(i&7) of byte i>>3, as restriction masks) and group B, every restriction
+ * of the basis must lie inside A, inside B or cover all modes, and rows are
+ * grouped by their B configuration. Terms that move B are stored once per
+ * group and applied to all of a group's rows with contiguous loads; terms on
+ * A alone form small per-A-set matrices. Same results as an ASSEMBLED binding
+ * (OperatorPsiConserveBasis, Core/OperatorWaveFunction.cpp:245) up to the
+ * summation order; vectors passed to qs_hop_apply, qs_lanczos and the block
+ * and ground-state entry points stay in canonical order. QS_EUNSUPPORTED with
+ * a message when the sector does not factor over the split. No counterpart in
+ * the reference (its hash-set H*psi has no layout); for impurity models whose
+ * bath moves dominate. qs_batch_add does not take such bindings. */
+qs_status qs_hop_create_product(qs_ctx *ctx, const qs_op *op, const qs_basis *basis, const uint8_t *amask,
+                                qs_hop **out);
+/* Shape of a product-layout binding: info[0..7] = groups, slices, A-sets,
+ * entries, mapped entries, 0 (reserved), A-matrix slots, maps; load_cost
+ * (may be NULL) = the estimated 64-lane 16-byte vector loads of one apply,
+ * for choosing the mode split. QS_EINVAL for other bindings. */
+qs_status qs_hop_product_info(const qs_hop *h, int64_t *info, double *load_cost);
+/* The canonical index of every row of h in the order its kernels keep
+ * vectors (n int64, host): the identity except for product-layout bindings. */
+qs_status qs_hop_row_order(qs_ctx *ctx, const qs_hop *h, int64_t *canon);
+/* y = O x with x and y in h's own row order (qs_hop_row_order): the apply
+ * kernel alone, without order conversions. */
+qs_status qs_hop_apply_rows(qs_ctx *ctx, const qs_hop *h, const double *x, double *y);
 /* Number of (row, term) pairs with nonzero phase whose source lies in cols,
  * the stored off-diagonal stream length (incl. slice padding; diagonal terms
  * are pre-summed), its bytes, and the off-diagonal entries without padding.

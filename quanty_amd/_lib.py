@@ -59,6 +59,10 @@ _SIGS = [
     ("qs_op_destroy", C.c_int, [_P]),
     ("qs_hop_create", C.c_int, [_P, _P, _P, _P, C.c_int, C.c_size_t, C.POINTER(_P)]),
     ("qs_hop_create_like", C.c_int, [_P, _P, _P, C.POINTER(_P)]),
+    ("qs_hop_create_product", C.c_int, [_P, _P, _P, _P, C.POINTER(_P)]),
+    ("qs_hop_row_order", C.c_int, [_P, _P, _P]),
+    ("qs_hop_product_info", C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    ("qs_hop_apply_rows", C.c_int, [_P, _P, _P, _P]),
     ("qs_hop_stats", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),
                                C.POINTER(_I64)]),
     ("qs_hop_layout", C.c_int, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(C.c_int),

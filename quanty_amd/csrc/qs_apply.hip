@@ -429,7 +429,7 @@ k_sell_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *
           bhits += (unsigned long long)ng;
         } else if ((grp >> lane) & 1u) {
           if (FILL) {
-            const uint64_t e = ebase + (uint64_t)nell * 64 + lane;
+            const uint64_t e = ell_slot(ebase, nell, lane);
             if (packed) {
               colv[e] = (uint32_t)col | (tj << colbits);
             } else {
@@ -448,7 +448,7 @@ k_sell_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *
     // padding slots of the lane's ELL column: column 0, term 0
     const uint32_t slen = slice_len[slice];
     for (uint32_t j = nell; j < slen; ++j) {
-      const uint64_t e = ebase + (uint64_t)j * 64 + lane;
+      const uint64_t e = ell_slot(ebase, j, lane);
       colv[e] = 0;
       if (!packed) tidv[e] = 0;
     }
@@ -468,7 +468,7 @@ k_sell_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *
     e += (unsigned long long)__shfl_xor((long long)e, o);
   }
   if (lane == 0) {
-    slice_len[slice] = m;
+    slice_len[slice] = (m + 3u) & ~3u;  // whole quads of positions
     blk_cnt[slice] = nblk;
     atomicAdd(ctr, e);
     atomicAdd(ctr + 2, bhits);
@@ -491,13 +491,14 @@ k_sell_sort_rows(int64_t nrows, const uint16_t *__restrict__ rowcnt, const uint6
   if (row >= nrows) return;
   const int cnt = rowcnt[row];
   if (cnt < 2 || cnt > QS_SORT_MAX) return;
-  const uint64_t base = slice_ptr[row >> 6] + (uint64_t)(row & 63);
+  const uint64_t base = slice_ptr[row >> 6];
+  const int lane = (int)(row & 63);
   const uint32_t cmask = PACKED ? ((1u << colbits) - 1u) : 0xFFFFFFFFu;
   uint32_t ev[QS_SORT_MAX];
   uint16_t tv[QS_SORT_MAX];
   for (int j = 0; j < cnt; ++j) {
-    ev[j] = colv[base + (uint64_t)j * 64];
-    tv[j] = PACKED ? 0 : tidv[base + (uint64_t)j * 64];
+    ev[j] = colv[ell_slot(base, (uint32_t)j, lane)];
+    tv[j] = PACKED ? 0 : tidv[ell_slot(base, (uint32_t)j, lane)];
   }
   for (int j = 1; j < cnt; ++j) {  // insertion sort by column
     const uint32_t e = ev[j];
@@ -512,8 +513,8 @@ k_sell_sort_rows(int64_t nrows, const uint16_t *__restrict__ rowcnt, const uint6
     tv[k + 1] = t;
   }
   for (int j = 0; j < cnt; ++j) {
-    colv[base + (uint64_t)j * 64] = ev[j];
-    if (!PACKED) tidv[base + (uint64_t)j * 64] = tv[j];
+    colv[ell_slot(base, (uint32_t)j, lane)] = ev[j];
+    if (!PACKED) tidv[ell_slot(base, (uint32_t)j, lane)] = tv[j];
   }
 }
 
@@ -680,6 +681,7 @@ static void free_stream(qs_hop *h) {
       qs_free(b);
   }
   qs_free(h->d_diag);
+  h->d_inv = nullptr;
   h->d_blk_ptr = nullptr;
   h->d_blk = nullptr;
   h->d_slice_ptr = nullptr;
@@ -809,8 +811,7 @@ static qs_status assemble(qs_ctx *ctx, qs_hop *h, size_t budget) {
   A.par = op->d_par;
   A.diag = h->d_diag;
   h->index = std::make_shared<HopIndex>();
-  void *own[7] = {h->d_blk_ptr, h->d_blk, h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid};
-  std::copy(own, own + 7, h->index->buf);
+  h->index->buf = {h->d_blk_ptr, h->d_blk, h->d_slice_ptr, h->d_slice_len, h->d_rowcnt, h->d_col, h->d_tid};
   h->assembled = 1;
   return QS_OK;
 }
@@ -855,6 +856,23 @@ k_diag_build(const uint64_t *__restrict__ rows, int64_t nrows, const TermD<NW> *
   }
 }
 
+// Pre-summed diagonal of n rows (determinants at `rows`, op->nw words each)
+// over the terms listed in d_which (device, nwhich term indices, in term
+// order), into diag (doubles when real_diag, else complex).
+qs_status qs_diag_rows(qs_ctx *ctx, const qs_op *op, const uint64_t *rows, int64_t n, const int *d_which, int nwhich,
+                       void *diag, int real_diag) {
+  if (n == 0) return QS_OK;
+  const unsigned blocks = (unsigned)((n + QS_BLOCK - 1) / QS_BLOCK);
+  if (op->nw == 1)
+    hipLaunchKernelGGL(k_diag_build<1>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows, n,
+                       (const TermD<1> *)op->d_terms, d_which, nwhich, diag, real_diag);
+  else
+    hipLaunchKernelGGL(k_diag_build<2>, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, rows, n,
+                       (const TermD<2> *)op->d_terms, d_which, nwhich, diag, real_diag);
+  QS_HIP(ctx, hipGetLastError());
+  return QS_OK;
+}
+
 extern "C" qs_status qs_hop_create_like(qs_ctx *ctx, const qs_op *op, const qs_hop *like, qs_hop **out) {
   if (!ctx || !op || !like || !out) return QS_EINVAL;
   *out = nullptr;
@@ -873,6 +891,7 @@ extern "C" qs_status qs_hop_create_like(qs_ctx *ctx, const qs_op *op, const qs_h
         return qs_fail(ctx, QS_EINVAL, "qs_hop_create_like: term %d acts differently from the shared binding's", t);
   }
   QS_HIP(ctx, hipSetDevice(ctx->device));
+  if (like->view.kind == 1) return qs_prod_create_like(ctx, op, like, out);
   qs_hop *h = new qs_hop();
   h->op = op;
   h->rows = like->rows;
@@ -1013,9 +1032,12 @@ extern "C" qs_status qs_hop_diagonal(qs_ctx *ctx, const qs_hop *h, double *d) {
   const int64_t n = h->rows->n;
   if (n > 0) {
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + QS_BLOCK - 1) / QS_BLOCK, 4096));
+    double2 *out = (double2 *)d;
+    if (h->view.kind == 1) QS_TRY(qs_scratch(ctx, (size_t)n * 16, (void **)&out));  // row order first
     hipLaunchKernelGGL(k_diag_out, dim3(blocks), dim3(QS_BLOCK), 0, ctx->stream, n, (const void *)h->d_diag,
-                       h->view.real_diag, (double2 *)d);
+                       h->view.real_diag, out);
     QS_HIP(ctx, hipGetLastError());
+    if (h->view.kind == 1) QS_TRY(qs_prod_from_rows(ctx, h, 1, out, n, (double2 *)d, n));
   }
   if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QS_OK;
@@ -1072,6 +1094,17 @@ qs_status qs_hop_apply_vectors(qs_ctx *ctx, const qs_hop *h, int k, const double
                                double *y, int64_t ldy, int64_t ldvy) {
   const int64_t n = h->rows->n;
   if (n == 0 || k == 0) return QS_OK;
+  if (h->view.kind == 1) {
+    // product layout: through the binding's row order
+    if (ldx != 1 || ldy != 1)
+      return qs_fail(ctx, QS_EUNSUPPORTED, "product layout: vectors must be stored one after another");
+    double2 *xp;
+    QS_TRY(qs_scratch(ctx, (size_t)2 * k * n * 16, (void **)&xp));
+    double2 *yp = xp + (size_t)k * n;
+    QS_TRY(qs_prod_to_rows(ctx, h, k, (const double2 *)x, ldvx, xp, n));
+    QS_TRY(qs_prod_apply_rows(ctx, h, k, xp, n, yp, n));
+    return qs_prod_from_rows(ctx, h, k, yp, n, (double2 *)y, ldvy);
+  }
   for (int v0 = 0; v0 < k;) {
     const int kc = k - v0 >= 4 ? 4 : (k - v0 >= 2 ? 2 : 1);
     const double2 *xs = (const double2 *)x + v0 * ldvx;

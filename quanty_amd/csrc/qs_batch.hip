@@ -494,6 +494,8 @@ extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double 
   qs_ctx *ctx = B->ctx;
   if (!h->assembled || h->rows != h->cols)
     return qs_fail(ctx, QS_EINVAL, "qs_batch_add needs an assembled operator with rows == cols");
+  if (h->view.kind != 0)
+    return qs_fail(ctx, QS_EUNSUPPORTED, "qs_batch_add: product-layout bindings are not batched");
   int slot = -1;
   for (int s = 0; s < B->cap; ++s)
     if (!B->used[s]) {

@@ -922,6 +922,22 @@ extern "C" qs_status qs_basis_index(qs_ctx *ctx, const qs_basis *b, const uint64
   return QS_OK;
 }
 
+// Index in b of n determinants already in device memory (d_words), into
+// d_idx (device); -1 for determinants outside b. Enqueued on ctx->stream.
+qs_status qs_basis_index_dev(qs_ctx *ctx, const qs_basis *b, const uint64_t *d_words, int64_t n, int64_t *d_idx) {
+  if (n == 0) return QS_OK;
+  const int64_t blocks = (n + 255) / 256;
+  const uint2 *tbl = b->has_rank ? b->d_tbl : nullptr;
+  if (b->nw == 1)
+    hipLaunchKernelGGL(k_index<1>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, d_words, n, tbl, b->smax,
+                       b->nchunks, b->d_words, b->n, d_idx);
+  else
+    hipLaunchKernelGGL(k_index<2>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, d_words, n, tbl, b->smax,
+                       b->nchunks, b->d_words, b->n, d_idx);
+  QS_HIP(ctx, hipGetLastError());
+  return QS_OK;
+}
+
 extern "C" qs_status qs_basis_destroy(qs_basis *b) {
   if (!b) return QS_OK;
   qs_free(b->d_words);

@@ -153,13 +153,37 @@ struct SellD {
   const double2 *val = nullptr;        // V_t
   const uint64_t *par = nullptr;       // P_t
   const void *diag = nullptr;          // pre-summed diagonal (rows == cols) or null: double2, or double if real_diag
+  // Product layout (kind 1; qs_product.hip). The modes are split into two
+  // groups, A and B. Rows are ordered by (B part, A part): a "group" holds the
+  // rows of one B configuration b, one row per allowed A configuration a
+  // (its A-set), and a slice is up to 64 consecutive rows of one group. Terms
+  // that move B are stored once per group as entries (source group's first
+  // row, term, sign from b); a lane applies an entry to its own
+  // a, either unchanged (identity: the source rows are contiguous) or through
+  // a map a -> source index. Terms acting on A alone form a small matrix per
+  // A-set (4-byte slots: source index | term | sign).
+  int kind = 0;                        // 0: SELL slices of 64 rows, 1: product layout
+  const int4 *pslice = nullptr;        // per slice: {first row, rows (<= 64), group, A-set | chunk << 16}
+  const uint2 *pent_ptr = nullptr;     // per group: {first entry, first mapped entry} (ngroups + 1)
+  const uint4 *pent = nullptr;         // {source group's first row, term | sign << 31, map offset or ~0, 0}
+  const int32_t *amap = nullptr;       // per map and destination index: source index (bit 30: sign) or -1
+  const int4 *aset = nullptr;          // per A-set: {rows m, A-matrix ELL width k, ELL offset, 0}
+  const uint32_t *hA = nullptr;        // A-matrix ELL, column-major per A-set: src | term << 16 | sign << 31
+};
+
+// Host side of a product layout, kept with the shared index.
+struct ProdHost {
+  std::vector<int> diag_terms;         // terms with F == 0, in term order
+  int64_t ngroups = 0, nsets = 0, nmaps = 0, nentries = 0, nmapped = 0, hA_slots = 0;
+  double load_cost = 0.0;              // estimated 64-lane 16-byte vector loads per apply
 };
 
 // Device arrays of an assembled index stream (everything but the term tables
 // and the diagonal), shared by the bindings made from it with
 // qs_hop_create_like and freed when the last of them goes.
 struct HopIndex {
-  void *buf[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  std::vector<void *> buf;
+  std::shared_ptr<ProdHost> prod;  // product layout only
   ~HopIndex();
 };
 
@@ -188,6 +212,8 @@ struct qs_hop {
   uint16_t *d_tid = nullptr;
   double *d_diag = nullptr;
   std::shared_ptr<HopIndex> index;  // owner of the d_* arrays above except d_diag, once assembled
+  // product layout: canonical index of each row (the index owns it)
+  const int32_t *d_inv = nullptr;
 };
 
 // ---- error helpers --------------------------------------------------------
@@ -232,6 +258,24 @@ qs_status qs_build_rank_table(int nf, int nres, const uint8_t *masks, const int 
                               const int *hi, RankTable &rt, std::string &err, bool count_only);
 qs_status qs_dot_internal(qs_ctx *ctx, int64_t n, const double *a, const double *b,
                           double *re, double *im);
+
+// Index in b of n determinants in device memory (-1 when absent), into d_idx
+// (device), enqueued on ctx->stream (qs_core.hip).
+qs_status qs_basis_index_dev(qs_ctx *ctx, const qs_basis *b, const uint64_t *d_words, int64_t n, int64_t *d_idx);
+// Pre-summed diagonal of n rows over the terms d_which (device, in term order)
+// into diag, enqueued on ctx->stream (qs_apply.hip).
+qs_status qs_diag_rows(qs_ctx *ctx, const qs_op *op, const uint64_t *rows, int64_t n, const int *d_which, int nwhich,
+                       void *diag, int real_diag);
+// Product layout (qs_product.hip): a binding sharing `like`'s product index
+// with op's own values; the vector order conversions; apply in the binding's
+// row order.
+qs_status qs_prod_create_like(qs_ctx *ctx, const qs_op *op, const qs_hop *like, qs_hop **out);
+qs_status qs_prod_to_rows(qs_ctx *ctx, const qs_hop *h, int k, const double2 *x, int64_t ldvx, double2 *xp,
+                          int64_t ldvp);
+qs_status qs_prod_from_rows(qs_ctx *ctx, const qs_hop *h, int k, const double2 *yp, int64_t ldvp, double2 *y,
+                            int64_t ldvy);
+qs_status qs_prod_apply_rows(qs_ctx *ctx, const qs_hop *h, int k, const double2 *x, int64_t ldvx, double2 *y,
+                             int64_t ldvy);
 
 // Host-side dense Hermitian eigen-solver (column-major, complex interleaved).
 // A: n*n complex (overwritten), w: n eigenvalues ascending, Z: n*n eigenvectors

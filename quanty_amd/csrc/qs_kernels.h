@@ -23,6 +23,14 @@ __device__ __forceinline__ int64_t wave_uniform(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// Position j of lane `lane` in the ELL part of a slice starting at `base`:
+// a lane's positions are stored four at a time (quad j/4 of the slice holds
+// 64 lanes x 4 consecutive positions), so one 16-byte load per lane fetches
+// four positions.
+__host__ __device__ __forceinline__ uint64_t ell_slot(uint64_t base, uint32_t j, int lane) {
+  return base + ((uint64_t)(j >> 2) << 8) + (uint64_t)lane * 4 + (j & 3u);
+}
+
 // Term t applied to the row's determinant d: accumulate
 // (-1)^popcount(d & P_t) V_t x[c] for KC interleaved vectors when `hit`.
 template <int KC, int NW>
@@ -118,22 +126,39 @@ __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, co
   }
   // ELL remainder, U positions per batch, software-pipelined: the stream
   // loads of batch b+1 are issued before the gathers of batch b, so the two
-  // latencies overlap. Padding slots hold column 0 / term 0 and are masked.
+  // latencies overlap. A lane's positions come four per 16-byte load (slices
+  // hold whole quads). Padding slots hold column 0 / term 0 and are masked.
   // The stream is read once per apply, so non-temporal loads keep it from
   // evicting the gathered vector from L2 / the Infinity Cache.
+  typedef uint32_t u32x4q __attribute__((ext_vector_type(4)));
+  typedef uint32_t u32x2q __attribute__((ext_vector_type(2)));
+  constexpr int NQ = U / 4;
   const uint64_t sb = A.slice_ptr[s];
   const uint32_t slen = A.slice_len[s];
-  const uint32_t *cp = A.col + sb + lane;
-  const uint16_t *tq = PACKED ? nullptr : A.tid + sb + lane;
+  const uint32_t nq = slen >> 2;
+  const u32x4q *cp = reinterpret_cast<const u32x4q *>(A.col + sb) + lane;
+  const u32x2q *tq = PACKED ? nullptr : reinterpret_cast<const u32x2q *>(A.tid + sb) + lane;
   const uint32_t cbits = A.colbits;
   const uint32_t cmask = PACKED ? ((1u << cbits) - 1u) : 0xFFFFFFFFu;
   uint32_t e[U], tt[U];
   auto load_batch = [&](uint32_t j0, uint32_t *eo, uint32_t *to) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint32_t j = j0 + u < slen ? j0 + u : slen - 1;
-      eo[u] = __builtin_nontemporal_load(cp + (size_t)j * 64);
-      to[u] = PACKED ? 0u : (uint32_t)__builtin_nontemporal_load(tq + (size_t)j * 64);
+    for (int q = 0; q < NQ; ++q) {
+      const uint32_t qi = (j0 >> 2) + q < nq ? (j0 >> 2) + q : nq - 1;
+      const u32x4q v = __builtin_nontemporal_load(cp + (size_t)qi * 64);
+      eo[4 * q + 0] = v.x;
+      eo[4 * q + 1] = v.y;
+      eo[4 * q + 2] = v.z;
+      eo[4 * q + 3] = v.w;
+      if (PACKED) {
+        to[4 * q + 0] = to[4 * q + 1] = to[4 * q + 2] = to[4 * q + 3] = 0u;
+      } else {
+        const u32x2q t = __builtin_nontemporal_load(tq + (size_t)qi * 64);
+        to[4 * q + 0] = t.x & 0xFFFFu;
+        to[4 * q + 1] = t.x >> 16;
+        to[4 * q + 2] = t.y & 0xFFFFu;
+        to[4 * q + 3] = t.y >> 16;
+      }
     }
   };
   if (slen > 0) load_batch(0, e, tt);
@@ -156,6 +181,177 @@ __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, co
       }
     }
   }
+}
+
+// Rows of one group that a wavefront stages in LDS for the A-matrix (product
+// layout, one vector): groups with more rows read their own rows from memory.
+#define QS_PROD_STAGE 256
+
+// Row of lane `lane` in product slice s (see SellD, kind 1) applied to KC
+// vectors (entry j of vector k at x[j*ldx + k*ldv]); returns the row, or -1
+// for a lane past the slice's rows (its sums stay 0). `s` must be
+// wave-uniform, so the slice header and the group's entries are scalar
+// loads. tv: the term values (an LDS copy or A.val). xs: this wavefront's LDS
+// area of QS_PROD_STAGE vector entries, or null; with KC == 1 and a group of
+// at most QS_PROD_STAGE rows, the wavefront copies the group's rows there and
+// the A-matrix reads them from LDS.
+//   diagonal: the row's pre-summed diagonal;
+//   A-matrix: per row, 4-byte slots (source index in the group | term | sign)
+//     over the row's own group;
+//   entries: per B-moving term, its source group; lanes of an identity entry
+//     read consecutive rows (one contiguous load per entry), the others go
+//     through the entry's map (source index, sign in bit 30).
+// Lanes past the slice's rows load nothing.
+template <int KC>
+__device__ __forceinline__ int64_t prod_row(const SellD &A, int64_t s, int lane, const double2 *tv,
+                                            const double2 *__restrict__ x, int64_t ldx, int64_t ldv, double *ar,
+                                            double *ai, double2 *xs) {
+  constexpr int U = KC == 1 ? 8 : 4;   // entries per batch
+  constexpr int UA = 8;                // A-matrix slots per batch
+#pragma unroll
+  for (int k = 0; k < KC; ++k) {
+    ar[k] = 0.0;
+    ai[k] = 0.0;
+  }
+  const int4 d = A.pslice[s];
+  const bool active = lane < d.y;
+  const int chunk = d.w >> 16;
+  const int ap = chunk * 64 + lane;                 // index of the row's A configuration in its A-set
+  const int64_t g0 = (int64_t)d.x - chunk * 64;     // first row of the group
+  const int64_t r = (int64_t)d.x + lane;
+  const int4 as = A.aset[d.w & 0xFFFF];             // {m, k, offset, 0}
+  const uint2 pp = A.pent_ptr[d.z];
+  const uint32_t e0 = pp.x, em = pp.y, e1 = A.pent_ptr[d.z + 1].x;
+  const bool staged = KC == 1 && xs != nullptr && as.x <= QS_PROD_STAGE;
+  if (staged) {
+    // the previous slice's reads of xs are done before it is overwritten
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int i = lane; i < as.x; i += 64) xs[i] = x[(g0 + i) * ldx];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (!active) return -1;
+  if (A.diag) {
+    double2 dg;
+    if (A.real_diag) dg = make_double2(static_cast<const double *>(A.diag)[r], 0.0);
+    else dg = static_cast<const double2 *>(A.diag)[r];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) {
+      const double2 xv = staged ? xs[ap] : x[r * ldx + k * ldv];
+      ar[k] = dg.x * xv.x - dg.y * xv.y;
+      ai[k] = dg.x * xv.y + dg.y * xv.x;
+    }
+  }
+  // A-matrix, UA slots at a time (one 4-byte load per slot and lane)
+  for (int eb = 0; eb < as.y; eb += UA) {
+    uint32_t sl[UA];
+#pragma unroll
+    for (int u = 0; u < UA; ++u) {
+      const int e = eb + u < as.y ? eb + u : as.y - 1;
+      sl[u] = A.hA[(int64_t)as.z + (int64_t)e * as.x + ap];
+    }
+#pragma unroll
+    for (int u = 0; u < UA; ++u) {
+      const uint32_t src = sl[u] & 0xFFFFu;
+      const bool hit = eb + u < as.y && src != 0xFFFFu;
+      double2 v = tv[(sl[u] >> 16) & 0x7FFFu];
+      const uint64_t sb = (uint64_t)(sl[u] >> 31) << 63;
+      v.x = __longlong_as_double(__double_as_longlong(v.x) ^ sb);
+      v.y = __longlong_as_double(__double_as_longlong(v.y) ^ sb);
+      if (!hit) v = make_double2(0.0, 0.0);
+      const int64_t c = hit ? g0 + src : r;
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        const double2 xv = staged ? xs[hit ? src : ap] : x[c * ldx + k * ldv];
+        ar[k] = fma(v.x, xv.x, fma(-v.y, xv.y, ar[k]));
+        ai[k] = fma(v.x, xv.y, fma(v.y, xv.x, ai[k]));
+      }
+    }
+  }
+  // identity entries: one coefficient per entry (its terms summed), one
+  // contiguous load of the source group's rows
+  for (uint32_t eb = e0; eb < em; eb += U) {
+    int64_t col[U];
+    double2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool ok = eb + u < em;
+      const uint4 E = A.pent[ok ? eb + u : em - 1];
+      double2 vv = tv[E.y & 0xFFFFu];
+      const uint64_t sb = (uint64_t)(E.y >> 31) << 63;
+      vv.x = __longlong_as_double(__double_as_longlong(vv.x) ^ sb);
+      vv.y = __longlong_as_double(__double_as_longlong(vv.y) ^ sb);
+      v[u] = ok ? vv : make_double2(0.0, 0.0);
+      col[u] = (int64_t)E.x + ap;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        const double2 xv = x[col[u] * ldx + k * ldv];
+        ar[k] = fma(v[u].x, xv.x, fma(-v[u].y, xv.y, ar[k]));
+        ai[k] = fma(v[u].x, xv.y, fma(v[u].y, xv.x, ai[k]));
+      }
+    }
+  }
+  // mapped entries: a source index and sign per lane from the map
+  constexpr int UM = 4;
+  for (uint32_t eb = em; eb < e1; eb += UM) {
+    uint4 E[UM];
+    int32_t q[UM];
+#pragma unroll
+    for (int u = 0; u < UM; ++u) E[u] = A.pent[eb + u < e1 ? eb + u : e1 - 1];
+#pragma unroll
+    for (int u = 0; u < UM; ++u) q[u] = A.amap[E[u].z + ap];
+    int64_t col[UM];
+    double2 v[UM];
+#pragma unroll
+    for (int u = 0; u < UM; ++u) {
+      const bool h = eb + u < e1 && q[u] >= 0;
+      double2 vv = tv[E[u].y & 0xFFFFu];
+      const uint64_t sb = (uint64_t)((E[u].y >> 31) ^ (((uint32_t)q[u] >> 30) & 1u)) << 63;
+      vv.x = __longlong_as_double(__double_as_longlong(vv.x) ^ sb);
+      vv.y = __longlong_as_double(__double_as_longlong(vv.y) ^ sb);
+      v[u] = h ? vv : make_double2(0.0, 0.0);
+      col[u] = h ? (int64_t)E[u].x + (q[u] & 0x3FFFFFFF) : r;
+    }
+#pragma unroll
+    for (int u = 0; u < UM; ++u) {
+#pragma unroll
+      for (int k = 0; k < KC; ++k) {
+        const double2 xv = x[col[u] * ldx + k * ldv];
+        ar[k] = fma(v[u].x, xv.x, fma(-v[u].y, xv.y, ar[k]));
+        ai[k] = fma(v[u].x, xv.y, fma(v[u].y, xv.x, ai[k]));
+      }
+    }
+  }
+  return r;
+}
+
+// Dynamic LDS of a product-layout kernel: the term values when `values`,
+// then one staging area per wavefront (KC == 1 only).
+__host__ __device__ inline size_t prod_lds_bytes(int nterms, bool values, int kc) {
+  const size_t v = values ? ((size_t)nterms * 16 + 15) / 16 * 16 : 0;
+  return v + (kc == 1 ? (size_t)(QS_BLOCK / 64) * QS_PROD_STAGE * 16 : 0);
+}
+
+// This wavefront's staging area in a product-layout kernel's dynamic LDS.
+__device__ __forceinline__ double2 *prod_stage_area(char *smem, int nterms, bool values, int kc) {
+  if (kc != 1) return nullptr;
+  const size_t v = values ? ((size_t)nterms * 16 + 15) / 16 * 16 : 0;
+  return reinterpret_cast<double2 *>(smem + v) + (size_t)(threadIdx.x >> 6) * QS_PROD_STAGE;
+}
+
+// Bytes of LDS that stage_term_values uses.
+__host__ __device__ inline size_t term_value_bytes(int nterms) { return (size_t)nterms * 16; }
+
+// Stage the term values of A into LDS at smem (no barrier).
+__device__ __forceinline__ void stage_term_values(const SellD &A, char *smem) {
+  double2 *sv = reinterpret_cast<double2 *>(smem);
+  for (int i = threadIdx.x; i < A.nterms; i += blockDim.x) sv[i] = A.val[i];
 }
 
 // Bytes of LDS that stage_term_tables uses.

@@ -155,6 +155,39 @@ k_lz1_asm(int step, int nb2, LzState S, SellD A, const double2 *__restrict__ uc,
   if (threadIdx.x == 0) S.pd[blockIdx.x] = t;
 }
 
+// K1, product layout (fused apply in the binding's row order + epilogue)
+template <bool LDS>
+__global__ void __launch_bounds__(QS_BLOCK)
+k_lz1_prod(int step, int nb2, LzState S, SellD A, const double2 *__restrict__ uc, double2 *__restrict__ up) {
+  __shared__ double red[QS_BLOCK / 64];
+  extern __shared__ char smem[];
+  const double2 *tv = A.val;
+  if (LDS) {
+    stage_term_values(A, smem);
+    tv = reinterpret_cast<const double2 *>(smem);
+  }
+  double2 *xs = prod_stage_area(smem, A.nterms, LDS, 1);
+  double sc, cp;
+  lz_k1_prologue(step, nb2, S, red, sc, cp);  // ends with a barrier
+  const int lane = threadIdx.x & 63;
+  int64_t s0, s1, ds;
+  xcd_items(A.nslices, s0, s1, ds);
+  s0 = wave_uniform(s0);
+  double dot = 0.0;
+  for (int64_t s = s0; s < s1; s += ds) {
+    double hr[1], hi[1];
+    const int64_t r = prod_row<1>(A, s, lane, tv, uc, 1, 0, hr, hi, xs);
+    if (r >= 0) {
+      const double2 u = uc[r];
+      dot += hr[0] * u.x + hi[0] * u.y;
+      const double2 p = up[r];
+      up[r] = make_double2(sc * hr[0] - cp * p.x, sc * hi[0] - cp * p.y);
+    }
+  }
+  const double t = block_sum_values(dot, red);
+  if (threadIdx.x == 0) S.pd[blockIdx.x] = t;
+}
+
 // Kernel, dynamic LDS and grid for K1 on an assembled operator.
 struct Lz1Launch {
   const void *kern;
@@ -178,6 +211,18 @@ static Lz1Launch lz1_choose(qs_ctx *ctx, const SellD &A) {
 }
 
 static Lz1Launch lz1_launch_for(qs_ctx *ctx, const SellD &A) {
+  if (A.kind == 1) {
+    Lz1Launch L;
+    if (term_value_bytes(A.nterms) <= (size_t)ctx->term_lds) {
+      L.kern = (const void *)k_lz1_prod<true>;
+      L.shm = prod_lds_bytes(A.nterms, true, 1);
+    } else {
+      L.kern = (const void *)k_lz1_prod<false>;
+      L.shm = prod_lds_bytes(A.nterms, false, 1);
+    }
+    L.blocks = (int)std::max<int64_t>(1, std::min<int64_t>((A.nslices + 3) / 4, qs_resident_blocks(ctx, L.kern, L.shm)));
+    return L;
+  }
   if (A.nw == 1) return A.packed ? lz1_choose<1, true>(ctx, A) : lz1_choose<1, false>(ctx, A);
   return A.packed ? lz1_choose<2, true>(ctx, A) : lz1_choose<2, false>(ctx, A);
 }
@@ -252,11 +297,17 @@ extern "C" qs_status qs_lanczos(qs_ctx *ctx, const qs_hop *h, const double *v0, 
   S.pd = S.pn + nb2;
   qs_status st = QS_OK;
   hipError_t e = hipMemsetAsync(S.a, 0, (size_t)(nsteps + 1) * 8, ctx->stream);
-  if (e == hipSuccess) e = hipMemsetAsync(u1, 0, vbytes, ctx->stream);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_lz_init, dim3(nb2), dim3(QS_BLOCK), 0, ctx->stream, n, (const double2 *)v0, u0, S.pn);
+  // a product-layout binding runs in its own row order: v0 goes there first
+  const double2 *vs = (const double2 *)v0;
+  if (e == hipSuccess && h->view.kind == 1) {
+    st = qs_prod_to_rows(ctx, h, 1, (const double2 *)v0, n, u1, n);
+    vs = u1;
+  }
+  if (e == hipSuccess && st == QS_OK) {
+    hipLaunchKernelGGL(k_lz_init, dim3(nb2), dim3(QS_BLOCK), 0, ctx->stream, n, vs, u0, S.pn);
     e = hipGetLastError();
   }
+  if (e == hipSuccess && st == QS_OK) e = hipMemsetAsync(u1, 0, vbytes, ctx->stream);
   double2 *uc = u0, *up = u1;
   for (int i = 0; i < nsteps && e == hipSuccess && st == QS_OK; ++i) {
     if (h->assembled) {

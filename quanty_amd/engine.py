@@ -136,6 +136,23 @@ class Basis:
             pass
 
 
+def product_candidates(nf: int, restrictions):
+    """Mode groups that may serve as group A of a product layout: the mode set
+    of every restriction that does not cover all modes (mask strings with
+    spaces ignored, or byte masks)."""
+    out = []
+    for m, _lo, _hi in restrictions:
+        if isinstance(m, str):
+            bits = m.replace(" ", "")
+            modes = [i for i, ch in enumerate(bits[:nf]) if ch == "1"]
+        else:
+            mb = np.asarray(m, dtype=np.uint8)
+            modes = [i for i in range(nf) if (int(mb[i >> 3]) >> (7 - (i & 7))) & 1]
+        if 0 < len(modes) < nf and modes not in out:
+            out.append(modes)
+    return out
+
+
 def pack_terms(terms):
     """[(codes (o[0] first), complex value), ...] -> (len int32, codes uint16, values float64)."""
     terms = list(terms)
@@ -201,6 +218,65 @@ class Hop:
         self.ctx.check(self.lib.qs_hop_create_like(self.ctx.h, op.h, template.h, C.byref(h)), "qs_hop_create_like")
         self.h = h
         return self
+
+    @classmethod
+    def product(cls, op: Op, basis: Basis, amodes=None) -> "Hop":
+        """op on the sector `basis` in the product layout (qs_hop_create_product):
+        modes `amodes` form group A, the others group B; rows are grouped by
+        their B configuration. Without `amodes`, every restriction mask that
+        does not cover all modes is tried as group A and the split with the
+        fewest estimated vector loads is kept. Vectors stay in canonical order
+        at apply/lanczos; `row_order()` gives the layout's own order."""
+        if amodes is None:
+            best = None
+            for amodes_c in product_candidates(basis.nf, basis.restrictions or []):
+                try:
+                    h = cls.product(op, basis, amodes_c)
+                except QSError:
+                    continue
+                cost = h.product_info()["load_cost"]
+                if best is None or cost < best[0]:
+                    best = (cost, h)
+            if best is None:
+                raise QSError("qs_hop_create_product: no restriction group gives a product layout")
+            return best[1]
+        self = cls.__new__(cls)
+        self.op, self.rows, self.cols = op, basis, basis
+        self.ctx = op.ctx
+        self.lib = self.ctx.lib
+        mask = np.zeros(D.nbytes(basis.nf), dtype=np.uint8)
+        for i in amodes:
+            mask[i >> 3] |= np.uint8(1 << (7 - (i & 7)))
+        h = C.c_void_p()
+        self.ctx.check(self.lib.qs_hop_create_product(self.ctx.h, op.h, basis.h, ptr(mask), C.byref(h)),
+                       "qs_hop_create_product")
+        self.h = h
+        self.amodes = sorted(int(i) for i in amodes)
+        return self
+
+    def product_info(self) -> dict:
+        info = np.zeros(8, dtype=np.int64)
+        cost = C.c_double()
+        self.ctx.check(self.lib.qs_hop_product_info(self.h, ptr(info), C.byref(cost)), "qs_hop_product_info")
+        keys = ("groups", "slices", "asets", "entries", "mapped", "reserved", "a_slots", "maps")
+        out = dict(zip(keys, (int(v) for v in info)))
+        out["load_cost"] = cost.value
+        return out
+
+    def row_order(self) -> np.ndarray:
+        """Canonical index of every row in the order the binding's kernels keep vectors."""
+        out = np.zeros(self.rows.n, dtype=np.int64)
+        self.ctx.check(self.lib.qs_hop_row_order(self.ctx.h, self.h, ptr(out)), "qs_hop_row_order")
+        return out
+
+    def apply_rows(self, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+        """y = O x with x, y in the binding's own row order (row_order())."""
+        assert x.dtype == torch.complex128 and x.is_contiguous() and x.numel() == self.cols.n
+        if y is None:
+            y = self.rows.zeros()
+        _torch_ready(x.device)
+        self.ctx.check(self.lib.qs_hop_apply_rows(self.ctx.h, self.h, ptr(x), ptr(y)), "qs_hop_apply_rows")
+        return y
 
     def stats(self) -> dict:
         a, b, c, d, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int(), C.c_int64()

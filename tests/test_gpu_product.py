@@ -1,0 +1,155 @@
+"""Product-layout bindings (qs_hop_create_product) against the oracle and the
+SELL layout: H*psi to 1e-13 on every split the restrictions allow (one and
+two words per determinant), the same off-diagonal entry count as SELL, Lanczos
+coefficients to 1e-10, the order conversions, the diagonal, bindings sharing
+an index (create_like) and the refusal of splits the sector does not factor
+over. All calls go through the C ABI."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from quanty_amd import det as D
+from quanty_amd import problems as P
+from quanty_amd._lib import QSError
+from quanty_amd.engine import Basis, Hop, Op
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda", 0)
+
+
+def cvec(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=n) + 1j * rng.normal(size=n)
+
+
+def oracle_apply(nf, terms, basis, x, restrictions):
+    psi = oracle.Vec.from_arrays(nf, D.words_to_bytes(basis.words(), nf), x)
+    d, v = oracle.Op(nf, terms).apply(psi, restrictions=restrictions).arrays()
+    out = np.zeros(basis.n, dtype=complex)
+    if len(v):
+        idx = basis.index(D.bytes_to_words(d, nf))
+        assert np.all(idx >= 0)
+        out[idx] = v
+    return out
+
+
+# (problem, group A) pairs: the d shell, a group of bath shells, and a
+# two-word case
+CASES = [
+    (dict(seed=5, nbath=2, nelec=18, full_baths=1, max_holes=2, max_elec=2), "d"),
+    (dict(seed=5, nbath=2, nelec=18, full_baths=1, max_holes=2, max_elec=2), "full"),
+    (dict(seed=9, nbath=3, nelec=22, full_baths=1, max_holes=2, max_elec=3), "d"),
+    (dict(seed=9, nbath=3, nelec=22, full_baths=1, max_holes=2, max_elec=3), "empty"),
+    (dict(seed=4, nbath=6, nelec=38, full_baths=3, max_holes=1, max_elec=1), "d"),
+]
+
+
+def group(pr, which, kw):
+    """Group A: the d shell, the nominally full bath shells or the empty ones."""
+    if which == "d":
+        return pr["dmodes"]
+    nfull = kw["full_baths"]
+    sel = pr["baths"][:nfull] if which == "full" else pr["baths"][nfull:]
+    return [m for b in sel for m in b]
+
+
+@pytest.mark.parametrize("kw,which", CASES)
+def test_product_apply_matches_oracle_and_sell(kw, which):
+    pr = P.anderson(**kw)
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"])
+    H = Op(nf, pr["H"])
+    prod = Hop.product(H, basis, group(pr, which, kw))
+    sell = Hop(H, basis, mode="assembled")
+    sp, ss = prod.stats(), sell.stats()
+    assert sp["assembled"]
+    assert sp["nnz_offdiag"] == ss["nnz_offdiag"]
+    assert sp["nnz_eff"] == ss["nnz_eff"]
+    x = cvec(basis.n, 11)
+    want = oracle_apply(nf, pr["H"], basis, x, pr["restrictions"])
+    y = prod.apply(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    err = np.max(np.abs(y - want)) / np.max(np.abs(want))
+    assert err < 1e-13, err
+    ys = sell.apply(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(y - ys)) / np.max(np.abs(ys)) < 1e-13
+    # several vectors at once (launches of 4, 2 and 1)
+    X = np.stack([cvec(basis.n, 20 + k) for k in range(7)])
+    Y = torch.zeros((7, basis.n), dtype=torch.complex128, device=DEV)
+    prod.ctx.check(prod.lib.qs_hop_apply_multi(prod.ctx.h, prod.h, 7, torch.from_numpy(X).to(DEV).data_ptr(),
+                                               Y.data_ptr()), "qs_hop_apply_multi")
+    Ys = sell.apply(torch.from_numpy(X[5]).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(Y[5].cpu().numpy() - Ys)) / np.max(np.abs(Ys)) < 1e-13
+
+
+@pytest.mark.parametrize("kw,which", CASES[:3])
+def test_product_lanczos_and_orders(kw, which):
+    pr = P.anderson(**kw)
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"])
+    H = Op(nf, pr["H"])
+    prod = Hop.product(H, basis, group(pr, which, kw))
+    sell = Hop(H, basis, mode="assembled")
+    x = cvec(basis.n, 3)
+    a1, b1 = sell.lanczos(torch.from_numpy(x).to(DEV), 60)
+    a2, b2 = prod.lanczos(torch.from_numpy(x).to(DEV), 60)
+    assert np.allclose(a2, a1, rtol=1e-10, atol=1e-10 * np.max(np.abs(a1)))
+    assert np.allclose(b2, b1, rtol=1e-10, atol=0)
+    psi = oracle.Vec.from_arrays(nf, D.words_to_bytes(basis.words(), nf), x)
+    ra, rb = oracle.Op(nf, pr["H"]).lanczos(psi, 30, restrictions=pr["restrictions"])
+    assert np.allclose(a2[:31], ra, rtol=1e-10, atol=1e-10) and np.allclose(b2[:30], rb, rtol=1e-10, atol=0)
+    # the row order is a permutation; apply_rows equals apply in that order
+    order = prod.row_order()
+    assert np.array_equal(np.sort(order), np.arange(basis.n))
+    xr = torch.from_numpy(np.ascontiguousarray(x[order])).to(DEV)
+    yr = prod.apply_rows(xr).cpu().numpy()
+    y = sell.apply(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(yr - y[order])) / np.max(np.abs(y)) < 1e-13
+    # the diagonal in canonical order
+    assert np.allclose(prod.diagonal().cpu().numpy(), sell.diagonal().cpu().numpy(), rtol=0, atol=1e-12)
+
+
+def test_product_create_like_binds_new_values():
+    kw = dict(seed=5, nbath=2, nelec=18, full_baths=1, max_holes=2, max_elec=2)
+    pr, pr2 = P.anderson(**kw), P.anderson(**dict(kw, seed=6))
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"])
+    H1, H2 = Op(nf, pr["H"]), Op(nf, pr2["H"])
+    p1 = Hop.product(H1, basis, pr["dmodes"])
+    p2 = Hop.like(H2, p1)
+    x = cvec(basis.n, 8)
+    want = oracle_apply(nf, pr2["H"], basis, x, pr2["restrictions"])
+    y = p2.apply(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(y - want)) / np.max(np.abs(want)) < 1e-13
+    del p1  # the shared index outlives the binding it came from
+    y = p2.apply(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(y - want)) / np.max(np.abs(want)) < 1e-13
+
+
+def test_product_refuses_a_split_that_does_not_factor():
+    kw = dict(seed=5, nbath=2, nelec=18, full_baths=1, max_holes=2, max_elec=2)
+    pr = P.anderson(**kw)
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"])
+    H = Op(nf, pr["H"])
+    # half the d shell: the d-occupation restriction then spans both groups
+    with pytest.raises(QSError, match="both groups"):
+        Hop.product(H, basis, pr["dmodes"][:5])
+
+
+def test_product_chooses_a_split_by_its_load_estimate():
+    kw = dict(seed=9, nbath=3, nelec=22, full_baths=1, max_holes=2, max_elec=3)
+    pr = P.anderson(**kw)
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"])
+    H = Op(nf, pr["H"])
+    auto = Hop.product(H, basis)
+    costs = {}
+    for which in ("d", "full", "empty"):
+        costs[which] = Hop.product(H, basis, group(pr, which, kw)).product_info()["load_cost"]
+    assert auto.product_info()["load_cost"] == min(costs.values())
+    x = cvec(basis.n, 4)
+    want = oracle_apply(nf, pr["H"], basis, x, pr["restrictions"])
+    y = auto.apply(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    assert np.max(np.abs(y - want)) / np.max(np.abs(want)) < 1e-13
