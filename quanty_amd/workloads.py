@@ -6,6 +6,7 @@
     python3 -m quanty_amd.workloads --config 2b   # 512 small impurity problems, batched vs one at a time
     python3 -m quanty_amd.workloads --config 3    # the serving stream at 10, 50, 200 and 500 req/s
     python3 -m quanty_amd.workloads --config 4    # 76-mode (two-word) L2,3 Green's function
+    python3 -m quanty_amd.workloads --config layouts  # H*psi of the 50-mode impurity: SELL vs product layouts
 
 Each run prints one JSON line: what was run, its sizes, wall times (HIP work
 synchronized on both sides) and the derived rates. Accuracy against the oracle
@@ -209,9 +210,74 @@ def config3(requests: int = 64, rates=(10.0, 50.0, 200.0, 500.0), seed: int = 10
             "rates": rows}
 
 
+def _spectrum_diff(a, b, a0, b0, ctx):
+    g = cf_spectrum(a, b, 0.0, -10.0, 20.0, 2000, 0.4, ctx)
+    g0 = cf_spectrum(a0, b0, 0.0, -10.0, 20.0, 2000, 0.4, ctx)
+    return float(np.max(np.abs(g - g0)) / np.max(np.abs(g0)))
+
+
+def layouts(seed: int = 1, reps: int = 30, nsteps: int = 200):
+    """H*psi of the 50-mode Anderson impurity (2,179,235 determinants) in the
+    assembled SELL layout and in the product layout for each restriction group
+    as group A (and the automatic choice): one apply in the binding's own row
+    order timed with HIP events over `reps` launches on the library's stream,
+    and `nsteps` Lanczos steps from one start vector; the coefficients of every
+    layout against SELL's."""
+    ctx = default_context()
+    dev = torch.device("cuda", ctx.device)
+    pr = P.anderson(seed=seed)
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"], ctx)
+    H = Op(nf, pr["H"], ctx)
+    stream = torch.cuda.ExternalStream(ctx.lib.qs_ctx_stream(ctx.h), device=dev)
+    g = np.random.default_rng(seed)
+    x = torch.from_numpy(g.normal(size=basis.n) + 1j * g.normal(size=basis.n)).to(dev)
+
+    def kernel_us(hop):
+        order = torch.from_numpy(hop.row_order()).to(dev)
+        xr = x[order].contiguous()
+        yr = basis.zeros()
+        hop.apply_rows(xr, yr)
+        torch.cuda.synchronize()
+        ctx.lib.qs_ctx_set_async(ctx.h, 1)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.check(ctx.lib.qs_hop_apply_rows(ctx.h, hop.h, xr.data_ptr(), yr.data_ptr()), "qs_hop_apply_rows")
+        e1.record(stream)
+        e1.synchronize()
+        ctx.lib.qs_ctx_set_async(ctx.h, 0)
+        return 1e3 * e0.elapsed_time(e1) / reps
+
+    sell, t_sell = _sync_time(lambda: Hop(H, basis, mode="assembled"), "assemble SELL")
+    (a0, b0), t_lz = _sync_time(lambda: sell.lanczos(x, nsteps), "SELL Lanczos")
+    rows = [{"layout": "sell", "build_s": t_sell, "apply_us": kernel_us(sell), "lanczos_ms": 1e3 * t_lz,
+             "stream_bytes": sell.stats()["stream_bytes"]}]
+    del sell
+    groups = [("auto", None), ("d", pr["dmodes"]), ("B1+B2", [m for b in pr["baths"][:2] for m in b]),
+              ("B3+B4", [m for b in pr["baths"][2:] for m in b])]
+    for name, am in groups:
+        hop, t_b = _sync_time(lambda: Hop.product(H, basis, am), f"product layout, A = {name}")
+        (a, b), t_lz = _sync_time(lambda: hop.lanczos(x, nsteps), f"product Lanczos, A = {name}")
+        info = hop.product_info()
+        rows.append({"layout": f"product A={name}", "group_a": hop.amodes, "build_s": t_b,
+                     "apply_us": kernel_us(hop), "lanczos_ms": 1e3 * t_lz,
+                     "stream_bytes": hop.stats()["stream_bytes"], "load_cost": info["load_cost"],
+                     "groups": info["groups"], "slices": info["slices"], "entries": info["entries"],
+                     "mapped": info["mapped"],
+                     # the first 40 coefficients (later ones of a recurrence without
+                     # re-orthogonalization follow rounding differences) and the
+                     # broadened spectra of all nsteps
+                     "coef40_max_rel_diff": float(max(np.max(np.abs(a[:41] - a0[:41])) / np.max(np.abs(a0[:41])),
+                                                      np.max(np.abs(b[:40] - b0[:40])) / np.max(np.abs(b0[:40])))),
+                     "spectrum_max_rel_diff": _spectrum_diff(a, b, a0, b0, ctx)})
+        del hop
+    return {"config": "layouts", "dim": basis.n, "lanczos_steps": nsteps, "rows": rows}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--config", required=True, choices=["0", "1", "2a", "2b", "3", "4", "dense"])
+    ap.add_argument("--config", required=True, choices=["0", "1", "2a", "2b", "3", "4", "dense", "layouts"])
     ap.add_argument("--requests", type=int, default=64, help="requests per rate (config 3)")
     args = ap.parse_args()
     if args.config == "0":
@@ -226,6 +292,8 @@ def main():
         out = config2b()
     elif args.config == "3":
         out = config3(args.requests)
+    elif args.config == "layouts":
+        out = layouts()
     else:
         out = config4()
     print(json.dumps(out), flush=True)
