@@ -153,3 +153,33 @@ def test_product_chooses_a_split_by_its_load_estimate():
     want = oracle_apply(nf, pr["H"], basis, x, pr["restrictions"])
     y = auto.apply(torch.from_numpy(x).to(DEV)).cpu().numpy()
     assert np.max(np.abs(y - want)) / np.max(np.abs(want)) < 1e-13
+
+
+def test_product_binding_through_block_krylov_and_ground_state_solvers():
+    """The multi-vector entry points reach H*psi through the canonical-order
+    conversions of a product binding: their results equal the SELL binding's."""
+    kw = dict(seed=9, nbath=3, nelec=22, full_baths=1, max_holes=2, max_elec=3)
+    pr = P.anderson(**kw)
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"])
+    H = Op(nf, pr["H"])
+    prod = Hop.product(H, basis)
+    sell = Hop(H, basis, mode="assembled")
+    E1, V1, var1, _ = sell.ground_states(3, tol=1e-12)
+    E2, V2, var2, _ = prod.ground_states(3, tol=1e-12)
+    assert np.allclose(E2, E1, rtol=1e-10, atol=0)
+    # the same states up to a phase: |<v1|v2>| = 1
+    ov = torch.abs(torch.sum(V1.conj() * V2, dim=1)).cpu().numpy()
+    assert np.all(np.abs(ov - 1.0) < 1e-8), ov
+    X = torch.from_numpy(np.stack([cvec(basis.n, 30 + k) for k in range(3)])).to(DEV)
+    A1, B1, k1 = sell.block_lanczos(X, 12)
+    A2, B2, k2 = prod.block_lanczos(X, 12)
+    assert np.array_equal(k1, k2)
+    scale = np.max(np.abs(A1))
+    assert np.max(np.abs(A2[:8] - A1[:8])) < 1e-10 * scale and np.max(np.abs(B2[:8] - B1[:8])) < 1e-10 * scale
+    x = X[0].contiguous()
+    a1, b1, W1 = sell.lanczos_krylov(x, 30)
+    a2, b2, W2 = prod.lanczos_krylov(x, 30)
+    assert np.allclose(a2, a1, rtol=1e-10, atol=1e-10 * np.max(np.abs(a1)))
+    assert np.allclose(b2, b1, rtol=1e-10, atol=0)
+    assert torch.max(torch.abs(W2 - W1)).item() < 1e-8
