@@ -107,9 +107,14 @@ qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
  * QS_PARAM_ALLOC_TOP (default 0): 1 places this context's later device
  * allocations at the top of the library's free device memory, for data that
  * stays long (a cache of assembled operators): the free space below then
- * stays in large pieces for the short-lived blocks placed best-fit. */
+ * stays in large pieces for the short-lived blocks placed best-fit.
+ * QS_PARAM_HETRD (default 1): the Householder reduction of the dense method
+ * (qs_dense_poles without eigenvectors, qs_dense_lowest); 1 runs it as one
+ * cooperative launch with one grid barrier per column when n <= 800 and the
+ * device holds the grid, else rocSOLVER's zhetrd; 0 always uses zhetrd. Both
+ * give LAPACK's zhetd2 reduction; results agree to rounding. */
 enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3, QS_PARAM_TERM_LDS = 4,
-       QS_PARAM_ELL_ORDER = 5, QS_PARAM_ALLOC_TOP = 6 };
+       QS_PARAM_ELL_ORDER = 5, QS_PARAM_ALLOC_TOP = 6, QS_PARAM_HETRD = 7 };
 qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value);
 qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
 

@@ -342,6 +342,10 @@ extern "C" qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value) {
       if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_ALLOC_TOP must be 0 or 1");
       ctx->alloc_top = (int)value;
       return QS_OK;
+    case QS_PARAM_HETRD:
+      if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_HETRD must be 0 or 1");
+      ctx->hetrd = (int)value;
+      return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
   }
@@ -367,6 +371,9 @@ extern "C" qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value) {
       return QS_OK;
     case QS_PARAM_ALLOC_TOP:
       *value = ctx->alloc_top;
+      return QS_OK;
+    case QS_PARAM_HETRD:
+      *value = ctx->hetrd;
       return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);

@@ -40,6 +40,7 @@ struct qs_ctx {
   int term_lds = QS_TERM_LDS_DEFAULT;  // QS_PARAM_TERM_LDS
   int ell_order = 1;               // QS_PARAM_ELL_ORDER
   int alloc_top = 0;               // QS_PARAM_ALLOC_TOP: device blocks from the top of free memory
+  int hetrd = 1;                   // QS_PARAM_HETRD: the dense method's own cooperative reduction
   void *blas = nullptr;            // rocblas handle for the dense eigensolver, made on first use
   void *pinned = nullptr;          // the ctx's own pinned staging buffer (qs_h2d / qs_d2h)
   size_t pinned_bytes = 0;
@@ -276,6 +277,12 @@ qs_status qs_prod_from_rows(qs_ctx *ctx, const qs_hop *h, int k, const double2 *
                             int64_t ldvy);
 qs_status qs_prod_apply_rows(qs_ctx *ctx, const qs_hop *h, int k, const double2 *x, int64_t ldvx, double2 *y,
                              int64_t ldvy);
+
+// Householder reduction of the n x n Hermitian A (device, column-major, lower
+// triangle as zhetrd uses it) in one cooperative launch, with Y <- Q^+ Y for
+// nv vectors (qs_tridiag.hip); QS_EUNSUPPORTED, nothing done, when n is too
+// large or the device cannot hold the grid.
+qs_status qs_hetrd_fused(qs_ctx *ctx, int n, double2 *A, double *d, double *e, double2 *tau, int nv, double2 *Y);
 
 // Host-side dense Hermitian eigen-solver (column-major, complex interleaved).
 // A: n*n complex (overwritten), w: n eigenvalues ascending, Z: n*n eigenvectors

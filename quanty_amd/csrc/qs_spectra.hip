@@ -364,12 +364,20 @@ static qs_status dense_poles_householder(qs_ctx *ctx, rocblas_handle hb, int n, 
   double *dd = (double *)(mem + ybytes + (size_t)n * 16), *de = dd + n;
   qs_status st = QS_OK;
   hipError_t e = nv ? hipMemcpyAsync(dY, X, (size_t)n * nv * 16, hipMemcpyDeviceToDevice, ctx->stream) : hipSuccess;
-  if (e == hipSuccess) {
+  // the library's own reduction, which applies Q^+ to Y as it goes; rocSOLVER's
+  // zhetrd and k_apply_qh otherwise
+  bool fused = false;
+  if (e == hipSuccess && ctx->hetrd) {
+    st = qs_hetrd_fused(ctx, n, (double2 *)A, dd, de, dtau, nv, dY);
+    fused = st == QS_OK;
+    if (st == QS_EUNSUPPORTED) st = QS_OK;
+  }
+  if (e == hipSuccess && st == QS_OK && !fused) {
     const rocblas_status rs = rocsolver_zhetrd(hb, rocblas_fill_lower, n, (rocblas_double_complex *)A, n, dd, de,
                                                (rocblas_double_complex *)dtau);
     if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zhetrd failed (status %d)", (int)rs);
   }
-  if (e == hipSuccess && st == QS_OK && nv > 0) {
+  if (e == hipSuccess && st == QS_OK && nv > 0 && !fused) {
     hipLaunchKernelGGL(k_apply_qh, dim3((unsigned)nv), dim3(256), 0, ctx->stream, n, (const double2 *)A,
                        (const double2 *)dtau, dY);
     e = hipGetLastError();
@@ -450,9 +458,17 @@ extern "C" qs_status qs_dense_lowest(qs_ctx *ctx, int n, double *A, int k, doubl
   double2 *dtau = (double2 *)mem;
   double *dd = (double *)(mem + (size_t)n * 16), *de = dd + n;
   qs_status st = QS_OK;
-  const rocblas_status rs = rocsolver_zhetrd(hb, rocblas_fill_lower, n, (rocblas_double_complex *)A, n, dd, de,
-                                             (rocblas_double_complex *)dtau);
-  if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zhetrd failed (status %d)", (int)rs);
+  bool fused = false;
+  if (ctx->hetrd) {
+    st = qs_hetrd_fused(ctx, n, (double2 *)A, dd, de, dtau, 0, nullptr);
+    fused = st == QS_OK;
+    if (st == QS_EUNSUPPORTED) st = QS_OK;
+  }
+  if (st == QS_OK && !fused) {
+    const rocblas_status rs = rocsolver_zhetrd(hb, rocblas_fill_lower, n, (rocblas_double_complex *)A, n, dd, de,
+                                               (rocblas_double_complex *)dtau);
+    if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zhetrd failed (status %d)", (int)rs);
+  }
   std::vector<double> hd(n), he(n);
   hipError_t e = hipSuccess;
   if (st == QS_OK) e = hipMemcpyAsync(hd.data(), dd, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);

@@ -103,11 +103,40 @@ def dense_small(seed: int = 5, reps: int = 5):
     # (Householder, as CreateSpectra uses), or every eigenvector (Jacobi, as
     # KrylovBasis = true needs)
     from .engine import dense_poles
+    from ._lib import QS_PARAM_HETRD
     Hd, t_build = _sync_time(lambda: Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense())
     _, t_poles = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=False))
+    # the same with rocSOLVER's zhetrd + Q^+ X instead of the cooperative reduction
+    Hd = Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense()
+    ctx.set_param(QS_PARAM_HETRD, 0)
+    try:
+        _, t_poles_rs = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=False))
+    finally:
+        ctx.set_param(QS_PARAM_HETRD, 1)
     Hd = Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense()
     _, t_eig = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=True))
     diff = float(np.max(np.abs(sd.G - sl.G)) / np.max(np.abs(sd.G)))
+    # eigenvalues + projections of 3 vectors for random dense complex matrices,
+    # by the cooperative reduction (up to 800 rows) and by rocSOLVER's zhetrd
+    rng = np.random.default_rng(seed)
+    reduction = []
+    for m in (120, 190, 500, 800):
+        B = rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))
+        M = B + B.conj().T
+        Xm = torch.from_numpy(rng.normal(size=(3, m)) + 1j * rng.normal(size=(3, m))).to(vecs.device)
+        row = {"n": m}
+        for mode, key in ((1, "ms_cooperative"), (0, "ms_rocsolver_zhetrd")):
+            ctx.set_param(QS_PARAM_HETRD, mode)
+            try:
+                best = None
+                for _ in range(4):
+                    Mt = torch.from_numpy(np.ascontiguousarray(M.T)).to(vecs.device).t()
+                    _, t = _sync_time(lambda: dense_poles(Mt, Xm, ctx, vectors=False))
+                    best = t if best is None else min(best, t)
+            finally:
+                ctx.set_param(QS_PARAM_HETRD, 1)
+            row[key] = 1e3 * best
+        reduction.append(row)
     # broadening of the spectrum: Gaussian 0.5, Lorentzian from an energy table
     table = [[-10.0, 0.2], [3.0, 0.2], [3.01, 0.6], [20.0, 0.6]]
     sd.Broaden(0.5, table)
@@ -115,7 +144,8 @@ def dense_small(seed: int = 5, reps: int = 5):
     return {"config": "dense", "dim": basis.n, "seconds_dense": td / reps, "seconds_lanczos": tl / reps,
             "seconds_broaden_2001_points": tb / reps,
             "seconds_dense_matrix": t_build, "seconds_householder_poles": t_poles,
-            "seconds_jacobi_all_eigenvectors": t_eig,
+            "seconds_householder_poles_rocsolver_zhetrd": t_poles_rs,
+            "seconds_jacobi_all_eigenvectors": t_eig, "reduction_by_size": reduction,
             "lanczos200_vs_dense_max_rel_diff": diff}
 
 

@@ -538,3 +538,73 @@ def test_create_spectra_block_deflation(cplx):
         assert d < 1e-10, (case, d)
         if case == 2:
             assert [int(k) for k in tab["kept"][1:] if k > 0] == [5, 3]
+
+
+def _hermitian(n, rng, kind):
+    if kind == "complex":
+        B = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        return B + B.conj().T
+    if kind == "real":
+        B = rng.normal(size=(n, n))
+        return (B + B.T).astype(complex)
+    if kind == "blocks":  # reflectors with tau = 0 at every block edge
+        m = max(1, n // 4)
+        A = rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))
+        M = np.zeros((n, n), dtype=complex)
+        for b in range(0, n, m):
+            e = min(n, b + m)
+            M[b:e, b:e] = (A + A.conj().T)[: e - b, : e - b]
+        return M
+    if kind == "diagonal":  # nothing to reduce: every tau is 0
+        return np.diag(rng.normal(size=n)).astype(complex)
+    raise ValueError(kind)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 257, 800])
+@pytest.mark.parametrize("kind", ["complex", "real", "blocks", "diagonal"])
+def test_dense_reduction_cooperative_equals_rocsolver(n, kind, ctx):
+    """The dense method's own reduction (one cooperative launch, Q^+ X formed
+    on the way) against rocSOLVER's zhetrd + Q^+ X and numpy: eigenvalues to
+    1e-12, the weight sums |<z_k|x>|^2 conserved, spectra to 1e-10."""
+    from quanty_amd._lib import QS_PARAM_HETRD
+    rng = np.random.default_rng(1000 * n + len(kind))
+    M = _hermitian(n, rng, kind)
+    Xh = rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))
+    X = torch.from_numpy(Xh).cuda()
+    out = {}
+    try:
+        for mode in (1, 0):
+            ctx.set_param(QS_PARAM_HETRD, mode)
+            T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+            out[mode] = dense_poles(T.t(), X, ctx, vectors=False)
+    finally:
+        ctx.set_param(QS_PARAM_HETRD, 1)
+    (E1, P1), (E0, P0) = out[1], out[0]
+    scale = max(1.0, np.max(np.abs(E0)))
+    assert np.max(np.abs(E1 - np.linalg.eigvalsh(M))) < 1e-12 * scale
+    assert np.max(np.abs(E1 - E0)) < 1e-12 * scale
+    assert np.allclose((np.abs(P1) ** 2).sum(0), (np.abs(Xh) ** 2).sum(1), rtol=1e-12)
+    G1 = pole_spectrum(E1, P1, 0.0, -40.0, 40.0, 300, 0.5, ctx=ctx)
+    G0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 300, 0.5, ctx=ctx)
+    assert np.max(np.abs(G1 - G0)) < 1e-10 * np.max(np.abs(G0))
+    Gt1 = pole_spectrum(E1, P1, 0.0, -40.0, 40.0, 300, 0.5, tensor=True, ctx=ctx)
+    Gt0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 300, 0.5, tensor=True, ctx=ctx)
+    assert np.max(np.abs(Gt1 - Gt0)) < 1e-10 * np.max(np.abs(Gt0))
+
+
+@pytest.mark.parametrize("n,kind", [(2, "complex"), (130, "blocks"), (800, "complex"), (801, "complex")])
+def test_dense_lowest_after_cooperative_reduction(n, kind, ctx):
+    """qs_dense_lowest through the cooperative reduction (n <= 800; rocSOLVER
+    above): the stored reflectors back-transform the tridiagonal eigenvectors
+    into orthonormal eigenvectors of H."""
+    from quanty_amd.engine import dense_lowest
+    rng = np.random.default_rng(n)
+    M = _hermitian(n, rng, kind)
+    k = min(n, 4)
+    T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+    E, V = dense_lowest(T.t(), k, ctx)
+    Z = V.cpu().numpy().T
+    scale = np.max(np.abs(E))
+    assert np.max(np.abs(E - np.linalg.eigvalsh(M))) < 1e-11 * scale
+    assert np.max(np.abs(M @ Z - Z * E[None, :k])) < 1e-10 * scale
+    assert np.max(np.abs(Z.conj().T @ Z - np.eye(k))) < 1e-11
