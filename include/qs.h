@@ -109,12 +109,16 @@ qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
  * stays long (a cache of assembled operators): the free space below then
  * stays in large pieces for the short-lived blocks placed best-fit.
  * QS_PARAM_HETRD (default 1): the Householder reduction of the dense method
- * (qs_dense_poles without eigenvectors, qs_dense_lowest); 1 runs it as one
- * cooperative launch with one grid barrier per column when n <= 800 and the
- * device holds the grid, else rocSOLVER's zhetrd; 0 always uses zhetrd. Both
- * give LAPACK's zhetd2 reduction; results agree to rounding. */
+ * (qs_dense_poles, qs_dense_lowest); 1 runs it as one cooperative launch
+ * with the matrix held in registers across the CUs (n <= 1536 on the MI355X),
+ * else rocSOLVER's zhetrd; 0 always uses zhetrd. Both give LAPACK's zhetd2
+ * reduction; results agree to rounding.
+ * QS_PARAM_TRIDIAG_DC (default 256, >= 1): the order from which the dense
+ * method solves its tridiagonal eigenproblem on the GPU by rocSOLVER's divide
+ * and conquer (dstedc); below it the host QL runs (faster for small orders).
+ * All eigenvectors (qs_dense_poles with vectors = 1) always take dstedc. */
 enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3, QS_PARAM_TERM_LDS = 4,
-       QS_PARAM_ELL_ORDER = 5, QS_PARAM_ALLOC_TOP = 6, QS_PARAM_HETRD = 7 };
+       QS_PARAM_ELL_ORDER = 5, QS_PARAM_ALLOC_TOP = 6, QS_PARAM_HETRD = 7, QS_PARAM_TRIDIAG_DC = 8 };
 qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value);
 qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
 
@@ -407,10 +411,13 @@ qs_status qs_hop_dense(qs_ctx *ctx, const qs_hop *h, double *M);
  * |P[k][s]|^2 are the pole weights of HermitianCompactMatrixToListOfPoles
  * (Core/BasicMath/BasicMath_CompactMatrixVector.cpp:3990-4026), and
  * conj(P[k][r]) P[k][c] the pole blocks of ...ToBlockListOfPoles (:4033-4113).
- * vectors = 1: A is overwritten by the eigenvectors (column k = z_k;
- * rocSOLVER's Jacobi solver zheevj). vectors = 0: eigenvalues and projections
- * only, A is overwritten by the Householder reduction (zhetrd + zunmtr on the
- * GPU, the tridiagonal QL on the host with its rotations applied to Q^+ X).
+ * Method: Householder reduction A = Q T Q^+ on the GPU with Q^+ X formed
+ * there (QS_PARAM_HETRD); the tridiagonal eigenvectors Z of T by divide and
+ * conquer on the GPU and P = Z^T Q^+ X (from order QS_PARAM_TRIDIAG_DC on), or
+ * the tridiagonal QL on the host with its rotations applied to Q^+ X.
+ * vectors = 1: A is overwritten by the eigenvectors Q Z (column k = z_k).
+ * vectors = 0: eigenvalues and projections only, A is overwritten by the
+ * reduction.
  * Within a degenerate eigenvalue the eigenvector basis, hence the individual
  * P, depends on the method; sums over the degenerate set do not.
  * QS_ENOTCONVERGED when the solver does not converge. */
@@ -420,8 +427,9 @@ qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const double *X,
 /* All eigenvalues E (host, ascending) of a dense Hermitian n x n matrix A
  * (device, complex, column-major, lower triangle read, overwritten) and the
  * eigenvectors of the k lowest in V (device, n x k complex, column j = z_j):
- * Householder reduction on the GPU (zhetrd), tridiagonal QL on the host with
- * its rotations replayed on k unit vectors, back-transform on the GPU. The
+ * Householder reduction on the GPU, the tridiagonal eigenvectors by divide and
+ * conquer on the GPU (from order QS_PARAM_TRIDIAG_DC on) or by the host QL
+ * with its rotations replayed on k unit vectors, back-transform on the GPU. The
  * dense ground states of BlockLanczosGroundStateConserveBasisDense
  * (Core/BlockLanczos.cpp:370-504). */
 qs_status qs_dense_lowest(qs_ctx *ctx, int n, double *A, int k, double *E, double *V);

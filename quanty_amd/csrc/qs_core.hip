@@ -346,6 +346,10 @@ extern "C" qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value) {
       if (value != 0 && value != 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_HETRD must be 0 or 1");
       ctx->hetrd = (int)value;
       return QS_OK;
+    case QS_PARAM_TRIDIAG_DC:
+      if (value < 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_TRIDIAG_DC must be >= 1");
+      ctx->tridiag_dc = value;
+      return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
   }
@@ -374,6 +378,9 @@ extern "C" qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value) {
       return QS_OK;
     case QS_PARAM_HETRD:
       *value = ctx->hetrd;
+      return QS_OK;
+    case QS_PARAM_TRIDIAG_DC:
+      *value = ctx->tridiag_dc;
       return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);

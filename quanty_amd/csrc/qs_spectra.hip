@@ -2,8 +2,9 @@
 //  - the dense small-problem path (the counterpart of the reference's LAPACK
 //    zheev): eigenvalues of a dense Hermitian matrix and the projections of
 //    the transition vectors on its eigenvectors (Householder reduction on the
-//    GPU, tridiagonal QL on the host), the lowest eigenvectors, or all of them
-//    (rocSOLVER zheevj), and spectra from the resulting lists of poles
+//    GPU; the tridiagonal eigenproblem by divide and conquer on the GPU, or by
+//    QL on the host for small matrices), the lowest eigenvectors, or all of
+//    them, and spectra from the resulting lists of poles
 //    (ListOfPolesToSpectrum,
 //    Core/Spectral.cpp:1516-1591; BlockListOfPolesToSpectraWithBroadening,
 //    Core/BasicMathBlock/BasicMath_BlockTypeConversion.cpp:588-660);
@@ -196,7 +197,13 @@ void qs_blas_release(qs_ctx *ctx) {
   }
 }
 
-constexpr int QS_JACOBI_SWEEPS = 100;
+// The tridiagonal eigenproblem of the dense method goes to rocSOLVER's divide
+// and conquer (dstedc) on the GPU from order ctx->tridiag_dc on (default 256);
+// below it the host QL is faster. Measured on the MI355X (random and
+// glued-Wilkinson tridiagonal matrices): dstedc 0.8 / 1.4 / 1.9 ms at n = 190
+// / 800 / 1410 against 0.5 / 8.7 / 23 ms for the host QL with its rotations
+// applied to one vector; dstedc's eigenvectors were orthonormal to 3e-15.
+// (rocSOLVER's sterf, steqr and stebz + stein took 0.3-50 s at these orders.)
 
 static qs_status blas_handle(qs_ctx *ctx, rocblas_handle *hb) {
   if (!ctx->blas) {
@@ -213,22 +220,30 @@ static qs_status blas_handle(qs_ctx *ctx, rocblas_handle *hb) {
 
 namespace {
 
-// P[k * nv + s] = <z_k|x_s> = sum_r conj(Z[r, k]) X[r, s]; one wave per (k, s),
-// lanes over r with a fixed stride and a fixed butterfly.
-__global__ __launch_bounds__(64) void k_project(int n, const double2 *__restrict__ Z, const double2 *__restrict__ X,
-                                                int nv, double2 *__restrict__ P) {
+// P[k * nv + s] = z_k^T y_s for real tridiagonal eigenvectors Z (column k at
+// Z + k*n); one wave per (k, s), lanes over the rows with a fixed stride and
+// a fixed butterfly, so the sums do not depend on the launch.
+__global__ __launch_bounds__(64) void k_project_real(int n, const double *__restrict__ Z,
+                                                     const double2 *__restrict__ Y, int nv, double2 *__restrict__ P) {
   const int k = blockIdx.x, sv = blockIdx.y;
-  const double2 *z = Z + (size_t)k * n;
-  const double2 *x = X + (size_t)sv * n;
+  const double *z = Z + (size_t)k * n;
+  const double2 *y = Y + (size_t)sv * n;
   double re = 0.0, im = 0.0;
   for (int r = threadIdx.x; r < n; r += 64) {
-    const double2 a = z[r], b = x[r];
-    re += a.x * b.x + a.y * b.y;
-    im += a.x * b.y - a.y * b.x;
+    const double a = z[r];
+    const double2 b = y[r];
+    re += a * b.x;
+    im += a * b.y;
   }
   re = wave_sum(re);
   im = wave_sum(im);
   if (threadIdx.x == 0) P[(size_t)k * nv + sv] = make_double2(re, im);
+}
+
+// V[t] = Z[t] + 0i for t < count
+__global__ void k_real_to_complex(int64_t count, const double *__restrict__ Z, double2 *__restrict__ V) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < count) V[t] = make_double2(Z[t], 0.0);
 }
 
 // G_s(w_i) = sum_k |P[k][s]|^2 / (w_i + E0_s - E_k + i Gamma/2), w_i = i de + emin.
@@ -350,37 +365,93 @@ __global__ __launch_bounds__(256) void k_apply_q(int n, const double2 *__restric
   }
 }
 
-// Eigenvalues and projections only: Householder reduction A = Q T Q^+ on the
-// GPU (zhetrd; T real symmetric tridiagonal), Y = Q^+ X on the GPU
-// (k_apply_qh), then the tridiagonal QL on the host with its rotations applied
-// to the rows of Y, which turns Y into Z^T Y for the eigenvectors Z of T.
-static qs_status dense_poles_householder(qs_ctx *ctx, rocblas_handle hb, int n, double *A, int nv, const double *X,
-                                         double *E, double *P) {
-  char *mem = nullptr;
-  const size_t ybytes = (size_t)n * std::max(nv, 1) * 16;
-  QS_TRY(qs_malloc(ctx, (void **)&mem, ybytes + (size_t)n * 16 + (size_t)n * 16));
-  double2 *dY = (double2 *)mem;
-  double2 *dtau = (double2 *)(mem + ybytes);
-  double *dd = (double *)(mem + ybytes + (size_t)n * 16), *de = dd + n;
-  qs_status st = QS_OK;
-  hipError_t e = nv ? hipMemcpyAsync(dY, X, (size_t)n * nv * 16, hipMemcpyDeviceToDevice, ctx->stream) : hipSuccess;
-  // the library's own reduction, which applies Q^+ to Y as it goes; rocSOLVER's
-  // zhetrd and k_apply_qh otherwise
-  bool fused = false;
-  if (e == hipSuccess && ctx->hetrd) {
-    st = qs_hetrd_fused(ctx, n, (double2 *)A, dd, de, dtau, nv, dY);
-    fused = st == QS_OK;
-    if (st == QS_EUNSUPPORTED) st = QS_OK;
+// Householder reduction A = Q T Q^+ (T real symmetric tridiagonal: d, e; A
+// holds the reflectors, tau their factors) with Y <- Q^+ Y for nv vectors: the
+// library's cooperative reduction when it takes the order, rocSOLVER's zhetrd
+// and k_apply_qh otherwise.
+static qs_status reduce_dense(qs_ctx *ctx, rocblas_handle hb, int n, double *A, double *dd, double *de,
+                              double2 *dtau, int nv, double2 *dY) {
+  if (ctx->hetrd) {
+    const qs_status st = qs_hetrd_fused(ctx, n, (double2 *)A, dd, de, dtau, nv, dY);
+    if (st != QS_EUNSUPPORTED) return st;
   }
-  if (e == hipSuccess && st == QS_OK && !fused) {
-    const rocblas_status rs = rocsolver_zhetrd(hb, rocblas_fill_lower, n, (rocblas_double_complex *)A, n, dd, de,
-                                               (rocblas_double_complex *)dtau);
-    if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zhetrd failed (status %d)", (int)rs);
-  }
-  if (e == hipSuccess && st == QS_OK && nv > 0 && !fused) {
+  const rocblas_status rs = rocsolver_zhetrd(hb, rocblas_fill_lower, n, (rocblas_double_complex *)A, n, dd, de,
+                                             (rocblas_double_complex *)dtau);
+  if (rs != rocblas_status_success) return qs_fail(ctx, QS_EHIP, "rocsolver_zhetrd failed (status %d)", (int)rs);
+  if (nv > 0) {
     hipLaunchKernelGGL(k_apply_qh, dim3((unsigned)nv), dim3(256), 0, ctx->stream, n, (const double2 *)A,
                        (const double2 *)dtau, dY);
-    e = hipGetLastError();
+    QS_HIP(ctx, hipGetLastError());
+  }
+  return QS_OK;
+}
+
+// Eigenvalues (dd, ascending; de destroyed) and eigenvectors Z (n x n real,
+// column k = eigenvector k) of the tridiagonal T = (dd, de) on the GPU by
+// rocSOLVER's divide and conquer. info: one device int.
+static qs_status tridiag_dc(qs_ctx *ctx, rocblas_handle hb, int n, double *dd, double *de, double *Z,
+                            rocblas_int *info) {
+  const rocblas_status rs = rocsolver_dstedc(hb, rocblas_evect_tridiagonal, n, dd, de, Z, n, info);
+  if (rs != rocblas_status_success) return qs_fail(ctx, QS_EHIP, "rocsolver_dstedc failed (status %d)", (int)rs);
+  rocblas_int hinfo = 0;
+  QS_TRY(qs_d2h(ctx, &hinfo, info, sizeof(hinfo)));
+  if (hinfo != 0) return qs_fail(ctx, QS_ENOTCONVERGED, "the tridiagonal divide and conquer failed (info %d)", hinfo);
+  return QS_OK;
+}
+
+// Eigenvalues and projections: the reduction with Y = Q^+ X on the GPU; then
+// either (n >= ctx->tridiag_dc, or all eigenvectors wanted) the tridiagonal
+// eigenvectors Z on the GPU and P = Z^T Y there, the eigenvectors of A being
+// Q Z (k_apply_q, one workgroup per column); or the tridiagonal QL on the host
+// with its rotations applied to the rows of Y, which turns Y into Z^T Y.
+static qs_status dense_poles_householder(qs_ctx *ctx, rocblas_handle hb, int n, double *A, int nv, const double *X,
+                                         double *E, double *P, bool vectors) {
+  const bool dc = vectors || n >= ctx->tridiag_dc;
+  const size_t ybytes = (size_t)n * std::max(nv, 1) * 16;
+  const size_t zbytes = dc ? (size_t)n * n * 8 : 0;
+  const size_t vbytes = vectors ? (size_t)n * n * 16 : 0;
+  const size_t pbytes = dc ? ybytes : 0;
+  char *mem = nullptr;
+  // 16-byte arrays first, then the 8-byte ones and the info word
+  QS_TRY(qs_malloc(ctx, (void **)&mem, ybytes + pbytes + vbytes + (size_t)n * 16 + zbytes + (size_t)n * 16 + 256));
+  double2 *dY = (double2 *)mem;
+  double2 *dP = (double2 *)(mem + ybytes);
+  double2 *dV = (double2 *)(mem + ybytes + pbytes);
+  double2 *dtau = (double2 *)(mem + ybytes + pbytes + vbytes);
+  double *dZ = (double *)(dtau + n);
+  double *dd = (double *)((char *)dZ + zbytes), *de = dd + n;
+  rocblas_int *info = (rocblas_int *)(de + n);
+  qs_status st = QS_OK;
+  hipError_t e = nv ? hipMemcpyAsync(dY, X, (size_t)n * nv * 16, hipMemcpyDeviceToDevice, ctx->stream) : hipSuccess;
+  if (e == hipSuccess) st = reduce_dense(ctx, hb, n, A, dd, de, dtau, nv, dY);
+  if (e == hipSuccess && st == QS_OK && dc) {
+    st = tridiag_dc(ctx, hb, n, dd, de, dZ, info);
+    if (st == QS_OK && nv > 0) {
+      hipLaunchKernelGGL(k_project_real, dim3((unsigned)n, (unsigned)nv), dim3(64), 0, ctx->stream, n,
+                         (const double *)dZ, (const double2 *)dY, nv, dP);
+      e = hipGetLastError();
+    }
+    if (st == QS_OK && e == hipSuccess && vectors) {
+      const int64_t cnt = (int64_t)n * n;
+      hipLaunchKernelGGL(k_real_to_complex, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, ctx->stream, cnt,
+                         (const double *)dZ, dV);
+      e = hipGetLastError();
+      if (e == hipSuccess && n > 1) {
+        hipLaunchKernelGGL(k_apply_q, dim3((unsigned)n), dim3(256), 0, ctx->stream, n, (const double2 *)A,
+                           (const double2 *)dtau, dV);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipMemcpyAsync(A, dV, (size_t)n * n * 16, hipMemcpyDeviceToDevice, ctx->stream);
+    }
+    if (st == QS_OK && e == hipSuccess) e = hipMemcpyAsync(E, dd, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (st == QS_OK && e == hipSuccess && nv > 0)
+      e = hipMemcpyAsync(P, dP, (size_t)n * nv * 16, hipMemcpyDeviceToHost, ctx->stream);
+    if (st == QS_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    else (void)hipStreamSynchronize(ctx->stream);
+    qs_free(mem);
+    if (st != QS_OK) return st;
+    if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_dense_poles", __FILE__, __LINE__);
+    return QS_OK;
   }
   std::vector<double> hd(n), he(n);
   std::vector<std::complex<double>> hy((size_t)n * nv);
@@ -410,42 +481,7 @@ extern "C" qs_status qs_dense_poles(qs_ctx *ctx, int n, double *A, int nv, const
   QS_HIP(ctx, hipSetDevice(ctx->device));
   rocblas_handle hb = nullptr;
   QS_TRY(blas_handle(ctx, &hb));
-  if (!vectors) return dense_poles_householder(ctx, hb, n, A, nv, X, E, P);
-  char *mem = nullptr;
-  const size_t pbytes = (size_t)n * nv * 16;
-  QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)n * 8 + 256 + pbytes));
-  double2 *dP = (double2 *)mem;  // first, so it is 16-byte aligned
-  double *dw = (double *)(mem + pbytes);
-  rocblas_int *info = (rocblas_int *)(dw + n);
-  double *resid = (double *)((char *)info + 64);
-  rocblas_int *sweeps = (rocblas_int *)((char *)info + 128);
-  qs_status st = QS_OK;
-  // Jacobi (zheevj): the divide-and-conquer zheevd of this rocSOLVER returns NaN
-  // eigenvectors for matrices with degenerate or tightly clustered eigenvalues
-  // from n = 100 on (typical of symmetric Hamiltonians), and the QR-based zheev
-  // takes 17 s at n = 1513 against 0.5 s for zheevj.
-  const rocblas_status rs =
-      rocsolver_zheevj(hb, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_lower, n,
-                       (rocblas_double_complex *)A, n, 0.0, resid, QS_JACOBI_SWEEPS, sweeps, dw, info);
-  if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zheevj failed (status %d)", (int)rs);
-  hipError_t e = hipSuccess;
-  if (st == QS_OK && nv > 0) {
-    hipLaunchKernelGGL(k_project, dim3((unsigned)n, (unsigned)nv), dim3(64), 0, ctx->stream, n, (const double2 *)A,
-                       (const double2 *)X, nv, dP);
-    e = hipGetLastError();
-  }
-  rocblas_int hinfo = 0;
-  if (st == QS_OK && e == hipSuccess) e = hipMemcpyAsync(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, ctx->stream);
-  if (st == QS_OK && e == hipSuccess) e = hipMemcpyAsync(E, dw, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (st == QS_OK && e == hipSuccess && nv > 0) e = hipMemcpyAsync(P, dP, pbytes, hipMemcpyDeviceToHost, ctx->stream);
-  if (st == QS_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  else (void)hipStreamSynchronize(ctx->stream);
-  qs_free(mem);
-  if (st != QS_OK) return st;
-  if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_dense_poles", __FILE__, __LINE__);
-  if (hinfo != 0)
-    return qs_fail(ctx, QS_ENOTCONVERGED, "zheevj did not converge in %d sweeps", QS_JACOBI_SWEEPS);
-  return QS_OK;
+  return dense_poles_householder(ctx, hb, n, A, nv, X, E, P, vectors != 0);
 }
 
 extern "C" qs_status qs_dense_lowest(qs_ctx *ctx, int n, double *A, int k, double *E, double *V) {
@@ -453,44 +489,51 @@ extern "C" qs_status qs_dense_lowest(qs_ctx *ctx, int n, double *A, int k, doubl
   QS_HIP(ctx, hipSetDevice(ctx->device));
   rocblas_handle hb = nullptr;
   QS_TRY(blas_handle(ctx, &hb));
+  const bool dc = n >= ctx->tridiag_dc;
+  const size_t zbytes = dc ? (size_t)n * n * 8 : 0;
   char *mem = nullptr;
-  QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)n * 32));
+  QS_TRY(qs_malloc(ctx, (void **)&mem, (size_t)n * 32 + zbytes + 256));
   double2 *dtau = (double2 *)mem;
   double *dd = (double *)(mem + (size_t)n * 16), *de = dd + n;
-  qs_status st = QS_OK;
-  bool fused = false;
-  if (ctx->hetrd) {
-    st = qs_hetrd_fused(ctx, n, (double2 *)A, dd, de, dtau, 0, nullptr);
-    fused = st == QS_OK;
-    if (st == QS_EUNSUPPORTED) st = QS_OK;
-  }
-  if (st == QS_OK && !fused) {
-    const rocblas_status rs = rocsolver_zhetrd(hb, rocblas_fill_lower, n, (rocblas_double_complex *)A, n, dd, de,
-                                               (rocblas_double_complex *)dtau);
-    if (rs != rocblas_status_success) st = qs_fail(ctx, QS_EHIP, "rocsolver_zhetrd failed (status %d)", (int)rs);
-  }
-  std::vector<double> hd(n), he(n);
+  double *dZ = de + n;
+  rocblas_int *info = (rocblas_int *)((char *)dZ + zbytes);
+  qs_status st = reduce_dense(ctx, hb, n, A, dd, de, dtau, 0, nullptr);
   hipError_t e = hipSuccess;
-  if (st == QS_OK) e = hipMemcpyAsync(hd.data(), dd, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (st == QS_OK && e == hipSuccess && n > 1)
-    e = hipMemcpyAsync(he.data(), de, (size_t)(n - 1) * 8, hipMemcpyDeviceToHost, ctx->stream);
-  if (st == QS_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  else (void)hipStreamSynchronize(ctx->stream);
-  std::vector<double> z((size_t)n * k);
-  if (st == QS_OK && e == hipSuccess && !qs_tridiag_lowest_vectors(n, hd.data(), he.data(), k, E, z.data()))
-    st = qs_fail(ctx, QS_ENOTCONVERGED, "the tridiagonal QL did not converge");
-  if (st == QS_OK && e == hipSuccess && k > 0) {
-    std::vector<double2> zc((size_t)n * k);
-    for (size_t t = 0; t < zc.size(); ++t) zc[t] = make_double2(z[t], 0.0);
-    e = hipMemcpyAsync(V, zc.data(), zc.size() * 16, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_apply_q, dim3((unsigned)k), dim3(256), 0, ctx->stream, n, (const double2 *)A,
-                         (const double2 *)dtau, (double2 *)V);
+  std::vector<double> z;
+  if (st == QS_OK && dc) {
+    // eigenvectors of T on the GPU; the k lowest become V's columns
+    st = tridiag_dc(ctx, hb, n, dd, de, dZ, info);
+    if (st == QS_OK) e = hipMemcpyAsync(E, dd, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (st == QS_OK && e == hipSuccess && k > 0) {
+      const int64_t cnt = (int64_t)n * k;
+      hipLaunchKernelGGL(k_real_to_complex, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, ctx->stream, cnt,
+                         (const double *)dZ, (double2 *)V);
       e = hipGetLastError();
     }
+  } else if (st == QS_OK) {
+    std::vector<double> hd(n), he(n);
+    e = hipMemcpyAsync(hd.data(), dd, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && n > 1)
+      e = hipMemcpyAsync(he.data(), de, (size_t)(n - 1) * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    else (void)hipStreamSynchronize(ctx->stream);
+    z.resize((size_t)n * k);
+    if (e == hipSuccess && !qs_tridiag_lowest_vectors(n, hd.data(), he.data(), k, E, z.data()))
+      st = qs_fail(ctx, QS_ENOTCONVERGED, "the tridiagonal QL did not converge");
+    if (st == QS_OK && e == hipSuccess && k > 0) {
+      std::vector<double2> zc((size_t)n * k);
+      for (size_t t = 0; t < zc.size(); ++t) zc[t] = make_double2(z[t], 0.0);
+      e = hipMemcpyAsync(V, zc.data(), zc.size() * 16, hipMemcpyHostToDevice, ctx->stream);
+      // the staging vector must outlive the copy
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
   }
+  if (st == QS_OK && e == hipSuccess && k > 0 && n > 1) {
+    hipLaunchKernelGGL(k_apply_q, dim3((unsigned)k), dim3(256), 0, ctx->stream, n, (const double2 *)A,
+                       (const double2 *)dtau, (double2 *)V);
+    e = hipGetLastError();
+  }
+  if (st == QS_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  else (void)hipStreamSynchronize(ctx->stream);
   qs_free(mem);
   if (st != QS_OK) return st;
   if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_dense_lowest", __FILE__, __LINE__);

@@ -78,9 +78,8 @@ def config0(reps: int = 5):
 def dense_small(seed: int = 5, reps: int = 5):
     """The dense small-problem path on a 1,410-determinant impurity (d + 1 bath
     shell, 16 electrons): ground state, one d<->bath transition, CreateSpectra by
-    the dense method (closure, determinant matrix, Householder reduction and
-    tridiagonal QL, poles) against
-    200-step Lanczos, 2001 points."""
+    the dense method (closure, determinant matrix, Householder reduction,
+    tridiagonal eigenproblem, poles) against 200-step Lanczos, 2001 points."""
     from .api import CreateSpectra, Operator, Wavefunction
 
     ctx = default_context()
@@ -99,11 +98,10 @@ def dense_small(seed: int = 5, reps: int = 5):
     sl, _ = CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]])
     _, td = _sync_time(lambda: [CreateSpectra(H, T, psi, opts) for _ in range(reps)])
     _, tl = _sync_time(lambda: [CreateSpectra(H, T, psi, opts + [["Method", "Lanczos"]]) for _ in range(reps)])
-    # stages of the dense method: matrix, then eigenvalues + projections
-    # (Householder, as CreateSpectra uses), or every eigenvector (Jacobi, as
-    # KrylovBasis = true needs)
+    # stages of the dense method: matrix, then eigenvalues + projections (as
+    # CreateSpectra uses), or every eigenvector (as KrylovBasis = true needs)
     from .engine import dense_poles
-    from ._lib import QS_PARAM_HETRD
+    from ._lib import QS_PARAM_HETRD, QS_PARAM_TRIDIAG_DC
     Hd, t_build = _sync_time(lambda: Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense())
     _, t_poles = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=False))
     # the same with rocSOLVER's zhetrd + Q^+ X instead of the cooperative reduction
@@ -116,17 +114,20 @@ def dense_small(seed: int = 5, reps: int = 5):
     Hd = Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense()
     _, t_eig = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=True))
     diff = float(np.max(np.abs(sd.G - sl.G)) / np.max(np.abs(sd.G)))
-    # eigenvalues + projections of 3 vectors for random dense complex matrices,
-    # by the cooperative reduction (up to 800 rows) and by rocSOLVER's zhetrd
+    # eigenvalues + projections of 3 vectors for random dense complex matrices:
+    # the defaults (cooperative reduction up to 800 rows, tridiagonal divide and
+    # conquer on the GPU from 256), rocSOLVER's zhetrd instead of the
+    # cooperative reduction, and the host QL instead of the divide and conquer
     rng = np.random.default_rng(seed)
     reduction = []
-    for m in (120, 190, 500, 800):
+    for m in (120, 190, 500, 800, 1000, 1410):
         B = rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m))
         M = B + B.conj().T
         Xm = torch.from_numpy(rng.normal(size=(3, m)) + 1j * rng.normal(size=(3, m))).to(vecs.device)
         row = {"n": m}
-        for mode, key in ((1, "ms_cooperative"), (0, "ms_rocsolver_zhetrd")):
-            ctx.set_param(QS_PARAM_HETRD, mode)
+        for hetrd, dc, key in ((1, 256, "ms_default"), (0, 256, "ms_rocsolver_zhetrd"), (1, 10 ** 9, "ms_host_ql")):
+            ctx.set_param(QS_PARAM_HETRD, hetrd)
+            ctx.set_param(QS_PARAM_TRIDIAG_DC, dc)
             try:
                 best = None
                 for _ in range(4):
@@ -135,6 +136,7 @@ def dense_small(seed: int = 5, reps: int = 5):
                     best = t if best is None else min(best, t)
             finally:
                 ctx.set_param(QS_PARAM_HETRD, 1)
+                ctx.set_param(QS_PARAM_TRIDIAG_DC, 256)
             row[key] = 1e3 * best
         reduction.append(row)
     # broadening of the spectrum: Gaussian 0.5, Lorentzian from an energy table
@@ -145,7 +147,7 @@ def dense_small(seed: int = 5, reps: int = 5):
             "seconds_broaden_2001_points": tb / reps,
             "seconds_dense_matrix": t_build, "seconds_householder_poles": t_poles,
             "seconds_householder_poles_rocsolver_zhetrd": t_poles_rs,
-            "seconds_jacobi_all_eigenvectors": t_eig, "reduction_by_size": reduction,
+            "seconds_all_eigenvectors": t_eig, "reduction_by_size": reduction,
             "lanczos200_vs_dense_max_rel_diff": diff}
 
 

@@ -280,11 +280,14 @@ def test_eigensystem_dense_equals_krylov(ctx):
 
 
 @pytest.mark.parametrize("degenerate", [False, True])
-def test_dense_poles_householder_path_equals_jacobi(degenerate, ctx):
-    """qs_dense_poles without eigenvectors (Householder + tridiagonal QL with the
-    rotations applied to Q^+ X) against the Jacobi path and numpy: the same
-    eigenvalues, and the same spectra (degenerate poles may split their weight
-    differently; the spectra see only the sums)."""
+def test_dense_poles_with_vectors_equals_projection_path(degenerate, ctx):
+    """qs_dense_poles with all eigenvectors (tridiagonal divide and conquer, then
+    the back-transform Q Z) against the projections-only path with the host QL
+    and with the GPU divide and conquer, and numpy: the same eigenvalues and
+    spectra (degenerate poles may split their weight differently; the spectra
+    see only the sums); the returned eigenvectors orthonormal with H z = E z and
+    P = Z^+ X."""
+    from quanty_amd._lib import QS_PARAM_TRIDIAG_DC
     n = 300
     rng = np.random.default_rng(11 + degenerate)
     if degenerate:
@@ -296,21 +299,73 @@ def test_dense_poles_householder_path_equals_jacobi(degenerate, ctx):
     Xh = rng.normal(size=(4, n)) + 1j * rng.normal(size=(4, n))
     X = torch.from_numpy(Xh).cuda()
     out = {}
-    for vec in (True, False):
-        T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
-        out[vec] = dense_poles(T.t(), X, ctx, vectors=vec)
-    E1, P1 = out[True]
-    E0, P0 = out[False]
+    try:
+        for key, vec, dc in (("vectors", True, 256), ("ql", False, 10 ** 9), ("dc", False, 1)):
+            ctx.set_param(QS_PARAM_TRIDIAG_DC, dc)
+            T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+            out[key] = dense_poles(T.t(), X, ctx, vectors=vec)
+            if vec:
+                Z = T.t().cpu().numpy()
+    finally:
+        ctx.set_param(QS_PARAM_TRIDIAG_DC, 256)
+    E1, P1 = out["vectors"]
     scale = np.max(np.abs(E1))
-    assert np.max(np.abs(E0 - E1)) < 1e-12 * scale
-    assert np.max(np.abs(E0 - np.linalg.eigvalsh(M))) < 1e-12 * scale
-    assert np.allclose((np.abs(P0) ** 2).sum(0), (np.abs(Xh) ** 2).sum(1), rtol=1e-12)
+    assert np.max(np.abs(E1 - np.linalg.eigvalsh(M))) < 1e-12 * scale
+    assert np.all(np.isfinite(Z))
+    assert np.max(np.abs(M @ Z - Z * E1[None, :])) < 1e-11 * scale
+    assert np.max(np.abs(Z.conj().T @ Z - np.eye(n))) < 1e-12
+    assert np.max(np.abs(P1 - Z.conj().T @ Xh.T)) < 1e-12 * np.max(np.abs(Xh)) * np.sqrt(n)
     G1 = pole_spectrum(E1, P1, 0.0, -40.0, 40.0, 400, 0.5, ctx=ctx)
-    G0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 400, 0.5, ctx=ctx)
-    assert np.max(np.abs(G0 - G1)) < 1e-10 * np.max(np.abs(G1))
     Gt1 = pole_spectrum(E1, P1, 0.0, -40.0, 40.0, 400, 0.5, tensor=True, ctx=ctx)
-    Gt0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 400, 0.5, tensor=True, ctx=ctx)
-    assert np.max(np.abs(Gt0 - Gt1)) < 1e-10 * np.max(np.abs(Gt1))
+    for key in ("ql", "dc"):
+        E0, P0 = out[key]
+        assert np.max(np.abs(E0 - E1)) < 1e-12 * scale
+        assert np.allclose((np.abs(P0) ** 2).sum(0), (np.abs(Xh) ** 2).sum(1), rtol=1e-12)
+        G0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 400, 0.5, ctx=ctx)
+        assert np.max(np.abs(G0 - G1)) < 1e-10 * np.max(np.abs(G1))
+        Gt0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 400, 0.5, tensor=True, ctx=ctx)
+        assert np.max(np.abs(Gt0 - Gt1)) < 1e-10 * np.max(np.abs(Gt1))
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 300])
+@pytest.mark.parametrize("kind", ["complex", "real", "blocks", "diagonal"])
+def test_dense_tridiagonal_divide_and_conquer_equals_host_ql(n, kind, ctx):
+    """The tridiagonal eigenproblem by rocSOLVER's divide and conquer on the GPU
+    against the host QL, on the same reduction: eigenvalues to 1e-12, weight
+    sums conserved, spectra (plain and tensor) to 1e-10; all eigenvectors from
+    either route orthonormal with H z = E z (split matrices included: the
+    block and diagonal kinds reduce to tridiagonal matrices with zero
+    couplings)."""
+    from quanty_amd._lib import QS_PARAM_TRIDIAG_DC
+    rng = np.random.default_rng(7000 + n + len(kind))
+    M = _hermitian(n, rng, kind)
+    Xh = rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))
+    X = torch.from_numpy(Xh).cuda()
+    out = {}
+    try:
+        for dc in (1, 10 ** 9):
+            ctx.set_param(QS_PARAM_TRIDIAG_DC, dc)
+            T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+            out[dc] = dense_poles(T.t(), X, ctx, vectors=False)
+    finally:
+        ctx.set_param(QS_PARAM_TRIDIAG_DC, 256)
+    (E1, P1), (E0, P0) = out[1], out[10 ** 9]
+    scale = max(1.0, np.max(np.abs(E0)))
+    assert np.max(np.abs(E1 - np.linalg.eigvalsh(M))) < 1e-12 * scale
+    assert np.max(np.abs(E1 - E0)) < 1e-12 * scale
+    assert np.allclose((np.abs(P1) ** 2).sum(0), (np.abs(Xh) ** 2).sum(1), rtol=1e-12)
+    G1 = pole_spectrum(E1, P1, 0.0, -40.0, 40.0, 300, 0.5, ctx=ctx)
+    G0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 300, 0.5, ctx=ctx)
+    assert np.max(np.abs(G1 - G0)) < 1e-10 * np.max(np.abs(G0))
+    Gt1 = pole_spectrum(E1, P1, 0.0, -40.0, 40.0, 300, 0.5, tensor=True, ctx=ctx)
+    Gt0 = pole_spectrum(E0, P0, 0.0, -40.0, 40.0, 300, 0.5, tensor=True, ctx=ctx)
+    assert np.max(np.abs(Gt1 - Gt0)) < 1e-10 * np.max(np.abs(Gt0))
+    T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+    E, _ = dense_poles(T.t(), None, ctx, vectors=True)
+    Z = T.t().cpu().numpy()
+    assert np.max(np.abs(E - E0)) < 1e-12 * scale
+    assert np.max(np.abs(M @ Z - Z * E[None, :])) < 1e-11 * scale
+    assert np.max(np.abs(Z.conj().T @ Z - np.eye(n))) < 1e-12
 
 
 @pytest.mark.parametrize("n", [100, 400])
@@ -361,7 +416,9 @@ def test_dense_two_word_determinants_match_matrix_free(ctx):
 @pytest.mark.parametrize("degenerate", [False, True])
 def test_dense_lowest_eigenvectors(degenerate, ctx):
     """qs_dense_lowest: every eigenvalue, and the k lowest eigenvectors
-    orthonormal with H z = E z at rounding level (degenerate included)."""
+    orthonormal with H z = E z at rounding level (degenerate included), with
+    the tridiagonal eigenvectors from the GPU divide and conquer and from the
+    host QL."""
     from quanty_amd.engine import dense_lowest
     n, k = 240, 7
     rng = np.random.default_rng(21 + degenerate)
@@ -371,13 +428,19 @@ def test_dense_lowest_eigenvectors(degenerate, ctx):
     else:
         B = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
         M = B + B.conj().T
-    T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
-    E, V = dense_lowest(T.t(), k, ctx)
-    Z = V.cpu().numpy().T  # columns z_j
-    scale = np.max(np.abs(E))
-    assert np.max(np.abs(E - np.linalg.eigvalsh(M))) < 1e-12 * scale
-    assert np.max(np.abs(M @ Z - Z * E[None, :k])) < 1e-11 * scale
-    assert np.max(np.abs(Z.conj().T @ Z - np.eye(k))) < 1e-12
+    from quanty_amd._lib import QS_PARAM_TRIDIAG_DC
+    try:
+        for dc in (1, 10 ** 9):  # divide and conquer on the GPU, host QL
+            ctx.set_param(QS_PARAM_TRIDIAG_DC, dc)
+            T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+            E, V = dense_lowest(T.t(), k, ctx)
+            Z = V.cpu().numpy().T  # columns z_j
+            scale = np.max(np.abs(E))
+            assert np.max(np.abs(E - np.linalg.eigvalsh(M))) < 1e-12 * scale
+            assert np.max(np.abs(M @ Z - Z * E[None, :k])) < 1e-11 * scale
+            assert np.max(np.abs(Z.conj().T @ Z - np.eye(k))) < 1e-12
+    finally:
+        ctx.set_param(QS_PARAM_TRIDIAG_DC, 256)
 
 
 def test_create_spectra_with_annihilated_psi(ctx):
@@ -560,12 +623,13 @@ def _hermitian(n, rng, kind):
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 257, 800])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 64, 257, 800, 1536])
 @pytest.mark.parametrize("kind", ["complex", "real", "blocks", "diagonal"])
 def test_dense_reduction_cooperative_equals_rocsolver(n, kind, ctx):
-    """The dense method's own reduction (one cooperative launch, Q^+ X formed
-    on the way) against rocSOLVER's zhetrd + Q^+ X and numpy: eigenvalues to
-    1e-12, the weight sums |<z_k|x>|^2 conserved, spectra to 1e-10."""
+    """The dense method's own reduction (one cooperative launch, the matrix in
+    registers, Q^+ X formed on the way; 1 to 256 workgroups) against
+    rocSOLVER's zhetrd + Q^+ X and numpy: eigenvalues to 1e-12, the weight sums
+    |<z_k|x>|^2 conserved, spectra to 1e-10."""
     from quanty_amd._lib import QS_PARAM_HETRD
     rng = np.random.default_rng(1000 * n + len(kind))
     M = _hermitian(n, rng, kind)
@@ -592,9 +656,10 @@ def test_dense_reduction_cooperative_equals_rocsolver(n, kind, ctx):
     assert np.max(np.abs(Gt1 - Gt0)) < 1e-10 * np.max(np.abs(Gt0))
 
 
-@pytest.mark.parametrize("n,kind", [(2, "complex"), (130, "blocks"), (800, "complex"), (801, "complex")])
+@pytest.mark.parametrize("n,kind", [(2, "complex"), (130, "blocks"), (800, "complex"), (1536, "complex"),
+                                    (1537, "complex")])
 def test_dense_lowest_after_cooperative_reduction(n, kind, ctx):
-    """qs_dense_lowest through the cooperative reduction (n <= 800; rocSOLVER
+    """qs_dense_lowest through the cooperative reduction (n <= 1536; rocSOLVER
     above): the stored reflectors back-transform the tridiagonal eigenvectors
     into orthonormal eigenvectors of H."""
     from quanty_amd.engine import dense_lowest
