@@ -319,7 +319,9 @@ qs_status qs_block_lanczos(qs_ctx *ctx, const qs_hop *h, int k, const double *V0
  * bit-identical whether it runs alone or with others. */
 typedef struct qs_batch qs_batch;
 qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out);
-/* Start a recurrence of `depth` steps on an assembled h (rows == cols) from
+/* Start a recurrence of `depth` steps on an assembled h (rows == cols; the
+ * sliced ELL or the product layout, whose slot keeps its vectors in the
+ * layout's row order while v0, Ritz vectors and y stay canonical) from
  * device vector v0 (copied); max_depth (>= depth) sizes the coefficient arrays
  * so qs_batch_set_depth can extend the run. ycoef (host, depth doubles, or
  * NULL): also build y = sum_i ycoef[i] v_i, the Ritz vector of a previous pass

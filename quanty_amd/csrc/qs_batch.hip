@@ -1,7 +1,9 @@
 // Batched Lanczos: many independent recurrences (different operators, bases,
 // start vectors and depths) advanced in lockstep by one set of launches per
-// step. Each slot holds an assembled operator, its two Lanczos vectors and
-// its coefficient arrays; a device worklist of (slot, slice) and (slot, row
+// step. Each slot holds an assembled operator (sliced ELL or product layout),
+// its two Lanczos vectors and its coefficient arrays; a product-layout slot
+// keeps its vectors in the layout's row order (converted on the way in and
+// out, so callers see canonical order); a device worklist of (slot, slice) and (slot, row
 // chunk) entries drives the kernels, and per-slot partial sums are reduced in a
 // fixed order, so a slot's coefficients are bit-identical whether it runs
 // alone or among others. Step counters live on the device; a slot stops when
@@ -68,15 +70,18 @@ struct qs_batch {
   bool dirty = true;
   WorkItem *d_w1 = nullptr, *d_w2 = nullptr;
   int64_t n1 = 0, n2 = 0, cap1 = 0, cap2 = 0;
+  int64_t n1p = 0;  // K1 items of product-layout slots (the last n1p of the n1)
   int *d_r1 = nullptr, *d_r2 = nullptr;  // per slot: [begin, end) of its entries, 2 ints each
   int *d_off = nullptr;                    // per slot: first K1 item, first K2 item (worklist build)
   double *d_p1 = nullptr, *d_p2 = nullptr;  // partials per slice (K1) and per work item (K2)
   size_t lds1 = 0;                          // K1 dynamic LDS: the largest term table that is staged
-  bool fast1 = true;                        // every slot: one-word determinants, packed entries
+  size_t ldsp = 0;                          // K1p: the largest term-value table that is staged
+  bool any1 = false, anyp = false;          // sliced-ELL / product-layout slots present
+  bool fast1 = true;                        // every sliced-ELL slot: one-word determinants, packed entries
   int64_t *d_n = nullptr;                   // work item counts of K1 and K2 (device, read by the kernels)
-  // graphs of `steps` steps by (K1 variant, K1 dynamic LDS, steps);
+  // graphs of `steps` steps by (K1 variant and presence, K1 LDS, K1p presence, K1p LDS, steps);
   // valid while the worklist and partial buffers keep their addresses
-  std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<int, int, int, int, int>, hipGraphExec_t> graphs;
 };
 
 static void drop_graphs(qs_batch *B) {
@@ -128,10 +133,11 @@ k_b_init_finish(SlotDev *slots, int slot, const double *__restrict__ part, int n
   }
 }
 
-// K1 over the work items of all active slots. A workgroup takes one item (4
-// consecutive slices of one slot, one per wavefront) per iteration and keeps
-// the slot's term tables in LDS while consecutive items stay in that slot;
-// tables larger than the dynamic LDS are read from global memory. The items
+// K1 over the work items of all active sliced-ELL slots (k_b1p takes the
+// product-layout ones). A workgroup takes one item (4 consecutive slices of
+// one slot, one per wavefront) per iteration and keeps the slot's term tables
+// in LDS while consecutive items stay in that slot; tables larger than the
+// dynamic LDS are read from global memory. The items
 // of XCD x (workgroups b with b % 8 == x) are a contiguous range, so a
 // workgroup meets few slot changes.
 template <int NW, bool PACKED>
@@ -144,14 +150,15 @@ __device__ __forceinline__ void b1_row(const SellD &A, int64_t sl, int lane, boo
     sell_row<1, NW, PACKED>(A, sl, lane, A.val, A.par, uc, 1, 0, hr, hi);
 }
 
-// FAST: every active slot has one-word determinants and packed entries (the
-// common case), which keeps the kernel within 64 VGPRs (8 waves per SIMD).
+// FAST: every active sliced-ELL slot has one-word determinants and packed
+// entries (the common case), which keeps the kernel within 64 VGPRs (8 waves
+// per SIMD).
 template <bool FAST>
 __global__ void __launch_bounds__(QS_BLOCK, FAST ? 8 : 1)
 k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, const int64_t *__restrict__ counts,
      double *__restrict__ part, int lds_bytes) {
   extern __shared__ char smem[];
-  const int64_t nwork = counts[0];
+  const int64_t nwork = counts[2];  // the sliced-ELL slots' items come first
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   constexpr int NX = 8;
@@ -172,7 +179,12 @@ k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, const
     const WorkItem it = work[i];
     const SlotDev &S = slots[it.slot];
     if (!S.active || S.step >= S.depth) continue;  // uniform across the workgroup
-    const SellD &A = S.A;
+    if (S.A.kind != 0) continue;  // k_b1p's item; uniform across the workgroup
+    // the item's operator, vectors and scalars as locals
+    const SellD A = S.A;
+    double2 *const uc = S.uc, *const up = S.up;
+    const int64_t n = S.n, nsl = S.nslices;
+    const double sc = S.sc, cp = S.cp;
     const bool lds = term_table_bytes(A.nterms, A.nw) <= (size_t)lds_bytes;
     if (lds && staged != it.slot) {
       __syncthreads();  // the previous table is no longer read
@@ -183,24 +195,86 @@ k_b1(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, const
     }
     const int64_t sl = (int64_t)it.idx * QS_K1_GROUP + wv;
     double dot = 0.0;
-    if (sl < S.nslices) {  // uniform across the wavefront
-      const double sc = S.sc, cp = S.cp;
+    if (sl < nsl) {  // uniform across the wavefront
       double hr[1], hi[1];
       if (FAST) {
-        b1_row<1, true>(A, sl, lane, lds, smem, S.uc, hr, hi);
+        b1_row<1, true>(A, sl, lane, lds, smem, uc, hr, hi);
       } else if (A.nw == 1) {
-        if (A.packed) b1_row<1, true>(A, sl, lane, lds, smem, S.uc, hr, hi);
-        else b1_row<1, false>(A, sl, lane, lds, smem, S.uc, hr, hi);
+        if (A.packed) b1_row<1, true>(A, sl, lane, lds, smem, uc, hr, hi);
+        else b1_row<1, false>(A, sl, lane, lds, smem, uc, hr, hi);
       } else {
-        if (A.packed) b1_row<2, true>(A, sl, lane, lds, smem, S.uc, hr, hi);
-        else b1_row<2, false>(A, sl, lane, lds, smem, S.uc, hr, hi);
+        if (A.packed) b1_row<2, true>(A, sl, lane, lds, smem, uc, hr, hi);
+        else b1_row<2, false>(A, sl, lane, lds, smem, uc, hr, hi);
       }
       const int64_t r = sl * 64 + lane;
-      if (r < S.n) {
-        const double2 u = S.uc[r];
+      if (r < n) {
+        const double2 u = uc[r];
         dot = hr[0] * u.x + hi[0] * u.y;
-        const double2 p = S.up[r];
-        S.up[r] = make_double2(sc * hr[0] - cp * p.x, sc * hi[0] - cp * p.y);
+        const double2 p = up[r];
+        up[r] = make_double2(sc * hr[0] - cp * p.x, sc * hi[0] - cp * p.y);
+      }
+    }
+    dot = wave_sum(dot);  // fixed butterfly: same order every call
+    if (lane == 0) part[i * QS_K1_GROUP + wv] = dot;
+  }
+}
+
+// K1 over the work items of the product-layout slots, whose vectors are in
+// the layout's row order: the slot's term values in LDS when they fit in
+// lds_bytes, then a staging area of QS_PROD_STAGE vector entries per
+// wavefront for the rows of a slice's group.
+__global__ void __launch_bounds__(QS_BLOCK)
+k_b1p(const SlotDev *__restrict__ slots, const WorkItem *__restrict__ work, const int64_t *__restrict__ counts,
+      double *__restrict__ part, int lds_bytes) {
+  extern __shared__ char smem[];
+  const int64_t w0 = counts[2], nwork = counts[0] - w0;  // the product slots' items come last
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  constexpr int NX = 8;
+  const int64_t nblk = gridDim.x;
+  int64_t i0, i1, di;
+  if (nblk < NX) {
+    i0 = w0 + blockIdx.x;
+    i1 = w0 + nwork;
+    di = nblk;
+  } else {
+    const int xcd = blockIdx.x % NX;
+    i0 = w0 + nwork * xcd / NX + blockIdx.x / NX;
+    i1 = w0 + nwork * (xcd + 1) / NX;
+    di = (nblk - xcd + NX - 1) / NX;
+  }
+  double2 *xs = reinterpret_cast<double2 *>(smem + lds_bytes) + (size_t)wv * QS_PROD_STAGE;
+  int staged = -1;
+  for (int64_t i = i0; i < i1; i += di) {
+    const WorkItem it = work[i];
+    const SlotDev &S = slots[it.slot];
+    if (!S.active || S.step >= S.depth) continue;  // uniform across the workgroup
+    if (S.A.kind != 1) continue;  // k_b1's item
+    // the item's operator, vectors and scalars as locals
+    const SellD A = S.A;
+    double2 *const uc = S.uc, *const up = S.up;
+    const int64_t nsl = S.nslices;
+    const double sc = S.sc, cp = S.cp;
+    const bool lds = term_value_bytes(A.nterms) <= (size_t)lds_bytes;
+    if (lds && staged != it.slot) {
+      __syncthreads();  // the previous values are no longer read
+      stage_term_values(A, smem);
+      __syncthreads();
+      staged = it.slot;
+    }
+    const int64_t sl = (int64_t)it.idx * QS_K1_GROUP + wv;
+    double dot = 0.0;
+    if (sl < nsl) {  // uniform across the wavefront
+      double hr[1], hi[1];
+      // separate calls, so the term values are read as LDS or as global memory
+      // (one pointer to either would make every read a flat load)
+      const int64_t r = lds ? prod_row<1>(A, sl, lane, reinterpret_cast<const double2 *>(smem), uc, 1, 0, hr, hi, xs)
+                            : prod_row<1>(A, sl, lane, A.val, uc, 1, 0, hr, hi, xs);
+      if (r >= 0) {
+        const double2 u = uc[r];
+        dot = hr[0] * u.x + hi[0] * u.y;
+        const double2 p = up[r];
+        up[r] = make_double2(sc * hr[0] - cp * p.x, sc * hi[0] - cp * p.y);
       }
     }
     dot = wave_sum(dot);  // fixed butterfly: same order every call
@@ -299,9 +373,11 @@ k_b_r2(SlotDev *slots, int nslots, const int *__restrict__ range, const double *
 }
 
 // y = sum_{j<m} z_j s_j u_j over the kept vectors, j ascending per row (the
-// order of the second-pass accumulation in k_b2)
+// order of the second-pass accumulation in k_b2); row r goes to y[inv[r]]
+// when the slot keeps its vectors in a product layout's row order
 __global__ void __launch_bounds__(QS_BLOCK)
-k_b_ritz(const SlotDev *__restrict__ slots, int slot, int m, const double *__restrict__ z, double2 *__restrict__ y) {
+k_b_ritz(const SlotDev *__restrict__ slots, int slot, int m, const double *__restrict__ z,
+         const int32_t *__restrict__ inv, double2 *__restrict__ y) {
   const SlotDev &S = slots[slot];
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < S.n; r += (int64_t)gridDim.x * blockDim.x) {
     double2 acc = make_double2(0.0, 0.0);
@@ -310,7 +386,7 @@ k_b_ritz(const SlotDev *__restrict__ slots, int slot, int m, const double *__res
       const double2 u = S.V[(size_t)j * S.n + r];
       acc = make_double2(fma(c, u.x, acc.x), fma(c, u.y, acc.y));
     }
-    y[r] = acc;
+    y[inv ? (int64_t)inv[r] : r] = acc;
   }
 }
 
@@ -326,37 +402,46 @@ k_b_worklist_scan(const SlotDev *__restrict__ slots, int cap, int *__restrict__ 
                   int *__restrict__ r2, int64_t *__restrict__ counts) {
   __shared__ int s1[QS_WL_BLOCK], s2[QS_WL_BLOCK];
   int base1 = 0, base2 = 0;
-  for (int t0 = 0; t0 < cap; t0 += blockDim.x) {
-    const int s = t0 + (int)threadIdx.x;
-    int c1 = 0, c2 = 0;
-    if (s < cap && slots[s].active) {
-      c1 = (int)((slots[s].nslices + QS_K1_GROUP - 1) / QS_K1_GROUP);
-      c2 = (int)((slots[s].n + QS_K2_CHUNK - 1) / QS_K2_CHUNK);
-    }
-    s1[threadIdx.x] = c1;
-    s2[threadIdx.x] = c2;
-    __syncthreads();
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-      const int a1 = threadIdx.x >= (unsigned)o ? s1[threadIdx.x - o] : 0;
-      const int a2 = threadIdx.x >= (unsigned)o ? s2[threadIdx.x - o] : 0;
+  // K1 items: the sliced-ELL slots' first (pass 0), then the product-layout
+  // slots' (pass 1), so k_b1 and k_b1p each walk a range of their own; K2
+  // items in pass 0
+  for (int pass = 0; pass < 2; ++pass) {
+    if (pass == 1 && threadIdx.x == 0) counts[2] = base1;
+    for (int t0 = 0; t0 < cap; t0 += blockDim.x) {
+      const int s = t0 + (int)threadIdx.x;
+      const bool act = s < cap && slots[s].active;
+      const bool mine = s < cap && (act ? (slots[s].A.kind == 1) == (pass == 1) : pass == 0);
+      int c1 = 0, c2 = 0;
+      if (act && mine) c1 = (int)((slots[s].nslices + QS_K1_GROUP - 1) / QS_K1_GROUP);
+      if (act && pass == 0) c2 = (int)((slots[s].n + QS_K2_CHUNK - 1) / QS_K2_CHUNK);
+      s1[threadIdx.x] = c1;
+      s2[threadIdx.x] = c2;
       __syncthreads();
-      s1[threadIdx.x] += a1;
-      s2[threadIdx.x] += a2;
+      for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+        const int a1 = threadIdx.x >= (unsigned)o ? s1[threadIdx.x - o] : 0;
+        const int a2 = threadIdx.x >= (unsigned)o ? s2[threadIdx.x - o] : 0;
+        __syncthreads();
+        s1[threadIdx.x] += a1;
+        s2[threadIdx.x] += a2;
+        __syncthreads();
+      }
+      if (mine) {
+        const int e1 = base1 + s1[threadIdx.x];
+        off[2 * s] = e1 - c1;
+        // K1 partials are kept per slice position of an item
+        r1[2 * s] = (e1 - c1) * QS_K1_GROUP;
+        r1[2 * s + 1] = e1 * QS_K1_GROUP;
+      }
+      if (s < cap && pass == 0) {
+        const int e2 = base2 + s2[threadIdx.x];
+        off[2 * s + 1] = e2 - c2;
+        r2[2 * s] = e2 - c2;
+        r2[2 * s + 1] = e2;
+      }
+      base1 += s1[blockDim.x - 1];
+      base2 += s2[blockDim.x - 1];
       __syncthreads();
     }
-    if (s < cap) {
-      const int e1 = base1 + s1[threadIdx.x], e2 = base2 + s2[threadIdx.x];
-      off[2 * s] = e1 - c1;
-      off[2 * s + 1] = e2 - c2;
-      // K1 partials are kept per slice position of an item
-      r1[2 * s] = (e1 - c1) * QS_K1_GROUP;
-      r1[2 * s + 1] = e1 * QS_K1_GROUP;
-      r2[2 * s] = e2 - c2;
-      r2[2 * s + 1] = e2;
-    }
-    base1 += s1[blockDim.x - 1];
-    base2 += s2[blockDim.x - 1];
-    __syncthreads();
   }
   if (threadIdx.x == 0) {
     counts[0] = base1;
@@ -364,7 +449,6 @@ k_b_worklist_scan(const SlotDev *__restrict__ slots, int cap, int *__restrict__ 
   }
 }
 
-// One block per slot writes its items at the offsets of k_b_worklist_scan.
 __global__ void __launch_bounds__(QS_BLOCK)
 k_b_worklist_fill(const SlotDev *__restrict__ slots, const int *__restrict__ off, WorkItem *__restrict__ w1,
                   WorkItem *__restrict__ w2) {
@@ -398,22 +482,34 @@ static qs_status rebuild(qs_batch *B) {
   qs_ctx *ctx = B->ctx;
   // item counts from the host's copy of the slot table (the device builds
   // the same lists from its own copy)
-  int64_t n1 = 0, n2 = 0;
-  size_t lds = 0;
-  bool fast = true;
+  int64_t n1 = 0, n2 = 0, n1p = 0;
+  size_t lds = 0, ldsp = 0;
+  bool fast = true, any1 = false, anyp = false;
   for (int s = 0; s < B->cap; ++s)
     if (B->used[s]) {
       const SlotDev &S = B->h_slots[s];
       n1 += (S.nslices + QS_K1_GROUP - 1) / QS_K1_GROUP;
       n2 += (S.n + QS_K2_CHUNK - 1) / QS_K2_CHUNK;
       const SellD &A = S.A;
-      const size_t tb = term_table_bytes(A.nterms, A.nw);
-      if (tb <= (size_t)ctx->term_lds) lds = std::max(lds, tb);
-      fast = fast && A.nw == 1 && A.packed;
+      if (A.kind == 1) {
+        n1p += (S.nslices + QS_K1_GROUP - 1) / QS_K1_GROUP;
+        const size_t tb = (term_value_bytes(A.nterms) + 15) / 16 * 16;
+        if (tb <= (size_t)ctx->term_lds) ldsp = std::max(ldsp, tb);
+        anyp = true;
+      } else {
+        const size_t tb = term_table_bytes(A.nterms, A.nw);
+        if (tb <= (size_t)ctx->term_lds) lds = std::max(lds, tb);
+        fast = fast && A.nw == 1 && A.packed;
+        any1 = true;
+      }
     }
   if (n1 >= ((int64_t)1 << 31) / QS_K1_GROUP || n2 >= ((int64_t)1 << 31))
     return qs_fail(ctx, QS_EINVAL, "the batch's work lists would exceed 2^31 items");
   B->lds1 = lds;
+  B->ldsp = ldsp;
+  B->n1p = n1p;
+  B->any1 = any1;
+  B->anyp = anyp;
   B->fast1 = fast;
   auto grow = [&](WorkItem **p, int64_t *cap, int64_t need) -> qs_status {
     if (need > *cap) {
@@ -467,7 +563,7 @@ extern "C" qs_status qs_batch_create(qs_ctx *ctx, int max_slots, qs_batch **out)
   qs_status st = qs_malloc(ctx, (void **)&B->d_slots, (size_t)max_slots * sizeof(SlotDev));
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r1, (size_t)max_slots * 8);
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_r2, (size_t)max_slots * 8);
-  if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_n, 2 * sizeof(int64_t));
+  if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_n, 4 * sizeof(int64_t));
   if (st == QS_OK) st = qs_malloc(ctx, (void **)&B->d_off, (size_t)max_slots * 8);
   if (st == QS_OK) {
     hipError_t e = hipMemsetAsync(B->d_slots, 0, (size_t)max_slots * sizeof(SlotDev), ctx->stream);
@@ -494,8 +590,6 @@ extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double 
   qs_ctx *ctx = B->ctx;
   if (!h->assembled || h->rows != h->cols)
     return qs_fail(ctx, QS_EINVAL, "qs_batch_add needs an assembled operator with rows == cols");
-  if (h->view.kind != 0)
-    return qs_fail(ctx, QS_EUNSUPPORTED, "qs_batch_add: product-layout bindings are not batched");
   int slot = -1;
   for (int s = 0; s < B->cap; ++s)
     if (!B->used[s]) {
@@ -553,8 +647,15 @@ extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double 
   void *scr = nullptr;
   if (st == QS_OK) st = qs_scratch(ctx, (size_t)nb * 8, &scr);
   part = (double *)scr;
+  // a product-layout slot starts from v0 in the layout's row order (staged in
+  // its second vector, which k_b_init reads before clearing)
+  const double2 *v0s = (const double2 *)v0;
+  if (st == QS_OK && e == hipSuccess && h->view.kind == 1) {
+    st = qs_prod_to_rows(ctx, h, 1, (const double2 *)v0, n, S.up, n);
+    v0s = S.up;
+  }
   if (st == QS_OK && e == hipSuccess) {
-    hipLaunchKernelGGL(k_b_init, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, (const double2 *)v0, part);
+    hipLaunchKernelGGL(k_b_init, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, v0s, part);
     hipLaunchKernelGGL(k_b_init_finish, dim3(1), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, part, nb);
     e = hipGetLastError();
   }
@@ -577,9 +678,14 @@ extern "C" qs_status qs_batch_add_ex(qs_batch *B, const qs_hop *h, const double 
   return QS_OK;
 }
 
+// K1p's dynamic LDS: the term values, then the staging area
+static int kp_shm(const qs_batch *B) {
+  return (int)B->ldsp + (QS_BLOCK / 64) * QS_PROD_STAGE * (int)sizeof(double2);
+}
+
 // `steps` steps as a graph of kernel nodes in a chain (the same launches as
 // qs_batch_run's kernel-by-kernel path, with grids that fit any worklist).
-static qs_status build_graph(qs_batch *B, int steps, int g1, int g2, int lds1, hipGraphExec_t *out) {
+static qs_status build_graph(qs_batch *B, int steps, int g1, int gp, int g2, hipGraphExec_t *out) {
   qs_ctx *ctx = B->ctx;
   hipGraph_t g = nullptr;
   QS_HIP(ctx, hipGraphCreate(&g, 0));
@@ -590,8 +696,9 @@ static qs_status build_graph(qs_batch *B, int steps, int g1, int g2, int lds1, h
   double *p1 = B->d_p1, *p2 = B->d_p2;
   const double *cp1 = B->d_p1, *cp2 = B->d_p2;
   const int *r1 = B->d_r1, *r2 = B->d_r2;
-  int cap = B->cap, lds = lds1;
+  int cap = B->cap, lds = (int)B->lds1, ldsp = (int)B->ldsp;
   void *a1[] = {&cs, &w1, &cn, &p1, &lds};
+  void *ap[] = {&cs, &w1, &cn, &p1, &ldsp};
   void *ar1[] = {&ms, &cap, &r1, &cp1};
   void *a2[] = {&cs, &w2, &cn, &p2};
   void *ar2[] = {&ms, &cap, &r2, &cp2};
@@ -602,26 +709,42 @@ static qs_status build_graph(qs_batch *B, int steps, int g1, int g2, int lds1, h
     int block;
     int shm;
     void **args;
-  } const nodes[4] = {{k1, g1, QS_BLOCK, lds1, a1},
-                      {reinterpret_cast<void *>(&k_b_r1), B->cap, QS_R1_BLOCK, 0, ar1},
-                      {reinterpret_cast<void *>(&k_b2), g2, QS_BLOCK, 0, a2},
-                      {reinterpret_cast<void *>(&k_b_r2), B->cap, QS_BLOCK, 0, ar2}};
+  };
+  // per step: K1 and K1p side by side (both after the previous step, R1 after
+  // both, so the tail of one overlaps the other), then R1, K2, R2 in a chain
+  std::vector<Node> k1s, chain;
+  if (B->any1) k1s.push_back({k1, g1, QS_BLOCK, lds, a1});
+  if (B->anyp) k1s.push_back({reinterpret_cast<void *>(&k_b1p), gp, QS_BLOCK, kp_shm(B), ap});
+  chain.push_back({reinterpret_cast<void *>(&k_b_r1), B->cap, QS_R1_BLOCK, 0, ar1});
+  chain.push_back({reinterpret_cast<void *>(&k_b2), g2, QS_BLOCK, 0, a2});
+  chain.push_back({reinterpret_cast<void *>(&k_b_r2), B->cap, QS_BLOCK, 0, ar2});
+  auto add = [&](const Node &nd, const hipGraphNode_t *deps, size_t ndeps, hipGraphNode_t *node) {
+    hipKernelNodeParams p{};
+    p.func = nd.f;
+    p.gridDim = dim3((unsigned)nd.grid);
+    p.blockDim = dim3((unsigned)nd.block);
+    p.sharedMemBytes = (unsigned)nd.shm;
+    p.kernelParams = nd.args;
+    p.extra = nullptr;
+    return hipGraphAddKernelNode(node, g, deps, ndeps, &p);
+  };
   hipGraphNode_t prev = nullptr;
   hipError_t e = hipSuccess;
-  for (int k = 0; k < steps && e == hipSuccess; ++k)
-    for (const Node &nd : nodes) {
-      hipKernelNodeParams p{};
-      p.func = nd.f;
-      p.gridDim = dim3((unsigned)nd.grid);
-      p.blockDim = dim3((unsigned)nd.block);
-      p.sharedMemBytes = (unsigned)nd.shm;
-      p.kernelParams = nd.args;
-      p.extra = nullptr;
-      hipGraphNode_t node = nullptr;
-      e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &p);
+  for (int k = 0; k < steps && e == hipSuccess; ++k) {
+    hipGraphNode_t k1n[2];
+    size_t nk = 0;
+    for (const Node &nd : k1s) {
+      e = add(nd, prev ? &prev : nullptr, prev ? 1 : 0, &k1n[nk]);
       if (e != hipSuccess) break;
+      ++nk;
+    }
+    for (size_t c = 0; c < chain.size() && e == hipSuccess; ++c) {
+      hipGraphNode_t node = nullptr;
+      if (c == 0) e = add(chain[c], nk ? k1n : (prev ? &prev : nullptr), nk ? nk : (prev ? 1 : 0), &node);
+      else e = add(chain[c], &prev, 1, &node);
       prev = node;
     }
+  }
   if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return qs_hip_fail(ctx, e, "qs_batch_run (graph)", __FILE__, __LINE__);
@@ -634,21 +757,27 @@ extern "C" qs_status qs_batch_run(qs_batch *B, int steps) {
   QS_HIP(ctx, hipSetDevice(ctx->device));
   if (B->dirty) QS_TRY(rebuild(B));
   if (B->n1 == 0) return QS_OK;
-  const int lds1 = (int)B->lds1;
+  const int lds1 = (int)B->lds1, ldsp = (int)B->ldsp, shmp = kp_shm(B);
   const void *k1 = B->fast1 ? (const void *)k_b1<true> : (const void *)k_b1<false>;
   const int64_t res1 = qs_resident_blocks(ctx, k1, (size_t)lds1);
+  const int64_t resp = qs_resident_blocks(ctx, (const void *)k_b1p, (size_t)shmp);
   const int64_t res2 = (int64_t)ctx->num_cu * 4;
   // the kernels read the work item counts from the device, so any grid
   // covers the worklist; a graph uses the resident grids, which fit every
   // later worklist
-  auto enqueue = [&](int g1, int g2, int n) {
+  auto enqueue = [&](int g1, int gp, int g2, int n) {
     for (int k = 0; k < n; ++k) {
-      if (B->fast1)
-        hipLaunchKernelGGL(k_b1<true>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1,
-                           B->d_n, B->d_p1, lds1);
-      else
-        hipLaunchKernelGGL(k_b1<false>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1,
-                           B->d_n, B->d_p1, lds1);
+      if (B->any1) {
+        if (B->fast1)
+          hipLaunchKernelGGL(k_b1<true>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1,
+                             B->d_n, B->d_p1, lds1);
+        else
+          hipLaunchKernelGGL(k_b1<false>, dim3(g1), dim3(QS_BLOCK), (size_t)lds1, ctx->stream, B->d_slots, B->d_w1,
+                             B->d_n, B->d_p1, lds1);
+      }
+      if (B->anyp)
+        hipLaunchKernelGGL(k_b1p, dim3(gp), dim3(QS_BLOCK), (size_t)shmp, ctx->stream, B->d_slots, B->d_w1, B->d_n,
+                           B->d_p1, ldsp);
       hipLaunchKernelGGL(k_b_r1, dim3(B->cap), dim3(QS_R1_BLOCK), 0, ctx->stream, B->d_slots, B->cap, B->d_r1,
                          B->d_p1);
       hipLaunchKernelGGL(k_b2, dim3(g2), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, B->d_w2, B->d_n, B->d_p2);
@@ -656,20 +785,21 @@ extern "C" qs_status qs_batch_run(qs_batch *B, int steps) {
     }
   };
   if (ctx->batch_graph && steps > 0) {
-    const auto key = std::make_tuple(B->fast1 ? 1 : 0, lds1, steps);
+    const auto key = std::make_tuple((B->any1 ? 1 : 0) + (B->fast1 ? 2 : 0), lds1, B->anyp ? 1 : 0, ldsp, steps);
     auto it = B->graphs.find(key);
     if (it == B->graphs.end()) {
       // The graph is assembled node by node rather than captured from the
       // stream: other host threads may synchronize the device meanwhile,
       // which would invalidate a capture.
       hipGraphExec_t ex = nullptr;
-      QS_TRY(build_graph(B, steps, (int)std::max<int64_t>(1, res1), (int)std::max<int64_t>(1, res2), lds1, &ex));
+      QS_TRY(build_graph(B, steps, (int)std::max<int64_t>(1, res1), (int)std::max<int64_t>(1, resp),
+                         (int)std::max<int64_t>(1, res2), &ex));
       it = B->graphs.emplace(key, ex).first;
     }
     QS_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
   } else {
-    enqueue((int)std::max<int64_t>(1, std::min(B->n1, res1)), (int)std::max<int64_t>(1, std::min(B->n2, res2)),
-            steps);
+    enqueue((int)std::max<int64_t>(1, std::min(B->n1 - B->n1p, res1)), (int)std::max<int64_t>(1, std::min(B->n1p, resp)),
+            (int)std::max<int64_t>(1, std::min(B->n2, res2)), steps);
     QS_HIP(ctx, hipGetLastError());
   }
   if (!ctx->async) QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -705,7 +835,8 @@ extern "C" qs_status qs_batch_poll(qs_batch *B, int slot, int *step, double *a, 
   if (b) QS_TRY(qs_d2h(ctx, b, H.b, (size_t)(H.depth + 1) * 8));
   if (y) {
     if (!H.y) return qs_fail(ctx, QS_EINVAL, "slot %d accumulates no vector", slot);
-    QS_HIP(ctx, hipMemcpyAsync(y, H.y, (size_t)H.n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    if (H.A.kind == 1) QS_TRY(qs_prod_from_rows(ctx, B->hops[slot], 1, H.y, H.n, (double2 *)y, H.n));
+    else QS_HIP(ctx, hipMemcpyAsync(y, H.y, (size_t)H.n * 16, hipMemcpyDeviceToDevice, ctx->stream));
   }
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QS_OK;
@@ -725,8 +856,9 @@ extern "C" qs_status qs_batch_ritz(qs_batch *B, int slot, int m, const double *z
   QS_TRY(qs_scratch(ctx, (size_t)std::max(m, 1) * 8, &scr));
   if (m > 0) QS_TRY(qs_h2d(ctx, scr, z, (size_t)m * 8));
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((H.n + QS_BLOCK - 1) / QS_BLOCK, (int64_t)ctx->num_cu * 8));
+  const int32_t *inv = H.A.kind == 1 ? B->hops[slot]->d_inv : nullptr;
   hipLaunchKernelGGL(k_b_ritz, dim3(nb), dim3(QS_BLOCK), 0, ctx->stream, B->d_slots, slot, m, (const double *)scr,
-                     (double2 *)y);
+                     inv, (double2 *)y);
   QS_HIP(ctx, hipGetLastError());
   QS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return QS_OK;

@@ -23,6 +23,24 @@ __device__ __forceinline__ int64_t wave_uniform(int64_t v) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// Element i of read-only data that every lane of a wave reads at the same
+// address: loads through the constant address space go to the scalar cache
+// (s_load, adjacent words merged) instead
+// of returning one copy per lane through the vector memory path (texture
+// addresser and data units). Only for buffers no kernel writes while the
+// reading kernel runs.
+template <typename T>
+__device__ __forceinline__ T ld_cst(const T *p, int64_t i) {
+  static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+  typedef const __attribute__((address_space(4))) uint32_t *cword;
+  const cword q = (cword)(p + i);
+  T out;
+  uint32_t *o = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) o[k] = q[k];
+  return out;
+}
+
 // Position j of lane `lane` in the ELL part of a slice starting at `base`:
 // a lane's positions are stored four at a time (quad j/4 of the slice holds
 // 64 lanes x 4 consecutive positions), so one 16-byte load per lane fetches
@@ -186,6 +204,9 @@ __device__ __forceinline__ void sell_row(const SellD &A, int64_t s, int lane, co
 // Rows of one group that a wavefront stages in LDS for the A-matrix (product
 // layout, one vector): groups with more rows read their own rows from memory.
 #define QS_PROD_STAGE 256
+// zero entries stored past the end of a product binding's entry list, so a
+// batch of entries is loaded without bounds tests
+#define QS_PROD_ENT_PAD 8
 
 // Row of lane `lane` in product slice s (see SellD, kind 1) applied to KC
 // vectors (entry j of vector k at x[j*ldx + k*ldv]); returns the row, or -1
@@ -213,15 +234,18 @@ __device__ __forceinline__ int64_t prod_row(const SellD &A, int64_t s, int lane,
     ar[k] = 0.0;
     ai[k] = 0.0;
   }
-  const int4 d = A.pslice[s];
+  s = wave_uniform(s);
+  // the slice, its A-set and its group's entries are the same for every lane:
+  // scalar loads
+  const int4 d = ld_cst(A.pslice, s);
   const bool active = lane < d.y;
   const int chunk = d.w >> 16;
   const int ap = chunk * 64 + lane;                 // index of the row's A configuration in its A-set
   const int64_t g0 = (int64_t)d.x - chunk * 64;     // first row of the group
   const int64_t r = (int64_t)d.x + lane;
-  const int4 as = A.aset[d.w & 0xFFFF];             // {m, k, offset, 0}
-  const uint2 pp = A.pent_ptr[d.z];
-  const uint32_t e0 = pp.x, em = pp.y, e1 = A.pent_ptr[d.z + 1].x;
+  const int4 as = ld_cst(A.aset, d.w & 0xFFFF);    // {m, k, offset, 0}
+  const uint2 pp = ld_cst(A.pent_ptr, d.z);
+  const uint32_t e0 = pp.x, em = pp.y, e1 = ld_cst(A.pent_ptr, (int64_t)d.z + 1).x;
   const bool staged = KC == 1 && xs != nullptr && as.x <= QS_PROD_STAGE;
   if (staged) {
     // the previous slice's reads of xs are done before it is overwritten
@@ -273,19 +297,27 @@ __device__ __forceinline__ int64_t prod_row(const SellD &A, int64_t s, int lane,
   }
   // identity entries: one coefficient per entry (its terms summed), one
   // contiguous load of the source group's rows
+  // (the U entries of a batch are loaded unconditionally, as scalar loads
+  // issued together: the list has QS_PROD_ENT_PAD >= U entries of padding)
+  static_assert(U <= QS_PROD_ENT_PAD, "entry batch beyond the padding");
   for (uint32_t eb = e0; eb < em; eb += U) {
     int64_t col[U];
     double2 v[U];
+    uint2 E[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint4 e4 = ld_cst(A.pent, (int64_t)eb + u);
+      E[u] = make_uint2(e4.x, e4.y);
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const bool ok = eb + u < em;
-      const uint4 E = A.pent[ok ? eb + u : em - 1];
-      double2 vv = tv[E.y & 0xFFFFu];
-      const uint64_t sb = (uint64_t)(E.y >> 31) << 63;
+      double2 vv = tv[E[u].y & 0xFFFFu];
+      const uint64_t sb = (uint64_t)(E[u].y >> 31) << 63;
       vv.x = __longlong_as_double(__double_as_longlong(vv.x) ^ sb);
       vv.y = __longlong_as_double(__double_as_longlong(vv.y) ^ sb);
       v[u] = ok ? vv : make_double2(0.0, 0.0);
-      col[u] = (int64_t)E.x + ap;
+      col[u] = ok ? (int64_t)E[u].x + ap : r;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -303,9 +335,9 @@ __device__ __forceinline__ int64_t prod_row(const SellD &A, int64_t s, int lane,
     uint4 E[UM];
     int32_t q[UM];
 #pragma unroll
-    for (int u = 0; u < UM; ++u) E[u] = A.pent[eb + u < e1 ? eb + u : e1 - 1];
+    for (int u = 0; u < UM; ++u) E[u] = ld_cst(A.pent, (int64_t)eb + u);  // padded list, as above
 #pragma unroll
-    for (int u = 0; u < UM; ++u) q[u] = A.amap[E[u].z + ap];
+    for (int u = 0; u < UM; ++u) q[u] = A.amap[eb + u < e1 ? E[u].z + ap : 0];
     int64_t col[UM];
     double2 v[UM];
 #pragma unroll

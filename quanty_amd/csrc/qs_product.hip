@@ -566,6 +566,8 @@ static qs_status prod_build(qs_ctx *ctx, qs_hop *h, const uint8_t *amask) {
   void *p_slice, *p_eptr, *p_ent, *p_amap, *p_aset, *p_hA, *p_inv;
   QS_TRY(up(slices.data(), slices.size() * sizeof(int4), &p_slice));
   QS_TRY(up(eptr.data(), eptr.size() * sizeof(uint2), &p_eptr));
+  // entry batches are loaded without bounds tests: zero entries past the end
+  ent.resize(ent.size() + QS_PROD_ENT_PAD, make_uint4(0u, 0u, 0u, 0u));
   QS_TRY(up(ent.data(), ent.size() * sizeof(uint4), &p_ent));
   QS_TRY(up(amap.data(), amap.size() * 4, &p_amap));
   QS_TRY(up(asets.data(), asets.size() * sizeof(int4), &p_aset));

@@ -217,6 +217,8 @@ class Hop:
         h = C.c_void_p()
         self.ctx.check(self.lib.qs_hop_create_like(self.ctx.h, op.h, template.h, C.byref(h)), "qs_hop_create_like")
         self.h = h
+        if getattr(template, "amodes", None):
+            self.amodes = template.amodes  # the same product layout
         return self
 
     @classmethod

@@ -30,8 +30,8 @@ import torch
 
 from . import problems as P
 from ._lib import QS_PARAM_ALLOC_TOP, QS_PARAM_SHIFT_MIN, QSError
-from .engine import Basis, Batch, Hop, Op, cf_spectrum, tridiag_lowest
-from .tune import layout_for
+from .engine import Basis, Batch, Hop, Op, cf_spectrum, product_candidates, tridiag_lowest
+from .tune import layout_for, product_for
 
 # (nbath, full_baths, nelec, d_range, max_holes, max_elec) -> sector size, cached
 _FAMILY = None
@@ -111,6 +111,7 @@ class Request:
     rejected: str | None = field(compare=False, default=None)  # why the server refused it
     a: np.ndarray | None = field(compare=False, default=None)
     b: np.ndarray | None = field(compare=False, default=None)
+    layout: str = field(compare=False, default="")  # "sell" or "product": how its H was bound
     # the request's payload: NF, H and T terms, restrictions (made from seed and
     # shape when absent)
     problem: dict | None = field(compare=False, default=None, repr=False)
@@ -205,12 +206,15 @@ class PatternCache:
 
 PATTERNS = PatternCache()
 
+# the product layout is kept only when its groups average at least this many rows
+PRODUCT_MIN_GROUP_ROWS = 128
+
 
 class _Job:
     """Device-resident state of an admitted request."""
 
     def __init__(self, req: Request, ctx, gs_depth: int, start_temperature: float | None = 0.3,
-                 start_floor: float = 1e-3, patterns: PatternCache | None = PATTERNS):
+                 start_floor: float = 1e-3, patterns: PatternCache | None = PATTERNS, product: bool = False):
         self.req = req
         pr = req.problem if req.problem is not None else make_problem(req.seed, req.shape)
         self.nf = pr["NF"]
@@ -233,17 +237,32 @@ class _Job:
                 ctx.set_param(QS_PARAM_ALLOC_TOP, 1)
             try:
                 self.basis = Basis.enumerate(self.nf, pr["restrictions"], ctx)
-                # the layout the autotuner fitted for this basis size (tune.fit_family)
-                sm = layout_for(self.basis.n)
-                if sm is not None:
-                    ctx.set_param(QS_PARAM_SHIFT_MIN, sm)
-                self.hop = Hop(self.H, self.basis, mode="assembled")
+                # the layout the autotuner fitted for this basis size
+                # (tune.fit_family): the product layout where it was faster,
+                # else the sliced ELL with the fitted shift-block threshold
+                cand = product_for(self.basis.n) if product else None
+                if cand is not None:
+                    groups = product_candidates(self.nf, pr["restrictions"])
+                    try:
+                        h = Hop.product(self.H, self.basis, groups[cand] if cand < len(groups) else None)
+                        # groups of fewer rows than two wavefronts leave lanes idle that cost
+                        # more than the layout saves (a 56-row group set measured slower)
+                        if self.basis.n >= PRODUCT_MIN_GROUP_ROWS * h.product_info()["groups"]:
+                            self.hop = h
+                    except QSError:
+                        self.hop = None
+                if self.hop is None:
+                    sm = layout_for(self.basis.n)
+                    if sm is not None:
+                        ctx.set_param(QS_PARAM_SHIFT_MIN, sm)
+                    self.hop = Hop(self.H, self.basis, mode="assembled")
             finally:
                 if patterns is not None:
                     ctx.set_param(QS_PARAM_ALLOC_TOP, 0)
             if patterns is not None:
                 patterns.put(key, self.basis, self.hop, self.hop.stats()["stream_bytes"])
         self.T = Op(self.nf, pr["T"], ctx)
+        req.layout = "product" if getattr(self.hop, "amodes", None) else "sell"
         self.thop = Hop(self.T, self.basis, mode="matfree")
         g = torch.Generator(device="cuda").manual_seed(req.seed)
         dev = torch.device("cuda", ctx.device)
@@ -295,10 +314,12 @@ class _Prep:
     stream, start vector) on worker threads, each with its own library context,
     so that the preparation overlaps the batch's GPU steps."""
 
-    def __init__(self, device: int, gs_depth: int, workers: int = 1, start=(0.3, 1e-3), patterns=None):
+    def __init__(self, device: int, gs_depth: int, workers: int = 1, start=(0.3, 1e-3), patterns=None,
+                 product: bool = False):
         self.gs_depth = gs_depth
         self.start = start
         self.patterns = patterns
+        self.product = product
         self.todo: queue.Queue = queue.Queue()
         self.ready: queue.Queue = queue.Queue()
         self.ctxs = _worker_contexts(device, max(1, workers))
@@ -312,7 +333,7 @@ class _Prep:
             if r is None:
                 return
             try:
-                job = _Job(r, ctx, self.gs_depth, *self.start, patterns=self.patterns)
+                job = _Job(r, ctx, self.gs_depth, *self.start, patterns=self.patterns, product=self.product)
                 torch.cuda.current_stream().synchronize()  # the start vector; library calls are synchronous
                 self.ready.put(job)
             except BaseException as e:  # handed to the serving loop, which raises it
@@ -339,8 +360,14 @@ class SpectrumServer:
                  chunk: int = 8, ne: int = 2000,
                  emin: float = -10.0, emax: float = 20.0, keep_basis: bool = True, prepare_ahead: bool = True,
                  prep_workers: int = 2, start_temperature: float | None = 0.3, start_floor: float = 1e-3,
-                 share_patterns: bool = True):
+                 share_patterns: bool = True, product_layout: bool = False):
         self.ctx = ctx
+        # product_layout: requests the fitted layout table (tune.product_for) and
+        # their group sizes favour are bound in the product layout. Off by
+        # default: on the 10,000-request stream it served 52.8 spectra/s against
+        # 53.4 with the sliced ELL alone, although a lone 2.18 M-row slot steps
+        # 8.6% faster in it
+        self.product_layout = product_layout
         # requests of one operator structure share one assembled stream (PatternCache)
         self.patterns = PATTERNS if share_patterns else None
         self.max_slots = max_slots
@@ -512,7 +539,7 @@ class SpectrumServer:
         waiting: list[Request] = []
         done: list[Request] = []
         prep = _Prep(self.ctx.device, self.gs_depth, self.prep_workers,
-                     (self.start_temperature, self.start_floor), self.patterns) \
+                     (self.start_temperature, self.start_floor), self.patterns, self.product_layout) \
             if (self.prepare_ahead and not plain) else None
         inflight: dict[int, int] = {}  # rid -> reserved bytes of requests being prepared
         kept = self.gs_max + 1 if self.keep_basis else 0
@@ -551,7 +578,7 @@ class SpectrumServer:
                     r.t_admit = time.perf_counter() - t0
                     if prep is None:
                         self._start(_Job(r, self.ctx, self.gs_depth, self.start_temperature, self.start_floor,
-                                         patterns=self.patterns))
+                                         patterns=self.patterns, product=self.product_layout))
                     else:
                         inflight[r.rid] = est
                         self.used += est

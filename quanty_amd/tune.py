@@ -23,8 +23,8 @@ import time
 import numpy as np
 import torch
 
-from ._lib import QS_PARAM_SHIFT_MIN, QS_PARAM_TERM_LDS
-from .engine import Basis, Hop, Op
+from ._lib import QS_PARAM_SHIFT_MIN, QS_PARAM_TERM_LDS, QSError
+from .engine import Basis, Hop, Op, product_candidates
 
 CANDIDATES = (65, 64, 56, 48, 32, 24, 16, 8)
 LDS_CANDIDATES = (0, 24 * 1024, 64 * 1024)
@@ -47,6 +47,42 @@ def time_apply(hop: Hop, x: torch.Tensor, y: torch.Tensor, reps: int) -> float:
     finally:
         ctx.lib.qs_ctx_set_async(ctx.h, 0)
     return e0.elapsed_time(e1) / reps * 1e3
+
+
+def time_apply_rows(hop: Hop, x: torch.Tensor, y: torch.Tensor, reps: int) -> float:
+    """Average H*psi time in microseconds of a product-layout binding in its own
+    row order (qs_hop_apply_rows, what the batch engine runs), over `reps`
+    back-to-back launches."""
+    ctx = hop.ctx
+    stream = torch.cuda.ExternalStream(ctx.lib.qs_ctx_stream(ctx.h), device=x.device)
+    torch.cuda.synchronize()
+    ctx.lib.qs_ctx_set_async(ctx.h, 1)
+    try:
+        ctx.check(ctx.lib.qs_hop_apply_rows(ctx.h, hop.h, x.data_ptr(), y.data_ptr()), "qs_hop_apply_rows")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.check(ctx.lib.qs_hop_apply_rows(ctx.h, hop.h, x.data_ptr(), y.data_ptr()), "qs_hop_apply_rows")
+        e1.record(stream)
+        e1.synchronize()
+    finally:
+        ctx.lib.qs_ctx_set_async(ctx.h, 0)
+    return e0.elapsed_time(e1) / reps * 1e3
+
+
+def time_product(op: Op, basis: Basis, reps: int = 10, seed: int = 1):
+    """(microseconds, group A modes, index in product_candidates) of the
+    product layout with the split of fewest estimated loads, or (None, None,
+    None) when no restriction gives one."""
+    try:
+        hop = Hop.product(op, basis)
+    except QSError:
+        return None, None, None
+    cands = product_candidates(basis.nf, basis.restrictions or [])
+    g = np.random.default_rng(seed)
+    x = torch.from_numpy(g.normal(size=basis.n) + 1j * g.normal(size=basis.n)).cuda()
+    y = basis.zeros()
+    return time_apply_rows(hop, x, y, reps), hop.amodes, cands.index(hop.amodes)
 
 
 def tune_shift_min(op: Op, basis: Basis, candidates=CANDIDATES, reps: int = 10, apply: bool = True,
@@ -110,11 +146,13 @@ _TABLE = None
 
 def fit_family(dims=(1e3, 1e4, 1e5, 5e5, 1.5e6, 3e6), candidates=(65, 56, 48, 32), reps: int = 10,
                seed: int = 1, out: str = LAYOUT_TABLE):
-    """The shift-block threshold the server assembles each request with: for
-    impurity shapes of the serving family near each of `dims`, the fastest
-    threshold of `candidates` by timed H*psi; written to `out` as rows
-    {dim, threshold, timings}. layout_for() picks the row nearest a
-    request's dim on a log scale."""
+    """The layout the server assembles each request with: for impurity shapes
+    of the serving family near each of `dims`, the fastest shift-block
+    threshold of `candidates` by timed H*psi, and the product layout (the
+    split of fewest estimated loads, timed in its own row order) where it is
+    faster than that; written to `out` as rows {dim, threshold, timings,
+    product timing, layout}. layout_for() and product_for() pick the row
+    nearest a request's dim on a log scale."""
     from . import default_context
     from . import problems as P
     from . import serve
@@ -127,27 +165,55 @@ def fit_family(dims=(1e3, 1e4, 1e5, 5e5, 1.5e6, 3e6), candidates=(65, 56, 48, 32
         basis = Basis.enumerate(pr["NF"], pr["restrictions"], ctx)
         op = Op(pr["NF"], pr["H"], ctx)
         best, rs = tune_shift_min(op, basis, candidates=candidates, reps=reps, apply=False, seed=seed)
+        best_us = min(r["us"] for r in rs)
+        pus, amodes, cand = time_product(op, basis, reps=reps, seed=seed)
+        layout = "product" if pus is not None and pus < best_us else "sell"
         rows.append({"dim": int(basis.n), "shape": {k: list(v) if isinstance(v, tuple) else v for k, v in shape.items()},
-                     "shift_min": int(best), "us": {str(r["shift_min"]): round(r["us"], 2) for r in rs}})
-        print(f"dim {basis.n}: threshold {best}  " + "  ".join(f"{r['shift_min']}: {r['us']:.1f} us" for r in rs),
-              flush=True)
+                     "shift_min": int(best), "us": {str(r["shift_min"]): round(r["us"], 2) for r in rs},
+                     "product_us": None if pus is None else round(pus, 2), "product_group_a": amodes,
+                     "product_candidate": cand, "layout": layout})
+        print(f"dim {basis.n}: threshold {best}  " + "  ".join(f"{r['shift_min']}: {r['us']:.1f} us" for r in rs)
+              + (f"  product: {pus:.1f} us" if pus is not None else "  product: none") + f"  -> {layout}", flush=True)
     with open(out, "w") as f:
-        json.dump({"what": "shift-block threshold by basis size, fastest timed H*psi per impurity shape",
+        json.dump({"what": "H*psi layout by basis size (shift-block threshold of the sliced ELL, or the product "
+                           "layout where it is faster), fastest timed H*psi per impurity shape",
                    "candidates": list(candidates), "rows": rows}, f, indent=1)
     return rows
+
+
+def _row_for(dim: int):
+    global _TABLE
+    if _TABLE is None:
+        _TABLE = json.load(open(LAYOUT_TABLE))["rows"] if os.path.exists(LAYOUT_TABLE) else []
+    if not _TABLE or dim < 1:
+        return None
+    return min(_TABLE, key=lambda r: abs(math.log(r["dim"]) - math.log(dim)))
 
 
 def layout_for(dim: int) -> int | None:
     """The shift-block threshold for a basis of `dim` determinants from the
     fitted table (the measured shape nearest in log dim), or None without a
     table."""
+    row = _row_for(dim)
+    return None if row is None else int(row["shift_min"])
+
+
+def product_for(dim: int) -> int | None:
+    """For a basis of `dim` determinants at least as large as where the fitted
+    table starts to have the product layout faster than the sliced ELL (the
+    geometric mean of the first such row's dim and the row before it): the
+    index, in engine.product_candidates order, of the restriction group that
+    row used as group A; else None. Whether the product layout wins depends on
+    the shape as well (the rows per group); the server checks that on the
+    binding it builds."""
     global _TABLE
-    if _TABLE is None:
-        _TABLE = json.load(open(LAYOUT_TABLE))["rows"] if os.path.exists(LAYOUT_TABLE) else []
-    if not _TABLE or dim < 1:
-        return None
-    row = min(_TABLE, key=lambda r: abs(math.log(r["dim"]) - math.log(dim)))
-    return int(row["shift_min"])
+    _row_for(1)
+    rows = sorted(_TABLE or [], key=lambda r: r["dim"])
+    for k, r in enumerate(rows):
+        if r.get("layout") == "product":
+            start = math.sqrt(r["dim"] * rows[k - 1]["dim"]) if k > 0 else 0.0
+            return r.get("product_candidate") if dim >= start else None
+    return None
 
 
 def main():
@@ -158,10 +224,11 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--family", action="store_true",
-                    help="fit the per-size threshold table of the serving family (layout_table.json)")
+                    help="fit the per-size layout table of the serving family (layout_table.json)")
+    ap.add_argument("--out", default=LAYOUT_TABLE, help="where --family writes the table")
     args = ap.parse_args()
     if args.family:
-        fit_family(reps=args.reps, seed=args.seed)
+        fit_family(reps=args.reps, seed=args.seed, out=args.out)
         return
     ctx = default_context()
     pr = P.anderson(seed=args.seed)

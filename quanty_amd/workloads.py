@@ -307,9 +307,27 @@ def layouts(seed: int = 1, reps: int = 30, nsteps: int = 200):
     return {"config": "layouts", "dim": basis.n, "lanczos_steps": nsteps, "rows": rows}
 
 
+def product_apply(seed: int = 1, reps: int = 10):
+    """The 50-mode impurity in the product layout with group A = B3+B4 (the
+    automatic choice), applied `reps` times in its own row order: the kernel
+    the product-layout counters are taken on (one kernel, one split)."""
+    ctx = default_context()
+    pr = P.anderson(seed=seed)
+    nf = pr["NF"]
+    basis = Basis.enumerate(nf, pr["restrictions"], ctx)
+    hop = Hop.product(Op(nf, pr["H"], ctx), basis, [m for b in pr["baths"][2:] for m in b])
+    g = np.random.default_rng(0)
+    x = torch.from_numpy(g.normal(size=basis.n) + 1j * g.normal(size=basis.n)).to(basis.zeros().device)
+    y = basis.zeros()
+    _, t = _sync_time(lambda: [hop.apply_rows(x, y) for _ in range(reps)])
+    return {"config": "product_apply", "dim": basis.n, "group_a": hop.amodes, "applies": reps,
+            "us_per_apply": 1e6 * t / reps}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--config", required=True, choices=["0", "1", "2a", "2b", "3", "4", "dense", "layouts"])
+    ap.add_argument("--config", required=True,
+                    choices=["0", "1", "2a", "2b", "3", "4", "dense", "layouts", "product_apply"])
     ap.add_argument("--requests", type=int, default=64, help="requests per rate (config 3)")
     args = ap.parse_args()
     if args.config == "0":
@@ -326,6 +344,8 @@ def main():
         out = config3(args.requests)
     elif args.config == "layouts":
         out = layouts()
+    elif args.config == "product_apply":
+        out = product_apply()
     else:
         out = config4()
     print(json.dumps(out), flush=True)

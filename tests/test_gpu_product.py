@@ -3,7 +3,7 @@ SELL layout: H*psi to 1e-13 on every split the restrictions allow (one and
 two words per determinant), the same off-diagonal entry count as SELL, Lanczos
 coefficients to 1e-10, the order conversions, the diagonal, bindings sharing
 an index (create_like) and the refusal of splits the sector does not factor
-over. All calls go through the C ABI."""
+over, and product slots in the batch engine. All calls go through the C ABI."""
 import numpy as np
 import pytest
 import torch
@@ -183,3 +183,72 @@ def test_product_binding_through_block_krylov_and_ground_state_solvers():
     assert np.allclose(a2, a1, rtol=1e-10, atol=1e-10 * np.max(np.abs(a1)))
     assert np.allclose(b2, b1, rtol=1e-10, atol=0)
     assert torch.max(torch.abs(W2 - W1)).item() < 1e-8
+
+
+def test_product_slots_in_the_batch():
+    """Product-layout bindings as batch slots (k_b1p), next to sliced-ELL slots:
+    every product slot's coefficients equal its SELL binding's single-problem
+    Lanczos to 1e-10 (its sums run in the layout's row order) and are
+    bit-identical alone and in the mix; the SELL slots are bit-identical to a
+    batch without product slots; the Ritz vector from kept vectors and the
+    second-pass vector come back in canonical order and equal the SELL slot's
+    to 1e-12."""
+    from quanty_amd.engine import Batch, heev_host
+    kws = [CASES[0][0], CASES[2][0]]
+    probs = []
+    for kw, which in ((kws[0], "full"), (kws[1], "empty")):
+        pr = P.anderson(**kw)
+        b = Basis.enumerate(pr["NF"], pr["restrictions"])
+        H = Op(pr["NF"], pr["H"])
+        sell = Hop(H, b, mode="assembled")
+        prod = Hop.product(H, b, group(pr, which, kw))
+        v0 = torch.from_numpy(cvec(b.n, kw["seed"])).to(DEV)
+        probs.append((b, sell, prod, v0))
+    depths = [45, 38]
+    mixed = Batch(8)
+    ps = [mixed.add(prod, v0, d) for (b, sell, prod, v0), d in zip(probs, depths)]
+    ss = [mixed.add(sell, v0, d) for (b, sell, prod, v0), d in zip(probs, depths)]
+    for _ in range(7):
+        mixed.run(8)
+    for (b, sell, prod, v0), d, sp, sq in zip(probs, depths, ps, ss):
+        st, a, bb = mixed.poll(sp)
+        assert st == d
+        ra, rb = sell.lanczos(v0, d)
+        assert np.allclose(a, ra, rtol=1e-10, atol=1e-12)
+        assert np.allclose(bb[:d], rb, rtol=1e-10, atol=0)
+        alone = Batch(1)
+        s1 = alone.add(prod, v0, d)
+        alone.run(d)
+        _, a1, b1 = alone.poll(s1)
+        assert np.array_equal(a, a1) and np.array_equal(bb, b1)
+        _, a2, b2 = mixed.poll(sq)
+        only = Batch(1)
+        s2 = only.add(sell, v0, d)
+        only.run(d)
+        _, a3, b3 = only.poll(s2)
+        assert np.array_equal(a2, a3) and np.array_equal(b2, b3)
+    # Ritz vectors: kept basis and second pass, canonical order
+    b, sell, prod, v0 = probs[0]
+    m = 60
+    bt = Batch(4)
+    sk = bt.add(prod, v0, m, keep_basis=True)
+    sr = bt.add(sell, v0, m, keep_basis=True)
+    bt.run(m)
+    _, a, bb = bt.poll(sk)
+    T = np.diag(a[1:m + 1]).astype(complex)
+    for i in range(1, m):
+        T[i, i - 1] = T[i - 1, i] = bb[i]
+    w, Z = heev_host(T)
+    z = (Z[:, 0] * np.exp(-1j * np.angle(Z[np.argmax(np.abs(Z[:, 0])), 0]))).real
+    yp = bt.ritz(sk, z, b.zeros())
+    ys = bt.ritz(sr, z, b.zeros())
+    assert torch.linalg.vector_norm(yp - ys).item() < 1e-12 * torch.linalg.vector_norm(ys).item()
+    s2 = bt.add(prod, v0, m, ycoef=z)
+    bt.run(m)
+    y2 = b.zeros()
+    bt.poll(s2, want_coef=False, y=y2)
+    assert torch.linalg.vector_norm(y2 - ys).item() < 1e-12 * torch.linalg.vector_norm(ys).item()
+    # a Ritz vector's Rayleigh quotient is its Ritz value
+    hy = sell.apply(yp.contiguous())
+    rq = (torch.vdot(yp, hy) / torch.vdot(yp, yp)).item()
+    assert abs(rq - w[0]) < 1e-9 * abs(w[0])
