@@ -22,8 +22,9 @@
 //     for one another; fixed-order sums make the copies identical;
 //   * w_k needs all of H22 v_k and the sum of the owners' partial dot
 //     products: the one exchange between workgroups per column.
-// Workgroup s < nv also applies each reflector to vector s of Y (Y <- Q^+ Y),
-// so the projections of the spectra come out of the same launch.
+// Workgroup s < nv also applies each reflector to vector s of Y (Y <- Q^+ Y,
+// the vector in LDS), so the projections of the spectra come out of the same
+// launch.
 #include <algorithm>
 
 #include "qs_internal.h"
@@ -42,9 +43,9 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
 // ---------------------------------------------------------------------------
 // Layout and exchange. Up to 256 workgroups of 512 threads (one per CU, two waves per SIMD, so a
 // thread has 256 VGPRs); workgroup g owns columns g, g + G, ... (at most
-// QS_HETRD_REG_NC of them) and thread t holds rows t, t + 512, t + 1024,
-// t + 1536 of each owned column for the whole launch, so the matrix is read
-// once and never written back.
+// QS_HETRD_REG_NC of them) and thread t holds rows t, t + 512, t + 1024 of
+// each owned column for the whole launch, so the matrix is read once and
+// never written back.
 //
 // What workgroups exchange per column (the owners' entries of H22 v, one
 // partial dot product per workgroup, the next column as its owner left it)
@@ -56,21 +57,24 @@ __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
 // dot products, one per workgroup and written every column, make the wait a
 // barrier: a workgroup that holds all partials of step k knows that every
 // workgroup has finished reading the buffers of step k-1. The buffers rotate
-// over three copies (step k reads copy k mod 3, writes copy k+1 mod 3), and in
-// step k a workgroup puts the marker back into its own words of copy
-// k+2 mod 3, read last in step k-1; those stores complete before it writes
-// anything of step k+1's, so a reader of copy k+2 mod 3 sees the marker or
-// the new value, never the old one. Compared with per-workgroup flags (store,
-// wait for completion, set the flag, poll it, then load the data: about four
-// dependent round trips to memory per column) this is one round trip.
+// over three copies (step k reads copy k mod 3, writes copy k+1 mod 3), and at
+// the end of step k a workgroup puts the marker back into its own words of
+// copy k+2 mod 3, read last in step k-1; those stores complete before it
+// writes anything of step k+1's, so a reader of copy k+2 mod 3 sees the marker
+// or the new value, never the old one. Compared with per-workgroup flags
+// (store, wait for completion, set the flag, poll it, then load the data:
+// about four dependent round trips to memory per column) this is one round
+// trip.
 //
-// Between writing its step k+1 data and polling for it, a workgroup does the
-// work nobody waits for: storing reflector k into A and applying reflector
-// k+1 to its vector of Y.
+// Inside a workgroup a column takes three barriers (the exchanged data in, the
+// norm of the next column and v^+ y, the next reflector in LDS) and one more
+// in the owners' pass; wave sums go through lane exchanges (permlane swaps,
+// DPP) rather than LDS. Between writing its step k+1 data and polling for it,
+// the lead workgroup stores reflector k into A, which nobody waits for.
 #define QS_HETRD_REG_BLOCK 512
 #define QS_HETRD_REG_NC 6   // owned columns per workgroup
-#define QS_HETRD_REG_RPT 4  // rows per thread
-#define QS_HETRD_REG_MAX (QS_HETRD_REG_BLOCK * QS_HETRD_REG_RPT)
+#define QS_HETRD_REG_RPT 3  // rows per thread
+#define QS_HETRD_REG_MAX (QS_HETRD_REG_BLOCK * QS_HETRD_REG_RPT)  // = 256 workgroups x 6 columns
 
 constexpr unsigned long long QS_MARK = ~0ull;
 
@@ -114,7 +118,7 @@ __device__ __forceinline__ bool ld_wait_n(const double2 *const (&p)[NP], const b
       __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
-    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if ((spins & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     __builtin_amdgcn_s_sleep(1);
 #pragma unroll
     for (int q = 0; q < 2 * NP; ++q)
@@ -128,42 +132,76 @@ __device__ __forceinline__ bool ld_wait_n(const double2 *const (&p)[NP], const b
   return true;
 }
 
-// Wave sums of 8 values per lane by recursive halving: at lane distances 32,
-// 16 and 8 each lane keeps half of its values and adds its partner's copy of
-// them, then the last value is summed over lane distances 4, 2 and 1. Returns
-// the wave sum of value 4 b5 + 2 b4 + b3 (lane bits b5, b4, b3); 10 shuffles
-// instead of 48, in a fixed order.
+// Lane exchanges without LDS (gfx950 permlane swaps, DPP), on the two halves
+// of a double. fold32(a, b): a(l) + a(l+32) in lanes l < 32, b(l-32) + b(l)
+// in lanes l >= 32; fold16 likewise between rows 2r and 2r+1 of 16 lanes.
+struct double4_ {
+  double v[4];
+};
+__device__ __forceinline__ double join_f64(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double fold32(double a, double b) {
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  return join_f64(lo[0], hi[0]) + join_f64(lo[1], hi[1]);
+}
+__device__ __forceinline__ double fold16(double a, double b) {
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  return join_f64(lo[0], hi[0]) + join_f64(lo[1], hi[1]);
+}
+// value of the lane that DPP control CTRL names (row_mirror 0x140: 15 - l in
+// a row of 16; row_half_mirror 0x141: 7 - l in 8; quad_perm 0xB1: l xor 1;
+// 0x4E: l xor 2)
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return join_f64((unsigned)lo, (unsigned)hi);
+}
+// the value in lane l (the same in every lane)
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  return join_f64((unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l),
+                  (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l));
+}
+// sum over the 8 lanes of a half-row, the same in each of them
+__device__ __forceinline__ double sum8_lanes(double x) {
+  x += dpp_f64<0xB1>(x);
+  x += dpp_f64<0x4E>(x);
+  x += dpp_f64<0x141>(x);
+  return x;
+}
+
+// Wave sums of 8 values per lane by recursive halving: at lane distances 32
+// and 16 each lane keeps half of its values and adds its partner's copy of
+// them, then pairs lanes l and 15-l for one value and sums over 8 lanes.
+// Returns the wave sum of value 4 b5 + 2 b4 + b3 (lane bits b5, b4, b3), in a
+// fixed order; whole wave active.
 __device__ __forceinline__ double wave_sum8(const double (&v)[8], int lane) {
   double w4[4], w2[2];
-  {
-    const bool hi = lane & 32;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const double send = hi ? v[i] : v[4 + i];
-      const double keep = hi ? v[4 + i] : v[i];
-      w4[i] = keep + __shfl_xor(send, 32);
-    }
-  }
-  {
-    const bool hi = lane & 16;
+  for (int i = 0; i < 4; ++i) w4[i] = fold32(v[i], v[4 + i]);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const double send = hi ? w4[i] : w4[2 + i];
-      const double keep = hi ? w4[2 + i] : w4[i];
-      w2[i] = keep + __shfl_xor(send, 16);
-    }
-  }
-  double x;
-  {
-    const bool hi = lane & 8;
-    const double send = hi ? w2[0] : w2[1];
-    const double keep = hi ? w2[1] : w2[0];
-    x = keep + __shfl_xor(send, 8);
-  }
-  x += __shfl_xor(x, 4);
-  x += __shfl_xor(x, 2);
-  x += __shfl_xor(x, 1);
-  return x;
+  for (int i = 0; i < 2; ++i) w2[i] = fold16(w4[i], w4[2 + i]);
+  const bool hi = lane & 8;
+  const double send = hi ? w2[0] : w2[1];
+  const double keep = hi ? w2[1] : w2[0];
+  return sum8_lanes(keep + dpp_f64<0x140>(send));
+}
+
+// Wave sums of 4 values per lane: the sum of value 2 b5 + b4 in every lane of
+// row b5 b4 (lanes 16 (2 b5 + b4) onwards); fixed order, whole wave active.
+__device__ __forceinline__ double wave_sum4(const double (&v)[4]) {
+  const double w0 = fold32(v[0], v[2]), w1 = fold32(v[1], v[3]);
+  double x = fold16(w0, w1);
+  x = sum8_lanes(x);
+  return x + dpp_f64<0x140>(x);
 }
 
 // block sum of one complex value per thread, fixed order, valid in every
@@ -211,24 +249,6 @@ __device__ void reg_larfg(double2 *c, int k0, int n, double2 &tau, double &beta,
   __syncthreads();
 }
 
-// y <- (I - tau v v^+)^+ y, v = c[k0..n-1] (c[k0] = 1), one complex reduction
-__device__ void reg_apply_qh(const double2 *c, int k0, int n, double2 tau, double2 *y, double *red) {
-  double2 s = make_double2(0.0, 0.0);
-  for (int i = k0 + (int)threadIdx.x; i < n; i += blockDim.x) {
-    const double2 t = cmulc(c[i], y[i]);
-    s.x += t.x;
-    s.y += t.y;
-  }
-  s = block_sum_c(s, red);
-  const double2 f = cmulc(tau, s);
-  for (int i = k0 + (int)threadIdx.x; i < n; i += blockDim.x) {
-    const double2 t = cmul(c[i], f);
-    y[i].x -= t.x;
-    y[i].y -= t.y;
-  }
-  __syncthreads();
-}
-
 }  // namespace
 
 // A: n x n column-major, lower triangle read (the upper one is taken as its
@@ -240,15 +260,20 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
             double2 *__restrict__ Y, int *err) {
   extern __shared__ double2 sm[];
   double2 *vk = sm, *wk = sm + n, *vn = sm + 2 * n;
-  __shared__ double red[2 * (QS_HETRD_REG_BLOCK / 64)];
+  __shared__ double red[4 * (QS_HETRD_REG_BLOCK / 64)];
   __shared__ double colsum[QS_HETRD_REG_BLOCK / 64][16];
-  __shared__ double2 pcon[QS_HETRD_REG_NC];
-  __shared__ double2 scal;
-  __shared__ int bad;
+  __shared__ double2 alph;  // row k+2 of column k+1 (alpha of the next reflector)
+  __shared__ double2 shx[2];  // p^+ v over all workgroups, p at row k+1
+  __shared__ int shok;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, bs = QS_HETRD_REG_BLOCK;
   const int G = (int)gridDim.x, g = (int)blockIdx.x;
   const bool lead = g == 0;
   double2 *ys = g < nv ? Y + (size_t)g * n : nullptr;
+  // workgroup g < nv: vector g of Y in LDS, each row read and written only by
+  // the thread that holds it (rows as the owned columns')
+  double2 *yr = sm + 3 * n;
+  if (ys)
+    for (int i = tid; i < n; i += bs) yr[i] = ys[i];
 
   // owned columns into registers (rows below the diagonal from the column,
   // rows above it from the conjugated row of the lower triangle)
@@ -282,6 +307,10 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
     // next v read once per group) and the group's columns inner
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+      // the group's columns are g + (3h + c) G, c = 0, 1, 2: none in [r0, n)
+      // when the last is below r0 or the first at n or beyond (the same for
+      // the whole workgroup, so the wave sums below stay whole-wave)
+      if (g + (3 * h + 2) * G < r0 || g + 3 * h * G >= n) continue;
       double acc[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) acc[q] = 0.0;
@@ -321,31 +350,33 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
     }
     if (prod) {
       __syncthreads();
-      if (tid < QS_HETRD_REG_NC) {
-        const int j = g + tid * G;
-        double2 s = make_double2(0.0, 0.0);
-        for (int w = 0; w < bs / 64; ++w) {
-          const int q = 8 * (tid / 3) + 2 * (tid % 3);  // column tid: value 2 (tid mod 3) of group tid / 3
-          s.x += colsum[w][q];
-          s.y += colsum[w][q + 1];
-        }
+      if (wv == 0) {
+        // lanes 0..NC-1: one owned column each; their terms added in column
+        // order from scalar reads of those lanes (within wave 0, no barrier)
         double2 con = make_double2(0.0, 0.0);
-        if (j >= r0 && j < n) {
-          st_dev(&pw[j], s);
-          con = cmulc(s, vn[j]);
+        if (lane < QS_HETRD_REG_NC) {
+          const int j = g + lane * G;
+          double2 s = make_double2(0.0, 0.0);
+          for (int w = 0; w < bs / 64; ++w) {
+            const int q = 8 * (lane / 3) + 2 * (lane % 3);  // column lane: value 2 (lane mod 3) of group lane / 3
+            s.x += colsum[w][q];
+            s.y += colsum[w][q + 1];
+          }
+          if (j >= r0 && j < n) {
+            st_dev(&pw[j], s);
+            con = cmulc(s, vn[j]);
+          }
         }
-        pcon[tid] = con;
-      }
-      __syncthreads();
-    }
-    if (tid == 0) {
-      double2 s = make_double2(0.0, 0.0);
-      if (prod)
+        double2 tot = make_double2(0.0, 0.0);
+#pragma unroll
         for (int c = 0; c < QS_HETRD_REG_NC; ++c) {
-          s.x += pcon[c].x;
-          s.y += pcon[c].y;
+          tot.x += readlane_f64(con.x, c);
+          tot.y += readlane_f64(con.y, c);
         }
-      st_dev(&partw[g], s);
+        if (lane == 0) st_dev(&partw[g], tot);
+      }
+    } else if (tid == 0) {
+      st_dev(&partw[g], make_double2(0.0, 0.0));
     }
   };
 
@@ -363,7 +394,7 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
   }
   // p_0 = H22 v_0, column 1 as it is and the partials: copy 0
   owners(1, false, tk.x != 0.0 || tk.y != 0.0, pbuf, part, colbuf);
-  if (ys && (tk.x != 0.0 || tk.y != 0.0)) reg_apply_qh(vn, 1, n, tk, ys, red);
+  double ek = beta;  // e[k] of the loop's step k
   {
     double2 *t = vk;
     vk = vn;
@@ -374,32 +405,40 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
     const bool act = tk.x != 0.0 || tk.y != 0.0;
     const int cr = k % 3, cw = (k + 1) % 3, cz = (k + 2) % 3;
     const double2 *pr = pbuf + (size_t)cr * n, *partr = part + (size_t)cr * G, *colr = colbuf + (size_t)cr * n;
-    // step k's data: the partials of every workgroup (wave 0), this thread's
-    // rows of p_k and of column k+1
-    if (tid == 0) bad = 0;
-    __syncthreads();
+    // step k's data: the partials of every workgroup and p_k = H22 v_k at row
+    // k+1 (wave 0, into LDS), this thread's rows of column k+1 and of p_k
     bool ok = true;
-    double2 s = make_double2(0.0, 0.0);
     if (wv == 0) {
-      // the partials: G <= 256, four per lane
-      const double2 *pp[4];
-      bool use[4];
-      double2 v[4];
+      const double2 *pp[5];
+      bool use[5];
+      double2 v[5];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         use[q] = lane + 64 * q < G;
         pp[q] = partr + (use[q] ? lane + 64 * q : 0);
       }
-      ok = ld_wait_n<4>(pp, use, v, err);
+      use[4] = act && lane == 0;
+      pp[4] = pr + k + 1;
+      ok = ld_wait_n<5>(pp, use, v, err);
+      double2 s = make_double2(0.0, 0.0);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
         if (use[q]) {
           s.x += v[q].x;
           s.y += v[q].y;
         }
+      const double4_ q = {s.x, s.y, 0.0, 0.0};
+      const double t = wave_sum4(q.v);  // s.x in lanes 0.., s.y in lanes 16..
+      const bool all = __all(ok);
+      if (lane == 0) {
+        shx[0].x = t;
+        shx[1] = act ? v[4] : make_double2(0.0, 0.0);
+        shok = all;
+      }
+      if (lane == 16) shx[0].y = t;
     }
+    double2 cv[QS_HETRD_REG_RPT], pv[QS_HETRD_REG_RPT];
     {
-      // this thread's rows of column k+1 and of p_k
       const double2 *pp[2 * QS_HETRD_REG_RPT];
       bool use[2 * QS_HETRD_REG_RPT];
       double2 v[2 * QS_HETRD_REG_RPT];
@@ -414,24 +453,141 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
       ok = ld_wait_n<2 * QS_HETRD_REG_RPT>(pp, use, v, err) && ok;
 #pragma unroll
       for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
+        cv[r] = v[2 * r];
+        pv[r] = v[2 * r + 1];
+      }
+    }
+    __syncthreads();
+    const double2 s = shx[0], pk1 = shx[1];
+    ok = ok && shok;
+    // one pass over this thread's rows, held in registers: w_k = tau p +
+    // alpha v (alpha = -|tau|^2/2 p^+ v, w_k at row k+1 formed by every
+    // thread), column k+1 after step k's update, and the squares of its
+    // entries below row k+2 (the next reflector's norm)
+    double nrm = 0.0;
+    {
+      const double h = -0.5 * (tk.x * tk.x + tk.y * tk.y);
+      const double2 al = make_double2(h * s.x, h * s.y);
+      const double2 vkk = vk[k + 1];
+      const double2 wk1 = act ? make_double2(tk.x * pk1.x - tk.y * pk1.y + al.x * vkk.x - al.y * vkk.y,
+                                             tk.x * pk1.y + tk.y * pk1.x + al.x * vkk.y + al.y * vkk.x)
+                              : make_double2(0.0, 0.0);
+      const double2 wkc = make_double2(wk1.x, -wk1.y), vkc = make_double2(vkk.x, -vkk.y);
+#pragma unroll
+      for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
         const int i = k + 1 + tid + r * bs;
-        if (i < n) {
-          vn[i] = v[2 * r];
-          if (act) wk[i] = v[2 * r + 1];
+        if (i >= n) continue;
+        double2 c = cv[r];
+        if (act) {
+          const double2 vi = vk[i];
+          const double2 x = cmul(tk, pv[r]), y = cmul(al, vi);
+          const double2 w = make_double2(x.x + y.x, x.y + y.y);
+          wk[i] = w;
+          const double2 t1 = cmul(vi, wkc), t2 = cmul(w, vkc);
+          c.x -= t1.x + t2.x;
+          c.y -= t1.y + t2.y;
+        }
+        cv[r] = c;
+        if (i >= k + 3) nrm += c.x * c.x + c.y * c.y;
+        if (i == k + 2) alph = c;
+      }
+    }
+    // Y: this thread's rows of v_k^+ y (reflector k applied to y below)
+    double2 sy = make_double2(0.0, 0.0);
+    if (ys && act) {
+#pragma unroll
+      for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
+        const int i = tid + r * bs;
+        if (i > k && i < n) {
+          const double2 t = cmulc(vk[i], yr[i]);
+          sy.x += t.x;
+          sy.y += t.y;
         }
       }
     }
-    if (!ok) bad = 1;
-    if (wv == 0) {
-      s.x = wave_sum(s.x);
-      s.y = wave_sum(s.y);
-      if (lane == 0) scal = s;
+    // the norm, the count of failed waits and v_k^+ y, over the workgroup:
+    // one barrier (red is next written after the barrier ahead of the
+    // owners' pass)
+    {
+      const double4_ q = {nrm, ok ? 0.0 : 1.0, sy.x, sy.y};
+      const double t = wave_sum4(q.v);
+      if ((lane & 15) == 0) red[4 * wv + (lane >> 4)] = t;
     }
     __syncthreads();
-    if (bad) return;
-    // every workgroup has finished step k-1: the marker back into this
-    // workgroup's words of copy k+2 mod 3 (its partial, its p entries, and
-    // column k+3 if it owns it)
+    double xn2 = 0.0, nbad = 0.0;
+    sy = make_double2(0.0, 0.0);
+    for (int w = 0; w < bs / 64; ++w) {
+      xn2 += red[4 * w];
+      nbad += red[4 * w + 1];
+      sy.x += red[4 * w + 2];
+      sy.y += red[4 * w + 3];
+    }
+    if (nbad != 0.0) return;
+    // y <- (I - tau_k v_k v_k^+)^+ y = y - conj(tau_k) (v_k^+ y) v_k
+    if (ys && act) {
+      const double2 f = cmulc(tk, sy);
+#pragma unroll
+      for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
+        const int i = tid + r * bs;
+        if (i > k && i < n) {
+          const double2 t = cmul(vk[i], f);
+          yr[i].x -= t.x;
+          yr[i].y -= t.y;
+        }
+      }
+    }
+    if (lead && tid == 0) d[k + 1] = cv[0].x;  // row k+1 is thread 0's first
+    if (k + 1 > n - 2) {
+      if (lead && tid == 0) A[(size_t)k * n + k + 1] = make_double2(ek, 0.0);
+      break;
+    }
+    // the next reflector (LAPACK zlarfg on rows k+2..n-1 of column k+1): every
+    // thread forms tau and beta from the norm and alpha, and writes its rows of
+    // the reflector (leading 1, tail scaled) into vn
+    double2 tn;
+    {
+      const double2 alpha = alph;
+      double2 sc = make_double2(1.0, 0.0);
+      if (xn2 == 0.0 && alpha.y == 0.0) {
+        tn = make_double2(0.0, 0.0);
+        beta = alpha.x;
+      } else {
+        const double nrm2 = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        beta = alpha.x >= 0.0 ? -nrm2 : nrm2;
+        tn = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+        const double dr = alpha.x - beta, di = alpha.y;
+        const double dd = dr * dr + di * di;
+        sc = make_double2(dr / dd, -di / dd);
+      }
+#pragma unroll
+      for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
+        const int i = k + 1 + tid + r * bs;
+        if (i < k + 2 || i >= n) continue;
+        vn[i] = i == k + 2 ? make_double2(1.0, 0.0) : (tn.x != 0.0 || tn.y != 0.0) ? cmul(sc, cv[r]) : cv[r];
+      }
+    }
+    if (lead && tid == 0) {
+      tau[k + 1] = tn;
+      e[k + 1] = beta;
+    }
+    const bool nact = tn.x != 0.0 || tn.y != 0.0;
+    // the reflector is in LDS, and every thread's markers (issued at the end
+    // of step k-1) have completed, before anything of step k+1 is stored (by
+    // whichever thread)
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    owners(k + 2, act, nact, pbuf + (size_t)cw * n, part + (size_t)cw * G, colbuf + (size_t)cw * n);
+    // off the critical path: reflector k into column k
+    if (lead) {
+      double2 *c0 = A + (size_t)k * n;
+      for (int i = k + 2 + tid; i < n; i += bs) c0[i] = vk[i];
+      if (tid == 0) c0[k + 1] = make_double2(ek, 0.0);
+    }
+    // every workgroup has finished step k-1 (it has stored its partial of
+    // step k): the marker back into this workgroup's words of copy k+2 mod 3
+    // (its partial, its p entries, and column k+3 if it owns it), which step
+    // k+1 writes. Issued last, they complete while step k+1 waits for its
+    // data; the wait for them is the one ahead of step k+1's owners' pass.
     {
       double2 *pz = pbuf + (size_t)cz * n, *colz = colbuf + (size_t)cz * n;
       if (tid == 0) st_mark(&part[(size_t)cz * G + g]);
@@ -439,61 +595,20 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
       if (k + 3 < n && (k + 3) % G == g)
         for (int i = k + 3 + tid; i < n; i += bs) st_mark(&colz[i]);
     }
-    // w_k = tau p + alpha v, alpha = -|tau|^2/2 (p^+ v)
-    if (act) {
-      const double h = -0.5 * (tk.x * tk.x + tk.y * tk.y);
-      const double2 al = make_double2(h * scal.x, h * scal.y);
-      for (int i = k + 1 + tid; i < n; i += bs) {
-        const double2 x = cmul(tk, wk[i]), y = cmul(al, vk[i]);
-        wk[i] = make_double2(x.x + y.x, x.y + y.y);
-      }
-    }
-    __syncthreads();
-    // column k+1 after step k's update
-    if (act) {
-      const double2 wkk = make_double2(wk[k + 1].x, -wk[k + 1].y);
-      const double2 vkk = make_double2(vk[k + 1].x, -vk[k + 1].y);
-      for (int i = k + 1 + tid; i < n; i += bs) {
-        double2 c = vn[i];
-        const double2 t1 = cmul(vk[i], wkk), t2 = cmul(wk[i], vkk);
-        c.x -= t1.x + t2.x;
-        c.y -= t1.y + t2.y;
-        vn[i] = c;
-      }
-    }
-    __syncthreads();
-    if (lead && tid == 0) d[k + 1] = vn[k + 1].x;
-    if (k + 1 > n - 2) {
-      if (lead && tid == 0) A[(size_t)k * n + k + 1] = make_double2(e[k], 0.0);
-      break;
-    }
-    double2 tn;
-    reg_larfg(vn, k + 2, n, tn, beta, red);
-    if (lead && tid == 0) {
-      tau[k + 1] = tn;
-      e[k + 1] = beta;
-    }
-    const bool nact = tn.x != 0.0 || tn.y != 0.0;
-    // every thread's markers have completed before anything of step k+1 is
-    // stored (by whichever thread)
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    owners(k + 2, act, nact, pbuf + (size_t)cw * n, part + (size_t)cw * G, colbuf + (size_t)cw * n);
-    // off the critical path: reflector k into column k, reflector k+1 on Y
-    if (lead) {
-      double2 *c0 = A + (size_t)k * n;
-      for (int i = k + 2 + tid; i < n; i += bs) c0[i] = vk[i];
-      if (tid == 0) c0[k + 1] = make_double2(e[k], 0.0);
-    }
-    if (ys && nact) reg_apply_qh(vn, k + 2, n, tn, ys, red);
-    __syncthreads();
+    // step k+1 overwrites vk (as its vn) and wk: the owners' pass has read
+    // them before its own barrier when it forms products, the lead's copy of
+    // the reflector needs one more
+    if (!nact || lead) __syncthreads();
     {
       double2 *t = vk;
       vk = vn;
       vn = t;
     }
     tk = tn;
+    ek = beta;
   }
+  if (ys)
+    for (int i = tid; i < n; i += bs) ys[i] = yr[i];
 }
 
 // In-place tridiagonal reduction of the n x n Hermitian matrix A (device,
@@ -503,14 +618,16 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
 // least nv, at most 256) does not fit the device, i.e. above 1536 rows on the
 // MI355X; the caller then takes another route. Measured on the MI355X
 // (eigenvalues and projections of 3 vectors for a random dense complex matrix,
-// tridiagonal eigenproblem included): 1.15 against 2.0 ms with rocSOLVER's
-// zhetrd at n = 120, 3.8 against 7.9 ms at 300, 10.0 against 20.3 ms at 800,
-// 19.3 against 38.7 ms at 1410. Per column (n = 1410, timestamps inside the
-// kernel): about 9 us, of which 3.4 waiting for the exchanged data (memory
-// latency), 2.7 the owners' update and products, 1.5 the reflector.
+// tridiagonal eigenproblem included): 0.90 against 2.0 ms with rocSOLVER's
+// zhetrd at n = 120, 3.3 against 7.9 ms at 300, 9.3 against 20.3 ms at 800,
+// 18.1 against 38.6 ms at 1410. Per column in the middle of the reduction
+// (n = 1410, timestamps inside the kernel): 7.3 us, of which 2.1 to 3.2
+// waiting for the exchanged data (memory latency), 0.8 to 1.0 the next column
+// and its norm, 0.7 to 1.0 the reflector, 1.7 to 2.0 the owners' update and
+// products.
 qs_status qs_hetrd_fused(qs_ctx *ctx, int n, double2 *A, double *d, double *e, double2 *tau, int nv, double2 *Y) {
   if (n < 1 || n > QS_HETRD_REG_MAX) return QS_EUNSUPPORTED;
-  const size_t shm = (size_t)3 * n * sizeof(double2);
+  const size_t shm = (size_t)4 * n * sizeof(double2);
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_hetrd_reg, QS_HETRD_REG_BLOCK, shm) != hipSuccess ||
       per_cu < 1) {
