@@ -261,7 +261,7 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
   extern __shared__ double2 sm[];
   double2 *vk = sm, *wk = sm + n, *vn = sm + 2 * n;
   __shared__ double red[4 * (QS_HETRD_REG_BLOCK / 64)];
-  __shared__ double colsum[QS_HETRD_REG_BLOCK / 64][16];
+  __shared__ __align__(16) double colsum[QS_HETRD_REG_BLOCK / 64][16];
   __shared__ double2 alph;  // row k+2 of column k+1 (alpha of the next reflector)
   __shared__ double2 shx[2];  // p^+ v over all workgroups, p at row k+1
   __shared__ int shok;
@@ -311,16 +311,26 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
       // when the last is below r0 or the first at n or beyond (the same for
       // the whole workgroup, so the wave sums below stay whole-wave)
       if (g + (3 * h + 2) * G < r0 || g + 3 * h * G >= n) continue;
+      // the group's (conjugated) entries of w_k and v_k, read once
+      double2 wj[3], vj[3];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const int j = min(g + (3 * h + cc) * G, n - 1);
+        wj[cc] = wk[j];
+        vj[cc] = vk[j];
+        wj[cc].y = -wj[cc].y;
+        vj[cc].y = -vj[cc].y;
+      }
       double acc[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) acc[q] = 0.0;
 #pragma unroll
       for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
         const int i = tid + r * bs;
+        // read whether used or not (in one batch; the index kept in range)
+        const int ic = min(i, n - 1);
+        const double2 vi = vk[ic], wi = wk[ic], ni = vn[ic];
         if (i < r0 || i >= n) continue;
-        const double2 vi = upd ? vk[i] : make_double2(0.0, 0.0);
-        const double2 wi = upd ? wk[i] : make_double2(0.0, 0.0);
-        const double2 ni = prod ? vn[i] : make_double2(0.0, 0.0);
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
           const int c = 3 * h + cc;
@@ -328,8 +338,7 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
           if (j < r0 || j >= n) continue;
           double2 x = a[c][r];
           if (upd) {
-            const double2 wj = make_double2(wk[j].x, -wk[j].y), vj = make_double2(vk[j].x, -vk[j].y);
-            const double2 t1 = cmul(vi, wj), t2 = cmul(wi, vj);
+            const double2 t1 = cmul(vi, wj[cc]), t2 = cmul(wi, vj[cc]);
             x.x -= t1.x + t2.x;
             x.y -= t1.y + t2.y;
             a[c][r] = x;
@@ -345,33 +354,31 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
       if (prod) {
         const double x = wave_sum8(acc, lane);
         if ((lane & 7) == 0) colsum[wv][8 * h + (lane >> 3)] = x;  // value 4 b5 + 2 b4 + b3 of group h
-
       }
     }
     if (prod) {
       __syncthreads();
       if (wv == 0) {
-        // lanes 0..NC-1: one owned column each; their terms added in column
-        // order from scalar reads of those lanes (within wave 0, no barrier)
+        // lane 8 c + w (c < 6): wave w's sum for owned column c, all read at
+        // once; the 8 lanes of column c add them (DPP), lane 8 c stores p_j
+        // and forms conj(p_j) v_j; the six terms added in column order from
+        // scalar reads of those lanes (within wave 0, no barrier)
+        const int c = lane >> 3, w = lane & 7;
+        double2 sv = make_double2(0.0, 0.0);
+        if (c < QS_HETRD_REG_NC) sv = *reinterpret_cast<const double2 *>(&colsum[w][8 * (c / 3) + 2 * (c % 3)]);
+        sv.x = sum8_lanes(sv.x);
+        sv.y = sum8_lanes(sv.y);
         double2 con = make_double2(0.0, 0.0);
-        if (lane < QS_HETRD_REG_NC) {
-          const int j = g + lane * G;
-          double2 s = make_double2(0.0, 0.0);
-          for (int w = 0; w < bs / 64; ++w) {
-            const int q = 8 * (lane / 3) + 2 * (lane % 3);  // column lane: value 2 (lane mod 3) of group lane / 3
-            s.x += colsum[w][q];
-            s.y += colsum[w][q + 1];
-          }
-          if (j >= r0 && j < n) {
-            st_dev(&pw[j], s);
-            con = cmulc(s, vn[j]);
-          }
+        const int j = g + c * G;
+        if (w == 0 && c < QS_HETRD_REG_NC && j >= r0 && j < n) {
+          st_dev(&pw[j], sv);
+          con = cmulc(sv, vn[j]);
         }
         double2 tot = make_double2(0.0, 0.0);
 #pragma unroll
-        for (int c = 0; c < QS_HETRD_REG_NC; ++c) {
-          tot.x += readlane_f64(con.x, c);
-          tot.y += readlane_f64(con.y, c);
+        for (int cc = 0; cc < QS_HETRD_REG_NC; ++cc) {
+          tot.x += readlane_f64(con.x, 8 * cc);
+          tot.y += readlane_f64(con.y, 8 * cc);
         }
         if (lane == 0) st_dev(&partw[g], tot);
       }
@@ -473,13 +480,16 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
                                              tk.x * pk1.y + tk.y * pk1.x + al.x * vkk.y + al.y * vkk.x)
                               : make_double2(0.0, 0.0);
       const double2 wkc = make_double2(wk1.x, -wk1.y), vkc = make_double2(vkk.x, -vkk.y);
+      double2 vr[QS_HETRD_REG_RPT];  // read in one batch, the index kept in range
+#pragma unroll
+      for (int r = 0; r < QS_HETRD_REG_RPT; ++r) vr[r] = vk[min(k + 1 + tid + r * bs, n - 1)];
 #pragma unroll
       for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
         const int i = k + 1 + tid + r * bs;
         if (i >= n) continue;
         double2 c = cv[r];
         if (act) {
-          const double2 vi = vk[i];
+          const double2 vi = vr[r];
           const double2 x = cmul(tk, pv[r]), y = cmul(al, vi);
           const double2 w = make_double2(x.x + y.x, x.y + y.y);
           wk[i] = w;
@@ -514,17 +524,24 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
       if ((lane & 15) == 0) red[4 * wv + (lane >> 4)] = t;
     }
     __syncthreads();
-    double xn2 = 0.0, nbad = 0.0;
-    sy = make_double2(0.0, 0.0);
-    for (int w = 0; w < bs / 64; ++w) {
-      xn2 += red[4 * w];
-      nbad += red[4 * w + 1];
-      sy.x += red[4 * w + 2];
-      sy.y += red[4 * w + 3];
+    // lane 8 q + w reads value q of wave w (one read each), the 8 lanes of
+    // value q add them (DPP), every lane takes the four totals
+    double xn2, nbad;
+    {
+      static_assert(QS_HETRD_REG_BLOCK == 512, "8 waves");
+      const int q = lane >> 3, w = lane & 7;
+      double x = q < 4 ? red[4 * w + q] : 0.0;
+      x = sum8_lanes(x);
+      xn2 = readlane_f64(x, 0);
+      nbad = readlane_f64(x, 8);
+      sy = make_double2(readlane_f64(x, 16), readlane_f64(x, 24));
     }
     if (nbad != 0.0) return;
-    // y <- (I - tau_k v_k v_k^+)^+ y = y - conj(tau_k) (v_k^+ y) v_k
-    if (ys && act) {
+    // y <- (I - tau_k v_k v_k^+)^+ y = y - conj(tau_k) (v_k^+ y) v_k, after
+    // the owners' pass (nobody waits for it; v_k stays in LDS until step
+    // k+1's first barrier), on the last step here
+    auto apply_y = [&]() {
+      if (!ys || !act) return;
       const double2 f = cmulc(tk, sy);
 #pragma unroll
       for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
@@ -535,10 +552,14 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
           yr[i].y -= t.y;
         }
       }
-    }
-    if (lead && tid == 0) d[k + 1] = cv[0].x;  // row k+1 is thread 0's first
+    };
+    const double dk1 = cv[0].x;  // d[k+1]: row k+1 is thread 0's first
     if (k + 1 > n - 2) {
-      if (lead && tid == 0) A[(size_t)k * n + k + 1] = make_double2(ek, 0.0);
+      if (lead && tid == 0) {
+        d[k + 1] = dk1;
+        A[(size_t)k * n + k + 1] = make_double2(ek, 0.0);
+      }
+      apply_y();
       break;
     }
     // the next reflector (LAPACK zlarfg on rows k+2..n-1 of column k+1): every
@@ -554,10 +575,13 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
       } else {
         const double nrm2 = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
         beta = alpha.x >= 0.0 ? -nrm2 : nrm2;
-        tn = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+        // two reciprocals instead of four divisions (each a long dependent
+        // sequence): tau = (beta - alpha) / beta, scale = 1 / (alpha - beta)
+        const double rb = 1.0 / beta;
+        tn = make_double2((beta - alpha.x) * rb, -alpha.y * rb);
         const double dr = alpha.x - beta, di = alpha.y;
-        const double dd = dr * dr + di * di;
-        sc = make_double2(dr / dd, -di / dd);
+        const double rdd = 1.0 / (dr * dr + di * di);
+        sc = make_double2(dr * rdd, -di * rdd);
       }
 #pragma unroll
       for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
@@ -566,10 +590,6 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
         vn[i] = i == k + 2 ? make_double2(1.0, 0.0) : (tn.x != 0.0 || tn.y != 0.0) ? cmul(sc, cv[r]) : cv[r];
       }
     }
-    if (lead && tid == 0) {
-      tau[k + 1] = tn;
-      e[k + 1] = beta;
-    }
     const bool nact = tn.x != 0.0 || tn.y != 0.0;
     // the reflector is in LDS, and every thread's markers (issued at the end
     // of step k-1) have completed, before anything of step k+1 is stored (by
@@ -577,11 +597,17 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     owners(k + 2, act, nact, pbuf + (size_t)cw * n, part + (size_t)cw * G, colbuf + (size_t)cw * n);
-    // off the critical path: reflector k into column k
+    // off the critical path: reflector k on y, and into column k; T's entries
+    apply_y();
     if (lead) {
       double2 *c0 = A + (size_t)k * n;
       for (int i = k + 2 + tid; i < n; i += bs) c0[i] = vk[i];
-      if (tid == 0) c0[k + 1] = make_double2(ek, 0.0);
+      if (tid == 0) {
+        c0[k + 1] = make_double2(ek, 0.0);
+        d[k + 1] = dk1;
+        tau[k + 1] = tn;
+        e[k + 1] = beta;
+      }
     }
     // every workgroup has finished step k-1 (it has stored its partial of
     // step k): the marker back into this workgroup's words of copy k+2 mod 3
@@ -618,13 +644,13 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
 // least nv, at most 256) does not fit the device, i.e. above 1536 rows on the
 // MI355X; the caller then takes another route. Measured on the MI355X
 // (eigenvalues and projections of 3 vectors for a random dense complex matrix,
-// tridiagonal eigenproblem included): 0.90 against 2.0 ms with rocSOLVER's
-// zhetrd at n = 120, 3.3 against 7.9 ms at 300, 9.3 against 20.3 ms at 800,
-// 18.1 against 38.6 ms at 1410. Per column in the middle of the reduction
-// (n = 1410, timestamps inside the kernel): 7.3 us, of which 2.1 to 3.2
-// waiting for the exchanged data (memory latency), 0.8 to 1.0 the next column
-// and its norm, 0.7 to 1.0 the reflector, 1.7 to 2.0 the owners' update and
-// products.
+// tridiagonal eigenproblem included): 0.83 against 2.0 ms with rocSOLVER's
+// zhetrd at n = 120, 3.1 against 7.9 ms at 300, 8.0 against 20.3 ms at 800,
+// 15.5 against 38.7 ms at 1410. Per column in the middle of the reduction
+// (n = 1410, timestamps inside the kernel): 5.9 us, of which 2.4 waiting for
+// the exchanged data (memory latency), 0.7 the next column and its norm, 0.5
+// the reflector, 1.7 the owners' update and products (0.5 of it waiting at
+// their barrier for the waves with more rows).
 qs_status qs_hetrd_fused(qs_ctx *ctx, int n, double2 *A, double *d, double *e, double2 *tau, int nv, double2 *Y) {
   if (n < 1 || n > QS_HETRD_REG_MAX) return QS_EUNSUPPORTED;
   const size_t shm = (size_t)4 * n * sizeof(double2);
