@@ -199,10 +199,12 @@ void qs_blas_release(qs_ctx *ctx) {
 
 // The tridiagonal eigenproblem of the dense method goes to rocSOLVER's divide
 // and conquer (dstedc) on the GPU from order ctx->tridiag_dc on (default 256);
-// below it the host QL is faster. Measured on the MI355X (random and
-// glued-Wilkinson tridiagonal matrices): dstedc 0.8 / 1.4 / 1.9 ms at n = 190
-// / 800 / 1410 against 0.5 / 8.7 / 23 ms for the host QL with its rotations
-// applied to one vector; dstedc's eigenvectors were orthonormal to 3e-15.
+// below it the host QL is faster. Measured on the MI355X: dstedc 0.8 / 1.4 /
+// 1.9 ms at n = 190 / 800 / 1410 for random and glued-Wilkinson tridiagonal
+// matrices (localized eigenvectors, much deflation), about 6 ms at 1410 for
+// those of dense Hermitian matrices (little deflation), against 0.5 / 8.7 /
+// 23 ms for the host QL with its rotations applied to one vector; dstedc's
+// eigenvectors were orthonormal to 3e-15.
 // (rocSOLVER's sterf, steqr and stebz + stein took 0.3-50 s at these orders.)
 
 static qs_status blas_handle(qs_ctx *ctx, rocblas_handle *hb) {
@@ -335,7 +337,11 @@ __global__ __launch_bounds__(256) void k_apply_qh(int n, const double2 *__restri
 }
 
 // Y <- Q Y (the back-transform of tridiagonal eigenvectors): H_{n-2} first,
-// y -= tau v (v^+ y). Layout and order of sums as in k_apply_qh.
+// y -= tau v (v^+ y). Layout and order of sums as in k_apply_qh. (Blocks of 64
+// reflectors as compact WY products applied by rocBLAS zgemm took as long at
+// n = 1410, 24.1 against 23.5 ms for all eigenvectors, and longer at 500 and
+// 1000: the GEMM shapes, 64-column panels and a 64 x 64 Gram matrix over the
+// rows, ran at about 2.4 TFLOP/s.)
 __global__ __launch_bounds__(256) void k_apply_q(int n, const double2 *__restrict__ A,
                                                  const double2 *__restrict__ tau, double2 *__restrict__ Y) {
   __shared__ double red[4];

@@ -103,16 +103,26 @@ def dense_small(seed: int = 5, reps: int = 5):
     from .engine import dense_poles
     from ._lib import QS_PARAM_HETRD, QS_PARAM_TRIDIAG_DC
     Hd, t_build = _sync_time(lambda: Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense())
-    _, t_poles = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=False))
+
+    def best_of(vectors, hetrd=1, times=3):
+        # the reduction overwrites its matrix: a fresh copy per call; one
+        # untimed call first (library kernels load on their first use)
+        ctx.set_param(QS_PARAM_HETRD, hetrd)
+        try:
+            best = float("inf")
+            for it in range(times + 1):
+                A = Hd.clone()
+                _, t = _sync_time(lambda: dense_poles(A, vecs[:1], ctx, vectors=vectors))
+                if it:
+                    best = min(best, t)
+            return best
+        finally:
+            ctx.set_param(QS_PARAM_HETRD, 1)
+
+    t_poles = best_of(False)
     # the same with rocSOLVER's zhetrd + Q^+ X instead of the cooperative reduction
-    Hd = Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense()
-    ctx.set_param(QS_PARAM_HETRD, 0)
-    try:
-        _, t_poles_rs = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=False))
-    finally:
-        ctx.set_param(QS_PARAM_HETRD, 1)
-    Hd = Hop(Op(nf, pr["H"], ctx), basis, mode="matfree").dense()
-    _, t_eig = _sync_time(lambda: dense_poles(Hd, vecs[:1], ctx, vectors=True))
+    t_poles_rs = best_of(False, hetrd=0)
+    t_eig = best_of(True)
     diff = float(np.max(np.abs(sd.G - sl.G)) / np.max(np.abs(sd.G)))
     # eigenvalues + projections of 3 vectors for random dense complex matrices:
     # the defaults (cooperative reduction up to 800 rows, tridiagonal divide and
