@@ -116,9 +116,14 @@ qs_status qs_ctx_set_async(qs_ctx *ctx, int async);
  * QS_PARAM_TRIDIAG_DC (default 256, >= 1): the order from which the dense
  * method solves its tridiagonal eigenproblem on the GPU by rocSOLVER's divide
  * and conquer (dstedc); below it the host QL runs (faster for small orders).
- * All eigenvectors (qs_dense_poles with vectors = 1) always take dstedc. */
+ * All eigenvectors (qs_dense_poles with vectors = 1) always take dstedc.
+ * QS_PARAM_WAIT_SPINS (default 2^22, 0 to 2^40): how many times a wait inside
+ * the register-resident reduction polls for data another workgroup has not
+ * written yet before the launch gives up (every workgroup then stops, and the
+ * call returns QS_EHIP naming the wait); a bound, not a timing knob. */
 enum { QS_PARAM_SHIFT_MIN = 1, QS_PARAM_ELL_PACK = 2, QS_PARAM_BATCH_GRAPH = 3, QS_PARAM_TERM_LDS = 4,
-       QS_PARAM_ELL_ORDER = 5, QS_PARAM_ALLOC_TOP = 6, QS_PARAM_HETRD = 7, QS_PARAM_TRIDIAG_DC = 8 };
+       QS_PARAM_ELL_ORDER = 5, QS_PARAM_ALLOC_TOP = 6, QS_PARAM_HETRD = 7, QS_PARAM_TRIDIAG_DC = 8,
+       QS_PARAM_WAIT_SPINS = 9 };
 qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value);
 qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value);
 

@@ -21,6 +21,7 @@ QS_PARAM_SHIFT_MIN, QS_PARAM_ELL_PACK, QS_PARAM_BATCH_GRAPH, QS_PARAM_TERM_LDS, 
 QS_PARAM_ALLOC_TOP = 6
 QS_PARAM_HETRD = 7
 QS_PARAM_TRIDIAG_DC = 8
+QS_PARAM_WAIT_SPINS = 9
 QS_BATCH_KEEP_BASIS = 1
 STATUS = {0: "QS_OK", 1: "QS_ENOMEM_DEVICE", 2: "QS_ENOMEM_HOST", 3: "QS_EINVAL",
           4: "QS_EHIP", 5: "QS_ENOTCONVERGED", 6: "QS_EUNSUPPORTED"}

@@ -350,6 +350,10 @@ extern "C" qs_status qs_ctx_set_param(qs_ctx *ctx, int param, int64_t value) {
       if (value < 1) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_TRIDIAG_DC must be >= 1");
       ctx->tridiag_dc = value;
       return QS_OK;
+    case QS_PARAM_WAIT_SPINS:
+      if (value < 0 || value > ((int64_t)1 << 40)) return qs_fail(ctx, QS_EINVAL, "QS_PARAM_WAIT_SPINS must be in [0, 2^40]");
+      ctx->wait_spins = value;
+      return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);
   }
@@ -381,6 +385,9 @@ extern "C" qs_status qs_ctx_get_param(qs_ctx *ctx, int param, int64_t *value) {
       return QS_OK;
     case QS_PARAM_TRIDIAG_DC:
       *value = ctx->tridiag_dc;
+      return QS_OK;
+    case QS_PARAM_WAIT_SPINS:
+      *value = ctx->wait_spins;
       return QS_OK;
     default:
       return qs_fail(ctx, QS_EINVAL, "unknown parameter %d", param);

@@ -42,6 +42,7 @@ struct qs_ctx {
   int alloc_top = 0;               // QS_PARAM_ALLOC_TOP: device blocks from the top of free memory
   int hetrd = 1;                   // QS_PARAM_HETRD: the dense method's own cooperative reduction
   int64_t tridiag_dc = 256;        // QS_PARAM_TRIDIAG_DC: order from which dstedc solves the tridiagonal problem
+  int64_t wait_spins = 1 << 22;    // QS_PARAM_WAIT_SPINS: polls a wait of the dense reduction makes before failing
   void *blas = nullptr;            // rocblas handle for the dense eigensolver, made on first use
   void *pinned = nullptr;          // the ctx's own pinned staging buffer (qs_h2d / qs_d2h)
   size_t pinned_bytes = 0;

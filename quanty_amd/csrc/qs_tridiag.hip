@@ -100,7 +100,7 @@ __device__ __forceinline__ void st_mark(double2 *p) {
 // false returned. use[q] = false skips value q.
 template <int NP>
 __device__ __forceinline__ bool ld_wait_n(const double2 *const (&p)[NP], const bool (&use)[NP], double2 (&v)[NP],
-                                          int *err) {
+                                          int *err, long max_spins) {
   unsigned long long w[2 * NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) {
@@ -114,7 +114,7 @@ __device__ __forceinline__ bool ld_wait_n(const double2 *const (&p)[NP], const b
 #pragma unroll
     for (int q = 0; q < 2 * NP; ++q) all = all && w[q] != QS_MARK;
     if (all) break;
-    if (++spins > (1L << 22)) {
+    if (++spins > max_spins) {
       __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
@@ -257,7 +257,7 @@ __device__ void reg_larfg(double2 *c, int k0, int n, double2 &tau, double &beta,
 __global__ void __launch_bounds__(QS_HETRD_REG_BLOCK)
 k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__restrict__ e,
             double2 *__restrict__ tau, double2 *pbuf, double2 *part, double2 *colbuf, int nv,
-            double2 *__restrict__ Y, int *err) {
+            double2 *__restrict__ Y, int *err, long max_spins) {
   extern __shared__ double2 sm[];
   double2 *vk = sm, *wk = sm + n, *vn = sm + 2 * n;
   __shared__ double red[4 * (QS_HETRD_REG_BLOCK / 64)];
@@ -426,7 +426,7 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
       }
       use[4] = act && lane == 0;
       pp[4] = pr + k + 1;
-      ok = ld_wait_n<5>(pp, use, v, err);
+      ok = ld_wait_n<5>(pp, use, v, err, max_spins);
       double2 s = make_double2(0.0, 0.0);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -457,7 +457,7 @@ k_hetrd_reg(int n, double2 *__restrict__ A, double *__restrict__ d, double *__re
         pp[2 * r] = colr + (i < n ? i : 0);
         pp[2 * r + 1] = pr + (i < n ? i : 0);
       }
-      ok = ld_wait_n<2 * QS_HETRD_REG_RPT>(pp, use, v, err) && ok;
+      ok = ld_wait_n<2 * QS_HETRD_REG_RPT>(pp, use, v, err, max_spins) && ok;
 #pragma unroll
       for (int r = 0; r < QS_HETRD_REG_RPT; ++r) {
         cv[r] = v[2 * r];
@@ -674,7 +674,8 @@ qs_status qs_hetrd_fused(qs_ctx *ctx, int n, double2 *A, double *d, double *e, d
   // every exchange word starts as the marker (all bits set), err as zero
   QS_HIP(ctx, hipMemsetAsync(scr, 0xff, xbytes, ctx->stream));
   QS_HIP(ctx, hipMemsetAsync(err, 0, 64, ctx->stream));
-  void *args[] = {&n, &A, &d, &e, &tau, &pbuf, &part, &colbuf, &nv, &Y, &err};
+  long max_spins = (long)ctx->wait_spins;
+  void *args[] = {&n, &A, &d, &e, &tau, &pbuf, &part, &colbuf, &nv, &Y, &err, &max_spins};
   const hipError_t herr = hipLaunchCooperativeKernel((const void *)k_hetrd_reg, dim3(G), dim3(QS_HETRD_REG_BLOCK),
                                                      args, shm, ctx->stream);
   if (herr == hipErrorCooperativeLaunchTooLarge || herr == hipErrorNotSupported) {

@@ -656,6 +656,34 @@ def test_dense_reduction_cooperative_equals_rocsolver(n, kind, ctx):
     assert np.max(np.abs(Gt1 - Gt0)) < 1e-10 * np.max(np.abs(Gt0))
 
 
+def test_dense_reduction_wait_bound_ends_the_launch(ctx):
+    """Every wait of the cooperative reduction is bounded. With
+    QS_PARAM_WAIT_SPINS = 0 a wait gives up at its first poll that finds no
+    data (at n = 800, 134 workgroups over 799 columns, some poll always does):
+    that workgroup raises the shared error flag, every workgroup leaves the
+    launch at its next check, and the call raises the error naming the wait.
+    With the bound restored the next call is correct (the exchange buffers are
+    reset per call)."""
+    from quanty_amd._lib import QS_PARAM_WAIT_SPINS, QSError
+    n = 800
+    rng = np.random.default_rng(4242)
+    M = _hermitian(n, rng, "complex")
+    X = torch.from_numpy(rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))).cuda()
+    default = ctx.get_param(QS_PARAM_WAIT_SPINS)
+    try:
+        ctx.set_param(QS_PARAM_WAIT_SPINS, 0)
+        T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+        with pytest.raises(QSError, match="waited too long"):
+            dense_poles(T.t(), X, ctx, vectors=False)
+    finally:
+        ctx.set_param(QS_PARAM_WAIT_SPINS, default)
+    assert default == 1 << 22
+    T = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()
+    E, P = dense_poles(T.t(), X, ctx, vectors=False)
+    assert np.max(np.abs(E - np.linalg.eigvalsh(M))) < 1e-12 * np.max(np.abs(E))
+    assert np.allclose((np.abs(P) ** 2).sum(0), (np.abs(X.cpu().numpy()) ** 2).sum(1), rtol=1e-12)
+
+
 @pytest.mark.parametrize("n,kind", [(2, "complex"), (130, "blocks"), (800, "complex"), (1536, "complex"),
                                     (1537, "complex")])
 def test_dense_lowest_after_cooperative_reduction(n, kind, ctx):
