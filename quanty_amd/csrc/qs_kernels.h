@@ -429,6 +429,42 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Lane exchanges without LDS on the two halves of a double: DPP row
+// operations and scalar reads of one lane. join_f64: a double from its halves.
+__device__ __forceinline__ double join_f64(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// value of the lane that DPP control CTRL names (row_mirror 0x140: 15 - l in
+// a row of 16; row_half_mirror 0x141: 7 - l in 8; quad_perm 0xB1: l xor 1;
+// 0x4E: l xor 2)
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return join_f64((unsigned)lo, (unsigned)hi);
+}
+// the value in lane l (the same in every lane)
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  return join_f64((unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l),
+                  (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l));
+}
+// sum over the 8 lanes of a half-row, the same in each of them
+__device__ __forceinline__ double sum8_lanes(double x) {
+  x += dpp_f64<0xB1>(x);
+  x += dpp_f64<0x4E>(x);
+  x += dpp_f64<0x141>(x);
+  return x;
+}
+
+// Sum over a wavefront in a fixed order, the same in every lane: DPP within
+// rows of 16 lanes, then the four row sums in row order; whole wave active.
+__device__ __forceinline__ double wave_total(double x) {
+  x = sum8_lanes(x);
+  x += dpp_f64<0x140>(x);
+  return (readlane_f64(x, 0) + readlane_f64(x, 16)) + (readlane_f64(x, 32) + readlane_f64(x, 48));
+}
+
 // Block-wide sum of one value per thread in a fixed order; the result is valid
 // in every thread. red: at least blockDim.x / 64 doubles of LDS.
 __device__ __forceinline__ double block_sum_values(double v, double *red) {

@@ -371,6 +371,83 @@ __global__ __launch_bounds__(256) void k_apply_q(int n, const double2 *__restric
   }
 }
 
+#define QS_AQ_RPT 6  // rows per thread of k_apply_q_reg: n <= 1536
+
+// Y <- Q Y as k_apply_q, for n <= 256 QS_AQ_RPT: the workgroup's column in
+// registers (thread t holds rows t, t + 256, ...), reflector k-1's rows loaded
+// while step k reduces, one barrier per reflector (wave sums by DPP, the four
+// wave totals through double-buffered LDS, added in wave order).
+__global__ __launch_bounds__(256) void k_apply_q_reg(int n, const double2 *__restrict__ A,
+                                                     const double2 *__restrict__ tau, double2 *__restrict__ Y) {
+  __shared__ double red[2][8];
+  const int tid = (int)threadIdx.x, wv = tid >> 6;
+  double2 *y = Y + (size_t)blockIdx.x * n;
+  double2 yr[QS_AQ_RPT], vc[QS_AQ_RPT], vx[QS_AQ_RPT];
+#pragma unroll
+  for (int r = 0; r < QS_AQ_RPT; ++r) {
+    const int i = tid + 256 * r;
+    yr[r] = i < n ? y[i] : make_double2(0.0, 0.0);
+  }
+  // rows of reflector k: zero above row k+1, its implicit unit at k+1
+  auto load_v = [&](int k, double2(&v)[QS_AQ_RPT]) {
+    const double2 *col = A + (size_t)k * n;
+#pragma unroll
+    for (int r = 0; r < QS_AQ_RPT; ++r) {
+      const int i = tid + 256 * r;
+      v[r] = i > k + 1 && i < n ? col[i] : make_double2(i == k + 1 ? 1.0 : 0.0, 0.0);
+    }
+  };
+  if (n < 2) return;
+  load_v(n - 2, vc);
+  int buf = 0;
+  for (int k = n - 2; k >= 0; --k) {
+    if (k > 0) load_v(k - 1, vx);
+    const double2 t = tau[k];
+    if (t.x != 0.0 || t.y != 0.0) {
+      double sr = 0.0, si = 0.0;  // v^+ y
+#pragma unroll
+      for (int r = 0; r < QS_AQ_RPT; ++r) {
+        sr += vc[r].x * yr[r].x + vc[r].y * yr[r].y;
+        si += vc[r].x * yr[r].y - vc[r].y * yr[r].x;
+      }
+      sr = wave_total(sr);
+      si = wave_total(si);
+      if ((tid & 63) == 0) {
+        red[buf][2 * wv] = sr;
+        red[buf][2 * wv + 1] = si;
+      }
+      __syncthreads();
+      sr = (red[buf][0] + red[buf][2]) + (red[buf][4] + red[buf][6]);
+      si = (red[buf][1] + red[buf][3]) + (red[buf][5] + red[buf][7]);
+      buf ^= 1;
+      const double fr = t.x * sr - t.y * si, fi = t.x * si + t.y * sr;  // tau (v^+ y)
+#pragma unroll
+      for (int r = 0; r < QS_AQ_RPT; ++r) {
+        yr[r].x -= vc[r].x * fr - vc[r].y * fi;
+        yr[r].y -= vc[r].x * fi + vc[r].y * fr;
+      }
+    }
+    if (k > 0)
+#pragma unroll
+      for (int r = 0; r < QS_AQ_RPT; ++r) vc[r] = vx[r];
+  }
+#pragma unroll
+  for (int r = 0; r < QS_AQ_RPT; ++r) {
+    const int i = tid + 256 * r;
+    if (i < n) y[i] = yr[r];
+  }
+}
+
+// Y <- Q Y for ncols columns of Y (n x ncols, column-major).
+static hipError_t launch_apply_q(qs_ctx *ctx, int n, const double2 *A, const double2 *tau, double2 *Y, int ncols) {
+  if (n < 2 || ncols < 1) return hipSuccess;
+  if (n <= 256 * QS_AQ_RPT)
+    hipLaunchKernelGGL(k_apply_q_reg, dim3((unsigned)ncols), dim3(256), 0, ctx->stream, n, A, tau, Y);
+  else
+    hipLaunchKernelGGL(k_apply_q, dim3((unsigned)ncols), dim3(256), 0, ctx->stream, n, A, tau, Y);
+  return hipGetLastError();
+}
+
 // Householder reduction A = Q T Q^+ (T real symmetric tridiagonal: d, e; A
 // holds the reflectors, tau their factors) with Y <- Q^+ Y for nv vectors: the
 // library's cooperative reduction when it takes the order, rocSOLVER's zhetrd
@@ -442,11 +519,7 @@ static qs_status dense_poles_householder(qs_ctx *ctx, rocblas_handle hb, int n, 
       hipLaunchKernelGGL(k_real_to_complex, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, ctx->stream, cnt,
                          (const double *)dZ, dV);
       e = hipGetLastError();
-      if (e == hipSuccess && n > 1) {
-        hipLaunchKernelGGL(k_apply_q, dim3((unsigned)n), dim3(256), 0, ctx->stream, n, (const double2 *)A,
-                           (const double2 *)dtau, dV);
-        e = hipGetLastError();
-      }
+      if (e == hipSuccess) e = launch_apply_q(ctx, n, (const double2 *)A, (const double2 *)dtau, dV, n);
       if (e == hipSuccess) e = hipMemcpyAsync(A, dV, (size_t)n * n * 16, hipMemcpyDeviceToDevice, ctx->stream);
     }
     if (st == QS_OK && e == hipSuccess) e = hipMemcpyAsync(E, dd, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream);
@@ -533,11 +606,8 @@ extern "C" qs_status qs_dense_lowest(qs_ctx *ctx, int n, double *A, int k, doubl
       if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     }
   }
-  if (st == QS_OK && e == hipSuccess && k > 0 && n > 1) {
-    hipLaunchKernelGGL(k_apply_q, dim3((unsigned)k), dim3(256), 0, ctx->stream, n, (const double2 *)A,
-                       (const double2 *)dtau, (double2 *)V);
-    e = hipGetLastError();
-  }
+  if (st == QS_OK && e == hipSuccess && k > 0 && n > 1)
+    e = launch_apply_q(ctx, n, (const double2 *)A, (const double2 *)dtau, (double2 *)V, k);
   if (st == QS_OK && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   else (void)hipStreamSynchronize(ctx->stream);
   qs_free(mem);

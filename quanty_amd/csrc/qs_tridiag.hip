@@ -132,15 +132,12 @@ __device__ __forceinline__ bool ld_wait_n(const double2 *const (&p)[NP], const b
   return true;
 }
 
-// Lane exchanges without LDS (gfx950 permlane swaps, DPP), on the two halves
-// of a double. fold32(a, b): a(l) + a(l+32) in lanes l < 32, b(l-32) + b(l)
-// in lanes l >= 32; fold16 likewise between rows 2r and 2r+1 of 16 lanes.
+// Lane exchanges without LDS (gfx950 permlane swaps), on the two halves of a
+// double. fold32(a, b): a(l) + a(l+32) in lanes l < 32, b(l-32) + b(l) in
+// lanes l >= 32; fold16 likewise between rows 2r and 2r+1 of 16 lanes.
 struct double4_ {
   double v[4];
 };
-__device__ __forceinline__ double join_f64(unsigned lo, unsigned hi) {
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 __device__ __forceinline__ double fold32(double a, double b) {
   const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
   const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
@@ -155,29 +152,6 @@ __device__ __forceinline__ double fold16(double a, double b) {
   const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
   return join_f64(lo[0], hi[0]) + join_f64(lo[1], hi[1]);
 }
-// value of the lane that DPP control CTRL names (row_mirror 0x140: 15 - l in
-// a row of 16; row_half_mirror 0x141: 7 - l in 8; quad_perm 0xB1: l xor 1;
-// 0x4E: l xor 2)
-template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
-  return join_f64((unsigned)lo, (unsigned)hi);
-}
-// the value in lane l (the same in every lane)
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-  return join_f64((unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l),
-                  (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l));
-}
-// sum over the 8 lanes of a half-row, the same in each of them
-__device__ __forceinline__ double sum8_lanes(double x) {
-  x += dpp_f64<0xB1>(x);
-  x += dpp_f64<0x4E>(x);
-  x += dpp_f64<0x141>(x);
-  return x;
-}
-
 // Wave sums of 8 values per lane by recursive halving: at lane distances 32
 // and 16 each lane keeps half of its values and adds its partner's copy of
 // them, then pairs lanes l and 15-l for one value and sums over 8 lanes.
