@@ -503,27 +503,42 @@ class SpectrumServer:
         if extend:
             self.batch.set_depths([j.slot for j in extend], [j.gs_depth for j in extend])
 
-    def _start(self, job: _Job):
-        """Put a prepared job into the batch."""
+    def _start(self, job: _Job) -> bool:
+        """Put a prepared job into the batch. False: its slot's device memory
+        is not to be had while other jobs run (the budget counts bytes, but a
+        slot's Krylov basis needs one contiguous block); the caller keeps the
+        job and tries again when jobs have finished. With no other job
+        running the error is raised."""
         job.gs_max = min(self.gs_max, job.basis.n)
+
+        def add():
+            job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max,
+                                      keep_basis=self.keep_basis)
+
         try:
-            job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max,
-                                      keep_basis=self.keep_basis)
+            add()
         except QSError as e:
-            if "QS_ENOMEM_DEVICE" not in str(e) or self.patterns is None or not self.patterns.entries:
+            if "QS_ENOMEM_DEVICE" not in str(e):
                 raise
-            # the cached streams may split the free device memory into pieces
-            # too small for this slot's Krylov basis: drop the cache (streams
-            # still in use stay with their requests) and try once more
-            import gc
-            self.patterns.clear()
-            gc.collect()
-            job.slot = self.batch.add(job.hop, job.v0, job.gs_depth, max_depth=job.gs_max,
-                                      keep_basis=self.keep_basis)
+            try:
+                if self.patterns is None or not self.patterns.entries:
+                    raise e
+                # the cached streams may split the free device memory into
+                # pieces too small for this slot's Krylov basis: drop the cache
+                # (streams still in use stay with their requests), try again
+                import gc
+                self.patterns.clear()
+                gc.collect()
+                add()
+            except QSError as e2:
+                if "QS_ENOMEM_DEVICE" not in str(e2) or not self.jobs:
+                    raise
+                return False
         if self.keep_basis:
             job.kept_bytes = 16 * (job.gs_max + 1) * job.basis.n
         self.jobs[job.req.rid] = job
         self.used += job.bytes + job.kept_bytes
+        return True
 
     def serve(self, requests: list[Request], realtime: bool = True, plain: bool = False,
               progress=None) -> list[Request]:
@@ -542,6 +557,12 @@ class SpectrumServer:
                      (self.start_temperature, self.start_floor), self.patterns, self.product_layout) \
             if (self.prepare_ahead and not plain) else None
         inflight: dict[int, int] = {}  # rid -> reserved bytes of requests being prepared
+        deferred: list[_Job] = []  # prepared jobs waiting for device memory (their bytes counted in used)
+
+        def start(job):
+            if not self._start(job):
+                deferred.append(job)
+                self.used += job.bytes
         kept = self.gs_max + 1 if self.keep_basis else 0
         slots = 1 if plain else self.max_slots
         # The preparation threads run Python between their library calls; the
@@ -561,7 +582,7 @@ class SpectrumServer:
                     i += 1
                 # admission: FIFO, by free slots and HBM budget; a request that
                 # would not fit the budget even alone is refused at once
-                while waiting and len(self.jobs) + len(inflight) < slots:
+                while waiting and len(self.jobs) + len(inflight) + len(deferred) < slots:
                     r = waiting[0]
                     est = estimate_bytes(r.dim, kept)
                     if est > self.budget:
@@ -577,8 +598,8 @@ class SpectrumServer:
                     waiting.pop(0)
                     r.t_admit = time.perf_counter() - t0
                     if prep is None:
-                        self._start(_Job(r, self.ctx, self.gs_depth, self.start_temperature, self.start_floor,
-                                         patterns=self.patterns, product=self.product_layout))
+                        start(_Job(r, self.ctx, self.gs_depth, self.start_temperature, self.start_floor,
+                                   patterns=self.patterns, product=self.product_layout))
                     else:
                         inflight[r.rid] = est
                         self.used += est
@@ -592,7 +613,7 @@ class SpectrumServer:
                     if isinstance(job, BaseException):
                         raise job
                     self.used -= inflight.pop(job.req.rid)
-                    self._start(job)
+                    start(job)
                 if not self.jobs:
                     if i < len(pending) and realtime and not inflight:
                         time.sleep(max(0.0, pending[i].arrival - (time.perf_counter() - t0)))
@@ -600,6 +621,14 @@ class SpectrumServer:
                 self.batch.run(self.chunk)
                 before = len(done)
                 self._step_jobs(t0, done)
+                # jobs that waited for device memory, in arrival order, now
+                # that finished jobs have released theirs
+                while deferred:
+                    self.used -= deferred[0].bytes
+                    if not self._start(deferred[0]):
+                        self.used += deferred[0].bytes
+                        break
+                    deferred.pop(0)
                 if progress is not None:
                     step = max(1, len(pending) // 20)
                     if len(done) // step > before // step:
